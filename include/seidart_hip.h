@@ -1,0 +1,160 @@
+/*
+ * seidart_hip.h -- C ABI of libseidart_hip.so, the MI355X (gfx950) FDTD
+ * time-stepping core behind SeidarT's run path.
+ *
+ * What it replaces.  The reference crosses a process boundary at
+ *     subprocess.call(['seidartfdtd', json, 'seismic=..', 'density_method=..'])
+ * in src/seidart/routines/classes.py:878-924 (Model.run), and the solver
+ * behind it is the Komatitsch/Martin C-PML staggered-grid scheme whose
+ * readable source is legacy/fortran/seismicfdtd.f95 (seismic2 :12-385,
+ * seismic25 :388-949) and legacy/fortran/electromagfdtd.f95 (electromag2
+ * :10-326, electromag25 :619-1092).  The receiver gather that the reference
+ * does in Python from per-step snapshot files (src/seidart/routines/
+ * arraybuild.py:199-295) happens on the device here.  The entry points below
+ * are what a ctypes / cgo / JNI binding of that boundary would call; the
+ * Python binding is seidart_amd/_hip.py.
+ *
+ * Conventions.
+ *   - All grids are the padded grid of the reference (nx = domain.nx + 2*cpml
+ *     and so on, classes.py:608-679), x fastest (Fortran column-major), with
+ *     every field of a 2-D run stored as nx*nz and of a 2.5-D run as nx*ny*nz
+ *     (cells outside a staggered field's Fortran extent stay zero).
+ *   - Every array pointer inside sdt_model / sdt_shot is DEVICE memory owned
+ *     by the caller, in the plan's precision (double for SDT_FP64, float for
+ *     SDT_FP32) except rec (int32).  The library never frees them.
+ *   - The library owns only its descriptor copies, work list and per-shot
+ *     instability flags (allocated in sdt_plan_create, freed in
+ *     sdt_plan_destroy).
+ *   - Errors: every call returns 0 on success or an SDT_E* code; the message
+ *     of the last failure on the calling thread is in sdt_last_error().
+ *     Nothing calls exit() (the reference's solver stops the process and its
+ *     caller ignores the status, classes.py:910-924).
+ *   - Threading: a plan is used by one thread at a time; plans are
+ *     independent.  sdt_plan_run only enqueues on the given stream.
+ */
+#ifndef SEIDART_HIP_H
+#define SEIDART_HIP_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* physics kinds: which reference subroutine's scheme runs */
+#define SDT_ELASTIC2  0 /* seismic2     seismicfdtd.f95:12-385       */
+#define SDT_ELASTIC25 1 /* seismic25    seismicfdtd.f95:388-949      */
+#define SDT_EM2       2 /* electromag2  electromagfdtd.f95:10-326    */
+#define SDT_EM25      3 /* electromag25 electromagfdtd.f95:619-1092  */
+
+/* arithmetic precision of fields, coefficients and traces */
+#define SDT_FP64 0
+#define SDT_FP32 1
+
+/* error codes */
+#define SDT_OK        0
+#define SDT_EINVAL    1 /* bad argument (sizes, indices, null pointers)  */
+#define SDT_EDEVICE   2 /* HIP runtime failure                           */
+#define SDT_EUNSTABLE 3 /* a field exceeded 1e25 or became non-finite    */
+
+/* A model: one padded grid and the kernel coefficient arrays built from its
+ * project (FP64 host setup, then stored in the plan's precision).
+ * coef[] slots per kind:
+ *   ELASTIC2 (12): n11 n13 n15 n33 n35 | s15 s35 s55 | dtr_x dtr_z | om_x om_z
+ *       n* = (c(i+1,j)+2c(i,j)+c(i,j-1))/4 at the normal-stress node,
+ *       s* = (c(i,j+1)+2c(i,j)+c(i-1,j))/4 at the shear node
+ *       (seismicfdtd.f95:269-278, :307-310); dtr_x = dt/rho(i,j),
+ *       dtr_z = dt/((2rho(i,j)+rho(i+1,j)+rho(i,j+1))/4) (:334, :342, :354);
+ *       om = 1 - gamma (:334, :354).
+ *   ELASTIC25 (39): the 4-point Cij averages of the four stress loops
+ *       (seismicfdtd.f95:651-673 [15], :716-721 [6], :761-767 [6],
+ *       :806-811 [6]), dt/deltarho for vx, vy, vz (:826, :853, :878),
+ *       1 - gamma for vx, vy, vz.  All 2-D (nx*nz), y-invariant.
+ *   EM2 (4): caEx cbEx caEz cbEz, already 2-point averaged
+ *       (electromagfdtd.f95:171-177, :281-282, :293-294).
+ *   EM25 (6): caEx cbEx, caEy cbEy, caEz cbEz; Ey/Ez already (4,1,1)/6
+ *       averaged (electromagfdtd.f95:770-781, :985-1026).  2-D, y-invariant.
+ * prof[axis][q]: 1-D C-PML profiles per axis (x: nx, y: ny, z: nz entries),
+ *   q = 0 1/kappa, 1 a, 2 b, 3 1/kappa_half, 4 a_half, 5 b_half
+ *   (the values the legacy loadcpml streams in, readwrite_routines.f95:26-37;
+ *   kappa is passed as its reciprocal).  Unused axes may be NULL.
+ * aux: ELASTIC2 only, nz entries: b_x_half indexed by the z row, the decay
+ *   the legacy normal-stress loop applies to memory_dvx_dx
+ *   (seismicfdtd.f95:255); NULL elsewhere.
+ * lx, ly, lz: psi slab widths.  Cells with i < lx or i >= nx - lx carry x
+ *   memory variables (likewise y, z); every other cell must have a = 0 and
+ *   kappa = 1 on that axis.  0 <= l <= (n+1)/2.
+ * scal[0]: EM only, dt/mu0 (electromagfdtd.f95:179-181).
+ */
+typedef struct sdt_model {
+  int nx, ny, nz;
+  int lx, ly, lz;
+  double dx, dy, dz, dt;
+  double scal[4];
+  const void* coef[40];
+  const void* prof[3][6];
+  const void* aux;
+} sdt_model;
+
+/* A shot: its state, C-PML memory, source and receivers, on one model.
+ * field[]: ELASTIC2 vx vz sxx szz sxz | ELASTIC25 vx vy vz sxx syy szz sxy sxz syz
+ *          EM2 Ex Ez Hy | EM25 Ex Ey Ez Hx Hy Hz.   In/out; initial values are
+ *          the initial conditions (classes.py:888-898).
+ * psi: sdt_psi_elems() elements, zero before the first step, kept between
+ *      calls of sdt_plan_run (a run may be split into segments).
+ * isrc, jsrc, ksrc: 0-based padded source cell (jsrc ignored in 2-D).
+ * src[c]: nt per-step increments added to component c at the source cell
+ *      after the update (seismicfdtd.f95:359-361: s(it)*dt/rho(src);
+ *      electromagfdtd.f95:300-301: s(it)*dt/eps_r(src)); NULL = none.
+ *      Components are (x, y, z); 2-D kinds use x and z.
+ * rec: nrec triples (i, j, k), 0-based padded; trace: nt * ncomp * nrec,
+ *      trace[it][c][r] = component c at receiver r after step it, with
+ *      ncomp = 2 (ELASTIC2: vx vz, EM2: Ex Ez) or 3 (vx vy vz / Ex Ey Ez).
+ */
+typedef struct sdt_shot {
+  int model;
+  int isrc, jsrc, ksrc;
+  void* field[9];
+  void* psi;
+  const void* src[3];
+  int nrec;
+  const int* rec;
+  void* trace;
+} sdt_shot;
+
+/* Elements of one shot's psi workspace for a kind and grid (slab storage:
+ * each memory variable lives only on the slabs of its own axis). */
+int sdt_psi_elems(int kind, int nx, int ny, int nz, int lx, int ly, int lz,
+                  long long* out_elems);
+
+/* Build a plan over nmodels models and nshots shots (each shot names its
+ * model).  One sdt_plan_run step advances every shot by one time step; the
+ * shots of all models run in the same kernel launches over one
+ * (shot, tile) work list.  nt bounds the step index.  Replaces one
+ * `seidartfdtd` process per shot (classes.py:910-924,
+ * src/seidart/simulations/common_offset.py:143-283). */
+int sdt_plan_create(int kind, int precision, const sdt_model* models, int nmodels,
+                    const sdt_shot* shots, int nshots, int nt, void** out_plan);
+
+/* Enqueue steps it0 .. it0+nsteps-1 on `stream` (a hipStream_t, NULL =
+ * default stream).  graph_block > 1 captures that many steps per hipGraph
+ * and replays it; 0 or 1 launches kernels directly. */
+int sdt_plan_run(void* plan, int it0, int nsteps, int graph_block, void* stream);
+
+/* Synchronise the plan's last stream and report instability: *first_step
+ * gets the first step at which any shot's field exceeded 1e25 in magnitude
+ * or became non-finite (seismicfdtd.f95:378-379), or -1.  Returns
+ * SDT_EUNSTABLE when such a step exists.  per_shot (optional, nshots ints)
+ * gets the step per shot (-1 = stable). */
+int sdt_plan_check(void* plan, int* first_step, int* per_shot);
+
+/* Release the plan's own device memory and graphs. */
+void sdt_plan_destroy(void* plan);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* sdt_last_error(void);
+
+/* Library version string. */
+const char* sdt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,150 @@
+"""ctypes binding of oracle/liboracle.so (the FP64 CPU restatement of the
+legacy solver loops).  TEST INFRASTRUCTURE ONLY: used by tests/, by
+__graft_entry__.smoke() and by bench.py's cpu_baseline leg, always as the
+checker or the CPU baseline, never as the product path."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+KIND = {"elastic2": 0, "elastic25": 1, "em2": 2, "em25": 3}
+_P = C.POINTER(C.c_double)
+
+
+class OrRun(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+        ("nt", C.c_int), ("it0", C.c_int), ("nsteps", C.c_int),
+        ("coef", _P * 22), ("gam", _P * 3), ("prof", (_P * 6) * 3),
+        ("isrc", C.c_int), ("jsrc", C.c_int), ("ksrc", C.c_int),
+        ("src", _P * 3),
+        ("nrec", C.c_int), ("rec", C.POINTER(C.c_int)), ("trace", _P),
+        ("f", _P * 9), ("psi", _P),
+        ("blowup_step", C.c_int), ("threads", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the Makefile beside this file."""
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-C", HERE, "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+        _lib.or_run.argtypes = [C.c_int, C.POINTER(OrRun)]
+        _lib.or_run.restype = C.c_int
+        _lib.or_psi_len.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+        _lib.or_psi_len.restype = C.c_longlong
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def _coef_list(p):
+    if p.kind.startswith("elastic"):
+        from seidart_amd.problem import STIFFNESS
+        names = STIFFNESS
+    else:
+        from seidart_amd.problem import PERMITTIVITY, CONDUCTIVITY
+        names = PERMITTIVITY + CONDUCTIVITY if p.kind == "em25" else \
+            ["e11", "e13", "e33", "s11", "s13", "s33"]
+    out = []
+    for n in names:
+        a = p.coef.get(n)
+        out.append(None if a is None else np.ascontiguousarray(a, dtype=np.float64))
+    return out
+
+
+class OracleRun:
+    """Run one shot of a Problem on the CPU oracle, optionally in segments
+    (state and psi are kept between calls)."""
+
+    def __init__(self, p, shot: int = 0, threads: int = 0):
+        self.p = p
+        self.shot = p.shots[shot]
+        self.kind = KIND[p.kind]
+        self.threads = threads
+        shape = p.field_shape
+        from seidart_amd.problem import FIELDS, ncomp
+        self.names = FIELDS[p.kind]
+        init = (p.init[shot] if p.init else None) or {}
+        self.fields = [np.ascontiguousarray(init.get(n, np.zeros(shape)), dtype=np.float64).copy()
+                       for n in self.names]
+        self.psi = np.zeros(lib().or_psi_len(self.kind, p.nx, p.ny, p.nz))
+        self.coefs = _coef_list(p)
+        self.gam = [p.gamma.get(a) for a in ("x", "y", "z")] if p.kind.startswith("elastic") else [None] * 3
+        self.gam = [None if g is None else np.ascontiguousarray(g, dtype=np.float64) for g in self.gam]
+        self.prof = []
+        for a in ("x", "y", "z"):
+            pr = p.prof.get(a)
+            if pr is None:
+                self.prof.append([None] * 6)
+            else:
+                self.prof.append([np.ascontiguousarray(v, dtype=np.float64)
+                                  for v in (pr.kappa, pr.a, pr.b, pr.kappa_h, pr.a_h, pr.b_h)])
+        self.src = [None if f is None else np.ascontiguousarray(f, dtype=np.float64)
+                    for f in self.shot.force]
+        self.rec = np.ascontiguousarray(np.asarray(self.shot.rec, dtype=np.int32).reshape(-1, 3))
+        self.ncomp = ncomp(p.kind)
+        self.trace = np.zeros((p.nt, self.ncomp, self.rec.shape[0]))
+        self.blowup = -1
+
+    def run(self, it0: int = 0, nsteps: int | None = None):
+        p = self.p
+        if nsteps is None:
+            nsteps = p.nt - it0
+        r = OrRun()
+        r.nx, r.ny, r.nz = p.nx, p.ny, p.nz
+        r.dx, r.dy, r.dz, r.dt = p.dx, p.dy, p.dz, p.dt
+        r.nt, r.it0, r.nsteps = p.nt, it0, nsteps
+        for q, a in enumerate(self.coefs):
+            r.coef[q] = _ptr(a)
+        for q, a in enumerate(self.gam):
+            r.gam[q] = _ptr(a)
+        for ax in range(3):
+            for q in range(6):
+                r.prof[ax][q] = _ptr(self.prof[ax][q])
+        r.isrc, r.jsrc, r.ksrc = (int(v) for v in self.shot.src)
+        for q in range(3):
+            r.src[q] = _ptr(self.src[q])
+        r.nrec = self.rec.shape[0]
+        r.rec = self.rec.ctypes.data_as(C.POINTER(C.c_int))
+        r.trace = _ptr(self.trace)
+        for q, a in enumerate(self.fields):
+            r.f[q] = _ptr(a)
+        r.psi = _ptr(self.psi)
+        r.blowup_step = self.blowup
+        r.threads = self.threads
+        rc = lib().or_run(self.kind, C.byref(r))
+        if rc:
+            raise RuntimeError(f"oracle failed with code {rc}")
+        self.blowup = r.blowup_step
+        return self
+
+    def field(self, name):
+        return self.fields[self.names.index(name)]
+
+
+def run_problem(p, shot: int = 0, threads: int = 0):
+    """Full run of one shot; returns (trace [nt, ncomp, nrec], fields dict, blowup step)."""
+    o = OracleRun(p, shot, threads).run()
+    return o.trace, dict(zip(o.names, o.fields)), o.blowup

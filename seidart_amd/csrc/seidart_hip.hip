@@ -1,0 +1,1152 @@
+// seidart_hip.hip -- MI355X (gfx950) FDTD time-stepping core of SeidarT.
+//
+// Hand-written HIP kernels for the four reference schemes plus the C ABI
+// declared in include/seidart_hip.h.  One source, templated on the
+// arithmetic type T (double or float).  The scheme of every kernel is the
+// legacy Fortran loop it cites; the numerical conventions (which psi array
+// decays with which profile, which coefficient average feeds which stress)
+// are kept exactly, including the legacy's shared psi arrays.
+//
+// Data layout in HBM (all x fastest, padded grid, one plane per shot):
+//   fields        nx*nz (2-D) or nx*ny*nz (2.5-D), precision T
+//   coefficients  nx*nz per model, shared by every shot on that model
+//   psi           slab storage: x-axis memory variables only on the
+//                 i < lx / i >= nx-lx columns, z-axis ones only on the
+//                 j < lz / j >= nz-lz rows (likewise y in 2.5-D), so the
+//                 interior carries no C-PML state at all.
+//   traces        [it][comp][rec] per shot
+//
+// Execution: every time step is two field-update launches (stress then
+// velocity, or H then E) over one (shot, tile) work list covering every
+// shot of every model, plus one receiver-gather launch.  Work items are
+// ordered so that all shots of one tile land on the same XCD (blocks are
+// dealt to the 8 XCDs round-robin), so a model's coefficient tile is read
+// from HBM once and served from that XCD's L2 to the other shots.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/seidart_hip.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail(SDT_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int TX = 64;  // tile width in x (one wavefront row)
+constexpr int TZ = 4;   // tile height in z
+constexpr int NT = TX * TZ;
+
+// Device copies of the descriptors.
+struct DModel {
+  int nx, ny, nz, lx, ly, lz;
+  double dx, dy, dz, dt;
+  double scal[4];
+  const void* coef[40];
+  const void* prof[3][6];
+  const void* aux;
+};
+
+struct DShot {
+  int model, isrc, jsrc, ksrc;
+  void* f[9];
+  void* psi;
+  const void* src[3];
+  int nrec;
+  const int* rec;
+  void* trace;
+  int* blowup;
+};
+
+enum { PK = 0, PA, PB, PKH, PAH, PBH };
+
+template <typename T>
+__device__ __forceinline__ const T* cp(const void* p) {
+  return static_cast<const T*>(p);
+}
+template <typename T>
+__device__ __forceinline__ T* mp(void* p) {
+  return static_cast<T*>(p);
+}
+
+// Flag instability: |v|^2 > 1e50 or non-finite (seismicfdtd.f95:378-379).
+__device__ __forceinline__ void check_blowup(double v2, int* flag, int it) {
+  if (!(v2 <= 1.0e50)) atomicMin(flag, it);
+}
+
+// slab offset helpers: position of column i inside the two x slabs
+__device__ __forceinline__ int slab_pos(int i, int n, int l) { return i < l ? i : i - n + 2 * l; }
+__device__ __forceinline__ bool in_slab(int i, int n, int l) { return i < l || i >= n - l; }
+
+// ===========================================================================
+// ELASTIC2: seismic2, legacy/fortran/seismicfdtd.f95:12-385
+// psi slabs: x [nz][2lx]: dvx_dx dvz_dx dsxx_dx dsxz_dx
+//            z [2lz][nx]: dvz_dz dvx_dz dszz_dz dsxz_dz
+// ===========================================================================
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_el2_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+             const int4* __restrict__ work) {
+  const int4 wi = work[blockIdx.x];
+  if (wi.x < 0) return;
+  const DShot& S = shots[wi.x];
+  const DModel& M = models[S.model];
+  const int nx = M.nx, nz = M.nz;
+  const int i = wi.y + threadIdx.x, j = wi.z + threadIdx.y;
+  if (i >= nx || j >= nz) return;
+  const T* __restrict__ vx = cp<T>(S.f[0]);
+  const T* __restrict__ vz = cp<T>(S.f[1]);
+  const long c = (long)j * nx + i;
+  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dt = T(M.dt);
+  const int lx = M.lx, lz = M.lz;
+  const bool inx = in_slab(i, nx, lx), inz = in_slab(j, nz, lz);
+  T* psi = mp<T>(S.psi);
+  const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
+  const long ox = (long)j * 2 * lx + slab_pos(i, nx, lx);
+  const long oz = (long)slab_pos(j, nz, lz) * nx + i;
+  T m_vxx = 0, m_vzx = 0, m_vzz = 0, m_vxz = 0;
+  if (inx) { m_vxx = psi[ox]; m_vzx = psi[SX + ox]; }
+  if (inz) { m_vzz = psi[4 * SX + oz]; m_vxz = psi[4 * SX + SZ + oz]; }
+  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
+  const T *ikx = P[0 * 6 + PK], *ax = P[0 * 6 + PA], *bx = P[0 * 6 + PB];
+  const T *ikxh = P[0 * 6 + PKH], *axh = P[0 * 6 + PAH], *bxh = P[0 * 6 + PBH];
+  const T *ikz = P[2 * 6 + PK], *az = P[2 * 6 + PA], *bz = P[2 * 6 + PB];
+  const T *ikzh = P[2 * 6 + PKH], *azh = P[2 * 6 + PAH], *bzh = P[2 * 6 + PBH];
+  const T vxc = vx[c], vzc = vz[c];
+  // normal stresses, seismicfdtd.f95:247-281
+  if (j >= 1 && i <= nx - 2) {
+    T dvx_dx = (vx[c + 1] - vxc) * idx;
+    T dvz_dz = (vzc - vz[c - nx]) * idz;
+    T dvz_dx = (vz[c + 1] - vzc) * idx;
+    T dvx_dz = (vxc - vx[c - nx]) * idz;
+    if (inx) {
+      m_vxx = cp<T>(M.aux)[j] * m_vxx + axh[i] * dvx_dx;  // b_x_half(j), :255
+      m_vzx = bx[i] * m_vzx + ax[i] * dvz_dx;
+      dvx_dx = dvx_dx * ikxh[i] + m_vxx;
+      dvz_dx = dvz_dx * ikx[i] + m_vzx;
+    }
+    if (inz) {
+      m_vzz = bz[j] * m_vzz + az[j] * dvz_dz;
+      m_vxz = bzh[j] * m_vxz + azh[j] * dvx_dz;
+      dvz_dz = dvz_dz * ikz[j] + m_vzz;
+      dvx_dz = dvx_dz * ikzh[j] + m_vxz;
+    }
+    const T sh = dvz_dx + dvx_dz;
+    T* sxx = mp<T>(S.f[2]);
+    T* szz = mp<T>(S.f[3]);
+    const T n13 = cp<T>(M.coef[1])[c];
+    sxx[c] = sxx[c] + (cp<T>(M.coef[0])[c] * dvx_dx + n13 * dvz_dz + cp<T>(M.coef[2])[c] * sh) * dt;
+    szz[c] = szz[c] + (n13 * dvx_dx + cp<T>(M.coef[3])[c] * dvz_dz + cp<T>(M.coef[4])[c] * sh) * dt;
+  }
+  // shear stress, seismicfdtd.f95:283-312 (same psi arrays: A1)
+  if (j <= nz - 2 && i >= 1) {
+    T dvx_dx = (vxc - vx[c - 1]) * idx;
+    T dvz_dz = (vz[c + nx] - vzc) * idz;
+    T dvz_dx = (vzc - vz[c - 1]) * idx;
+    T dvx_dz = (vx[c + nx] - vxc) * idz;
+    if (inx) {
+      m_vxx = bxh[i] * m_vxx + axh[i] * dvx_dx;
+      m_vzx = bx[i] * m_vzx + ax[i] * dvz_dx;
+      dvx_dx = dvx_dx * ikxh[i] + m_vxx;
+      dvz_dx = dvz_dx * ikx[i] + m_vzx;
+    }
+    if (inz) {
+      m_vzz = bz[j] * m_vzz + az[j] * dvz_dz;
+      m_vxz = bzh[j] * m_vxz + azh[j] * dvx_dz;
+      dvz_dz = dvz_dz * ikz[j] + m_vzz;
+      dvx_dz = dvx_dz * ikzh[j] + m_vxz;
+    }
+    T* sxz = mp<T>(S.f[4]);
+    sxz[c] = sxz[c] + (cp<T>(M.coef[5])[c] * dvx_dx + cp<T>(M.coef[6])[c] * dvz_dz +
+                       cp<T>(M.coef[7])[c] * (dvz_dx + dvx_dz)) * dt;
+  }
+  if (inx) { psi[ox] = m_vxx; psi[SX + ox] = m_vzx; }
+  if (inz) { psi[4 * SX + oz] = m_vzz; psi[4 * SX + SZ + oz] = m_vxz; }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_el2_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+          const int4* __restrict__ work, const int* __restrict__ base, int rel) {
+  const int4 wi = work[blockIdx.x];
+  if (wi.x < 0) return;
+  const DShot& S = shots[wi.x];
+  const DModel& M = models[S.model];
+  const int nx = M.nx, nz = M.nz;
+  const int i = wi.y + threadIdx.x, j = wi.z + threadIdx.y;
+  if (i >= nx || j >= nz) return;
+  const int it = *base + rel;
+  const T* __restrict__ sxx = cp<T>(S.f[2]);
+  const T* __restrict__ szz = cp<T>(S.f[3]);
+  const T* __restrict__ sxz = cp<T>(S.f[4]);
+  T* vx = mp<T>(S.f[0]);
+  T* vz = mp<T>(S.f[1]);
+  const long c = (long)j * nx + i;
+  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz);
+  const int lx = M.lx, lz = M.lz;
+  const bool inx = in_slab(i, nx, lx), inz = in_slab(j, nz, lz);
+  T* psi = mp<T>(S.psi);
+  const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
+  const long ox = (long)j * 2 * lx + slab_pos(i, nx, lx);
+  const long oz = (long)slab_pos(j, nz, lz) * nx + i;
+  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
+  T vxn = vx[c], vzn = vz[c];
+  // vx, seismicfdtd.f95:317-337
+  if (j >= 1 && i >= 1) {
+    T a = (sxx[c] - sxx[c - 1]) * idx;
+    T b = (sxz[c] - sxz[c - nx]) * idz;
+    if (inx) {
+      T m = psi[2 * SX + ox];
+      m = P[PB][i] * m + P[PA][i] * a;
+      a = a * P[PK][i] + m;
+      psi[2 * SX + ox] = m;
+    }
+    if (inz) {
+      T m = psi[4 * SX + 3 * SZ + oz];
+      m = P[12 + PB][j] * m + P[12 + PA][j] * b;
+      b = b * P[12 + PK][j] + m;
+      psi[4 * SX + 3 * SZ + oz] = m;
+    }
+    vxn = vxn * cp<T>(M.coef[10])[c] + (a + b) * cp<T>(M.coef[8])[c];
+  }
+  // vz, seismicfdtd.f95:339-357
+  if (j <= nz - 2 && i <= nx - 2) {
+    T a = (sxz[c + 1] - sxz[c]) * idx;
+    T b = (szz[c + nx] - szz[c]) * idz;
+    if (inx) {
+      T m = psi[3 * SX + ox];
+      m = P[PBH][i] * m + P[PAH][i] * a;
+      a = a * P[PKH][i] + m;
+      psi[3 * SX + ox] = m;
+    }
+    if (inz) {
+      T m = psi[4 * SX + 2 * SZ + oz];
+      m = P[12 + PBH][j] * m + P[12 + PAH][j] * b;
+      b = b * P[12 + PKH][j] + m;
+      psi[4 * SX + 2 * SZ + oz] = m;
+    }
+    vzn = vzn * cp<T>(M.coef[11])[c] + (a + b) * cp<T>(M.coef[9])[c];
+  }
+  // source, seismicfdtd.f95:359-361
+  if (i == S.isrc && j == S.ksrc) {
+    if (S.src[0]) vxn += cp<T>(S.src[0])[it];
+    if (S.src[2]) vzn += cp<T>(S.src[2])[it];
+  }
+  // Dirichlet edges, seismicfdtd.f95:365-375
+  if (i == 0 || i == nx - 1 || j == 0 || j == nz - 1) { vxn = 0; vzn = 0; }
+  check_blowup((double)vxn * vxn + (double)vzn * vzn, S.blowup, it);
+  vx[c] = vxn;
+  vz[c] = vzn;
+}
+
+// ===========================================================================
+// EM2: electromag2, legacy/fortran/electromagfdtd.f95:10-326
+// psi slabs: x [nz][2lx]: dEz_dx dHy_dx ; z [2lz][nx]: dEx_dz dHy_dz
+// fields: Ex (nx-1 x nz), Ez (nx x nz-1), Hy (nx-1 x nz-1) stored nx*nz.
+// The Hy Dirichlet zeroing that ends each legacy step (:312-315) is applied
+// by the next H update, which reads a zeroed boundary Hy as 0.
+// ===========================================================================
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_em2_h(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+        const int4* __restrict__ work) {
+  const int4 wi = work[blockIdx.x];
+  if (wi.x < 0) return;
+  const DShot& S = shots[wi.x];
+  const DModel& M = models[S.model];
+  const int nx = M.nx, nz = M.nz;
+  const int i = wi.y + threadIdx.x, j = wi.z + threadIdx.y;
+  if (i > nx - 2 || j > nz - 2) return;
+  const T* __restrict__ Ex = cp<T>(S.f[0]);
+  const T* __restrict__ Ez = cp<T>(S.f[1]);
+  T* Hy = mp<T>(S.f[2]);
+  const long c = (long)j * nx + i;
+  const int lx = M.lx, lz = M.lz;
+  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
+  T* psi = mp<T>(S.psi);
+  const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
+  // Hy, electromagfdtd.f95:249-266
+  T a = (Ex[c + nx] - Ex[c]) * T(1.0 / M.dz);
+  if (in_slab(j, nz, lz)) {
+    const long o = 2 * SX + (long)slab_pos(j, nz, lz) * nx + i;
+    T m = P[12 + PB][j] * psi[o] + P[12 + PA][j] * a;
+    a = a * P[12 + PK][j] + m;
+    psi[o] = m;
+  }
+  T b = (Ez[c + 1] - Ez[c]) * T(1.0 / M.dx);
+  if (in_slab(i, nx, lx)) {
+    const long o = (long)j * 2 * lx + slab_pos(i, nx, lx);
+    T m = P[PB][i] * psi[o] + P[PA][i] * b;
+    b = b * P[PK][i] + m;
+    psi[o] = m;
+  }
+  const bool edge = (i == 0 || i == nx - 2 || j == 0 || j == nz - 2);
+  const T hy = edge ? T(0) : Hy[c];
+  Hy[c] = hy + T(M.scal[0]) * (b + a);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_em2_e(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+        const int4* __restrict__ work, const int* __restrict__ base, int rel) {
+  const int4 wi = work[blockIdx.x];
+  if (wi.x < 0) return;
+  const DShot& S = shots[wi.x];
+  const DModel& M = models[S.model];
+  const int nx = M.nx, nz = M.nz;
+  const int i = wi.y + threadIdx.x, j = wi.z + threadIdx.y;
+  if (i >= nx || j >= nz) return;
+  const int it = *base + rel;
+  const T* __restrict__ Hy = cp<T>(S.f[2]);
+  T* Ex = mp<T>(S.f[0]);
+  T* Ez = mp<T>(S.f[1]);
+  const long c = (long)j * nx + i;
+  const int lx = M.lx, lz = M.lz;
+  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
+  T* psi = mp<T>(S.psi);
+  const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
+  T ex = Ex[c], ez = Ez[c];
+  // Ex, electromagfdtd.f95:273-283
+  if (j >= 1 && j <= nz - 2 && i <= nx - 2) {
+    T d = (Hy[c] - Hy[c - nx]) * T(1.0 / M.dz);
+    if (in_slab(j, nz, lz)) {
+      const long o = 2 * SX + SZ + (long)slab_pos(j, nz, lz) * nx + i;
+      T m = P[12 + PBH][j] * psi[o] + P[12 + PAH][j] * d;
+      d = d * P[12 + PKH][j] + m;
+      psi[o] = m;
+    }
+    ex = cp<T>(M.coef[0])[c] * ex + cp<T>(M.coef[1])[c] * d;
+  }
+  // Ez, electromagfdtd.f95:285-295
+  if (j <= nz - 2 && i >= 1 && i <= nx - 2) {
+    T d = (Hy[c] - Hy[c - 1]) * T(1.0 / M.dx);
+    if (in_slab(i, nx, lx)) {
+      const long o = SX + (long)j * 2 * lx + slab_pos(i, nx, lx);
+      T m = P[PBH][i] * psi[o] + P[PAH][i] * d;
+      d = d * P[PKH][i] + m;
+      psi[o] = m;
+    }
+    ez = cp<T>(M.coef[2])[c] * ez + cp<T>(M.coef[3])[c] * d;
+  }
+  // source, electromagfdtd.f95:300-301
+  if (i == S.isrc && j == S.ksrc) {
+    if (S.src[0]) ex += cp<T>(S.src[0])[it];
+    if (S.src[2]) ez += cp<T>(S.src[2])[it];
+  }
+  // Dirichlet, electromagfdtd.f95:304-311
+  if (i == 0 || i == nx - 2 || j == 0 || j == nz - 1) ex = 0;
+  if (i == 0 || i == nx - 1 || j == 0 || j == nz - 2) ez = 0;
+  if (i == nx - 1) ex = 0;  // outside the (nx-1) x nz Ex array
+  if (j == nz - 1) ez = 0;  // outside the nx x (nz-1) Ez array
+  check_blowup((double)ex * ex + (double)ez * ez, S.blowup, it);
+  Ex[c] = ex;
+  Ez[c] = ez;
+}
+
+// ===========================================================================
+// ELASTIC25: seismic25, legacy/fortran/seismicfdtd.f95:388-949
+// One thread per (x, z) column walks the y axis: the 2-D (y-invariant)
+// coefficients are loaded once into registers and reused for all ny cells.
+// psi slabs (per shot):
+//   x [nz][ny][2lx]: dvx_dx dvy_dx dvz_dx dsxx_dx dsxy_dx dsxz_dx
+//   y [nz][2ly][nx]: dvx_dy dvy_dy dvz_dy dsyy_dy dsxy_dy dsyz_dy
+//   z [2lz][ny][nx]: dvx_dz dvy_dz dvz_dz dszz_dz dsxz_dz dsyz_dz
+// ===========================================================================
+struct Slab3 {
+  long SX, SY, SZ;       // elements per memory-variable array on each axis
+  long bx, by, bz;       // base offsets of the x, y, z groups
+  __device__ Slab3(int nx, int ny, int nz, int lx, int ly, int lz) {
+    SX = (long)nz * ny * 2 * lx;
+    SY = (long)nz * 2 * ly * nx;
+    SZ = (long)2 * lz * ny * nx;
+    bx = 0;
+    by = 6 * SX;
+    bz = 6 * SX + 6 * SY;
+  }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+              const int4* __restrict__ work) {
+  const int4 wi = work[blockIdx.x];
+  if (wi.x < 0) return;
+  const DShot& S = shots[wi.x];
+  const DModel& M = models[S.model];
+  const int nx = M.nx, ny = M.ny, nz = M.nz;
+  const int i = wi.y + threadIdx.x, k = wi.z + threadIdx.y;
+  if (i >= nx || k >= nz) return;
+  const long nxy = (long)nx * ny;
+  const long p = (long)k * nx + i;
+  const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz), dt = T(M.dt);
+  const int lx = M.lx, ly = M.ly, lz = M.lz;
+  const bool inx = in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
+  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
+  const T *ikx = P[PK], *ax = P[PA], *bx = P[PB], *ikxh = P[PKH], *axh = P[PAH], *bxh = P[PBH];
+  const T *iky = P[6 + PK], *ay = P[6 + PA], *by = P[6 + PB];
+  const T *ikyh = P[6 + PKH], *ayh = P[6 + PAH], *byh = P[6 + PBH];
+  const T *ikz = P[12 + PK], *az = P[12 + PA], *bz = P[12 + PB];
+  const T *ikzh = P[12 + PKH], *azh = P[12 + PAH], *bzh = P[12 + PBH];
+  // per-column profile values (x and z are fixed for the thread)
+  const T bx_i = bx[i], ax_i = ax[i], ikx_i = ikx[i], bxh_i = bxh[i], axh_i = axh[i], ikxh_i = ikxh[i];
+  const T bz_k = bz[k], az_k = az[k], ikz_k = ikz[k], bzh_k = bzh[k], azh_k = azh[k], ikzh_k = ikzh[k];
+  const bool p1 = (k >= 1 && i <= nx - 2), p2 = (k >= 1 && i >= 1);
+  const bool p3 = (k <= nz - 2 && i >= 1), p4 = (k <= nz - 2 && i <= nx - 2);
+  T C1[15], C2[6], C3[6], C4[6];
+#pragma unroll
+  for (int q = 0; q < 15; ++q) C1[q] = p1 ? cp<T>(M.coef[q])[p] : T(0);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    C2[q] = p2 ? cp<T>(M.coef[15 + q])[p] : T(0);
+    C3[q] = p3 ? cp<T>(M.coef[21 + q])[p] : T(0);
+    C4[q] = p4 ? cp<T>(M.coef[27 + q])[p] : T(0);
+  }
+  const T* __restrict__ vx = cp<T>(S.f[0]);
+  const T* __restrict__ vy = cp<T>(S.f[1]);
+  const T* __restrict__ vz = cp<T>(S.f[2]);
+  T* sxx = mp<T>(S.f[3]);
+  T* syy = mp<T>(S.f[4]);
+  T* szz = mp<T>(S.f[5]);
+  T* sxy = mp<T>(S.f[6]);
+  T* sxz = mp<T>(S.f[7]);
+  T* syz = mp<T>(S.f[8]);
+  T* psi = mp<T>(S.psi);
+  const Slab3 L(nx, ny, nz, lx, ly, lz);
+  const int xs = slab_pos(i, nx, lx), zs = slab_pos(k, nz, lz);
+
+  for (int j = 0; j < ny; ++j) {
+    const long c = (long)k * nxy + (long)j * nx + i;
+    const bool iny = in_slab(j, ny, ly);
+    const long ox = L.bx + ((long)k * ny + j) * 2 * lx + xs;
+    const long oy = L.by + ((long)k * 2 * ly + slab_pos(j, ny, ly)) * nx + i;
+    const long oz = L.bz + ((long)zs * ny + j) * nx + i;
+    // velocity-derivative memory variables (shared by all four loops: A1)
+    T mxx = 0, myx = 0, mzx = 0, mxy = 0, myy = 0, mzy = 0, mxz = 0, myz = 0, mzz = 0;
+    if (inx) { mxx = psi[ox]; myx = psi[ox + L.SX]; mzx = psi[ox + 2 * L.SX]; }
+    if (iny) { mxy = psi[oy]; myy = psi[oy + L.SY]; mzy = psi[oy + 2 * L.SY]; }
+    if (inz) { mxz = psi[oz]; myz = psi[oz + L.SZ]; mzz = psi[oz + 2 * L.SZ]; }
+    const T vxc = vx[c], vyc = vy[c], vzc = vz[c];
+
+    // loop 1: sigma xx, yy, zz (seismicfdtd.f95:617-674)
+    if (p1 && j >= 1) {
+      const long e = c + 1, s = c - nx, d = c - nxy;
+      T dvx_dx = (vx[e] - vxc) * idx, dvy_dx = (vy[e] - vyc) * idx, dvz_dx = (vz[e] - vzc) * idx;
+      T dvy_dy = (vyc - vy[s]) * idy, dvx_dy = (vxc - vx[s]) * idy, dvz_dy = (vzc - vz[s]) * idy;
+      T dvz_dz = (vzc - vz[d]) * idz, dvx_dz = (vxc - vx[d]) * idz, dvy_dz = (vyc - vy[d]) * idz;
+      if (inx) {
+        mxx = bxh_i * mxx + axh_i * dvx_dx; myx = bxh_i * myx + axh_i * dvy_dx; mzx = bxh_i * mzx + axh_i * dvz_dx;
+        dvx_dx = dvx_dx * ikxh_i + mxx; dvy_dx = dvy_dx * ikxh_i + myx; dvz_dx = dvz_dx * ikxh_i + mzx;
+      }
+      if (iny) {
+        const T b = by[j], a = ay[j], ik = iky[j];
+        myy = b * myy + a * dvy_dy; mxy = b * mxy + a * dvx_dy; mzy = b * mzy + a * dvz_dy;
+        dvy_dy = dvy_dy * ik + myy; dvx_dy = dvx_dy * ik + mxy; dvz_dy = dvz_dy * ik + mzy;
+      }
+      if (inz) {
+        mzz = bz_k * mzz + az_k * dvz_dz; mxz = bz_k * mxz + az_k * dvx_dz; myz = bz_k * myz + az_k * dvy_dz;
+        dvz_dz = dvz_dz * ikz_k + mzz; dvx_dz = dvx_dz * ikz_k + mxz; dvy_dz = dvy_dz * ikz_k + myz;
+      }
+      sxx[c] = sxx[c] + (C1[0] * dvx_dx + C1[1] * dvy_dy + C1[2] * dvz_dz + C1[3] * (dvy_dz + dvz_dy) +
+                         C1[4] * (dvx_dz + dvz_dx) + C1[5] * (dvx_dy + dvz_dy)) * dt;  // A3 (c16)
+      syy[c] = syy[c] + (C1[1] * dvx_dx + C1[6] * dvy_dy + C1[7] * dvz_dz + C1[8] * (dvy_dz + dvz_dy) +
+                         C1[9] * (dvx_dz + dvz_dx) + C1[10] * (dvy_dx + dvx_dy)) * dt;
+      szz[c] = szz[c] + (C1[2] * dvx_dx + C1[7] * dvy_dy + C1[11] * dvz_dz + C1[12] * (dvy_dz + dvz_dy) +
+                         C1[13] * (dvx_dz + dvz_dx) + C1[14] * (dvy_dx + dvx_dy)) * dt;
+    }
+    // loop 2: sigma xy (seismicfdtd.f95:677-725); dvz_dx and dvx_dz get no correction
+    if (p2 && j <= ny - 2) {
+      const long w = c - 1, n = c + nx, d = c - nxy;
+      T dvx_dx = (vxc - vx[w]) * idx, dvy_dx = (vyc - vy[w]) * idx, dvz_dx = (vzc - vz[w]) * idx;
+      T dvy_dy = (vy[n] - vyc) * idy, dvx_dy = (vx[n] - vxc) * idy, dvz_dy = (vz[n] - vzc) * idy;
+      T dvz_dz = (vzc - vz[d]) * idz, dvx_dz = (vxc - vx[d]) * idz, dvy_dz = (vyc - vy[d]) * idz;
+      if (inx) {
+        mxx = bx_i * mxx + ax_i * dvx_dx; myx = bx_i * myx + ax_i * dvy_dx; mzx = bx_i * mzx + ax_i * dvz_dx;
+        dvx_dx = dvx_dx * ikx_i + mxx; dvy_dx = dvy_dx * ikx_i + myx;
+      }
+      if (iny) {
+        const T b = byh[j], a = ayh[j], ik = ikyh[j];
+        myy = b * myy + a * dvy_dy; mxy = b * mxy + a * dvx_dy; mzy = b * mzy + a * dvz_dy;
+        dvy_dy = dvy_dy * ik + myy; dvx_dy = dvx_dy * ik + mxy; dvz_dy = dvz_dy * ik + mzy;
+      }
+      if (inz) {
+        mzz = bzh_k * mzz + azh_k * dvz_dz; mxz = bzh_k * mxz + azh_k * dvx_dz; myz = bzh_k * myz + azh_k * dvy_dz;
+        dvz_dz = dvz_dz * ikzh_k + mzz; dvy_dz = dvy_dz * ikzh_k + myz;
+      }
+      sxy[c] = sxy[c] + (C2[0] * dvx_dx + C2[1] * dvy_dy + C2[2] * dvz_dz + C2[3] * (dvz_dy + dvy_dz) +
+                         C2[4] * (dvz_dx + dvx_dz) + C2[5] * (dvy_dx + dvx_dy)) * dt;
+    }
+    // loop 3: sigma xz (seismicfdtd.f95:728-768)
+    if (p3 && j >= 1) {
+      const long w = c - 1, s = c - nx, u = c + nxy;
+      T dvx_dx = (vxc - vx[w]) * idx, dvy_dx = (vyc - vy[w]) * idx, dvz_dx = (vzc - vz[w]) * idx;
+      T dvy_dy = (vyc - vy[s]) * idy, dvz_dy = (vzc - vz[s]) * idy, dvx_dy = (vxc - vx[s]) * idy;
+      T dvz_dz = (vz[u] - vzc) * idz, dvx_dz = (vx[u] - vxc) * idz, dvy_dz = (vy[u] - vyc) * idz;
+      if (inx) {
+        mxx = bx_i * mxx + ax_i * dvx_dx; myx = bx_i * myx + ax_i * dvy_dx; mzx = bx_i * mzx + ax_i * dvz_dx;
+        dvx_dx = dvx_dx * ikx_i + mxx; dvy_dx = dvy_dx * ikx_i + myx; dvz_dx = dvz_dx * ikx_i + mzx;
+      }
+      if (iny) {
+        const T b = by[j], a = ay[j], ik = iky[j];
+        myy = b * myy + a * dvy_dy; mxy = b * mxy + a * dvx_dy; mzy = b * mzy + a * dvz_dy;
+        dvy_dy = dvy_dy * ik + myy; dvx_dy = dvx_dy * ik + mxy; dvz_dy = dvz_dy * ik + mzy;
+      }
+      if (inz) {
+        mzz = bzh_k * mzz + azh_k * dvz_dz; mxz = bzh_k * mxz + azh_k * dvx_dz; myz = bzh_k * myz + azh_k * dvy_dz;
+        dvz_dz = dvz_dz * ikzh_k + mzz; dvx_dz = dvx_dz * ikzh_k + mxz; dvy_dz = dvy_dz * ikzh_k + myz;
+      }
+      sxz[c] = sxz[c] + (C3[0] * dvx_dx + C3[1] * dvy_dy + C3[2] * dvz_dz + C3[3] * (dvx_dz + dvz_dx) +
+                         C3[4] * (dvx_dz + dvz_dx) + C3[5] * (dvx_dy + dvy_dx)) * dt;  // A3 (c45)
+    }
+    // loop 4: sigma yz (seismicfdtd.f95:771-813)
+    if (p4 && j <= ny - 2) {
+      const long e = c + 1, n = c + nx, u = c + nxy;
+      T dvx_dx = (vx[e] - vxc) * idx, dvy_dx = (vy[e] - vyc) * idx, dvz_dx = (vz[e] - vzc) * idx;
+      T dvy_dy = (vy[n] - vyc) * idy, dvx_dy = (vx[n] - vxc) * idy, dvz_dy = (vz[n] - vzc) * idy;
+      T dvz_dz = (vz[u] - vzc) * idz, dvx_dz = (vx[u] - vxc) * idz, dvy_dz = (vy[u] - vyc) * idz;
+      if (inx) {
+        mxx = bxh_i * mxx + axh_i * dvx_dx; myx = bxh_i * myx + axh_i * dvy_dx; mzx = bxh_i * mzx + axh_i * dvz_dx;
+        dvx_dx = dvx_dx * ikxh_i + mxx; dvy_dx = dvy_dx * ikxh_i + myx; dvz_dx = dvz_dx * ikxh_i + mzx;
+      }
+      if (iny) {
+        const T b = byh[j], a = ayh[j], ik = ikyh[j];
+        myy = b * myy + a * dvy_dy; mxy = b * mxy + a * dvx_dy; mzy = b * mzy + a * dvz_dy;
+        dvy_dy = dvy_dy * ik + myy; dvx_dy = dvx_dy * ik + mxy; dvz_dy = dvz_dy * ik + mzy;
+      }
+      if (inz) {
+        mzz = bzh_k * mzz + azh_k * dvz_dz; mxz = bzh_k * mxz + azh_k * dvx_dz; myz = bzh_k * myz + azh_k * dvy_dz;
+        dvz_dz = dvz_dz * ikzh_k + mzz; dvx_dz = dvx_dz * ikzh_k + mxz; dvy_dz = dvy_dz * ikzh_k + myz;
+      }
+      syz[c] = syz[c] + (C4[0] * dvx_dx + C4[1] * dvy_dy + C4[2] * dvz_dz + C4[3] * (dvy_dz + dvz_dy) +
+                         C4[4] * (dvx_dz + dvz_dx) + C4[5] * (dvy_dx + dvx_dy)) * dt;
+    }
+    if (inx) { psi[ox] = mxx; psi[ox + L.SX] = myx; psi[ox + 2 * L.SX] = mzx; }
+    if (iny) { psi[oy] = mxy; psi[oy + L.SY] = myy; psi[oy + 2 * L.SY] = mzy; }
+    if (inz) { psi[oz] = mxz; psi[oz + L.SZ] = myz; psi[oz + 2 * L.SZ] = mzz; }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_el25_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+           const int4* __restrict__ work, const int* __restrict__ base, int rel) {
+  const int4 wi = work[blockIdx.x];
+  if (wi.x < 0) return;
+  const DShot& S = shots[wi.x];
+  const DModel& M = models[S.model];
+  const int nx = M.nx, ny = M.ny, nz = M.nz;
+  const int i = wi.y + threadIdx.x, k = wi.z + threadIdx.y;
+  if (i >= nx || k >= nz) return;
+  const int it = *base + rel;
+  const long nxy = (long)nx * ny;
+  const long p = (long)k * nx + i;
+  const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz);
+  const int lx = M.lx, ly = M.ly, lz = M.lz;
+  const bool inx = in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
+  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
+  const T bx_i = P[PB][i], ax_i = P[PA][i], ikx_i = P[PK][i];
+  const T bxh_i = P[PBH][i], axh_i = P[PAH][i], ikxh_i = P[PKH][i];
+  const T bz_k = P[12 + PB][k], az_k = P[12 + PA][k], ikz_k = P[12 + PK][k];
+  const T bzh_k = P[12 + PBH][k], azh_k = P[12 + PAH][k], ikzh_k = P[12 + PKH][k];
+  const T *by = P[6 + PB], *ay = P[6 + PA], *iky = P[6 + PK];
+  const T *byh = P[6 + PBH], *ayh = P[6 + PAH], *ikyh = P[6 + PKH];
+  const bool ux = (k >= 1 && i >= 1), uy = (k >= 1 && i <= nx - 2), uz = (k <= nz - 2 && i <= nx - 2);
+  const T dtrx = ux ? cp<T>(M.coef[33])[p] : T(0), dtry = uy ? cp<T>(M.coef[34])[p] : T(0);
+  const T dtrz = uz ? cp<T>(M.coef[35])[p] : T(0);
+  const T omx = cp<T>(M.coef[36])[p], omy = cp<T>(M.coef[37])[p], omz = cp<T>(M.coef[38])[p];
+  const T* __restrict__ sxx = cp<T>(S.f[3]);
+  const T* __restrict__ syy = cp<T>(S.f[4]);
+  const T* __restrict__ szz = cp<T>(S.f[5]);
+  const T* __restrict__ sxy = cp<T>(S.f[6]);
+  const T* __restrict__ sxz = cp<T>(S.f[7]);
+  const T* __restrict__ syz = cp<T>(S.f[8]);
+  T* vx = mp<T>(S.f[0]);
+  T* vy = mp<T>(S.f[1]);
+  T* vz = mp<T>(S.f[2]);
+  T* psi = mp<T>(S.psi);
+  const Slab3 L(nx, ny, nz, lx, ly, lz);
+  const int xs = slab_pos(i, nx, lx), zs = slab_pos(k, nz, lz);
+  const bool kedge = (i == 0 || k == 0 || i == nx - 1 || k == nz - 1);
+  const bool srccol = (i == S.isrc && k == S.ksrc);
+
+  for (int j = 0; j < ny; ++j) {
+    const long c = (long)k * nxy + (long)j * nx + i;
+    const bool iny = in_slab(j, ny, ly);
+    const long ox = L.bx + ((long)k * ny + j) * 2 * lx + xs;
+    const long oy = L.by + ((long)k * 2 * ly + slab_pos(j, ny, ly)) * nx + i;
+    const long oz = L.bz + ((long)zs * ny + j) * nx + i;
+    T vxn = vx[c], vyn = vy[c], vzn = vz[c];
+    // vx, seismicfdtd.f95:821-846
+    if (ux && j >= 1) {
+      T a = (sxx[c] - sxx[c - 1]) * idx, b = (sxy[c] - sxy[c - nx]) * idy, d = (sxz[c] - sxz[c - nxy]) * idz;
+      if (inx) { T m = bx_i * psi[ox + 3 * L.SX] + ax_i * a; a = a * ikx_i + m; psi[ox + 3 * L.SX] = m; }
+      if (iny) { T m = by[j] * psi[oy + 4 * L.SY] + ay[j] * b; b = b * iky[j] + m; psi[oy + 4 * L.SY] = m; }
+      if (inz) { T m = bz_k * psi[oz + 4 * L.SZ] + az_k * d; d = d * ikz_k + m; psi[oz + 4 * L.SZ] = m; }
+      vxn = vxn * omx + (a + b + d) * dtrx;
+    }
+    // vy, seismicfdtd.f95:848-870
+    if (uy && j <= ny - 2) {
+      T a = (sxy[c + 1] - sxy[c]) * idx, b = (syy[c + nx] - syy[c]) * idy, d = (syz[c] - syz[c - nxy]) * idz;
+      if (inx) { T m = bxh_i * psi[ox + 4 * L.SX] + axh_i * a; a = a * ikxh_i + m; psi[ox + 4 * L.SX] = m; }
+      if (iny) { T m = byh[j] * psi[oy + 3 * L.SY] + ayh[j] * b; b = b * ikyh[j] + m; psi[oy + 3 * L.SY] = m; }
+      if (inz) { T m = bz_k * psi[oz + 5 * L.SZ] + az_k * d; d = d * ikz_k + m; psi[oz + 5 * L.SZ] = m; }
+      vyn = vyn * omy + (a + b + d) * dtry;
+    }
+    // vz, seismicfdtd.f95:873-896
+    if (uz && j >= 1) {
+      T a = (sxz[c + 1] - sxz[c]) * idx, b = (syz[c] - syz[c - nx]) * idy, d = (szz[c + nxy] - szz[c]) * idz;
+      if (inx) { T m = bxh_i * psi[ox + 5 * L.SX] + axh_i * a; a = a * ikxh_i + m; psi[ox + 5 * L.SX] = m; }
+      if (iny) { T m = by[j] * psi[oy + 5 * L.SY] + ay[j] * b; b = b * iky[j] + m; psi[oy + 5 * L.SY] = m; }
+      if (inz) { T m = bzh_k * psi[oz + 3 * L.SZ] + azh_k * d; d = d * ikzh_k + m; psi[oz + 3 * L.SZ] = m; }
+      vzn = vzn * omz + (a + b + d) * dtrz;
+    }
+    // source, seismicfdtd.f95:898-900
+    if (srccol && j == S.jsrc) {
+      if (S.src[0]) vxn += cp<T>(S.src[0])[it];
+      if (S.src[1]) vyn += cp<T>(S.src[1])[it];
+      if (S.src[2]) vzn += cp<T>(S.src[2])[it];
+    }
+    // Dirichlet faces, seismicfdtd.f95:903-925
+    if (kedge || j == 0 || j == ny - 1) { vxn = 0; vyn = 0; vzn = 0; }
+    check_blowup((double)vxn * vxn + (double)vyn * vyn + (double)vzn * vzn, S.blowup, it);
+    vx[c] = vxn;
+    vy[c] = vyn;
+    vz[c] = vzn;
+  }
+}
+
+// ===========================================================================
+// EM25: electromag25, legacy/fortran/electromagfdtd.f95:619-1092
+// psi slabs: x [nz][ny][2lx]: dEz_dx dEy_dx dHz_dx dHy_dx
+//            y [nz][2ly][nx]: dEz_dy dEx_dy dHz_dy dHx_dy
+//            z [2lz][ny][nx]: dEy_dz dEx_dz dHy_dz dHx_dz
+// H boundary zeroing (:1059-1078) is applied by the next H update.
+// ===========================================================================
+struct Slab3e {
+  long SX, SY, SZ, bx, by, bz;
+  __device__ Slab3e(int nx, int ny, int nz, int lx, int ly, int lz) {
+    SX = (long)nz * ny * 2 * lx;
+    SY = (long)nz * 2 * ly * nx;
+    SZ = (long)2 * lz * ny * nx;
+    bx = 0;
+    by = 4 * SX;
+    bz = 4 * SX + 4 * SY;
+  }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_em25_h(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+         const int4* __restrict__ work) {
+  const int4 wi = work[blockIdx.x];
+  if (wi.x < 0) return;
+  const DShot& S = shots[wi.x];
+  const DModel& M = models[S.model];
+  const int nx = M.nx, ny = M.ny, nz = M.nz;
+  const int i = wi.y + threadIdx.x, k = wi.z + threadIdx.y;
+  if (i > nx - 2 || k > nz - 2) return;
+  const long nxy = (long)nx * ny;
+  const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz), dbH = T(M.scal[0]);
+  const int lx = M.lx, ly = M.ly, lz = M.lz;
+  const bool inx = in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
+  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
+  const T bx_i = P[PB][i], ax_i = P[PA][i], ikx_i = P[PK][i];
+  const T bz_k = P[12 + PB][k], az_k = P[12 + PA][k], ikz_k = P[12 + PK][k];
+  const T bzh_k = P[12 + PBH][k], azh_k = P[12 + PAH][k], ikzh_k = P[12 + PKH][k];
+  const T *by = P[6 + PB], *ay = P[6 + PA], *iky = P[6 + PK];
+  const T *byh = P[6 + PBH], *ayh = P[6 + PAH], *ikyh = P[6 + PKH];
+  const T* __restrict__ Ex = cp<T>(S.f[0]);
+  const T* __restrict__ Ey = cp<T>(S.f[1]);
+  const T* __restrict__ Ez = cp<T>(S.f[2]);
+  T* Hx = mp<T>(S.f[3]);
+  T* Hy = mp<T>(S.f[4]);
+  T* Hz = mp<T>(S.f[5]);
+  T* psi = mp<T>(S.psi);
+  const Slab3e L(nx, ny, nz, lx, ly, lz);
+  const int xs = slab_pos(i, nx, lx), zs = slab_pos(k, nz, lz);
+  for (int j = 0; j <= ny - 2; ++j) {
+    const long c = (long)k * nxy + (long)j * nx + i;
+    const bool iny = in_slab(j, ny, ly);
+    const long ox = L.bx + ((long)k * ny + j) * 2 * lx + xs;
+    const long oy = L.by + ((long)k * 2 * ly + slab_pos(j, ny, ly)) * nx + i;
+    const long oz = L.bz + ((long)zs * ny + j) * nx + i;
+    const bool lo = (i == 0 || j == 0 || k == 0);
+    // Hx, electromagfdtd.f95:901-920
+    {
+      T a = (Ez[c] - Ez[c + nx]) * idy;  // dEz_dy
+      if (iny) { T m = byh[j] * psi[oy] + ayh[j] * a; a = a * ikyh[j] + m; psi[oy] = m; }
+      T b = (Ey[c + nxy] - Ey[c]) * idz;  // dEy_dz
+      if (inz) { T m = bzh_k * psi[oz] + azh_k * b; b = b * ikzh_k + m; psi[oz] = m; }
+      const bool edge = lo || i == nx - 1 || j == ny - 2 || k == nz - 2;
+      Hx[c] = (edge ? T(0) : Hx[c]) + dbH * (b + a);
+    }
+    // Hy, electromagfdtd.f95:922-942
+    {
+      T a = (Ex[c] - Ex[c + nxy]) * idz;  // dEx_dz
+      if (inz) { T m = bz_k * psi[oz + L.SZ] + az_k * a; a = a * ikz_k + m; psi[oz + L.SZ] = m; }
+      T b = (Ez[c + 1] - Ez[c]) * idx;  // dEz_dx
+      if (inx) { T m = bx_i * psi[ox] + ax_i * b; b = b * ikx_i + m; psi[ox] = m; }
+      const bool edge = lo || i == nx - 2 || j == ny - 1 || k == nz - 2;
+      Hy[c] = (edge ? T(0) : Hy[c]) + dbH * (b + a);
+    }
+    // Hz, electromagfdtd.f95:944-962
+    if (k >= 1) {
+      T a = (Ex[c + nx] - Ex[c]) * idy;  // dEx_dy
+      if (iny) { T m = by[j] * psi[oy + L.SY] + ay[j] * a; a = a * iky[j] + m; psi[oy + L.SY] = m; }
+      T b = (Ey[c] - Ey[c + 1]) * idx;  // dEy_dx
+      if (inx) { T m = bx_i * psi[ox + L.SX] + ax_i * b; b = b * ikx_i + m; psi[ox + L.SX] = m; }
+      const bool edge = lo || i == nx - 2 || j == ny - 2 || k == nz - 1;
+      Hz[c] = (edge ? T(0) : Hz[c]) + dbH * (b + a);
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_em25_e(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+         const int4* __restrict__ work, const int* __restrict__ base, int rel) {
+  const int4 wi = work[blockIdx.x];
+  if (wi.x < 0) return;
+  const DShot& S = shots[wi.x];
+  const DModel& M = models[S.model];
+  const int nx = M.nx, ny = M.ny, nz = M.nz;
+  const int i = wi.y + threadIdx.x, k = wi.z + threadIdx.y;
+  if (i >= nx || k >= nz) return;
+  const int it = *base + rel;
+  const long nxy = (long)nx * ny;
+  const long p = (long)k * nx + i;
+  const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz);
+  const int lx = M.lx, ly = M.ly, lz = M.lz;
+  const bool inx = in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
+  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
+  const T bxh_i = P[PBH][i], axh_i = P[PAH][i], ikxh_i = P[PKH][i];
+  const T bz_k = P[12 + PB][k], az_k = P[12 + PA][k], ikz_k = P[12 + PK][k];
+  const T bzh_k = P[12 + PBH][k], azh_k = P[12 + PAH][k], ikzh_k = P[12 + PKH][k];
+  const T *byh = P[6 + PBH], *ayh = P[6 + PAH], *ikyh = P[6 + PKH];
+  const bool ux = (k >= 1 && k <= nz - 2 && i <= nx - 2);
+  const bool uy = (k >= 1 && k <= nz - 2 && i >= 1 && i <= nx - 2);
+  const bool uz = (k <= nz - 2 && i >= 1 && i <= nx - 2);
+  const T cax = ux ? cp<T>(M.coef[0])[p] : T(0), cbx = ux ? cp<T>(M.coef[1])[p] : T(0);
+  const T cay = uy ? cp<T>(M.coef[2])[p] : T(0), cby = uy ? cp<T>(M.coef[3])[p] : T(0);
+  const T caz = uz ? cp<T>(M.coef[4])[p] : T(0), cbz = uz ? cp<T>(M.coef[5])[p] : T(0);
+  const T* __restrict__ Hx = cp<T>(S.f[3]);
+  const T* __restrict__ Hy = cp<T>(S.f[4]);
+  const T* __restrict__ Hz = cp<T>(S.f[5]);
+  T* Ex = mp<T>(S.f[0]);
+  T* Ey = mp<T>(S.f[1]);
+  T* Ez = mp<T>(S.f[2]);
+  T* psi = mp<T>(S.psi);
+  const Slab3e L(nx, ny, nz, lx, ly, lz);
+  const int xs = slab_pos(i, nx, lx), zs = slab_pos(k, nz, lz);
+  const bool srccol = (i == S.isrc && k == S.ksrc);
+  for (int j = 0; j < ny; ++j) {
+    const long c = (long)k * nxy + (long)j * nx + i;
+    const bool iny = in_slab(j, ny, ly);
+    const long ox = L.bx + ((long)k * ny + j) * 2 * lx + xs;
+    const long oy = L.by + ((long)k * 2 * ly + slab_pos(j, ny, ly)) * nx + i;
+    const long oz = L.bz + ((long)zs * ny + j) * nx + i;
+    T ex = Ex[c], ey = Ey[c], ez = Ez[c];
+    // Ex, electromagfdtd.f95:968-985
+    if (ux && j >= 1 && j <= ny - 2) {
+      T a = (Hz[c] - Hz[c - nx]) * idy;  // dHz_dy
+      if (iny) { T m = byh[j] * psi[oy + 2 * L.SY] + ayh[j] * a; a = a * ikyh[j] + m; psi[oy + 2 * L.SY] = m; }
+      T b = (Hy[c - nxy] - Hy[c]) * idz;  // dHy_dz
+      if (inz) { T m = bz_k * psi[oz + 2 * L.SZ] + az_k * b; b = b * ikz_k + m; psi[oz + 2 * L.SZ] = m; }
+      ex = cax * ex + cbx * (a + b);
+    }
+    // Ey, electromagfdtd.f95:988-1006
+    if (uy && j <= ny - 2) {
+      T a = (Hz[c - 1] - Hz[c]) * idx;  // dHz_dx
+      if (inx) { T m = bxh_i * psi[ox + 2 * L.SX] + axh_i * a; a = a * ikxh_i + m; psi[ox + 2 * L.SX] = m; }
+      T b = (Hx[c] - Hx[c - nxy]) * idz;  // dHx_dz
+      if (inz) { T m = bzh_k * psi[oz + 3 * L.SZ] + azh_k * b; b = b * ikzh_k + m; psi[oz + 3 * L.SZ] = m; }
+      ey = cay * ey + cby * (a + b);
+    }
+    // Ez, electromagfdtd.f95:1011-1029
+    if (uz && j >= 1 && j <= ny - 2) {
+      T a = (Hx[c - nx] - Hx[c]) * idy;  // dHx_dy
+      if (iny) { T m = byh[j] * psi[oy + 3 * L.SY] + ayh[j] * a; a = a * ikyh[j] + m; psi[oy + 3 * L.SY] = m; }
+      T b = (Hy[c] - Hy[c - 1]) * idx;  // dHy_dx
+      if (inx) { T m = bxh_i * psi[ox + 3 * L.SX] + axh_i * b; b = b * ikxh_i + m; psi[ox + 3 * L.SX] = m; }
+      ez = caz * ez + cbz * (a + b);
+    }
+    // source, electromagfdtd.f95:1033-1035
+    if (srccol && j == S.jsrc) {
+      if (S.src[0]) ex += cp<T>(S.src[0])[it];
+      if (S.src[1]) ey += cp<T>(S.src[1])[it];
+      if (S.src[2]) ez += cp<T>(S.src[2])[it];
+    }
+    // Dirichlet faces, electromagfdtd.f95:1038-1057 (and cells outside each
+    // staggered field's Fortran extent)
+    const bool lo = (i == 0 || j == 0 || k == 0);
+    if (lo || i >= nx - 2 || j == ny - 1 || k == nz - 1) ex = 0;
+    if (lo || i == nx - 1 || j >= ny - 2 || k == nz - 1) ey = 0;
+    if (lo || i == nx - 1 || j == ny - 1 || k >= nz - 2) ez = 0;
+    check_blowup((double)ex * ex + (double)ey * ey + (double)ez * ez, S.blowup, it);
+    Ex[c] = ex;
+    Ey[c] = ey;
+    Ez[c] = ez;
+  }
+}
+
+// ===========================================================================
+// Receiver gather (replaces arraybuild.py:199-295 reading snapshot files):
+// trace[it][comp][r] = field_comp[rec_r] after the step.
+// ===========================================================================
+template <typename T>
+__global__ void k_gather(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+                         int nshots, int ncomp, const int* __restrict__ base, int rel) {
+  const int s = blockIdx.y;
+  if (s >= nshots) return;
+  const DShot& S = shots[s];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= S.nrec || S.trace == nullptr) return;
+  const DModel& M = models[S.model];
+  const int it = *base + rel;
+  const int* q = S.rec + 3 * r;
+  const long c = ((long)q[2] * M.ny + q[1]) * M.nx + q[0];
+  T* tr = mp<T>(S.trace) + ((long)it * ncomp) * S.nrec + r;
+  for (int f = 0; f < ncomp; ++f) tr[(long)f * S.nrec] = cp<T>(S.f[f])[c];
+}
+
+__global__ void k_set_int(int* p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+__global__ void k_add_int(int* p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = *p + v;
+}
+__global__ void k_fill_int(int* p, int n, int v) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+// ===========================================================================
+// Plan
+// ===========================================================================
+struct Plan {
+  int kind = 0, prec = 0, nmodels = 0, nshots = 0, nt = 0, ncomp = 2;
+  int maxrec = 0;
+  std::vector<sdt_model> hmodels;
+  std::vector<sdt_shot> hshots;
+  DModel* dmodels = nullptr;
+  DShot* dshots = nullptr;
+  int4* dwork = nullptr;
+  int nwork = 0;
+  int* dflags = nullptr;  // per shot first unstable step (INT_MAX = none)
+  int* dbase = nullptr;   // step index of rel = 0
+  hipStream_t last = nullptr;
+  hipStream_t cap = nullptr;  // capture stream
+  int graph_block = 0;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+};
+
+int kind_fields(int kind) {
+  switch (kind) {
+    case SDT_ELASTIC2: return 5;
+    case SDT_ELASTIC25: return 9;
+    case SDT_EM2: return 3;
+    case SDT_EM25: return 6;
+  }
+  return 0;
+}
+int kind_ncoef(int kind) {
+  switch (kind) {
+    case SDT_ELASTIC2: return 12;
+    case SDT_ELASTIC25: return 39;
+    case SDT_EM2: return 4;
+    case SDT_EM25: return 6;
+  }
+  return 0;
+}
+bool kind_3d(int kind) { return kind == SDT_ELASTIC25 || kind == SDT_EM25; }
+
+long long psi_elems(int kind, long long nx, long long ny, long long nz, long long lx,
+                    long long ly, long long lz) {
+  switch (kind) {
+    case SDT_ELASTIC2: return 4 * nz * 2 * lx + 4 * 2 * lz * nx;
+    case SDT_EM2: return 2 * nz * 2 * lx + 2 * 2 * lz * nx;
+    case SDT_ELASTIC25: return 6 * (nz * ny * 2 * lx + nz * 2 * ly * nx + 2 * lz * ny * nx);
+    case SDT_EM25: return 4 * (nz * ny * 2 * lx + nz * 2 * ly * nx + 2 * lz * ny * nx);
+  }
+  return -1;
+}
+
+template <typename T>
+void launch_step(Plan* p, hipStream_t st, int rel) {
+  const dim3 blk(TX, TZ);
+  const dim3 grd(p->nwork);
+  switch (p->kind) {
+    case SDT_ELASTIC2:
+      hipLaunchKernelGGL(k_el2_stress<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork);
+      hipLaunchKernelGGL(k_el2_vel<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
+      break;
+    case SDT_EM2:
+      hipLaunchKernelGGL(k_em2_h<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork);
+      hipLaunchKernelGGL(k_em2_e<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
+      break;
+    case SDT_ELASTIC25:
+      hipLaunchKernelGGL(k_el25_stress<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork);
+      hipLaunchKernelGGL(k_el25_vel<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
+      break;
+    case SDT_EM25:
+      hipLaunchKernelGGL(k_em25_h<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork);
+      hipLaunchKernelGGL(k_em25_e<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
+      break;
+  }
+  if (p->maxrec > 0) {
+    const dim3 gg((p->maxrec + 63) / 64, p->nshots);
+    hipLaunchKernelGGL(k_gather<T>, gg, dim3(64), 0, st, p->dmodels, p->dshots, p->nshots,
+                       p->ncomp, p->dbase, rel);
+  }
+}
+
+void launch_steps(Plan* p, hipStream_t st, int n) {
+  for (int r = 0; r < n; ++r) {
+    if (p->prec == SDT_FP64)
+      launch_step<double>(p, st, r);
+    else
+      launch_step<float>(p, st, r);
+  }
+}
+
+int validate(int kind, int prec, const sdt_model* models, int nmodels, const sdt_shot* shots,
+             int nshots, int nt) {
+  if (kind < 0 || kind > 3) return fail(SDT_EINVAL, "unknown kind");
+  if (prec != SDT_FP64 && prec != SDT_FP32) return fail(SDT_EINVAL, "unknown precision");
+  if (!models || nmodels <= 0 || !shots || nshots <= 0) return fail(SDT_EINVAL, "no models or shots");
+  if (nt <= 0) return fail(SDT_EINVAL, "nt must be positive");
+  const bool d3 = kind_3d(kind);
+  for (int m = 0; m < nmodels; ++m) {
+    const sdt_model& M = models[m];
+    const std::string tag = "model " + std::to_string(m) + ": ";
+    if (M.nx < 3 || M.nz < 3 || (d3 && M.ny < 3) || (!d3 && M.ny != 1))
+      return fail(SDT_EINVAL, tag + "grid too small (nx, nz >= 3; ny >= 3 in 2.5-D, 1 in 2-D)");
+    if ((long long)M.nx * M.ny * M.nz > (1LL << 40)) return fail(SDT_EINVAL, tag + "grid too large");
+    if (M.lx < 0 || M.lz < 0 || M.lx > (M.nx + 1) / 2 || M.lz > (M.nz + 1) / 2 ||
+        (d3 && (M.ly < 0 || M.ly > (M.ny + 1) / 2)))
+      return fail(SDT_EINVAL, tag + "slab width out of range");
+    if (!(M.dx > 0) || !(M.dz > 0) || (d3 && !(M.dy > 0)) || !(M.dt > 0))
+      return fail(SDT_EINVAL, tag + "dx, dy, dz, dt must be positive");
+    for (int q = 0; q < kind_ncoef(kind); ++q)
+      if (!M.coef[q]) return fail(SDT_EINVAL, tag + "coefficient slot " + std::to_string(q) + " is null");
+    for (int a = 0; a < 3; ++a) {
+      if (a == 1 && !d3) continue;
+      for (int q = 0; q < 6; ++q)
+        if (!M.prof[a][q]) return fail(SDT_EINVAL, tag + "C-PML profile missing");
+    }
+    if (kind == SDT_ELASTIC2 && !M.aux) return fail(SDT_EINVAL, tag + "aux (b_x_half by row) missing");
+  }
+  for (int s = 0; s < nshots; ++s) {
+    const sdt_shot& S = shots[s];
+    const std::string tag = "shot " + std::to_string(s) + ": ";
+    if (S.model < 0 || S.model >= nmodels) return fail(SDT_EINVAL, tag + "bad model index");
+    const sdt_model& M = models[S.model];
+    for (int f = 0; f < kind_fields(kind); ++f)
+      if (!S.field[f]) return fail(SDT_EINVAL, tag + "field pointer null");
+    if (!S.psi && psi_elems(kind, M.nx, M.ny, M.nz, M.lx, M.ly, M.lz) > 0)
+      return fail(SDT_EINVAL, tag + "psi workspace null");
+    if (S.isrc < 0 || S.isrc >= M.nx || S.ksrc < 0 || S.ksrc >= M.nz ||
+        (d3 && (S.jsrc < 0 || S.jsrc >= M.ny)))
+      return fail(SDT_EINVAL, tag + "source outside the padded grid");
+    if (S.nrec < 0 || (S.nrec > 0 && (!S.rec || !S.trace)))
+      return fail(SDT_EINVAL, tag + "receivers need rec and trace buffers");
+  }
+  return SDT_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* sdt_last_error(void) { return g_err.c_str(); }
+
+const char* sdt_version(void) { return "seidart_hip 0.1 gfx950"; }
+
+int sdt_psi_elems(int kind, int nx, int ny, int nz, int lx, int ly, int lz, long long* out) {
+  if (!out) return fail(SDT_EINVAL, "out is null");
+  const long long n = psi_elems(kind, nx, ny, nz, lx, ly, lz);
+  if (n < 0) return fail(SDT_EINVAL, "unknown kind");
+  *out = n;
+  return SDT_OK;
+}
+
+int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
+                    const sdt_shot* shots, int nshots, int nt, void** out_plan) {
+  if (!out_plan) return fail(SDT_EINVAL, "out_plan is null");
+  *out_plan = nullptr;
+  int rc = validate(kind, prec, models, nmodels, shots, nshots, nt);
+  if (rc) return rc;
+  // Receiver indices must be checked on the host before any kernel reads them.
+  // They live in device memory, so copy each shot's list back once.
+  Plan* p = new Plan();
+  p->kind = kind;
+  p->prec = prec;
+  p->nmodels = nmodels;
+  p->nshots = nshots;
+  p->nt = nt;
+  p->ncomp = kind_3d(kind) ? 3 : 2;
+  p->hmodels.assign(models, models + nmodels);
+  p->hshots.assign(shots, shots + nshots);
+  auto bail = [&](int code) {
+    sdt_plan_destroy(p);
+    return code;
+  };
+  for (int s = 0; s < nshots; ++s) {
+    const sdt_shot& S = shots[s];
+    if (S.nrec <= 0) continue;
+    std::vector<int> rec(3 * (size_t)S.nrec);
+    if (hipMemcpy(rec.data(), S.rec, rec.size() * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+      return bail(fail(SDT_EDEVICE, "cannot read receiver indices (not device memory?)"));
+    const sdt_model& M = models[S.model];
+    for (int r = 0; r < S.nrec; ++r) {
+      const int i = rec[3 * r], j = rec[3 * r + 1], k = rec[3 * r + 2];
+      if (i < 0 || i >= M.nx || k < 0 || k >= M.nz || j < 0 || j >= M.ny)
+        return bail(fail(SDT_EINVAL, "shot " + std::to_string(s) + ": receiver " +
+                                         std::to_string(r) + " outside the padded grid"));
+    }
+    p->maxrec = std::max(p->maxrec, S.nrec);
+  }
+  // device descriptors
+  std::vector<DModel> dm(nmodels);
+  for (int m = 0; m < nmodels; ++m) {
+    const sdt_model& M = models[m];
+    DModel& D = dm[m];
+    D.nx = M.nx; D.ny = M.ny; D.nz = M.nz;
+    D.lx = M.lx; D.ly = kind_3d(kind) ? M.ly : 0; D.lz = M.lz;
+    D.dx = M.dx; D.dy = M.dy; D.dz = M.dz; D.dt = M.dt;
+    for (int q = 0; q < 4; ++q) D.scal[q] = M.scal[q];
+    for (int q = 0; q < 40; ++q) D.coef[q] = M.coef[q];
+    for (int a = 0; a < 3; ++a)
+      for (int q = 0; q < 6; ++q) D.prof[a][q] = M.prof[a][q];
+    D.aux = M.aux;
+  }
+  if (hipMalloc(&p->dflags, sizeof(int) * nshots) != hipSuccess ||
+      hipMalloc(&p->dbase, sizeof(int)) != hipSuccess)
+    return bail(fail(SDT_EDEVICE, "hipMalloc failed (flags)"));
+  std::vector<DShot> ds(nshots);
+  for (int s = 0; s < nshots; ++s) {
+    const sdt_shot& S = shots[s];
+    DShot& D = ds[s];
+    D.model = S.model; D.isrc = S.isrc; D.jsrc = kind_3d(kind) ? S.jsrc : 0; D.ksrc = S.ksrc;
+    for (int f = 0; f < 9; ++f) D.f[f] = S.field[f];
+    D.psi = S.psi;
+    for (int c = 0; c < 3; ++c) D.src[c] = S.src[c];
+    D.nrec = S.nrec; D.rec = S.rec; D.trace = S.trace;
+    D.blowup = p->dflags + s;
+  }
+  // work list: (shot, x0, z0); all shots of one tile share blockIdx % 8 (XCD)
+  std::vector<int4> work;
+  for (int m = 0; m < nmodels; ++m) {
+    const sdt_model& M = models[m];
+    const int tx = (M.nx + TX - 1) / TX, tz = (M.nz + TZ - 1) / TZ, nt_ = tx * tz;
+    std::vector<int> ms;
+    for (int s = 0; s < nshots; ++s)
+      if (shots[s].model == m) ms.push_back(s);
+    if (ms.empty()) continue;
+    for (int tg = 0; tg < (nt_ + 7) / 8; ++tg)
+      for (int s : ms)
+        for (int tt = 0; tt < 8; ++tt) {
+          const int t = tg * 8 + tt;
+          if (t < nt_) work.push_back(make_int4(s, (t % tx) * TX, (t / tx) * TZ, 0));
+          else work.push_back(make_int4(-1, 0, 0, 0));
+        }
+  }
+  p->nwork = (int)work.size();
+  if (hipMalloc(&p->dmodels, sizeof(DModel) * nmodels) != hipSuccess ||
+      hipMalloc(&p->dshots, sizeof(DShot) * nshots) != hipSuccess ||
+      hipMalloc(&p->dwork, sizeof(int4) * work.size()) != hipSuccess)
+    return bail(fail(SDT_EDEVICE, "hipMalloc failed (descriptors)"));
+  if (hipMemcpy(p->dmodels, dm.data(), sizeof(DModel) * nmodels, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->dshots, ds.data(), sizeof(DShot) * nshots, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->dwork, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(SDT_EDEVICE, "hipMemcpy failed (descriptors)"));
+  hipLaunchKernelGGL(k_fill_int, dim3((nshots + 255) / 256), dim3(256), 0, nullptr, p->dflags,
+                     nshots, INT_MAX);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, nullptr, p->dbase, 0);
+  if (hipDeviceSynchronize() != hipSuccess) return bail(fail(SDT_EDEVICE, "init kernels failed"));
+  if (hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(SDT_EDEVICE, "hipStreamCreate failed"));
+  *out_plan = p;
+  return SDT_OK;
+}
+
+int sdt_plan_run(void* plan, int it0, int nsteps, int graph_block, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p) return fail(SDT_EINVAL, "plan is null");
+  if (it0 < 0 || nsteps < 0 || (long long)it0 + nsteps > p->nt)
+    return fail(SDT_EINVAL, "step range outside [0, nt)");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  p->last = st;
+  if (nsteps == 0) return SDT_OK;
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, p->dbase, it0);
+  int done = 0;
+  if (graph_block > 1 && nsteps >= graph_block) {
+    if (p->graph_block != graph_block || !p->gexec) {
+      if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+      if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+      HIP_TRY(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+      launch_steps(p, p->cap, graph_block);
+      hipLaunchKernelGGL(k_add_int, dim3(1), dim3(1), 0, p->cap, p->dbase, graph_block);
+      HIP_TRY(hipStreamEndCapture(p->cap, &p->graph));
+      HIP_TRY(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
+      p->graph_block = graph_block;
+    }
+    for (; done + graph_block <= nsteps; done += graph_block) HIP_TRY(hipGraphLaunch(p->gexec, st));
+  }
+  if (done < nsteps) launch_steps(p, st, nsteps - done);
+  HIP_TRY(hipGetLastError());
+  return SDT_OK;
+}
+
+int sdt_plan_check(void* plan, int* first_step, int* per_shot) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p) return fail(SDT_EINVAL, "plan is null");
+  HIP_TRY(hipStreamSynchronize(p->last));
+  std::vector<int> f(p->nshots);
+  HIP_TRY(hipMemcpy(f.data(), p->dflags, sizeof(int) * p->nshots, hipMemcpyDeviceToHost));
+  int first = INT_MAX;
+  for (int s = 0; s < p->nshots; ++s) {
+    first = std::min(first, f[s]);
+    if (per_shot) per_shot[s] = f[s] == INT_MAX ? -1 : f[s];
+  }
+  if (first_step) *first_step = first == INT_MAX ? -1 : first;
+  if (first != INT_MAX)
+    return fail(SDT_EUNSTABLE, "code became unstable and blew up at step " + std::to_string(first));
+  return SDT_OK;
+}
+
+void sdt_plan_destroy(void* plan) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p) return;
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->cap) (void)hipStreamDestroy(p->cap);
+  if (p->dmodels) (void)hipFree(p->dmodels);
+  if (p->dshots) (void)hipFree(p->dshots);
+  if (p->dwork) (void)hipFree(p->dwork);
+  if (p->dflags) (void)hipFree(p->dflags);
+  if (p->dbase) (void)hipFree(p->dbase);
+  delete p;
+}
+
+}  // extern "C"
