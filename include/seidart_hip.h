@@ -145,6 +145,13 @@ int sdt_plan_run(void* plan, int it0, int nsteps, int graph_block, void* stream)
  * gets the step per shot (-1 = stable). */
 int sdt_plan_check(void* plan, int* first_step, int* per_shot);
 
+/* Measurement hook (no counterpart in the reference): run steps it0 ..
+ * it0+nsteps-1 like sdt_plan_run (no graph), bracketing every kernel with
+ * HIP events on `stream`, and return the mean duration in ms of
+ * out_ms[0] the first field update (stress / H), out_ms[1] the second
+ * (velocity / E) and out_ms[2] the receiver gather. */
+int sdt_plan_profile(void* plan, int it0, int nsteps, void* stream, double* out_ms);
+
 /* Release the plan's own device memory and graphs. */
 void sdt_plan_destroy(void* plan);
 

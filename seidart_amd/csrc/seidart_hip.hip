@@ -281,7 +281,7 @@ k_em2_h(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   const int lx = M.lx, lz = M.lz;
   const T* const* P = reinterpret_cast<const T* const*>(M.prof);
   T* psi = mp<T>(S.psi);
-  const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
+  const long SX = (long)nz * 2 * lx;
   // Hy, electromagfdtd.f95:249-266
   T a = (Ex[c + nx] - Ex[c]) * T(1.0 / M.dz);
   if (in_slab(j, nz, lz)) {
@@ -825,6 +825,36 @@ __global__ void k_gather(const DModel* __restrict__ models, const DShot* __restr
   for (int f = 0; f < ncomp; ++f) tr[(long)f * S.nrec] = cp<T>(S.f[f])[c];
 }
 
+// The H Dirichlet zeroing that closes each legacy EM step (electromagfdtd.f95
+// :312-315, :1059-1078) is folded into the next H update; at the end of a
+// run it is applied here so the returned state equals the legacy state.
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_em_h_edges(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+             const int4* __restrict__ work, int d3) {
+  const int4 wi = work[blockIdx.x];
+  if (wi.x < 0) return;
+  const DShot& S = shots[wi.x];
+  const DModel& M = models[S.model];
+  const int nx = M.nx, ny = M.ny, nz = M.nz;
+  const int i = wi.y + threadIdx.x, k = wi.z + threadIdx.y;
+  if (i >= nx || k >= nz) return;
+  if (!d3) {
+    if (i == 0 || i == nx - 2 || k == 0 || k == nz - 2) mp<T>(S.f[2])[(long)k * nx + i] = T(0);
+    return;
+  }
+  T* Hx = mp<T>(S.f[3]);
+  T* Hy = mp<T>(S.f[4]);
+  T* Hz = mp<T>(S.f[5]);
+  for (int j = 0; j < ny; ++j) {
+    const long c = ((long)k * ny + j) * nx + i;
+    const bool lo = (i == 0 || j == 0 || k == 0);
+    if (lo || i == nx - 1 || j == ny - 2 || k == nz - 2) Hx[c] = T(0);
+    if (lo || i == nx - 2 || j == ny - 1 || k == nz - 2) Hy[c] = T(0);
+    if (lo || i == nx - 2 || j == ny - 2 || k == nz - 1) Hz[c] = T(0);
+  }
+}
+
 __global__ void k_set_int(int* p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
@@ -888,33 +918,58 @@ long long psi_elems(int kind, long long nx, long long ny, long long nz, long lon
   return -1;
 }
 
+// Launch kernel `which` (0: first field update, 1: second field update,
+// 2: receiver gather) of one step.
 template <typename T>
-void launch_step(Plan* p, hipStream_t st, int rel) {
+void launch_one(Plan* p, hipStream_t st, int rel, int which) {
   const dim3 blk(TX, TZ);
   const dim3 grd(p->nwork);
-  switch (p->kind) {
-    case SDT_ELASTIC2:
+  if (which == 2) {
+    if (p->maxrec > 0) {
+      const dim3 gg((p->maxrec + 63) / 64, p->nshots);
+      hipLaunchKernelGGL(k_gather<T>, gg, dim3(64), 0, st, p->dmodels, p->dshots, p->nshots,
+                         p->ncomp, p->dbase, rel);
+    }
+    return;
+  }
+  switch (p->kind * 2 + which) {
+    case SDT_ELASTIC2 * 2:
       hipLaunchKernelGGL(k_el2_stress<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork);
+      break;
+    case SDT_ELASTIC2 * 2 + 1:
       hipLaunchKernelGGL(k_el2_vel<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
       break;
-    case SDT_EM2:
+    case SDT_EM2 * 2:
       hipLaunchKernelGGL(k_em2_h<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork);
+      break;
+    case SDT_EM2 * 2 + 1:
       hipLaunchKernelGGL(k_em2_e<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
       break;
-    case SDT_ELASTIC25:
+    case SDT_ELASTIC25 * 2:
       hipLaunchKernelGGL(k_el25_stress<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork);
+      break;
+    case SDT_ELASTIC25 * 2 + 1:
       hipLaunchKernelGGL(k_el25_vel<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
       break;
-    case SDT_EM25:
+    case SDT_EM25 * 2:
       hipLaunchKernelGGL(k_em25_h<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork);
+      break;
+    case SDT_EM25 * 2 + 1:
       hipLaunchKernelGGL(k_em25_e<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
       break;
   }
-  if (p->maxrec > 0) {
-    const dim3 gg((p->maxrec + 63) / 64, p->nshots);
-    hipLaunchKernelGGL(k_gather<T>, gg, dim3(64), 0, st, p->dmodels, p->dshots, p->nshots,
-                       p->ncomp, p->dbase, rel);
-  }
+}
+
+template <typename T>
+void launch_finalize(Plan* p, hipStream_t st) {
+  if (p->kind == SDT_EM2 || p->kind == SDT_EM25)
+    hipLaunchKernelGGL(k_em_h_edges<T>, dim3(p->nwork), dim3(TX, TZ), 0, st, p->dmodels,
+                       p->dshots, p->dwork, p->kind == SDT_EM25 ? 1 : 0);
+}
+
+template <typename T>
+void launch_step(Plan* p, hipStream_t st, int rel) {
+  for (int w = 0; w < 3; ++w) launch_one<T>(p, st, rel, w);
 }
 
 void launch_steps(Plan* p, hipStream_t st, int n) {
@@ -1114,6 +1169,54 @@ int sdt_plan_run(void* plan, int it0, int nsteps, int graph_block, void* stream)
     for (; done + graph_block <= nsteps; done += graph_block) HIP_TRY(hipGraphLaunch(p->gexec, st));
   }
   if (done < nsteps) launch_steps(p, st, nsteps - done);
+  if (p->prec == SDT_FP64)
+    launch_finalize<double>(p, st);
+  else
+    launch_finalize<float>(p, st);
+  HIP_TRY(hipGetLastError());
+  return SDT_OK;
+}
+
+int sdt_plan_profile(void* plan, int it0, int nsteps, void* stream, double* out_ms) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p || !out_ms) return fail(SDT_EINVAL, "plan or out_ms is null");
+  if (it0 < 0 || nsteps <= 0 || (long long)it0 + nsteps > p->nt)
+    return fail(SDT_EINVAL, "step range outside [0, nt)");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  p->last = st;
+  hipEvent_t ev[4];
+  for (int q = 0; q < 4; ++q) HIP_TRY(hipEventCreate(&ev[q]));
+  double acc[3] = {0, 0, 0};
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, p->dbase, it0);
+  int rc = SDT_OK;
+  for (int s = 0; s < nsteps && rc == SDT_OK; ++s) {
+    for (int w = 0; w < 3; ++w) {
+      if (hipEventRecord(ev[w], st) != hipSuccess) { rc = fail(SDT_EDEVICE, "hipEventRecord"); break; }
+      if (p->prec == SDT_FP64)
+        launch_one<double>(p, st, s, w);
+      else
+        launch_one<float>(p, st, s, w);
+    }
+    if (rc) break;
+    if (hipEventRecord(ev[3], st) != hipSuccess || hipEventSynchronize(ev[3]) != hipSuccess) {
+      rc = fail(SDT_EDEVICE, "event sync failed");
+      break;
+    }
+    for (int w = 0; w < 3; ++w) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev[w], ev[w + 1]) != hipSuccess) { rc = fail(SDT_EDEVICE, "elapsed"); break; }
+      acc[w] += ms;
+    }
+  }
+  if (rc == SDT_OK) {
+    if (p->prec == SDT_FP64)
+      launch_finalize<double>(p, st);
+    else
+      launch_finalize<float>(p, st);
+  }
+  for (int q = 0; q < 4; ++q) (void)hipEventDestroy(ev[q]);
+  if (rc) return rc;
+  for (int w = 0; w < 3; ++w) out_ms[w] = acc[w] / nsteps;
   HIP_TRY(hipGetLastError());
   return SDT_OK;
 }

@@ -25,7 +25,34 @@ import numpy as np
 from .cpml import EPS0, MU0, AxisProfile, slab_width
 
 __all__ = ["KINDS", "FIELDS", "STIFFNESS", "PERMITTIVITY", "CONDUCTIVITY", "Shot", "Problem",
-           "kernel_coefs", "source_increments", "aux_b_x_half_by_row", "ncomp"]
+           "kernel_coefs", "source_increments", "aux_b_x_half_by_row", "ncomp",
+           "DENSITY_METHODS", "node_density"]
+
+# Density averaging at staggered velocity nodes.  The legacy solver has one
+# rule (weighted arithmetic mean); the reference binary takes a
+# density_method option whose arithmetic is not published: 'geometric' and
+# 'harmonic' are the weighted geometric and harmonic means with the same
+# weights, 'none' the cell's own density.
+DENSITY_METHODS = ("legacy", "arithmetic", "geometric", "harmonic", "none")
+
+
+def node_density(parts, weights, method: str = "legacy"):
+    """Weighted mean of the density samples ``parts`` with integer weights,
+    in the legacy expression order for 'legacy' / 'arithmetic'
+    (e.g. (2 rho + rho_e + rho_n) / 4, seismicfdtd.f95:342)."""
+    tot = float(sum(weights))
+    if method in ("legacy", "arithmetic"):
+        acc = weights[0] * parts[0] if weights[0] != 1 else parts[0]
+        for w, x in zip(weights[1:], parts[1:]):
+            acc = acc + (w * x if w != 1 else x)
+        return acc / tot
+    if method == "geometric":
+        return np.exp(sum(w * np.log(x) for w, x in zip(weights, parts)) / tot)
+    if method == "harmonic":
+        return tot / sum(w / x for w, x in zip(weights, parts))
+    if method == "none":
+        return np.array(parts[0], copy=True)
+    raise ValueError(f"density_method must be one of {DENSITY_METHODS}")
 
 KINDS = {"elastic2": 0, "elastic25": 1, "em2": 2, "em25": 3}
 FIELDS = {
@@ -172,9 +199,11 @@ def kernel_coefs(p: Problem):
         rho = C["rho"]
         gx = p.gamma.get("x", np.zeros_like(rho))
         gz = p.gamma.get("z", np.zeros_like(rho))
+        dm = getattr(p, "density_method", "legacy")
         dtr_z = np.zeros_like(rho)
         # deltarho = (2 rho(i,j) + rho(i+1,j) + rho(i,j+1)) / 4, seismicfdtd.f95:342
-        dtr_z[:-1, :-1] = dt / ((2 * rho[:-1, :-1] + rho[:-1, 1:] + rho[1:, :-1]) / 4)
+        dtr_z[:-1, :-1] = dt / node_density([rho[:-1, :-1], rho[:-1, 1:], rho[1:, :-1]],
+                                            [2, 1, 1], dm)
         return [
             _avg4_normal(C["c11"]), _avg4_normal(C["c13"]), _avg4_normal(C["c15"]),
             _avg4_normal(C["c33"]), _avg4_normal(C["c35"]),
@@ -201,12 +230,13 @@ def kernel_coefs(p: Problem):
         dx_ = np.zeros_like(rho)
         dy_ = np.zeros_like(rho)
         dz_ = np.zeros_like(rho)
+        dm = getattr(p, "density_method", "legacy")
         # (4 rho(i,k) + rho(i-1,k) + rho(i,k-1)) / 6, seismicfdtd.f95:826
-        dx_[1:, 1:] = dt / ((4 * rho[1:, 1:] + rho[1:, :-1] + rho[:-1, 1:]) / 6)
+        dx_[1:, 1:] = dt / node_density([rho[1:, 1:], rho[1:, :-1], rho[:-1, 1:]], [4, 1, 1], dm)
         # (4 rho(i,k) + rho(i+1,k) + rho(i,k-1)) / 6, :853
-        dy_[1:, :-1] = dt / ((4 * rho[1:, :-1] + rho[1:, 1:] + rho[:-1, :-1]) / 6)
+        dy_[1:, :-1] = dt / node_density([rho[1:, :-1], rho[1:, 1:], rho[:-1, :-1]], [4, 1, 1], dm)
         # (rho(i+1,k) + rho(i,k+1) + 4 rho(i,k)) / 6, :878
-        dz_[:-1, :-1] = dt / ((rho[:-1, 1:] + rho[1:, :-1] + 4 * rho[:-1, :-1]) / 6)
+        dz_[:-1, :-1] = dt / node_density([rho[:-1, 1:], rho[1:, :-1], rho[:-1, :-1]], [1, 1, 4], dm)
         z = np.zeros_like(rho)
         om = [1 - p.gamma.get(a, z) for a in ("x", "y", "z")]
         return L1 + L2 + L3 + L4 + [dx_, dy_, dz_] + om

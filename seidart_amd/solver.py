@@ -1,0 +1,178 @@
+"""Device-side run of the FDTD core: allocate buffers, build a plan, step.
+
+PyTorch is used only for device memory and the current stream; every
+update is a kernel of libseidart_hip.so reached through its C ABI.  One
+:class:`Simulation` holds any number of models of one physics kind (grids
+may differ in size) and any number of shots per model; each time step
+advances all of them in the same launches (the batched path that replaces
+one ``seidartfdtd`` process per shot, src/seidart/simulations/
+common_offset.py:143-283).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _hip
+from .problem import FIELDS, KINDS, Problem, aux_b_x_half_by_row, em_dbh, kernel_coefs, ncomp, \
+    source_increments
+
+__all__ = ["Simulation", "PRECISIONS"]
+
+PRECISIONS = {"fp64": (_hip.SDT_FP64, torch.float64), "fp32": (_hip.SDT_FP32, torch.float32)}
+
+
+def _dev_ptr(t: torch.Tensor | None):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class Simulation:
+    """All shots of one or more Problems (same kind) on one GPU."""
+
+    def __init__(self, problems, precision: str = "fp64", device: str = "cuda:0"):
+        if isinstance(problems, Problem):
+            problems = [problems]
+        if not problems:
+            raise ValueError("no problems")
+        kinds = {p.kind for p in problems}
+        if len(kinds) != 1:
+            raise ValueError("all problems of a Simulation must share one physics kind")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {list(PRECISIONS)}")
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU visible: the HIP path needs a ROCm device")
+        _hip.lib()  # fail early if the library is not built
+        self.kind = problems[0].kind
+        self.kid = KINDS[self.kind]
+        self.prec, self.dtype = PRECISIONS[precision]
+        self.precision = precision
+        self.device = torch.device(device)
+        self.problems = problems
+        self.nt = max(p.nt for p in problems)
+        self.ncomp = ncomp(self.kind)
+        self.names = FIELDS[self.kind]
+        self._keep = []  # every tensor a descriptor points to
+        T = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=self.dtype, device=self.device)
+
+        models = (_hip.SdtModel * len(problems))()
+        shot_list = []
+        self.shots = []  # (problem index, shot index, dict of tensors)
+        for m, p in enumerate(problems):
+            p.validate()
+            M = models[m]
+            M.nx, M.ny, M.nz = p.nx, p.ny, p.nz
+            M.lx, M.ly, M.lz = p.slabs()
+            M.dx, M.dy, M.dz, M.dt = p.dx, p.dy, p.dz, p.dt
+            if self.kind.startswith("em"):
+                M.scal[0] = em_dbh(p.dt)
+            coefs = [T(a) for a in kernel_coefs(p)]
+            self._keep += coefs
+            for q, t in enumerate(coefs):
+                M.coef[q] = t.data_ptr()
+            for ax, a in enumerate(("x", "y", "z")):
+                pr = p.prof.get(a)
+                if pr is None:
+                    continue
+                vals = [1.0 / pr.kappa, pr.a, pr.b, 1.0 / pr.kappa_h, pr.a_h, pr.b_h]
+                ts = [T(v) for v in vals]
+                self._keep += ts
+                for q, t in enumerate(ts):
+                    M.prof[ax][q] = t.data_ptr()
+            if self.kind == "elastic2":
+                aux = T(aux_b_x_half_by_row(p))
+                self._keep.append(aux)
+                M.aux = aux.data_ptr()
+            lx, ly, lz = p.slabs()
+            npsi = _hip.psi_elems(self.kid, p.nx, p.ny, p.nz, lx, ly, lz)
+            for s, sh in enumerate(p.shots):
+                init = (p.init[s] if p.init else None) or {}
+                bufs = {}
+                for n in self.names:
+                    if n in init:
+                        bufs[n] = T(np.asarray(init[n], dtype=np.float64).reshape(p.field_shape))
+                    else:
+                        bufs[n] = torch.zeros(p.field_shape, dtype=self.dtype, device=self.device)
+                bufs["psi"] = torch.zeros(max(npsi, 1), dtype=self.dtype, device=self.device)
+                inc = source_increments(p, sh)
+                bufs["src"] = [None if v is None else T(np.pad(v, (0, self.nt - v.size))) for v in inc]
+                rec = np.asarray(sh.rec, dtype=np.int32).reshape(-1, 3)
+                bufs["rec"] = torch.as_tensor(rec, device=self.device) if rec.size else None
+                bufs["trace"] = torch.zeros((self.nt, self.ncomp, max(rec.shape[0], 1)),
+                                            dtype=self.dtype, device=self.device)
+                bufs["nrec"] = rec.shape[0]
+                S = _hip.SdtShot()
+                S.model = m
+                S.isrc, S.jsrc, S.ksrc = (int(v) for v in sh.src)
+                for f, n in enumerate(self.names):
+                    S.field[f] = bufs[n].data_ptr()
+                S.psi = bufs["psi"].data_ptr()
+                for c in range(3):
+                    S.src[c] = bufs["src"][c].data_ptr() if bufs["src"][c] is not None else None
+                S.nrec = bufs["nrec"]
+                S.rec = bufs["rec"].data_ptr() if bufs["rec"] is not None else None
+                S.trace = bufs["trace"].data_ptr()
+                shot_list.append(S)
+                self.shots.append((m, s, bufs))
+        shots = (_hip.SdtShot * len(shot_list))(*shot_list)
+        torch.cuda.synchronize(self.device)
+        plan = C.c_void_p()
+        _hip.check(_hip.lib().sdt_plan_create(self.kid, self.prec, models, len(problems), shots,
+                                              len(shot_list), self.nt, C.byref(plan)))
+        self._plan = plan
+        self._models, self._shots = models, shots
+        self.step = 0
+
+    # ------------------------------------------------------------------
+    def run(self, nsteps: int | None = None, graph_block: int = 0, stream=None):
+        """Enqueue the next ``nsteps`` steps (default: to the end)."""
+        if nsteps is None:
+            nsteps = self.nt - self.step
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        _hip.check(_hip.lib().sdt_plan_run(self._plan, self.step, nsteps, graph_block,
+                                           C.c_void_p(stream.cuda_stream)))
+        self.step += nsteps
+        return self
+
+    def profile(self, nsteps: int, stream=None):
+        """Run the next ``nsteps`` steps with every kernel bracketed by HIP
+        events; return mean ms per launch of (first field kernel, second
+        field kernel, receiver gather)."""
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        out = (C.c_double * 3)()
+        _hip.check(_hip.lib().sdt_plan_profile(self._plan, self.step, nsteps,
+                                               C.c_void_p(stream.cuda_stream), out))
+        self.step += nsteps
+        return tuple(float(v) for v in out)
+
+    def check(self):
+        """Synchronise; raise UnstableError if any shot blew up."""
+        first = C.c_int(-1)
+        per = (C.c_int * len(self.shots))()
+        rc = _hip.lib().sdt_plan_check(self._plan, C.byref(first), per)
+        self.unstable = [int(v) for v in per]
+        _hip.check(rc)
+        return self
+
+    def traces(self):
+        """Per shot: numpy [nt, ncomp, nrec] (float64 copy)."""
+        torch.cuda.synchronize(self.device)
+        return [b["trace"][:, :, : b["nrec"]].double().cpu().numpy() for _, _, b in self.shots]
+
+    def field(self, shot: int, name: str) -> np.ndarray:
+        torch.cuda.synchronize(self.device)
+        return self.shots[shot][2][name].double().cpu().numpy()
+
+    def close(self):
+        if getattr(self, "_plan", None):
+            _hip.lib().sdt_plan_destroy(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,103 @@
+"""The C ABI: the library loads, exports every symbol include/seidart_hip.h
+declares, and its host-side argument checks reject bad input without a GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from seidart_amd import _hip
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "seidart_hip.h")
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(sdt_[a-z_]+)\s*\(", txt)))
+
+
+def test_header_declares_the_bound_entry_points():
+    assert declared_symbols() == sorted(_hip.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _hip.lib()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    assert b"gfx950" in lib.sdt_version()
+
+
+def test_struct_layout_matches_header():
+    # 6 ints, 4 doubles, scal[4], coef[40], prof[3][6], aux
+    assert C.sizeof(_hip.SdtModel) == 24 + 32 + 32 + 8 * 40 + 8 * 18 + 8
+    assert C.sizeof(_hip.SdtShot) == 16 + 8 * 9 + 8 + 8 * 3 + 8 + 8 + 8
+
+
+@pytest.mark.parametrize("kind,dims,expect", [
+    (_hip.SDT_ELASTIC2, (100, 1, 80, 10, 0, 12), 4 * 80 * 20 + 4 * 24 * 100),
+    (_hip.SDT_EM2, (100, 1, 80, 10, 0, 12), 2 * 80 * 20 + 2 * 24 * 100),
+    (_hip.SDT_ELASTIC25, (30, 20, 40, 5, 4, 6), 6 * (40 * 20 * 10 + 40 * 8 * 30 + 12 * 20 * 30)),
+    (_hip.SDT_EM25, (30, 20, 40, 5, 4, 6), 4 * (40 * 20 * 10 + 40 * 8 * 30 + 12 * 20 * 30)),
+])
+def test_psi_elems(kind, dims, expect):
+    assert _hip.psi_elems(kind, *dims) == expect
+
+
+def test_psi_elems_bad_kind():
+    with pytest.raises(_hip.SeidartHipError, match="unknown kind"):
+        _hip.psi_elems(9, 10, 1, 10, 1, 0, 1)
+
+
+def _model(nx=20, nz=20):
+    M = _hip.SdtModel()
+    M.nx, M.ny, M.nz = nx, 1, nz
+    M.lx, M.lz = 3, 3
+    M.dx = M.dy = M.dz = 1.0
+    M.dt = 1e-4
+    for q in range(12):
+        M.coef[q] = 0x1000  # never dereferenced: validation fails first
+    for a in (0, 2):
+        for q in range(6):
+            M.prof[a][q] = 0x1000
+    M.aux = 0x1000
+    return M
+
+
+def _shot(model=0, src=(5, 0, 5)):
+    S = _hip.SdtShot()
+    S.model = model
+    S.isrc, S.jsrc, S.ksrc = src
+    for f in range(5):
+        S.field[f] = 0x1000
+    S.psi = 0x1000
+    return S
+
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda M, S: setattr(M, "nx", 2), "grid too small"),
+    (lambda M, S: setattr(M, "lx", 50), "slab width"),
+    (lambda M, S: setattr(M, "dt", 0.0), "must be positive"),
+    (lambda M, S: setattr(S, "isrc", 99), "source outside"),
+    (lambda M, S: setattr(S, "model", 3), "bad model index"),
+    (lambda M, S: M.coef.__setitem__(4, None), "coefficient slot 4"),
+    (lambda M, S: setattr(M, "aux", None), "aux"),
+    (lambda M, S: setattr(S, "nrec", 4), "receivers need"),
+])
+def test_plan_create_rejects_bad_input(mutate, msg):
+    M, S = _model(), _shot()
+    mutate(M, S)
+    models = (_hip.SdtModel * 1)(M)
+    shots = (_hip.SdtShot * 1)(S)
+    plan = C.c_void_p()
+    rc = _hip.lib().sdt_plan_create(_hip.SDT_ELASTIC2, _hip.SDT_FP64, models, 1, shots, 1, 10,
+                                    C.byref(plan))
+    assert rc == _hip.SDT_EINVAL
+    assert msg in _hip.lib().sdt_last_error().decode()
+    assert not plan.value
+
+
+def test_plan_run_rejects_null_plan():
+    rc = _hip.lib().sdt_plan_run(None, 0, 1, 0, None)
+    assert rc == _hip.SDT_EINVAL

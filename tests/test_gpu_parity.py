@@ -1,0 +1,130 @@
+"""HIP path (through the C ABI) against the oracle and the legacy goldens.
+
+Bars: FP64 device traces and fields within 1e-10 relative L2 of the FP64
+oracle (itself bit-identical to the compiled legacy Fortran); FP32 device
+traces within the FP32 tolerance below."""
+import os
+
+import numpy as np
+import pytest
+
+from golden.make_golden import CASES, case_problem
+from helpers import rel_l2, tiny_problem
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FP32_TOL = 2e-5   # tiny grids, <= 80 steps: FP32 round-off vs the FP64 oracle
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+SMALL = {
+    "elastic2": dict(seed=31, nx=48, nz=40, npml=6, nt=90, gamma=1e-3),
+    "em2": dict(seed=32, nx=48, nz=40, npml=6, nt=90),
+    "elastic25": dict(seed=33, nx=28, ny=14, nz=22, npml=4, nt=45, gamma=1e-3),
+    "em25": dict(seed=34, nx=28, ny=14, nz=22, npml=4, nt=45),
+}
+
+
+def _sim(p, precision="fp64", **kw):
+    from seidart_amd.solver import Simulation
+    return Simulation(p, precision=precision, **kw)
+
+
+def _compare(sim, p, oracle_lib, tol, shot_offset=0):
+    tr = sim.traces()
+    for s in range(len(p.shots)):
+        o = oracle_lib.OracleRun(p, shot=s).run()
+        assert rel_l2(tr[shot_offset + s], o.trace) < tol, ("trace", s, rel_l2(tr[shot_offset + s], o.trace))
+        for n in o.names:
+            got = sim.field(shot_offset + s, n)
+            exp = o.field(n)
+            if np.abs(exp).max() == 0:
+                continue
+            assert rel_l2(got, exp) < tol, (n, s, rel_l2(got, exp))
+
+
+@pytest.mark.parametrize("kind", list(SMALL))
+def test_fp64_matches_oracle(oracle_lib, kind):
+    p = tiny_problem(kind, **SMALL[kind])
+    sim = _sim(p, "fp64").run().check()
+    _compare(sim, p, oracle_lib, FP64_TOL)
+
+
+@pytest.mark.parametrize("kind", list(SMALL))
+def test_fp32_close_to_oracle(oracle_lib, kind):
+    p = tiny_problem(kind, **SMALL[kind])
+    sim = _sim(p, "fp32").run().check()
+    tr = sim.traces()[0]
+    o = oracle_lib.OracleRun(p).run()
+    err = rel_l2(tr, o.trace)
+    print(f"{kind} fp32 trace rel L2 vs fp64 oracle: {err:.3e}")
+    assert err < FP32_TOL
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fp64_matches_legacy_golden(name):
+    """Straight against the legacy Fortran's own traces (tests/golden)."""
+    g = np.load(os.path.join(GOLDEN, f"legacy_{name}.npz"))
+    p = case_problem(name)
+    sim = _sim(p, "fp64").run().check()
+    tr = sim.traces()[0]
+    chans = {"elastic2": ["Vx", "Vz"], "em2": ["Ex", "Ez"], "elastic25": ["Vx", "Vy", "Vz"],
+             "em25": ["Ex", "Ey", "Ez"]}[p.kind]
+    for c, ch in enumerate(chans):
+        assert rel_l2(tr[:, c, :], g[f"trace_{ch}"]) < FP64_TOL, ch
+
+
+@pytest.mark.parametrize("kind", ["elastic2", "em2", "elastic25"])
+def test_batched_uneven_models(oracle_lib, kind):
+    """Several models of different size, several shots each, one plan: every
+    shot equals its own single-shot oracle run."""
+    is3d = kind.endswith("25")
+    dims = [dict(nx=40, nz=30), dict(nx=57, nz=45), dict(nx=33, nz=50)] if not is3d else \
+        [dict(nx=24, ny=10, nz=18), dict(nx=31, ny=12, nz=22)]
+    probs = [tiny_problem(kind, seed=40 + m, npml=4, nt=40, nshots=3, **d)
+             for m, d in enumerate(dims)]
+    sim = _sim(probs, "fp64").run().check()
+    off = 0
+    for p in probs:
+        _compare(sim, p, oracle_lib, FP64_TOL, shot_offset=off)
+        off += len(p.shots)
+
+
+def test_graph_capture_equals_direct_launch():
+    p = tiny_problem("elastic2", seed=51, nt=64, nshots=2)
+    a = _sim(p, "fp32").run(graph_block=0).check().traces()
+    b = _sim(p, "fp32").run(graph_block=8).check().traces()
+    c = _sim(p, "fp32").run(nsteps=13, graph_block=5).run(graph_block=5).check().traces()
+    for x, y, z in zip(a, b, c):
+        np.testing.assert_array_equal(x, y)
+        np.testing.assert_array_equal(x, z)
+
+
+def test_segments_equal_full_run():
+    p = tiny_problem("em2", seed=52, nt=60)
+    a = _sim(p).run().check().traces()[0]
+    s = _sim(p)
+    for n in (1, 10, 20, 29):
+        s.run(n)
+    b = s.check().traces()[0]
+    np.testing.assert_array_equal(a, b)
+
+
+def test_unstable_run_is_reported():
+    """dt far above the CFL limit: the device flag reports the step, the run
+    raises instead of the reference's silent process stop."""
+    from seidart_amd._hip import UnstableError
+    p = tiny_problem("elastic2", seed=53, nt=200)
+    p.dt *= 8
+    sim = _sim(p).run()
+    with pytest.raises(UnstableError):
+        sim.check()
+    assert sim.unstable[0] >= 0
+
+
+def test_receivers_on_edges_and_source_on_boundary(oracle_lib):
+    p = tiny_problem("elastic2", seed=54, nt=40)
+    p.shots[0].rec = np.array([[0, 0, 0], [p.nx - 1, 0, p.nz - 1], [5, 0, p.nz - 1]], dtype=np.int32)
+    p.shots[0].src = (1, 0, 1)
+    sim = _sim(p).run().check()
+    _compare(sim, p, oracle_lib, FP64_TOL)
