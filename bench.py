@@ -32,30 +32,31 @@ PROFILE_STEPS = 20
 
 
 def algorithmic_bytes(problems, nshots_by_model, itemsize):
-    """Compulsory HBM bytes per launch of the two field kernels of the
-    2-D elastic step, over the whole batch.
-
-    stress  : read vx vz, read+write sxx szz sxz (per shot); read the 8
-              staggered-node Cij averages (per model, shared by its shots);
-              read+write the 2+2 velocity-derivative psi slabs (per shot).
-    velocity: read sxx szz sxz, read+write vx vz (per shot); read dt/rho x2
-              and 1-gamma x2 (per model); read+write the 2+2 stress-derivative
-              psi slabs (per shot).
-    1-D C-PML profiles and source samples are counted too (negligible)."""
-    st = vel = 0
+    """Compulsory HBM bytes of one launch of the single-pass 2-D elastic step
+    kernel over the whole batch: per shot, read and write vx vz sxx szz sxz
+    once and read and write each of the 8 C-PML memory arrays on its slab
+    cells once; per model (shared by its shots), read its coefficient arrays
+    once (up to 12: 8 staggered-node Cij averages, dt/rho x2, 1-gamma x2;
+    the c15/c35 averages and 1-gamma are absent for isotropic, lossless
+    models) plus the 1-D C-PML profiles."""
+    from seidart_amd.problem import kernel_coefs
+    tot = 0
     for p, S in zip(problems, nshots_by_model):
         n = p.nx * p.nz
         lx, _, lz = p.slabs()
-        psi = 2 * (p.nz * 2 * lx) + 2 * (2 * lz * p.nx)     # elements, one kernel's half
+        psi = 4 * (p.nz * 2 * lx) + 4 * (2 * lz * p.nx)
         prof = 6 * p.nx + 7 * p.nz
-        st += S * (2 * n + 6 * n + 2 * psi) + 8 * n + prof
-        vel += S * (3 * n + 4 * n + 2 * psi + 2) + 4 * n + prof
-    return st * itemsize, vel * itemsize
+        K = kernel_coefs(p)
+        ncoef = 12 - (4 if all(not np.any(K[q]) for q in (2, 4, 5, 6)) else 0) \
+            - (2 if all(np.all(K[q] == 1.0) for q in (10, 11)) else 0)
+        tot += S * (10 * n + 2 * psi) + ncoef * n + prof
+    return tot * itemsize
 
 
 def cpu_baseline(problems, budget_s):
-    """Oracle on one shot of the middle model, on the host cores, for a
-    bounded number of steps (~budget_s seconds)."""
+    """Oracle (FP64 C restatement of the legacy loops, OpenMP) on one shot of
+    the largest model, on the host cores, for about budget_s seconds of
+    stepping; plus a short single-thread sample for the per-core rate."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     pyoracle.build()
@@ -65,20 +66,53 @@ def cpu_baseline(problems, budget_s):
         cores = os.cpu_count() or 1
     if os.environ.get("OMP_NUM_THREADS"):
         cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
-    p = problems[1]
+    p = problems[-1]
     shot = len(p.shots) // 2
-    o = pyoracle.OracleRun(p, shot=shot, threads=cores)
-    t0 = time.perf_counter()
-    o.run(0, 10)
-    probe = time.perf_counter() - t0
-    n = int(max(10, min(p.nt - 10, budget_s / max(probe / 10, 1e-6))))
-    t0 = time.perf_counter()
-    o.run(10, n)
-    el = time.perf_counter() - t0
-    cells = p.nx * p.nz * n
-    return {"value": cells / el / 1e9, "unit": "Gcell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"oracle (FP64 C/OpenMP restatement of seismic2), 1 shot of the "
-                      f"{p.nx}x{p.nz} padded model, {n} steps, {el:.1f} s"}
+
+    def timed(threads, budget, max_steps):
+        o = pyoracle.OracleRun(p, shot=shot, threads=threads)
+        o.run(0, 2)  # first touch of every array
+        t0 = time.perf_counter()
+        steps = 2
+        while time.perf_counter() - t0 < budget and steps < max_steps:
+            n = min(25, max_steps - steps)
+            o.run(steps, n)
+            steps += n
+        return steps - 2, time.perf_counter() - t0
+
+    n, el = timed(cores, budget_s, p.nt)
+    n1, el1 = timed(1, min(3.0, budget_s / 4), p.nt)
+    cells = p.nx * p.nz
+    return {"value": round(cells * n / el / 1e9, 4), "unit": "Gcell-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"oracle (FP64 C/OpenMP restatement of seismic2) on 1 shot of the "
+                      f"{p.nx}x{p.nz} padded model, {n} steps in {el:.1f} s on {cores} threads; "
+                      f"1 thread: {cells * n1 / el1 / 1e9:.4f} Gcell-updates/s ({n1} steps)",
+            "value_1core": round(cells * n1 / el1 / 1e9, 4)}
+
+
+def copy_calibration(gib: float) -> int:
+    """Time torch's device-to-device copy of `gib` GiB (read gib + write gib):
+    the achievable-bandwidth reference and the known-size kernel the HBM
+    byte counters are scaled by."""
+    import torch
+    n = int(gib * 2**30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "ms": round(ms, 4),
+                      "GBs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}), flush=True)
+    return 0
 
 
 def main(argv=None):
@@ -90,7 +124,11 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="only time a plain device copy of this many GiB (HBM calibration)")
     a = ap.parse_args(argv)
+    if a.copy_gib > 0:
+        return copy_calibration(a.copy_gib)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the batch shards by shot across GPUs by design; this build measures one GPU)", file=sys.stderr)
@@ -128,19 +166,14 @@ def main(argv=None):
     # per-kernel durations (HIP events around each launch, same stream)
     k_ms = sim.profile(PROFILE_STEPS)
     sim.check()
-    b_st, b_vel = algorithmic_bytes(problems, shots, 4)
-    dom = 0 if k_ms[0] >= k_ms[1] else 1
-    dom_bytes = (b_st, b_vel)[dom]
-    achieved = dom_bytes / (k_ms[dom] * 1e-3) / 1e9
-    step_bytes = b_st + b_vel
+    dom_bytes = algorithmic_bytes(problems, shots, 4)
+    achieved = dom_bytes / (k_ms[0] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": ("k_el2_stress<float>", "k_el2_vel<float>")[dom],
-                "kernel_ms": round(k_ms[dom], 4),
+                "kernel": "k_el2_step<float>", "kernel_ms": round(k_ms[0], 4),
                 "alg_bytes_per_launch": int(dom_bytes),
-                "other_kernels_ms": {"stress": round(k_ms[0], 4), "velocity": round(k_ms[1], 4),
-                                     "gather": round(k_ms[2], 4)},
-                "step_alg_GBs": round(step_bytes / (sum(k_ms[:2]) * 1e-3) / 1e9, 1)}
+                "alg_bytes_per_cell_update": round(dom_bytes / cells_per_step, 2),
+                "gather_ms": round(k_ms[2], 4)}
     traces32 = sim.traces()
     sim.close()
     del sim

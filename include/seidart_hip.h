@@ -62,7 +62,9 @@ extern "C" {
  *       s* = (c(i,j+1)+2c(i,j)+c(i-1,j))/4 at the shear node
  *       (seismicfdtd.f95:269-278, :307-310); dtr_x = dt/rho(i,j),
  *       dtr_z = dt/((2rho(i,j)+rho(i+1,j)+rho(i,j+1))/4) (:334, :342, :354);
- *       om = 1 - gamma (:334, :354).
+ *       om = 1 - gamma (:334, :354).  Slots 2, 4, 5, 6 (the c15 / c35
+ *       couplings) may be null together, meaning zero, and slots 10, 11 may
+ *       be null together, meaning no attenuation; the kernel then skips them.
  *   ELASTIC25 (39): the 4-point Cij averages of the four stress loops
  *       (seismicfdtd.f95:651-673 [15], :716-721 [6], :761-767 [6],
  *       :806-811 [6]), dt/deltarho for vx, vy, vz (:826, :853, :878),
@@ -99,6 +101,10 @@ typedef struct sdt_model {
  *          the initial conditions (classes.py:888-898).
  * psi: sdt_psi_elems() elements, zero before the first step, kept between
  *      calls of sdt_plan_run (a run may be split into segments).
+ * field_b, psi_b: ELASTIC2 only (NULL elsewhere): a second state buffer of
+ *      the same sizes.  Its single-pass step kernel reads one buffer and
+ *      writes the other; at the end of every sdt_plan_run the state is back
+ *      in field / psi, with field_b / psi_b holding scratch or a copy.
  * isrc, jsrc, ksrc: 0-based padded source cell (jsrc ignored in 2-D).
  * src[c]: nt per-step increments added to component c at the source cell
  *      after the update (seismicfdtd.f95:359-361: s(it)*dt/rho(src);
@@ -113,6 +119,8 @@ typedef struct sdt_shot {
   int isrc, jsrc, ksrc;
   void* field[9];
   void* psi;
+  void* field_b[9];
+  void* psi_b;
   const void* src[3];
   int nrec;
   const int* rec;

@@ -34,6 +34,8 @@ class SdtShot(C.Structure):
         ("model", C.c_int), ("isrc", C.c_int), ("jsrc", C.c_int), ("ksrc", C.c_int),
         ("field", C.c_void_p * 9),
         ("psi", C.c_void_p),
+        ("field_b", C.c_void_p * 9),
+        ("psi_b", C.c_void_p),
         ("src", C.c_void_p * 3),
         ("nrec", C.c_int),
         ("rec", C.c_void_p),

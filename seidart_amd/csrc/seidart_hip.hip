@@ -34,6 +34,13 @@
 
 #include "../../include/seidart_hip.h"
 
+#ifndef SDT_NT_STORE
+#define SDT_NT_STORE 0
+#endif
+#ifndef SDT_NT_LOAD
+#define SDT_NT_LOAD 0
+#endif
+
 namespace {
 
 thread_local std::string g_err;
@@ -68,6 +75,8 @@ struct DShot {
   int model, isrc, jsrc, ksrc;
   void* f[9];
   void* psi;
+  void* g[9];     // second state buffer (ping-pong kinds)
+  void* psi2;
   const void* src[3];
   int nrec;
   const int* rec;
@@ -77,14 +86,31 @@ struct DShot {
 
 enum { PK = 0, PA, PB, PKH, PAH, PBH };
 
+// Array pointers live in descriptors in device memory, so the compiler would
+// treat them as generic (flat) addresses; flat loads also count against the
+// LDS/scalar wait counter, which serialises every shuffle and scalar load
+// with all outstanding memory traffic.  Every array access therefore goes
+// through explicitly global (address space 1) pointers.
 template <typename T>
-__device__ __forceinline__ const T* cp(const void* p) {
-  return static_cast<const T*>(p);
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+using gcptr = const __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gcptr<T> cp(const void* p) {
+  return (gcptr<T>)p;
 }
 template <typename T>
-__device__ __forceinline__ T* mp(void* p) {
-  return static_cast<T*>(p);
+__device__ __forceinline__ gptr<T> mp(void* p) {
+  return (gptr<T>)p;
 }
+
+// C-PML profile table of a model: P[axis * 6 + q] (q as in PK..PBH).
+template <typename T>
+struct Prof {
+  const void* const* p;
+  __device__ explicit Prof(const void* const* q) : p(q) {}
+  __device__ gcptr<T> operator[](int q) const { return cp<T>(p[q]); }
+};
 
 // Flag instability: |v|^2 > 1e50 or non-finite (seismicfdtd.f95:378-379).
 __device__ __forceinline__ void check_blowup(double v2, int* flag, int it) {
@@ -97,163 +123,308 @@ __device__ __forceinline__ bool in_slab(int i, int n, int l) { return i < l || i
 
 // ===========================================================================
 // ELASTIC2: seismic2, legacy/fortran/seismicfdtd.f95:12-385
+//
+// One fused kernel per time step.  A wavefront owns a tile of EW columns x
+// EH rows of one shot and walks it top to bottom.  Lane l holds column
+// xo - 1 + l; lanes 1..EW own their column, lane 0 and lanes EW+1, EW+2 are
+// halo.  At row r the wave updates the stresses of row r (both legacy stress
+// loops, sharing the velocity-derivative psi as the legacy does) and then
+// the velocities of row r-1, which need the new stresses of rows r-2..r.
+// The previous and next velocity rows live in registers; x neighbours come
+// from the neighbouring lanes (wave shuffles).  Stresses of the halo row
+// above and below the tile and of the halo lanes are recomputed, so every
+// tile is independent; the state therefore ping-pongs between two buffers
+// (step `it` reads buffer it&1 and writes the other), which keeps halo reads
+// of neighbouring tiles on old values.  Each cell's fields, psi and
+// coefficients cross HBM once per step.
+//
 // psi slabs: x [nz][2lx]: dvx_dx dvz_dx dsxx_dx dsxz_dx
 //            z [2lz][nx]: dvz_dz dvx_dz dszz_dz dsxz_dz
 // ===========================================================================
+#ifndef SDT_EL2_ROWS
+#define SDT_EL2_ROWS 32
+#endif
+#ifndef SDT_EL2_GRID
+#define SDT_EL2_GRID 1000000
+#endif
+#ifndef SDT_EL2_WAVES_PER_SIMD
+#define SDT_EL2_WAVES_PER_SIMD 4
+#endif
+#ifndef SDT_EL2_WAVES_PER_BLOCK
+#define SDT_EL2_WAVES_PER_BLOCK 4
+#endif
+constexpr int EW = 61;            // owned columns per wavefront
+constexpr int EWB = SDT_EL2_WAVES_PER_BLOCK;  // wavefronts per block (side by side in x)
+constexpr int EH = SDT_EL2_ROWS;  // owned rows per wavefront
+
 template <typename T>
-__global__ void __launch_bounds__(NT)
-k_el2_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-             const int4* __restrict__ work) {
-  const int4 wi = work[blockIdx.x];
-  if (wi.x < 0) return;
-  const DShot& S = shots[wi.x];
-  const DModel& M = models[S.model];
-  const int nx = M.nx, nz = M.nz;
-  const int i = wi.y + threadIdx.x, j = wi.z + threadIdx.y;
-  if (i >= nx || j >= nz) return;
-  const T* __restrict__ vx = cp<T>(S.f[0]);
-  const T* __restrict__ vz = cp<T>(S.f[1]);
-  const long c = (long)j * nx + i;
-  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dt = T(M.dt);
-  const int lx = M.lx, lz = M.lz;
-  const bool inx = in_slab(i, nx, lx), inz = in_slab(j, nz, lz);
-  T* psi = mp<T>(S.psi);
-  const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
-  const long ox = (long)j * 2 * lx + slab_pos(i, nx, lx);
-  const long oz = (long)slab_pos(j, nz, lz) * nx + i;
-  T m_vxx = 0, m_vzx = 0, m_vzz = 0, m_vxz = 0;
-  if (inx) { m_vxx = psi[ox]; m_vzx = psi[SX + ox]; }
-  if (inz) { m_vzz = psi[4 * SX + oz]; m_vxz = psi[4 * SX + SZ + oz]; }
-  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
-  const T *ikx = P[0 * 6 + PK], *ax = P[0 * 6 + PA], *bx = P[0 * 6 + PB];
-  const T *ikxh = P[0 * 6 + PKH], *axh = P[0 * 6 + PAH], *bxh = P[0 * 6 + PBH];
-  const T *ikz = P[2 * 6 + PK], *az = P[2 * 6 + PA], *bz = P[2 * 6 + PB];
-  const T *ikzh = P[2 * 6 + PKH], *azh = P[2 * 6 + PAH], *bzh = P[2 * 6 + PBH];
-  const T vxc = vx[c], vzc = vz[c];
-  // normal stresses, seismicfdtd.f95:247-281
-  if (j >= 1 && i <= nx - 2) {
-    T dvx_dx = (vx[c + 1] - vxc) * idx;
-    T dvz_dz = (vzc - vz[c - nx]) * idz;
-    T dvz_dx = (vz[c + 1] - vzc) * idx;
-    T dvx_dz = (vxc - vx[c - nx]) * idz;
-    if (inx) {
-      m_vxx = cp<T>(M.aux)[j] * m_vxx + axh[i] * dvx_dx;  // b_x_half(j), :255
-      m_vzx = bx[i] * m_vzx + ax[i] * dvz_dx;
-      dvx_dx = dvx_dx * ikxh[i] + m_vxx;
-      dvz_dx = dvz_dx * ikx[i] + m_vzx;
-    }
-    if (inz) {
-      m_vzz = bz[j] * m_vzz + az[j] * dvz_dz;
-      m_vxz = bzh[j] * m_vxz + azh[j] * dvx_dz;
-      dvz_dz = dvz_dz * ikz[j] + m_vzz;
-      dvx_dz = dvx_dz * ikzh[j] + m_vxz;
-    }
-    const T sh = dvz_dx + dvx_dz;
-    T* sxx = mp<T>(S.f[2]);
-    T* szz = mp<T>(S.f[3]);
-    const T n13 = cp<T>(M.coef[1])[c];
-    sxx[c] = sxx[c] + (cp<T>(M.coef[0])[c] * dvx_dx + n13 * dvz_dz + cp<T>(M.coef[2])[c] * sh) * dt;
-    szz[c] = szz[c] + (n13 * dvx_dx + cp<T>(M.coef[3])[c] * dvz_dz + cp<T>(M.coef[4])[c] * sh) * dt;
-  }
-  // shear stress, seismicfdtd.f95:283-312 (same psi arrays: A1)
-  if (j <= nz - 2 && i >= 1) {
-    T dvx_dx = (vxc - vx[c - 1]) * idx;
-    T dvz_dz = (vz[c + nx] - vzc) * idz;
-    T dvz_dx = (vzc - vz[c - 1]) * idx;
-    T dvx_dz = (vx[c + nx] - vxc) * idz;
-    if (inx) {
-      m_vxx = bxh[i] * m_vxx + axh[i] * dvx_dx;
-      m_vzx = bx[i] * m_vzx + ax[i] * dvz_dx;
-      dvx_dx = dvx_dx * ikxh[i] + m_vxx;
-      dvz_dx = dvz_dx * ikx[i] + m_vzx;
-    }
-    if (inz) {
-      m_vzz = bz[j] * m_vzz + az[j] * dvz_dz;
-      m_vxz = bzh[j] * m_vxz + azh[j] * dvx_dz;
-      dvz_dz = dvz_dz * ikz[j] + m_vzz;
-      dvx_dz = dvx_dz * ikzh[j] + m_vxz;
-    }
-    T* sxz = mp<T>(S.f[4]);
-    sxz[c] = sxz[c] + (cp<T>(M.coef[5])[c] * dvx_dx + cp<T>(M.coef[6])[c] * dvz_dz +
-                       cp<T>(M.coef[7])[c] * (dvz_dx + dvx_dz)) * dt;
-  }
-  if (inx) { psi[ox] = m_vxx; psi[SX + ox] = m_vzx; }
-  if (inz) { psi[4 * SX + oz] = m_vzz; psi[4 * SX + SZ + oz] = m_vxz; }
+__device__ __forceinline__ T ld_stream(gcptr<T> p) {
+#if SDT_NT_LOAD
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+
+// Outputs of a step are not read again within it: stream them past L2.
+template <typename T>
+__device__ __forceinline__ void st_stream(gptr<T> p, T v) {
+#if SDT_NT_STORE
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
 }
 
 template <typename T>
-__global__ void __launch_bounds__(NT)
-k_el2_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-          const int4* __restrict__ work, const int* __restrict__ base, int rel) {
-  const int4 wi = work[blockIdx.x];
-  if (wi.x < 0) return;
+__device__ __forceinline__ T lane_next(T v) {  // value of lane l+1
+  return __shfl_down(v, 1, 64);
+}
+template <typename T>
+__device__ __forceinline__ T lane_prev(T v) {  // value of lane l-1
+  return __shfl_up(v, 1, 64);
+}
+
+template <typename T>
+__device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
+                                         const DShot* __restrict__ shots, const int4 wi, const int it,
+                                         const int lane);
+
+// Work: a block of EWB wavefronts takes one item (shot, xo, zo): wave w owns
+// columns xo + w*EW .. +EW-1 of rows zo .. zo+EH-1, so the block's row
+// accesses are EWB*EW contiguous cells of each array.  The grid is
+// persistent: block b takes items b, b + gridDim.x, ...; gridDim.x is a
+// multiple of 8, so item w always runs on XCD w % 8.
+template <typename T>
+__global__ void __launch_bounds__(64 * EWB, sizeof(T) == 4 ? SDT_EL2_WAVES_PER_SIMD : 3)
+k_el2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+           const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel) {
+  // wave-uniform values kept in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int it = __builtin_amdgcn_readfirstlane(*base + rel);
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const int4 wv = work[w];
+    const int4 wi = make_int4(__builtin_amdgcn_readfirstlane(wv.x),
+                              __builtin_amdgcn_readfirstlane(wv.y) + wave * EW,
+                              __builtin_amdgcn_readfirstlane(wv.z), 0);
+    if (wi.x >= 0 && wi.y < models[shots[wi.x].model].nx) el2_tile<T>(models, shots, wi, it, threadIdx.x & 63);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
+                                         const DShot* __restrict__ shots, const int4 wi, const int it,
+                                         const int lane) {
   const DShot& S = shots[wi.x];
   const DModel& M = models[S.model];
   const int nx = M.nx, nz = M.nz;
-  const int i = wi.y + threadIdx.x, j = wi.z + threadIdx.y;
-  if (i >= nx || j >= nz) return;
-  const int it = *base + rel;
-  const T* __restrict__ sxx = cp<T>(S.f[2]);
-  const T* __restrict__ szz = cp<T>(S.f[3]);
-  const T* __restrict__ sxz = cp<T>(S.f[4]);
-  T* vx = mp<T>(S.f[0]);
-  T* vz = mp<T>(S.f[1]);
-  const long c = (long)j * nx + i;
-  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz);
+  const int xo = wi.y, zo = wi.z;
+  const int i = xo - 1 + lane;
+  const bool incol = (i >= 0 && i < nx);
+  const int ic = incol ? i : (i < 0 ? 0 : nx - 1);  // clamped column: every load stays in bounds
+  const bool own = incol && lane >= 1 && lane <= EW;
+  const int zend = min(zo + EH, nz);
+  const int par = it & 1;
+  const auto vxi = cp<T>(par ? S.g[0] : S.f[0]);
+  const auto vzi = cp<T>(par ? S.g[1] : S.f[1]);
+  const auto sxxi = cp<T>(par ? S.g[2] : S.f[2]);
+  const auto szzi = cp<T>(par ? S.g[3] : S.f[3]);
+  const auto sxzi = cp<T>(par ? S.g[4] : S.f[4]);
+  auto vxo = mp<T>(par ? S.f[0] : S.g[0]);
+  auto vzo = mp<T>(par ? S.f[1] : S.g[1]);
+  auto sxxo = mp<T>(par ? S.f[2] : S.g[2]);
+  auto szzo = mp<T>(par ? S.f[3] : S.g[3]);
+  auto sxzo = mp<T>(par ? S.f[4] : S.g[4]);
+  const auto pin = cp<T>(par ? S.psi2 : S.psi);
+  auto pout = mp<T>(par ? S.psi : S.psi2);
+  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dt = T(M.dt);
   const int lx = M.lx, lz = M.lz;
-  const bool inx = in_slab(i, nx, lx), inz = in_slab(j, nz, lz);
-  T* psi = mp<T>(S.psi);
   const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
-  const long ox = (long)j * 2 * lx + slab_pos(i, nx, lx);
-  const long oz = (long)slab_pos(j, nz, lz) * nx + i;
-  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
-  T vxn = vx[c], vzn = vz[c];
-  // vx, seismicfdtd.f95:317-337
-  if (j >= 1 && i >= 1) {
-    T a = (sxx[c] - sxx[c - 1]) * idx;
-    T b = (sxz[c] - sxz[c - nx]) * idz;
-    if (inx) {
-      T m = psi[2 * SX + ox];
-      m = P[PB][i] * m + P[PA][i] * a;
-      a = a * P[PK][i] + m;
-      psi[2 * SX + ox] = m;
+  const bool inx = incol && in_slab(i, nx, lx);
+  const int xs = inx ? slab_pos(i, nx, lx) : 0;
+  const Prof<T> P(&M.prof[0][0]);
+  const T ikx = P[PK][ic], ax = P[PA][ic], bx = P[PB][ic];
+  const T ikxh = P[PKH][ic], axh = P[PAH][ic], bxh = P[PBH][ic];
+  const auto ikz = P[12 + PK], az = P[12 + PA], bz = P[12 + PB];
+  const auto ikzh = P[12 + PKH], azh = P[12 + PAH], bzh = P[12 + PBH];
+  const auto brow = cp<T>(M.aux);
+  const auto N11 = cp<T>(M.coef[0]), N13 = cp<T>(M.coef[1]), N15 = cp<T>(M.coef[2]);
+  const auto N33 = cp<T>(M.coef[3]), N35 = cp<T>(M.coef[4]);
+  const auto S15 = cp<T>(M.coef[5]), S35 = cp<T>(M.coef[6]), S55 = cp<T>(M.coef[7]);
+  const auto DTX = cp<T>(M.coef[8]), DTZ = cp<T>(M.coef[9]);
+  const auto OMX = cp<T>(M.coef[10]), OMZ = cp<T>(M.coef[11]);
+  const bool mono = M.coef[2] != nullptr;   // c15 / c35 terms present
+  const bool atten = M.coef[10] != nullptr; // gamma present
+  const bool can_n = incol && lane <= EW + 1 && i <= nx - 2;          // normal stress (needs i+1)
+  const bool can_s = incol && lane >= 1 && lane <= EW + 1 && i >= 1;  // shear (needs i-1)
+  const bool ux = own && i >= 1, uz = own && i <= nx - 2;              // vx / vz columns
+  const bool is_src_col = (i == S.isrc);
+  const int ksrc = S.ksrc;
+  const auto srcx = cp<T>(S.src[0]), srcz = cp<T>(S.src[2]);
+  const T incx = (srcx != nullptr) ? srcx[it] : T(0);
+  const T incz = (srcz != nullptr) ? srcz[it] : T(0);
+
+  auto row = [&](int r) { return (long)(r < 0 ? 0 : (r >= nz ? nz - 1 : r)) * nx + ic; };
+  auto valid = [&](int r) { return incol && r >= 0 && r < nz; };
+  T vx_m = valid(zo - 2) ? vxi[row(zo - 2)] : T(0);  // velocity row r-1
+  T vz_m = valid(zo - 2) ? vzi[row(zo - 2)] : T(0);
+  T vx_c = valid(zo - 1) ? vxi[row(zo - 1)] : T(0);  // velocity row r
+  T vz_c = valid(zo - 1) ? vzi[row(zo - 1)] : T(0);
+  T sxx_p = 0, szz_p = 0, sxz_p = 0, sxz_pp = 0;      // new stresses of rows r-1, r-2
+
+  for (int r = zo - 1; r <= zend; ++r) {
+    // ---- all loads of this row first (addresses clamped in bounds) ----
+    const int rr = r - 1;                             // velocity row
+    const long c = row(r), cv = row(rr), cn = row(r + 1);
+    const bool rv = valid(r);
+    const T ld_vxn = ld_stream(vxi + cn), ld_vzn = ld_stream(vzi + cn);
+    const T ld_sxx = ld_stream(sxxi + c), ld_szz = ld_stream(szzi + c), ld_sxz = ld_stream(sxzi + c);
+    const T n11 = N11[c], n13 = N13[c], n33 = N33[c], s55 = S55[c];
+    // monoclinic couplings and attenuation are optional (null = 0 and 1)
+    const T n15 = mono ? N15[c] : T(0), n35 = mono ? N35[c] : T(0);
+    const T s15 = mono ? S15[c] : T(0), s35 = mono ? S35[c] : T(0);
+    const T dtx = DTX[cv], dtz = DTZ[cv];
+    const T omx = atten ? OMX[cv] : T(1), omz = atten ? OMZ[cv] : T(1);
+    const int rs = r < 0 ? 0 : (r >= nz ? nz - 1 : r);
+    const int rvs = rr < 0 ? 0 : (rr >= nz ? nz - 1 : rr);
+    const bool inz = in_slab(rs, nz, lz), inzv = in_slab(rvs, nz, lz);
+    const long ox = (long)rs * 2 * lx + xs, oxv = (long)rvs * 2 * lx + xs;
+    const long oz = 4 * SX + (inz ? (long)slab_pos(rs, nz, lz) * nx + ic : 0);
+    const long ozv = 4 * SX + (inzv ? (long)slab_pos(rvs, nz, lz) * nx + ic : 0);
+    const long oxs = inx ? ox : 0, oxvs = inx ? oxv : 0;
+    T m_vxx = pin[oxs], m_vzx = pin[SX + oxs];            // x-slab psi of the stress row
+    T m_vzz = pin[oz], m_vxz = pin[SZ + oz];              // z-slab psi of the stress row
+    T m_sxx = pin[2 * SX + oxvs], m_sxzx = pin[3 * SX + oxvs];  // x-slab psi of the velocity row
+    T m_szz = pin[2 * SZ + ozv], m_sxzz = pin[3 * SZ + ozv];    // z-slab psi of the velocity row
+    const T bzr = bz[rs], azr = az[rs], ikzr = ikz[rs], bzhr = bzh[rs], azhr = azh[rs], ikzhr = ikzh[rs];
+    const T browr = brow[rs];
+    const T bzv = bz[rvs], azv = az[rvs], ikzv = ikz[rvs], bzhv = bzh[rvs], azhv = azh[rvs], ikzhv = ikzh[rvs];
+    const bool nxt = (r + 1 <= zend) && valid(r + 1);
+    const T vx_n = nxt ? ld_vxn : T(0), vz_n = nxt ? ld_vzn : T(0);  // velocity row r+1
+
+    // ---- stresses of row r (seismicfdtd.f95:247-312) ----
+    const T vx_e = lane_next(vx_c), vz_e = lane_next(vz_c);
+    const T vx_w = lane_prev(vx_c), vz_w = lane_prev(vz_c);
+    T sxx_c = rv ? ld_sxx : T(0), szz_c = rv ? ld_szz : T(0), sxz_c = rv ? ld_sxz : T(0);
+    const bool inx_r = inx && rv, inz_r = incol && rv && inz;
+    if (rv && can_n && r >= 1) {
+      T dvx_dx = (vx_e - vx_c) * idx;
+      T dvz_dz = (vz_c - vz_m) * idz;
+      T dvz_dx = (vz_e - vz_c) * idx;
+      T dvx_dz = (vx_c - vx_m) * idz;
+      if (inx_r) {
+        m_vxx = browr * m_vxx + axh * dvx_dx;  // b_x_half(j), :255
+        m_vzx = bx * m_vzx + ax * dvz_dx;
+        dvx_dx = dvx_dx * ikxh + m_vxx;
+        dvz_dx = dvz_dx * ikx + m_vzx;
+      }
+      if (inz_r) {
+        m_vzz = bzr * m_vzz + azr * dvz_dz;
+        m_vxz = bzhr * m_vxz + azhr * dvx_dz;
+        dvz_dz = dvz_dz * ikzr + m_vzz;
+        dvx_dz = dvx_dz * ikzhr + m_vxz;
+      }
+      const T sh = dvz_dx + dvx_dz;
+      sxx_c = sxx_c + (n11 * dvx_dx + n13 * dvz_dz + n15 * sh) * dt;
+      szz_c = szz_c + (n13 * dvx_dx + n33 * dvz_dz + n35 * sh) * dt;
     }
-    if (inz) {
-      T m = psi[4 * SX + 3 * SZ + oz];
-      m = P[12 + PB][j] * m + P[12 + PA][j] * b;
-      b = b * P[12 + PK][j] + m;
-      psi[4 * SX + 3 * SZ + oz] = m;
+    if (rv && can_s && r <= nz - 2 && r < zend) {  // same psi arrays (A1)
+      T dvx_dx = (vx_c - vx_w) * idx;
+      T dvz_dz = (vz_n - vz_c) * idz;
+      T dvz_dx = (vz_c - vz_w) * idx;
+      T dvx_dz = (vx_n - vx_c) * idz;
+      if (inx_r) {
+        m_vxx = bxh * m_vxx + axh * dvx_dx;
+        m_vzx = bx * m_vzx + ax * dvz_dx;
+        dvx_dx = dvx_dx * ikxh + m_vxx;
+        dvz_dx = dvz_dx * ikx + m_vzx;
+      }
+      if (inz_r) {
+        m_vzz = bzr * m_vzz + azr * dvz_dz;
+        m_vxz = bzhr * m_vxz + azhr * dvx_dz;
+        dvz_dz = dvz_dz * ikzr + m_vzz;
+        dvx_dz = dvx_dz * ikzhr + m_vxz;
+      }
+      sxz_c = sxz_c + (s15 * dvx_dx + s35 * dvz_dz + s55 * (dvz_dx + dvx_dz)) * dt;
     }
-    vxn = vxn * cp<T>(M.coef[10])[c] + (a + b) * cp<T>(M.coef[8])[c];
+    if (own && rv && r >= zo && r < zend) {
+      st_stream(sxxo + c, sxx_c);
+      st_stream(szzo + c, szz_c);
+      st_stream(sxzo + c, sxz_c);
+      if (inx) { pout[ox] = m_vxx; pout[SX + ox] = m_vzx; }
+      if (inz) { pout[oz] = m_vzz; pout[SZ + oz] = m_vxz; }
+    }
+
+    // ---- velocities of row rr = r - 1 (seismicfdtd.f95:317-375) ----
+    const T sxx_w = lane_prev(sxx_p);  // sigma_xx(i-1, rr)
+    const T sxz_e = lane_next(sxz_p);  // sigma_xz(i+1, rr)
+    if (own && rr >= zo && rr < zend) {
+      T vxn = vx_m, vzn = vz_m;
+      if (ux && rr >= 1) {
+        T a = (sxx_p - sxx_w) * idx;
+        T b = (sxz_p - sxz_pp) * idz;
+        if (inx) {
+          m_sxx = bx * m_sxx + ax * a;
+          a = a * ikx + m_sxx;
+        }
+        if (inzv) {
+          m_sxzz = bzv * m_sxzz + azv * b;
+          b = b * ikzv + m_sxzz;
+        }
+        vxn = vxn * omx + (a + b) * dtx;
+      }
+      if (uz && rr <= nz - 2) {
+        T a = (sxz_e - sxz_p) * idx;
+        T b = (szz_c - szz_p) * idz;
+        if (inx) {
+          m_sxzx = bxh * m_sxzx + axh * a;
+          a = a * ikxh + m_sxzx;
+        }
+        if (inzv) {
+          m_szz = bzhv * m_szz + azhv * b;
+          b = b * ikzhv + m_szz;
+        }
+        vzn = vzn * omz + (a + b) * dtz;
+      }
+      if (inx) { pout[2 * SX + oxv] = m_sxx; pout[3 * SX + oxv] = m_sxzx; }
+      if (inzv) { pout[2 * SZ + ozv] = m_szz; pout[3 * SZ + ozv] = m_sxzz; }
+      if (is_src_col && rr == ksrc) { vxn += incx; vzn += incz; }   // source, :359-361
+      if (i == 0 || i == nx - 1 || rr == 0 || rr == nz - 1) { vxn = 0; vzn = 0; }  // :365-375
+      check_blowup((double)vxn * vxn + (double)vzn * vzn, S.blowup, it);
+      st_stream(vxo + cv, vxn);
+      st_stream(vzo + cv, vzn);
+    }
+    sxz_pp = sxz_p;
+    sxx_p = sxx_c;
+    szz_p = szz_c;
+    sxz_p = sxz_c;
+    vx_m = vx_c;
+    vz_m = vz_c;
+    vx_c = vx_n;
+    vz_c = vz_n;
   }
-  // vz, seismicfdtd.f95:339-357
-  if (j <= nz - 2 && i <= nx - 2) {
-    T a = (sxz[c + 1] - sxz[c]) * idx;
-    T b = (szz[c + nx] - szz[c]) * idz;
-    if (inx) {
-      T m = psi[3 * SX + ox];
-      m = P[PBH][i] * m + P[PAH][i] * a;
-      a = a * P[PKH][i] + m;
-      psi[3 * SX + ox] = m;
+}
+
+// After a run whose last step wrote buffer B, copy B's fields and psi into A
+// so the caller always finds the state in field[] / psi (A); B keeps an
+// identical copy, which the next (odd) step reads.
+template <typename T>
+__global__ void k_copy_back(const DShot* __restrict__ shots, int nfield, const long* __restrict__ nel,
+                            const long* __restrict__ npsi) {
+  const int s = blockIdx.y;
+  const DShot& S = shots[s];
+  const long n = nel[s], np = npsi[s];
+  const long tot = nfield * n + np;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (long)gridDim.x * blockDim.x) {
+    if (q < nfield * n) {
+      const int f = (int)(q / n);
+      const long e = q - (long)f * n;
+      mp<T>(S.f[f])[e] = cp<T>(S.g[f])[e];
+    } else {
+      const long e = q - nfield * n;
+      mp<T>(S.psi)[e] = cp<T>(S.psi2)[e];
     }
-    if (inz) {
-      T m = psi[4 * SX + 2 * SZ + oz];
-      m = P[12 + PBH][j] * m + P[12 + PAH][j] * b;
-      b = b * P[12 + PKH][j] + m;
-      psi[4 * SX + 2 * SZ + oz] = m;
-    }
-    vzn = vzn * cp<T>(M.coef[11])[c] + (a + b) * cp<T>(M.coef[9])[c];
   }
-  // source, seismicfdtd.f95:359-361
-  if (i == S.isrc && j == S.ksrc) {
-    if (S.src[0]) vxn += cp<T>(S.src[0])[it];
-    if (S.src[2]) vzn += cp<T>(S.src[2])[it];
-  }
-  // Dirichlet edges, seismicfdtd.f95:365-375
-  if (i == 0 || i == nx - 1 || j == 0 || j == nz - 1) { vxn = 0; vzn = 0; }
-  check_blowup((double)vxn * vxn + (double)vzn * vzn, S.blowup, it);
-  vx[c] = vxn;
-  vz[c] = vzn;
 }
 
 // ===========================================================================
@@ -274,13 +445,13 @@ k_em2_h(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   const int nx = M.nx, nz = M.nz;
   const int i = wi.y + threadIdx.x, j = wi.z + threadIdx.y;
   if (i > nx - 2 || j > nz - 2) return;
-  const T* __restrict__ Ex = cp<T>(S.f[0]);
-  const T* __restrict__ Ez = cp<T>(S.f[1]);
-  T* Hy = mp<T>(S.f[2]);
+  const auto Ex = cp<T>(S.f[0]);
+  const auto Ez = cp<T>(S.f[1]);
+  auto Hy = mp<T>(S.f[2]);
   const long c = (long)j * nx + i;
   const int lx = M.lx, lz = M.lz;
-  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
-  T* psi = mp<T>(S.psi);
+  const Prof<T> P(&M.prof[0][0]);
+  auto psi = mp<T>(S.psi);
   const long SX = (long)nz * 2 * lx;
   // Hy, electromagfdtd.f95:249-266
   T a = (Ex[c + nx] - Ex[c]) * T(1.0 / M.dz);
@@ -314,13 +485,13 @@ k_em2_e(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   const int i = wi.y + threadIdx.x, j = wi.z + threadIdx.y;
   if (i >= nx || j >= nz) return;
   const int it = *base + rel;
-  const T* __restrict__ Hy = cp<T>(S.f[2]);
-  T* Ex = mp<T>(S.f[0]);
-  T* Ez = mp<T>(S.f[1]);
+  const auto Hy = cp<T>(S.f[2]);
+  auto Ex = mp<T>(S.f[0]);
+  auto Ez = mp<T>(S.f[1]);
   const long c = (long)j * nx + i;
   const int lx = M.lx, lz = M.lz;
-  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
-  T* psi = mp<T>(S.psi);
+  const Prof<T> P(&M.prof[0][0]);
+  auto psi = mp<T>(S.psi);
   const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
   T ex = Ex[c], ez = Ez[c];
   // Ex, electromagfdtd.f95:273-283
@@ -398,12 +569,12 @@ k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots
   const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz), dt = T(M.dt);
   const int lx = M.lx, ly = M.ly, lz = M.lz;
   const bool inx = in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
-  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
-  const T *ikx = P[PK], *ax = P[PA], *bx = P[PB], *ikxh = P[PKH], *axh = P[PAH], *bxh = P[PBH];
-  const T *iky = P[6 + PK], *ay = P[6 + PA], *by = P[6 + PB];
-  const T *ikyh = P[6 + PKH], *ayh = P[6 + PAH], *byh = P[6 + PBH];
-  const T *ikz = P[12 + PK], *az = P[12 + PA], *bz = P[12 + PB];
-  const T *ikzh = P[12 + PKH], *azh = P[12 + PAH], *bzh = P[12 + PBH];
+  const Prof<T> P(&M.prof[0][0]);
+  gcptr<T> ikx = P[PK], ax = P[PA], bx = P[PB], ikxh = P[PKH], axh = P[PAH], bxh = P[PBH];
+  gcptr<T> iky = P[6 + PK], ay = P[6 + PA], by = P[6 + PB];
+  gcptr<T> ikyh = P[6 + PKH], ayh = P[6 + PAH], byh = P[6 + PBH];
+  gcptr<T> ikz = P[12 + PK], az = P[12 + PA], bz = P[12 + PB];
+  gcptr<T> ikzh = P[12 + PKH], azh = P[12 + PAH], bzh = P[12 + PBH];
   // per-column profile values (x and z are fixed for the thread)
   const T bx_i = bx[i], ax_i = ax[i], ikx_i = ikx[i], bxh_i = bxh[i], axh_i = axh[i], ikxh_i = ikxh[i];
   const T bz_k = bz[k], az_k = az[k], ikz_k = ikz[k], bzh_k = bzh[k], azh_k = azh[k], ikzh_k = ikzh[k];
@@ -418,16 +589,16 @@ k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots
     C3[q] = p3 ? cp<T>(M.coef[21 + q])[p] : T(0);
     C4[q] = p4 ? cp<T>(M.coef[27 + q])[p] : T(0);
   }
-  const T* __restrict__ vx = cp<T>(S.f[0]);
-  const T* __restrict__ vy = cp<T>(S.f[1]);
-  const T* __restrict__ vz = cp<T>(S.f[2]);
-  T* sxx = mp<T>(S.f[3]);
-  T* syy = mp<T>(S.f[4]);
-  T* szz = mp<T>(S.f[5]);
-  T* sxy = mp<T>(S.f[6]);
-  T* sxz = mp<T>(S.f[7]);
-  T* syz = mp<T>(S.f[8]);
-  T* psi = mp<T>(S.psi);
+  const auto vx = cp<T>(S.f[0]);
+  const auto vy = cp<T>(S.f[1]);
+  const auto vz = cp<T>(S.f[2]);
+  auto sxx = mp<T>(S.f[3]);
+  auto syy = mp<T>(S.f[4]);
+  auto szz = mp<T>(S.f[5]);
+  auto sxy = mp<T>(S.f[6]);
+  auto sxz = mp<T>(S.f[7]);
+  auto syz = mp<T>(S.f[8]);
+  auto psi = mp<T>(S.psi);
   const Slab3 L(nx, ny, nz, lx, ly, lz);
   const int xs = slab_pos(i, nx, lx), zs = slab_pos(k, nz, lz);
 
@@ -559,27 +730,27 @@ k_el25_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz);
   const int lx = M.lx, ly = M.ly, lz = M.lz;
   const bool inx = in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
-  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
+  const Prof<T> P(&M.prof[0][0]);
   const T bx_i = P[PB][i], ax_i = P[PA][i], ikx_i = P[PK][i];
   const T bxh_i = P[PBH][i], axh_i = P[PAH][i], ikxh_i = P[PKH][i];
   const T bz_k = P[12 + PB][k], az_k = P[12 + PA][k], ikz_k = P[12 + PK][k];
   const T bzh_k = P[12 + PBH][k], azh_k = P[12 + PAH][k], ikzh_k = P[12 + PKH][k];
-  const T *by = P[6 + PB], *ay = P[6 + PA], *iky = P[6 + PK];
-  const T *byh = P[6 + PBH], *ayh = P[6 + PAH], *ikyh = P[6 + PKH];
+  gcptr<T> by = P[6 + PB], ay = P[6 + PA], iky = P[6 + PK];
+  gcptr<T> byh = P[6 + PBH], ayh = P[6 + PAH], ikyh = P[6 + PKH];
   const bool ux = (k >= 1 && i >= 1), uy = (k >= 1 && i <= nx - 2), uz = (k <= nz - 2 && i <= nx - 2);
   const T dtrx = ux ? cp<T>(M.coef[33])[p] : T(0), dtry = uy ? cp<T>(M.coef[34])[p] : T(0);
   const T dtrz = uz ? cp<T>(M.coef[35])[p] : T(0);
   const T omx = cp<T>(M.coef[36])[p], omy = cp<T>(M.coef[37])[p], omz = cp<T>(M.coef[38])[p];
-  const T* __restrict__ sxx = cp<T>(S.f[3]);
-  const T* __restrict__ syy = cp<T>(S.f[4]);
-  const T* __restrict__ szz = cp<T>(S.f[5]);
-  const T* __restrict__ sxy = cp<T>(S.f[6]);
-  const T* __restrict__ sxz = cp<T>(S.f[7]);
-  const T* __restrict__ syz = cp<T>(S.f[8]);
-  T* vx = mp<T>(S.f[0]);
-  T* vy = mp<T>(S.f[1]);
-  T* vz = mp<T>(S.f[2]);
-  T* psi = mp<T>(S.psi);
+  const auto sxx = cp<T>(S.f[3]);
+  const auto syy = cp<T>(S.f[4]);
+  const auto szz = cp<T>(S.f[5]);
+  const auto sxy = cp<T>(S.f[6]);
+  const auto sxz = cp<T>(S.f[7]);
+  const auto syz = cp<T>(S.f[8]);
+  auto vx = mp<T>(S.f[0]);
+  auto vy = mp<T>(S.f[1]);
+  auto vz = mp<T>(S.f[2]);
+  auto psi = mp<T>(S.psi);
   const Slab3 L(nx, ny, nz, lx, ly, lz);
   const int xs = slab_pos(i, nx, lx), zs = slab_pos(k, nz, lz);
   const bool kedge = (i == 0 || k == 0 || i == nx - 1 || k == nz - 1);
@@ -665,19 +836,19 @@ k_em25_h(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz), dbH = T(M.scal[0]);
   const int lx = M.lx, ly = M.ly, lz = M.lz;
   const bool inx = in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
-  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
+  const Prof<T> P(&M.prof[0][0]);
   const T bx_i = P[PB][i], ax_i = P[PA][i], ikx_i = P[PK][i];
   const T bz_k = P[12 + PB][k], az_k = P[12 + PA][k], ikz_k = P[12 + PK][k];
   const T bzh_k = P[12 + PBH][k], azh_k = P[12 + PAH][k], ikzh_k = P[12 + PKH][k];
-  const T *by = P[6 + PB], *ay = P[6 + PA], *iky = P[6 + PK];
-  const T *byh = P[6 + PBH], *ayh = P[6 + PAH], *ikyh = P[6 + PKH];
-  const T* __restrict__ Ex = cp<T>(S.f[0]);
-  const T* __restrict__ Ey = cp<T>(S.f[1]);
-  const T* __restrict__ Ez = cp<T>(S.f[2]);
-  T* Hx = mp<T>(S.f[3]);
-  T* Hy = mp<T>(S.f[4]);
-  T* Hz = mp<T>(S.f[5]);
-  T* psi = mp<T>(S.psi);
+  gcptr<T> by = P[6 + PB], ay = P[6 + PA], iky = P[6 + PK];
+  gcptr<T> byh = P[6 + PBH], ayh = P[6 + PAH], ikyh = P[6 + PKH];
+  const auto Ex = cp<T>(S.f[0]);
+  const auto Ey = cp<T>(S.f[1]);
+  const auto Ez = cp<T>(S.f[2]);
+  auto Hx = mp<T>(S.f[3]);
+  auto Hy = mp<T>(S.f[4]);
+  auto Hz = mp<T>(S.f[5]);
+  auto psi = mp<T>(S.psi);
   const Slab3e L(nx, ny, nz, lx, ly, lz);
   const int xs = slab_pos(i, nx, lx), zs = slab_pos(k, nz, lz);
   for (int j = 0; j <= ny - 2; ++j) {
@@ -734,24 +905,24 @@ k_em25_e(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz);
   const int lx = M.lx, ly = M.ly, lz = M.lz;
   const bool inx = in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
-  const T* const* P = reinterpret_cast<const T* const*>(M.prof);
+  const Prof<T> P(&M.prof[0][0]);
   const T bxh_i = P[PBH][i], axh_i = P[PAH][i], ikxh_i = P[PKH][i];
   const T bz_k = P[12 + PB][k], az_k = P[12 + PA][k], ikz_k = P[12 + PK][k];
   const T bzh_k = P[12 + PBH][k], azh_k = P[12 + PAH][k], ikzh_k = P[12 + PKH][k];
-  const T *byh = P[6 + PBH], *ayh = P[6 + PAH], *ikyh = P[6 + PKH];
+  gcptr<T> byh = P[6 + PBH], ayh = P[6 + PAH], ikyh = P[6 + PKH];
   const bool ux = (k >= 1 && k <= nz - 2 && i <= nx - 2);
   const bool uy = (k >= 1 && k <= nz - 2 && i >= 1 && i <= nx - 2);
   const bool uz = (k <= nz - 2 && i >= 1 && i <= nx - 2);
   const T cax = ux ? cp<T>(M.coef[0])[p] : T(0), cbx = ux ? cp<T>(M.coef[1])[p] : T(0);
   const T cay = uy ? cp<T>(M.coef[2])[p] : T(0), cby = uy ? cp<T>(M.coef[3])[p] : T(0);
   const T caz = uz ? cp<T>(M.coef[4])[p] : T(0), cbz = uz ? cp<T>(M.coef[5])[p] : T(0);
-  const T* __restrict__ Hx = cp<T>(S.f[3]);
-  const T* __restrict__ Hy = cp<T>(S.f[4]);
-  const T* __restrict__ Hz = cp<T>(S.f[5]);
-  T* Ex = mp<T>(S.f[0]);
-  T* Ey = mp<T>(S.f[1]);
-  T* Ez = mp<T>(S.f[2]);
-  T* psi = mp<T>(S.psi);
+  const auto Hx = cp<T>(S.f[3]);
+  const auto Hy = cp<T>(S.f[4]);
+  const auto Hz = cp<T>(S.f[5]);
+  auto Ex = mp<T>(S.f[0]);
+  auto Ey = mp<T>(S.f[1]);
+  auto Ez = mp<T>(S.f[2]);
+  auto psi = mp<T>(S.psi);
   const Slab3e L(nx, ny, nz, lx, ly, lz);
   const int xs = slab_pos(i, nx, lx), zs = slab_pos(k, nz, lz);
   const bool srccol = (i == S.isrc && k == S.ksrc);
@@ -811,7 +982,7 @@ k_em25_e(const DModel* __restrict__ models, const DShot* __restrict__ shots,
 // ===========================================================================
 template <typename T>
 __global__ void k_gather(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-                         int nshots, int ncomp, const int* __restrict__ base, int rel) {
+                         int nshots, int ncomp, const int* __restrict__ base, int rel, int pingpong) {
   const int s = blockIdx.y;
   if (s >= nshots) return;
   const DShot& S = shots[s];
@@ -819,10 +990,11 @@ __global__ void k_gather(const DModel* __restrict__ models, const DShot* __restr
   if (r >= S.nrec || S.trace == nullptr) return;
   const DModel& M = models[S.model];
   const int it = *base + rel;
-  const int* q = S.rec + 3 * r;
+  const auto q = cp<int>(S.rec) + 3 * r;
   const long c = ((long)q[2] * M.ny + q[1]) * M.nx + q[0];
-  T* tr = mp<T>(S.trace) + ((long)it * ncomp) * S.nrec + r;
-  for (int f = 0; f < ncomp; ++f) tr[(long)f * S.nrec] = cp<T>(S.f[f])[c];
+  auto tr = mp<T>(S.trace) + ((long)it * ncomp) * S.nrec + r;
+  const bool inb = pingpong && ((it + 1) & 1);  // state after step it
+  for (int f = 0; f < ncomp; ++f) tr[(long)f * S.nrec] = cp<T>(inb ? S.g[f] : S.f[f])[c];
 }
 
 // The H Dirichlet zeroing that closes each legacy EM step (electromagfdtd.f95
@@ -843,9 +1015,9 @@ k_em_h_edges(const DModel* __restrict__ models, const DShot* __restrict__ shots,
     if (i == 0 || i == nx - 2 || k == 0 || k == nz - 2) mp<T>(S.f[2])[(long)k * nx + i] = T(0);
     return;
   }
-  T* Hx = mp<T>(S.f[3]);
-  T* Hy = mp<T>(S.f[4]);
-  T* Hz = mp<T>(S.f[5]);
+  auto Hx = mp<T>(S.f[3]);
+  auto Hy = mp<T>(S.f[4]);
+  auto Hz = mp<T>(S.f[5]);
   for (int j = 0; j < ny; ++j) {
     const long c = ((long)k * ny + j) * nx + i;
     const bool lo = (i == 0 || j == 0 || k == 0);
@@ -878,8 +1050,11 @@ struct Plan {
   DShot* dshots = nullptr;
   int4* dwork = nullptr;
   int nwork = 0;
+  int el2_grid = 0;  // persistent grid of the ELASTIC2 step kernel
   int* dflags = nullptr;  // per shot first unstable step (INT_MAX = none)
   int* dbase = nullptr;   // step index of rel = 0
+  long* dnel = nullptr;   // per shot: field elements (ping-pong copy-back)
+  long* dnpsi = nullptr;  // per shot: psi elements
   hipStream_t last = nullptr;
   hipStream_t cap = nullptr;  // capture stream
   int graph_block = 0;
@@ -928,17 +1103,17 @@ void launch_one(Plan* p, hipStream_t st, int rel, int which) {
     if (p->maxrec > 0) {
       const dim3 gg((p->maxrec + 63) / 64, p->nshots);
       hipLaunchKernelGGL(k_gather<T>, gg, dim3(64), 0, st, p->dmodels, p->dshots, p->nshots,
-                         p->ncomp, p->dbase, rel);
+                         p->ncomp, p->dbase, rel, p->kind == SDT_ELASTIC2 ? 1 : 0);
     }
     return;
   }
   switch (p->kind * 2 + which) {
     case SDT_ELASTIC2 * 2:
-      hipLaunchKernelGGL(k_el2_stress<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork);
+      hipLaunchKernelGGL(k_el2_step<T>, dim3(p->el2_grid), dim3(64 * EWB), 0, st, p->dmodels,
+                         p->dshots, p->dwork, p->nwork, p->dbase, rel);
       break;
     case SDT_ELASTIC2 * 2 + 1:
-      hipLaunchKernelGGL(k_el2_vel<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
-      break;
+      break;  // the fused step kernel does both phases
     case SDT_EM2 * 2:
       hipLaunchKernelGGL(k_em2_h<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork);
       break;
@@ -958,6 +1133,12 @@ void launch_one(Plan* p, hipStream_t st, int rel, int which) {
       hipLaunchKernelGGL(k_em25_e<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
       break;
   }
+}
+
+template <typename T>
+void launch_copy_back(Plan* p, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy_back<T>, dim3(1024, p->nshots), dim3(256), 0, st, p->dshots,
+                     kind_fields(p->kind), p->dnel, p->dnpsi);
 }
 
 template <typename T>
@@ -999,8 +1180,17 @@ int validate(int kind, int prec, const sdt_model* models, int nmodels, const sdt
       return fail(SDT_EINVAL, tag + "slab width out of range");
     if (!(M.dx > 0) || !(M.dz > 0) || (d3 && !(M.dy > 0)) || !(M.dt > 0))
       return fail(SDT_EINVAL, tag + "dx, dy, dz, dt must be positive");
-    for (int q = 0; q < kind_ncoef(kind); ++q)
-      if (!M.coef[q]) return fail(SDT_EINVAL, tag + "coefficient slot " + std::to_string(q) + " is null");
+    for (int q = 0; q < kind_ncoef(kind); ++q) {
+      const bool optional = kind == SDT_ELASTIC2 && (q == 2 || q == 4 || q == 5 || q == 6 || q >= 10);
+      if (!M.coef[q] && !optional)
+        return fail(SDT_EINVAL, tag + "coefficient slot " + std::to_string(q) + " is null");
+    }
+    if (kind == SDT_ELASTIC2) {
+      const bool m1 = M.coef[2] || M.coef[4] || M.coef[5] || M.coef[6];
+      const bool m2 = M.coef[2] && M.coef[4] && M.coef[5] && M.coef[6];
+      if (m1 != m2 || (!M.coef[10]) != (!M.coef[11]))
+        return fail(SDT_EINVAL, tag + "ELASTIC2 slots 2,4,5,6 (and 10,11) must be all set or all null");
+    }
     for (int a = 0; a < 3; ++a) {
       if (a == 1 && !d3) continue;
       for (int q = 0; q < 6; ++q)
@@ -1017,6 +1207,12 @@ int validate(int kind, int prec, const sdt_model* models, int nmodels, const sdt
       if (!S.field[f]) return fail(SDT_EINVAL, tag + "field pointer null");
     if (!S.psi && psi_elems(kind, M.nx, M.ny, M.nz, M.lx, M.ly, M.lz) > 0)
       return fail(SDT_EINVAL, tag + "psi workspace null");
+    if (kind == SDT_ELASTIC2) {
+      for (int f = 0; f < kind_fields(kind); ++f)
+        if (!S.field_b[f]) return fail(SDT_EINVAL, tag + "field_b pointer null (ELASTIC2 ping-pongs)");
+      if (!S.psi_b && psi_elems(kind, M.nx, M.ny, M.nz, M.lx, M.ly, M.lz) > 0)
+        return fail(SDT_EINVAL, tag + "psi_b workspace null");
+    }
     if (S.isrc < 0 || S.isrc >= M.nx || S.ksrc < 0 || S.ksrc >= M.nz ||
         (d3 && (S.jsrc < 0 || S.jsrc >= M.ny)))
       return fail(SDT_EINVAL, tag + "source outside the padded grid");
@@ -1024,6 +1220,19 @@ int validate(int kind, int prec, const sdt_model* models, int nmodels, const sdt
       return fail(SDT_EINVAL, tag + "receivers need rec and trace buffers");
   }
   return SDT_OK;
+}
+
+// State after the last step of a run: EM H edges zeroed; ping-pong kinds
+// copied back into buffer A when the last step wrote buffer B.
+void end_of_run(Plan* p, hipStream_t st, int next_step) {
+  const bool b = (p->kind == SDT_ELASTIC2) && (next_step & 1);
+  if (p->prec == SDT_FP64) {
+    launch_finalize<double>(p, st);
+    if (b) launch_copy_back<double>(p, st);
+  } else {
+    launch_finalize<float>(p, st);
+    if (b) launch_copy_back<float>(p, st);
+  }
 }
 
 }  // namespace
@@ -1105,15 +1314,40 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
     D.model = S.model; D.isrc = S.isrc; D.jsrc = kind_3d(kind) ? S.jsrc : 0; D.ksrc = S.ksrc;
     for (int f = 0; f < 9; ++f) D.f[f] = S.field[f];
     D.psi = S.psi;
+    for (int f = 0; f < 9; ++f) D.g[f] = S.field_b[f];
+    D.psi2 = S.psi_b;
     for (int c = 0; c < 3; ++c) D.src[c] = S.src[c];
     D.nrec = S.nrec; D.rec = S.rec; D.trace = S.trace;
     D.blowup = p->dflags + s;
   }
   // work list: (shot, x0, z0); all shots of one tile share blockIdx % 8 (XCD)
   std::vector<int4> work;
-  for (int m = 0; m < nmodels; ++m) {
+  if (kind == SDT_ELASTIC2) {
+    // one item per block: EWB adjacent x tiles of one shot; items of one
+    // block-tile over all shots of the model share blockIdx % 8 (XCD)
+    for (int m = 0; m < nmodels; ++m) {
+      const sdt_model& M = models[m];
+      const int bw = EW * EWB;
+      const int tx = (M.nx + bw - 1) / bw, tz = (M.nz + EH - 1) / EH, ntile = tx * tz;
+      std::vector<int> ms;
+      for (int s = 0; s < nshots; ++s)
+        if (shots[s].model == m) ms.push_back(s);
+      if (ms.empty()) continue;
+      for (int tg = 0; tg < (ntile + 7) / 8; ++tg)
+        for (int s : ms)
+          for (int tt = 0; tt < 8; ++tt) {
+            const int t = tg * 8 + tt;
+            work.push_back(t < ntile ? make_int4(s, (t % tx) * bw, (t / tx) * EH, 0)
+                                     : make_int4(-1, 0, 0, 0));
+          }
+    }
+    const int nblk = (int)work.size();
+    p->el2_grid = std::max(8, std::min(nblk, SDT_EL2_GRID) / 8 * 8);
+  }
+  for (int m = 0; m < nmodels && kind != SDT_ELASTIC2; ++m) {
     const sdt_model& M = models[m];
-    const int tx = (M.nx + TX - 1) / TX, tz = (M.nz + TZ - 1) / TZ, nt_ = tx * tz;
+    const int wx = TX, wz = TZ;
+    const int tx = (M.nx + wx - 1) / wx, tz = (M.nz + wz - 1) / wz, nt_ = tx * tz;
     std::vector<int> ms;
     for (int s = 0; s < nshots; ++s)
       if (shots[s].model == m) ms.push_back(s);
@@ -1122,11 +1356,24 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
       for (int s : ms)
         for (int tt = 0; tt < 8; ++tt) {
           const int t = tg * 8 + tt;
-          if (t < nt_) work.push_back(make_int4(s, (t % tx) * TX, (t / tx) * TZ, 0));
+          if (t < nt_) work.push_back(make_int4(s, (t % tx) * wx, (t / tx) * wz, 0));
           else work.push_back(make_int4(-1, 0, 0, 0));
         }
   }
   p->nwork = (int)work.size();
+  {
+    std::vector<long> nel(nshots), npsi(nshots);
+    for (int s = 0; s < nshots; ++s) {
+      const sdt_model& M = models[shots[s].model];
+      nel[s] = (long)M.nx * M.ny * M.nz;
+      npsi[s] = (long)psi_elems(kind, M.nx, M.ny, M.nz, M.lx, kind_3d(kind) ? M.ly : 0, M.lz);
+    }
+    if (hipMalloc(&p->dnel, sizeof(long) * nshots) != hipSuccess ||
+        hipMalloc(&p->dnpsi, sizeof(long) * nshots) != hipSuccess ||
+        hipMemcpy(p->dnel, nel.data(), sizeof(long) * nshots, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->dnpsi, npsi.data(), sizeof(long) * nshots, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(SDT_EDEVICE, "hipMalloc/hipMemcpy failed (sizes)"));
+  }
   if (hipMalloc(&p->dmodels, sizeof(DModel) * nmodels) != hipSuccess ||
       hipMalloc(&p->dshots, sizeof(DShot) * nshots) != hipSuccess ||
       hipMalloc(&p->dwork, sizeof(int4) * work.size()) != hipSuccess)
@@ -1169,10 +1416,7 @@ int sdt_plan_run(void* plan, int it0, int nsteps, int graph_block, void* stream)
     for (; done + graph_block <= nsteps; done += graph_block) HIP_TRY(hipGraphLaunch(p->gexec, st));
   }
   if (done < nsteps) launch_steps(p, st, nsteps - done);
-  if (p->prec == SDT_FP64)
-    launch_finalize<double>(p, st);
-  else
-    launch_finalize<float>(p, st);
+  end_of_run(p, st, it0 + nsteps);
   HIP_TRY(hipGetLastError());
   return SDT_OK;
 }
@@ -1208,12 +1452,7 @@ int sdt_plan_profile(void* plan, int it0, int nsteps, void* stream, double* out_
       acc[w] += ms;
     }
   }
-  if (rc == SDT_OK) {
-    if (p->prec == SDT_FP64)
-      launch_finalize<double>(p, st);
-    else
-      launch_finalize<float>(p, st);
-  }
+  if (rc == SDT_OK) end_of_run(p, st, it0 + nsteps);
   for (int q = 0; q < 4; ++q) (void)hipEventDestroy(ev[q]);
   if (rc) return rc;
   for (int w = 0; w < 3; ++w) out_ms[w] = acc[w] / nsteps;
@@ -1249,6 +1488,8 @@ void sdt_plan_destroy(void* plan) {
   if (p->dwork) (void)hipFree(p->dwork);
   if (p->dflags) (void)hipFree(p->dflags);
   if (p->dbase) (void)hipFree(p->dbase);
+  if (p->dnel) (void)hipFree(p->dnel);
+  if (p->dnpsi) (void)hipFree(p->dnpsi);
   delete p;
 }
 

@@ -67,9 +67,20 @@ class Simulation:
             M.dx, M.dy, M.dz, M.dt = p.dx, p.dy, p.dz, p.dt
             if self.kind.startswith("em"):
                 M.scal[0] = em_dbh(p.dt)
-            coefs = [T(a) for a in kernel_coefs(p)]
-            self._keep += coefs
-            for q, t in enumerate(coefs):
+            host = kernel_coefs(p)
+            skip = set()
+            if self.kind == "elastic2":
+                # optional slots: c15/c35 couplings all zero, or no attenuation
+                if all(not np.any(host[q]) for q in (2, 4, 5, 6)):
+                    skip |= {2, 4, 5, 6}
+                if all(np.all(host[q] == 1.0) for q in (10, 11)):
+                    skip |= {10, 11}
+            for q, a in enumerate(host):
+                if q in skip:
+                    M.coef[q] = None
+                    continue
+                t = T(a)
+                self._keep.append(t)
                 M.coef[q] = t.data_ptr()
             for ax, a in enumerate(("x", "y", "z")):
                 pr = p.prof.get(a)
@@ -95,6 +106,10 @@ class Simulation:
                     else:
                         bufs[n] = torch.zeros(p.field_shape, dtype=self.dtype, device=self.device)
                 bufs["psi"] = torch.zeros(max(npsi, 1), dtype=self.dtype, device=self.device)
+                if self.kind == "elastic2":  # second state buffer of the single-pass kernel
+                    for n in self.names:
+                        bufs[n + "_b"] = torch.empty_like(bufs[n])
+                    bufs["psi_b"] = torch.empty_like(bufs["psi"])
                 inc = source_increments(p, sh)
                 bufs["src"] = [None if v is None else T(np.pad(v, (0, self.nt - v.size))) for v in inc]
                 rec = np.asarray(sh.rec, dtype=np.int32).reshape(-1, 3)
@@ -108,6 +123,10 @@ class Simulation:
                 for f, n in enumerate(self.names):
                     S.field[f] = bufs[n].data_ptr()
                 S.psi = bufs["psi"].data_ptr()
+                if self.kind == "elastic2":
+                    for f, n in enumerate(self.names):
+                        S.field_b[f] = bufs[n + "_b"].data_ptr()
+                    S.psi_b = bufs["psi_b"].data_ptr()
                 for c in range(3):
                     S.src[c] = bufs["src"][c].data_ptr() if bufs["src"][c] is not None else None
                 S.nrec = bufs["nrec"]

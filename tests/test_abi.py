@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layout_matches_header():
     # 6 ints, 4 doubles, scal[4], coef[40], prof[3][6], aux
     assert C.sizeof(_hip.SdtModel) == 24 + 32 + 32 + 8 * 40 + 8 * 18 + 8
-    assert C.sizeof(_hip.SdtShot) == 16 + 8 * 9 + 8 + 8 * 3 + 8 + 8 + 8
+    assert C.sizeof(_hip.SdtShot) == 16 + 8 * 9 + 8 + 8 * 9 + 8 + 8 * 3 + 8 + 8 + 8
 
 
 @pytest.mark.parametrize("kind,dims,expect", [
@@ -71,7 +71,9 @@ def _shot(model=0, src=(5, 0, 5)):
     S.isrc, S.jsrc, S.ksrc = src
     for f in range(5):
         S.field[f] = 0x1000
+        S.field_b[f] = 0x1000
     S.psi = 0x1000
+    S.psi_b = 0x1000
     return S
 
 
@@ -81,7 +83,8 @@ def _shot(model=0, src=(5, 0, 5)):
     (lambda M, S: setattr(M, "dt", 0.0), "must be positive"),
     (lambda M, S: setattr(S, "isrc", 99), "source outside"),
     (lambda M, S: setattr(S, "model", 3), "bad model index"),
-    (lambda M, S: M.coef.__setitem__(4, None), "coefficient slot 4"),
+    (lambda M, S: M.coef.__setitem__(3, None), "coefficient slot 3"),
+    (lambda M, S: M.coef.__setitem__(4, None), "must be all set or all null"),
     (lambda M, S: setattr(M, "aux", None), "aux"),
     (lambda M, S: setattr(S, "nrec", 4), "receivers need"),
 ])

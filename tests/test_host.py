@@ -1,0 +1,243 @@
+"""Host-side setup (FP64): C-PML profiles, sources, rotations, file formats,
+the drop-in file loader.  Pinned by closed forms and invariants."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from seidart_amd import configs, cpml, fortranio, problem, run as dropin, sources
+
+
+# --------------------------------------------------------------------- C-PML
+@pytest.mark.parametrize("seismic", [True, False])
+def test_cpml_interior_and_bounds(seismic):
+    n, c = 80, 12
+    dt = 1e-4 if seismic else 1e-11
+    sm = cpml.sigma_max_seismic(3000.0, c, 2, 1e-3) if seismic else cpml.sigma_max_em(c, 0.01, 2, 1e-3)
+    am = cpml.alpha_max_seismic(1.0, 100.0) if seismic else cpml.alpha_max_em(1.0, 5e8)
+    p = cpml.axis_profile(n, c, 0.5, sm, am, 5.0, 2, 2, dt, seismic)
+    inner = slice(c, n - c)
+    for arr, val in ((p.kappa, 1.0), (p.a, 0.0), (p.b, 1.0), (p.kappa_h, 1.0), (p.a_h, 0.0), (p.b_h, 1.0)):
+        np.testing.assert_array_equal(arr[inner], val)
+    for b in (p.b, p.b_h):
+        assert np.all((b > 0) & (b <= 1))
+    for a in (p.a, p.a_h):
+        assert np.all(a <= 0)
+    # mirror symmetry of the two slabs
+    np.testing.assert_array_equal(p.kappa[:c], p.kappa[::-1][:c])
+    np.testing.assert_array_equal(p.a_h[:c], p.a_h[::-1][:c])
+    assert cpml.slab_width(p) == c
+
+
+def test_cpml_closed_form_at_the_outer_cell():
+    """At the outermost cell the normalised distance is 1: sigma = sigma_max,
+    kappa = kappa_max, alpha = 0, b = exp(-sigma_max/kappa_max dt),
+    a = (b - 1)/kappa_max (definitions.py:940-996)."""
+    n, c, dt, kmax = 60, 10, 2e-4, 5.0
+    sm, am = 1234.0, 300.0
+    p = cpml.axis_profile(n, c, 1.0, sm, am, kmax, 2, 2, dt, True)
+    b = np.exp(-(sm / kmax) * dt)
+    for idx in (0, n - 1):
+        assert p.sigma[idx] == pytest.approx(sm, rel=1e-15)
+        assert p.kappa[idx] == pytest.approx(kmax, rel=1e-15)
+        assert p.b[idx] == pytest.approx(b, rel=1e-14)
+        assert p.a[idx] == pytest.approx((b - 1) / kmax, rel=1e-13)
+    # innermost slab cell: d = 1/c, r = d^2
+    r = (1.0 / c) ** 2
+    assert p.kappa[c - 1] == pytest.approx(1 + (kmax - 1) * r, rel=1e-14)
+    assert p.sigma[c - 1] == pytest.approx(sm * r, rel=1e-14)
+    assert p.alpha[c - 1] == pytest.approx(am * (1 - 1.0 / c) ** 2, rel=1e-14)
+
+
+def test_sigma_max_formulas():
+    # -ln(R) (NP+1) 0.7 v / (2 cpml) and -(NP+1) ln(R) eps0 c / (2 cpml dmin)
+    assert cpml.sigma_max_seismic(2000.0, 10, 2, 1e-3) == pytest.approx(
+        np.log(1e3) * 3 * 0.7 * 2000.0 / 20, rel=1e-15)
+    assert cpml.sigma_max_em(10, 0.02, 2, 1e-3) == pytest.approx(
+        3 * np.log(1e3) * cpml.EPS0 * cpml.CLIGHT / (20 * 0.02), rel=1e-15)
+
+
+def test_profiles_from_2d_fields():
+    n_x, n_z, c = 50, 40, 8
+    px = cpml.axis_profile(n_x, c, 1.0, 500.0, 100.0, 4.0, 2, 1, 1e-4, True)
+    pz = cpml.axis_profile(n_z, c, 1.0, 700.0, 100.0, 4.0, 2, 1, 1e-4, True)
+
+    def field(ax, az):
+        f = np.ones((n_z, n_x))
+        f[:, :c] = ax[None, :c]
+        f[:, n_x - c:] = ax[None, n_x - c:]
+        f[:c, c:n_x - c] = az[:c, None]
+        f[n_z - c:, c:n_x - c] = az[n_z - c:, None]
+        return f
+    flds = [field(getattr(px, q), getattr(pz, q)) for q in ("kappa", "a", "b", "kappa_h", "a_h", "b_h")]
+    # the interior of a and the exterior of the other axis hold the trivial value
+    for q, f in zip(("a", "a_h"), (flds[1], flds[4])):
+        f[c:n_z - c, c:n_x - c] = 0.0
+    gx, gz = cpml.profiles_from_2d(*flds, cpml=c)
+    np.testing.assert_array_equal(gx.kappa, px.kappa)
+    np.testing.assert_array_equal(gz.b_h, pz.b_h)
+
+
+# ------------------------------------------------------------------- sources
+def test_ricker_closed_form():
+    f = 100.0
+    dt = 1e-5
+    t = np.arange(4000) * dt
+    w = sources.wavelet(t, f, "gaus1")
+    k = int(round(1.0 / f / dt))
+    assert w[k] == pytest.approx(1.0, abs=1e-12)
+    assert np.argmax(w) == k
+    tz = 1 / f + 1 / (np.sqrt(2) * np.pi * f)   # zero crossing (1 - 2a^2 = 0)
+    a = np.pi * f * (tz - 1 / f)
+    assert (1 - 2 * a * a) == pytest.approx(0.0, abs=1e-12)
+    assert abs(np.interp(tz, t, w)) < 1e-3
+
+
+def test_rotate_sdr_and_force_mapping():
+    R = sources.rotate_sdr(30, 40, 50)
+    np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-14)
+    assert np.linalg.det(R) == pytest.approx(1.0, abs=1e-14)
+    fx, fy, fz = sources.point_force(50, 1e-4, 100.0)
+    assert np.abs(fy).max() == 0 and np.abs(fz).max() < 1e-16 and np.abs(fx).max() == pytest.approx(1.0)
+    fx, fy, fz = sources.point_force(50, 1e-4, 100.0, xz_rotation=90.0)
+    assert np.abs(fx).max() < 1e-15 and np.abs(fz).max() == pytest.approx(1.0)
+
+
+# -------------------------------------------------------------- anisotropy
+def _rand_rot(rng):
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    return q * np.sign(np.linalg.det(q))
+
+
+def test_rotate_stiffness_invariants():
+    rng = np.random.default_rng(1)
+    lam, mu = 3e9, 2e9
+    iso = np.zeros((6, 6))
+    iso[:3, :3] = lam
+    iso[np.arange(3), np.arange(3)] = lam + 2 * mu
+    iso[np.arange(3, 6), np.arange(3, 6)] = mu
+    R1, R2 = _rand_rot(rng), _rand_rot(rng)
+    np.testing.assert_allclose(configs.rotate_stiffness(iso, R1), iso, rtol=1e-12, atol=1e-3)
+    C = configs._ice_crystal_voigt()
+    a = configs.rotate_stiffness(configs.rotate_stiffness(C, R1), R2)
+    b = configs.rotate_stiffness(C, R2 @ R1)
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-2)
+    np.testing.assert_allclose(configs.rotate_stiffness(C, np.eye(3)), C, rtol=0, atol=0)
+    # phase velocity along R n in the rotated medium equals that along n
+    n = np.array([0.3, -0.5, 0.8])
+    row = lambda C6: dict({f"c{i + 1}{j + 1}": C6[i, j] for i in range(6) for j in range(i, 6)}, rho=917.0)
+    v0 = dropin.christoffel_vmax(row(C), n)
+    v1 = dropin.christoffel_vmax(row(configs.rotate_stiffness(C, R1)), R1 @ n)
+    assert v1 == pytest.approx(v0, rel=1e-12)
+    # ice along the c-axis: P speed sqrt(c33/rho)
+    assert dropin.christoffel_vmax(row(C), (0, 0, 1)) == pytest.approx(np.sqrt(15.01e9 / 917), rel=1e-12)
+
+
+# -------------------------------------------------------------- file formats
+def test_fortran_record_roundtrip_and_errors(tmp_path):
+    a = np.arange(12.0).reshape(3, 4)
+    f = str(tmp_path / "x.dat")
+    fortranio.write_record(f, a)
+    np.testing.assert_array_equal(fortranio.read_record(f, shape=(3, 4)), a)
+    raw = open(f, "rb").read()
+    assert raw[:4] == raw[-4:] == (96).to_bytes(4, "little")
+    with pytest.raises(ValueError, match="expected shape"):
+        fortranio.read_record(f, shape=(5, 4))
+    open(f, "wb").write(raw[:-6])
+    with pytest.raises(ValueError, match="truncated"):
+        fortranio.read_record(f)
+
+
+def test_snapshot_names():
+    assert fortranio.snapshot_name("Vx", 7, (128, 30)) == "Vx.000007.128.30.dat"
+    assert fortranio.snapshot_name("Vx", 7, (128, 30), legacy_2d_dots=True) == "Vx.000007.128.30..dat"
+    assert fortranio.snapshot_name("Ez", 123456, (1, 2, 3)) == "Ez.123456.1.2.3.dat"
+
+
+# ------------------------------------------------------------ kernel coefs
+def test_kernel_coef_averages_match_loops():
+    """The vectorised staggered-node averages equal the legacy loop bodies
+    (seismicfdtd.f95:269-278, :307-310, :342) evaluated cell by cell."""
+    from helpers import tiny_problem
+    p = tiny_problem("elastic2", seed=3, nx=12, nz=9, npml=2, nt=4, gamma=1e-3)
+    K = problem.kernel_coefs(p)
+    c11, c55, rho = p.coef["c11"], p.coef["c55"], p.coef["rho"]
+    for j in range(p.nz):
+        for i in range(p.nx):
+            n11 = (c11[j, i + 1] + 2 * c11[j, i] + c11[j - 1, i]) / 4 if (j >= 1 and i <= p.nx - 2) else 0
+            s55 = (c55[j + 1, i] + 2 * c55[j, i] + c55[j, i - 1]) / 4 if (j <= p.nz - 2 and i >= 1) else 0
+            dtz = p.dt / ((2 * rho[j, i] + rho[j, i + 1] + rho[j + 1, i]) / 4) \
+                if (j <= p.nz - 2 and i <= p.nx - 2) else 0
+            assert K[0][j, i] == n11 and K[7][j, i] == s55 and K[9][j, i] == dtz
+
+
+def test_node_density_methods():
+    a, b, c = np.array([2.0]), np.array([3.0]), np.array([5.0])
+    assert problem.node_density([a, b, c], [2, 1, 1], "legacy")[0] == (2 * 2.0 + 3.0 + 5.0) / 4
+    assert problem.node_density([a, b, c], [2, 1, 1], "geometric")[0] == pytest.approx(
+        (2.0 * 2.0 * 3.0 * 5.0) ** 0.25, rel=1e-14)
+    assert problem.node_density([a, b, c], [2, 1, 1], "harmonic")[0] == pytest.approx(
+        4 / (2 / 2.0 + 1 / 3.0 + 1 / 5.0), rel=1e-14)
+    assert problem.node_density([a, b, c], [2, 1, 1], "none")[0] == 2.0
+    with pytest.raises(ValueError):
+        problem.node_density([a, b, c], [2, 1, 1], "bogus")
+
+
+# ------------------------------------------------------------- file loader
+def _write_project(tmp, seismic=True):
+    """A small project the way Model.build leaves it on disk."""
+    nx0, nz0, c = 30, 20, 5
+    nx, nz = nx0 + 2 * c, nz0 + 2 * c
+    rng = np.random.default_rng(4)
+    prj = {"Domain": {"dim": 2, "nx": nx0, "ny": 1, "nz": nz0, "dx": 0.5, "dy": 1, "dz": 0.5,
+                      "cpml": c, "nmats": 1, "alpha_max_scalar": 1.0, "kappa_max": 5, "NP": 2,
+                      "NPA": 2, "Rcoef": 0.001, "image_file": None},
+           "Seismic": {"Source": {"dt": 5e-5, "time_steps": 40, "x": 5.0, "y": 0.0, "z": 3.0,
+                                  "xind": 10, "yind": 0, "zind": 6, "source_frequency": 200.0}},
+           "Electromagnetic": {"Source": {"dt": 1e-11, "time_steps": 40, "x": 5.0, "y": 0.0,
+                                          "z": 3.0, "xind": 10, "yind": 0, "zind": 6}}}
+    json.dump(prj, open(os.path.join(tmp, "p.json"), "w"))
+    vals = {}
+    for n in problem.STIFFNESS:
+        v = rng.uniform(1e9, 2e9, size=(nz, nx)) if n != "rho" else rng.uniform(900, 1000, size=(nz, nx))
+        vals[n] = v
+        fortranio.write_record(os.path.join(tmp, f"{n}.dat"), v)
+    for a in ("x", "z"):
+        fortranio.write_record(os.path.join(tmp, f"gamma_{a}.dat"), np.full((nz, nx), 1e-4))
+    px = cpml.axis_profile(nx, c, 0.5, 900.0, 600.0, 5.0, 2, 2, 5e-5, True)
+    pz = cpml.axis_profile(nz, c, 0.5, 800.0, 600.0, 5.0, 2, 2, 5e-5, True)
+    for q, h in (("kappa", ""), ("acoef", ""), ("bcoef", ""), ("kappa", "_half"), ("acoef", "_half"),
+                 ("bcoef", "_half")):
+        attr = {"kappa": "kappa", "acoef": "a", "bcoef": "b"}[q] + ("_h" if h else "")
+        ax, az = getattr(px, attr), getattr(pz, attr)
+        f = np.ones((nz, nx)) if q != "acoef" else np.zeros((nz, nx))
+        f[:, :c], f[:, nx - c:] = ax[None, :c], ax[None, nx - c:]
+        f[:c, c:nx - c], f[nz - c:, c:nx - c] = az[:c, None], az[nz - c:, None]
+        fortranio.write_record(os.path.join(tmp, f"{q}{h}_cpml.dat"), f)
+    src = sources.point_force(40, 5e-5, 200.0, xz_rotation=90.0)
+    for a, s in zip("xyz", src):
+        fortranio.write_record(os.path.join(tmp, f"seismicsource{a}.dat"), s)
+    return vals, px, pz
+
+
+def test_problem_from_files(tmp_path):
+    vals, px, pz = _write_project(str(tmp_path))
+    rec = np.array([[1.0, 0.0, 1.0], [7.5, 0.0, 2.0]])
+    p = dropin.problem_from_files(str(tmp_path / "p.json"), seismic=True, receivers=rec)
+    assert (p.kind, p.nx, p.nz, p.nt) == ("elastic2", 40, 30, 40)
+    np.testing.assert_array_equal(p.coef["c33"], vals["c33"])
+    np.testing.assert_array_equal(p.prof["x"].a_h, px.a_h)
+    np.testing.assert_array_equal(p.prof["z"].b, pz.b)
+    # legacy source placement: Fortran index xind + cpml -> 0-based xind + cpml - 1
+    assert p.shots[0].src == (10 + 5 - 1, 0, 6 + 5 - 1)
+    # receivers: int(xyz / d) + cpml (arraybuild.py:185-192)
+    np.testing.assert_array_equal(p.shots[0].rec, [[7, 0, 7], [20, 0, 9]])
+    assert p.gamma["x"][0, 0] == 1e-4
+    assert p.shots[0].force[1] is None
+
+
+def test_dropin_cli_reports_missing_project(capsys):
+    assert dropin.main(["/nonexistent/p.json", "seismic=true"]) == 1
+    assert "seidart_amd.run" in capsys.readouterr().err
+    assert dropin.main(["x.json", "density_method=bogus"]) == 2
