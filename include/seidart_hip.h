@@ -160,6 +160,65 @@ int sdt_plan_check(void* plan, int* first_step, int* per_shot);
  * (velocity / E) and out_ms[2] the receiver gather. */
 int sdt_plan_profile(void* plan, int it0, int nsteps, void* stream, double* out_ms);
 
+/* ------------------------------------------------------------------------
+ * Adjoint (gradients of a trace misfit).  No counterpart in the reference,
+ * which has no adjoint or gradient code; these entry points serve an FWI
+ * caller (a torch.autograd.Function in seidart_amd/adjoint.py).  ELASTIC2
+ * only.
+ *
+ * A "state" of one shot is one contiguous array of sdt_state_elems()
+ * elements: the kind's fields (field order of sdt_shot, nx*nz each) followed
+ * by its psi workspace.  The caller checkpoints forward states, copies one
+ * into slot 0 of a trail, recomputes the following states into the trail
+ * with sdt_plan_run_trail, and runs the adjoint steps over them with
+ * sdt_adjoint_run, last segment first.
+ * ------------------------------------------------------------------------ */
+
+/* Per-shot adjoint buffers (device memory, plan precision unless noted).
+ * trail:   trail_len states; slot k at trail + k * sdt_state_elems().
+ * lam[2]:  adjoint state ping-pong, one state each; lambda_t (dJ/du_t) lives
+ *          in lam[t & 1].  Zero both before the first (last-in-time) call.
+ * work:    sdt_adjoint_work_elems() elements of scratch.
+ * resid:   dJ/dtrace, laid out like sdt_shot.trace ([nt][ncomp][nrec]);
+ *          NULL = no residual.
+ * grad:    FP64 [12][nz][nx]: dJ/d(kernel coefficient slot q at cell),
+ *          accumulated (+=) over steps; shots of one model may share it
+ *          (atomic accumulation).
+ * gsrc:    FP64 [nt][3] or NULL: dJ/d(source increment of component c at
+ *          step t), written per step. */
+typedef struct sdt_adj_shot {
+  void* trail;
+  int trail_len;
+  void* lam[2];
+  void* work;
+  const void* resid;
+  double* grad;
+  double* gsrc;
+} sdt_adj_shot;
+
+/* Elements of one shot's state (fields then psi). */
+int sdt_state_elems(int kind, int nx, int ny, int nz, int lx, int ly, int lz,
+                    long long* out_elems);
+
+/* Elements of one shot's adjoint scratch. */
+int sdt_adjoint_work_elems(int kind, int nx, int ny, int nz, int lx, int ly, int lz,
+                           long long* out_elems);
+
+/* Hand the plan one sdt_adj_shot per shot (synchronises the device; call
+ * again only to change buffers). */
+int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots);
+
+/* Forward recomputation: with trail slot 0 holding u_{it0}, run steps it0 ..
+ * it0+nsteps-1 writing u_{it0+k} into slot k (no receiver gather).  Same
+ * kernels and arithmetic as sdt_plan_run. */
+int sdt_plan_run_trail(void* plan, int it0, int nsteps, void* stream);
+
+/* Adjoint steps t = t_hi-1 down to t_lo, with trail slot k holding
+ * u_{trail_t0 + k} for trail_t0 + k in [t_lo, t_hi].  Step t injects
+ * resid[t] at the receivers into lambda_{t+1}, then writes lambda_t and
+ * accumulates grad (and gsrc). */
+int sdt_adjoint_run(void* plan, int t_hi, int t_lo, int trail_t0, void* stream);
+
 /* Release the plan's own device memory and graphs. */
 void sdt_plan_destroy(void* plan);
 

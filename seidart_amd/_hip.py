@@ -43,6 +43,18 @@ class SdtShot(C.Structure):
     ]
 
 
+class SdtAdjShot(C.Structure):
+    _fields_ = [
+        ("trail", C.c_void_p),
+        ("trail_len", C.c_int),
+        ("lam", C.c_void_p * 2),
+        ("work", C.c_void_p),
+        ("resid", C.c_void_p),
+        ("grad", C.c_void_p),
+        ("gsrc", C.c_void_p),
+    ]
+
+
 class SeidartHipError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[sdt error {code}] {msg}")
@@ -57,7 +69,9 @@ class UnstableError(SeidartHipError):
 _lib = None
 
 EXPORTS = ["sdt_psi_elems", "sdt_plan_create", "sdt_plan_run", "sdt_plan_check",
-           "sdt_plan_profile", "sdt_plan_destroy", "sdt_last_error", "sdt_version"]
+           "sdt_plan_profile", "sdt_plan_destroy", "sdt_last_error", "sdt_version",
+           "sdt_state_elems", "sdt_adjoint_work_elems", "sdt_adjoint_setup",
+           "sdt_plan_run_trail", "sdt_adjoint_run"]
 
 
 def lib():
@@ -87,6 +101,16 @@ def lib():
     L.sdt_last_error.restype = C.c_char_p
     L.sdt_version.argtypes = []
     L.sdt_version.restype = C.c_char_p
+    L.sdt_state_elems.argtypes = [C.c_int] * 7 + [C.POINTER(C.c_longlong)]
+    L.sdt_state_elems.restype = C.c_int
+    L.sdt_adjoint_work_elems.argtypes = [C.c_int] * 7 + [C.POINTER(C.c_longlong)]
+    L.sdt_adjoint_work_elems.restype = C.c_int
+    L.sdt_adjoint_setup.argtypes = [C.c_void_p, C.POINTER(SdtAdjShot), C.c_int]
+    L.sdt_adjoint_setup.restype = C.c_int
+    L.sdt_plan_run_trail.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    L.sdt_plan_run_trail.restype = C.c_int
+    L.sdt_adjoint_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    L.sdt_adjoint_run.restype = C.c_int
     _lib = L
     return L
 
@@ -103,4 +127,16 @@ def check(rc: int) -> None:
 def psi_elems(kind: int, nx, ny, nz, lx, ly, lz) -> int:
     out = C.c_longlong(0)
     check(lib().sdt_psi_elems(kind, nx, ny, nz, lx, ly, lz, C.byref(out)))
+    return int(out.value)
+
+
+def state_elems(kind: int, nx, ny, nz, lx, ly, lz) -> int:
+    out = C.c_longlong(0)
+    check(lib().sdt_state_elems(kind, nx, ny, nz, lx, ly, lz, C.byref(out)))
+    return int(out.value)
+
+
+def adjoint_work_elems(kind: int, nx, ny, nz, lx, ly, lz) -> int:
+    out = C.c_longlong(0)
+    check(lib().sdt_adjoint_work_elems(kind, nx, ny, nz, lx, ly, lz, C.byref(out)))
     return int(out.value)

@@ -1,0 +1,252 @@
+"""Gradients of a receiver-trace misfit for the elastic 2-D scheme.
+
+The reference has no adjoint or gradient code; this module adds full
+waveform inversion (FWI) gradients on top of the same forward scheme
+(legacy/fortran/seismicfdtd.f95:12-385, seismic2).  Pieces:
+
+* :class:`Elastic2Engine` runs the forward of every shot of one
+  :class:`~seidart_amd.solver.Simulation`, keeping a full state (fields and
+  C-PML memory variables) every ``every`` steps.  Its backward pass walks
+  the segments last to first: it copies the segment's checkpoint into slot 0
+  of a trail, recomputes the segment's states into the trail
+  (``sdt_plan_run_trail``: the forward kernel writing each new state into
+  the next slot), then runs the hand-written adjoint kernels over the
+  segment (``sdt_adjoint_run``).  Memory: one checkpoint per segment plus
+  ``every + 1`` trail states per shot.
+* The adjoint yields dJ/d(kernel coefficient) per cell in FP64 (the slots of
+  ``include/seidart_hip.h``: n11 n13 n15 n33 n35 s15 s35 s55 dt/rho_x
+  dt/rho_z 1-gx 1-gz) and dJ/d(source increment) per step.
+* :class:`Elastic2Traces` is the ``torch.autograd.Function`` around it:
+  inputs are those coefficient slots and increments, output the traces.
+  :func:`elastic2_traces` composes it with the coefficient setup
+  (:func:`seidart_amd.problem.elastic2_coefs`, the averaging stencils and
+  dt / rho written with torch operations), so torch's autograd carries the
+  chain rule to the per-cell Cij, density and attenuation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _hip
+from .problem import elastic2_coefs
+from .solver import Simulation
+
+__all__ = ["Elastic2Engine", "Elastic2Traces", "elastic2_traces", "PARAMS"]
+
+NSLOT = 12
+# per-cell parameters of elastic2_traces
+PARAMS = ("c11", "c13", "c15", "c33", "c35", "c55", "rho", "gamma_x", "gamma_z")
+
+
+class Elastic2Engine:
+    """Checkpointed forward and adjoint of every shot of elastic 2-D problems.
+
+    ``problems``: one Problem or a list (one plan; shots of all models run in
+    the same launches).  ``every``: steps per checkpoint segment.
+    """
+
+    def __init__(self, problems, precision: str = "fp32", every: int = 64,
+                 device: str = "cuda:0", graph: bool = True):
+        self.sim = Simulation(problems, precision, device, all_slots=True)
+        if self.sim.kind != "elastic2":
+            raise ValueError("the adjoint engine handles elastic2 problems only")
+        if every < 1:
+            raise ValueError("every must be >= 1")
+        sim = self.sim
+        self.every = int(every)
+        self.graph = graph
+        self.nt = sim.nt
+        self.nseg = (self.nt + self.every - 1) // self.every
+        self.device = sim.device
+        self.dtype = sim.dtype
+        dev, dt = self.device, self.dtype
+        self.grad = []
+        for p in sim.problems:
+            self.grad.append(torch.zeros((NSLOT, p.nz, p.nx), dtype=torch.float64, device=dev))
+        self.ckpt, self.trail, self.lam, self.work, self.gsrc, self.resid, self.init = ([] for _ in range(7))
+        adj = (_hip.SdtAdjShot * len(sim.shots))()
+        for s, (m, _, b) in enumerate(sim.shots):
+            p = sim.problems[m]
+            lx, ly, lz = p.slabs()
+            nst = b["state"].numel()
+            nwk = _hip.adjoint_work_elems(sim.kid, p.nx, p.ny, p.nz, lx, ly, lz)
+            self.ckpt.append(torch.zeros((self.nseg, nst), dtype=dt, device=dev))
+            self.trail.append(torch.zeros((self.every + 1, nst), dtype=dt, device=dev))
+            self.lam.append(torch.zeros((2, nst), dtype=dt, device=dev))
+            self.work.append(torch.zeros(nwk, dtype=dt, device=dev))
+            self.gsrc.append(torch.zeros((self.nt, 3), dtype=torch.float64, device=dev))
+            self.resid.append(torch.zeros_like(b["trace"]))
+            self.init.append(b["state"].clone())
+            A = adj[s]
+            A.trail = self.trail[s].data_ptr()
+            A.trail_len = self.every + 1
+            A.lam[0] = self.lam[s][0].data_ptr()
+            A.lam[1] = self.lam[s][1].data_ptr()
+            A.work = self.work[s].data_ptr()
+            A.resid = self.resid[s].data_ptr()
+            A.grad = self.grad[m].data_ptr()
+            A.gsrc = self.gsrc[s].data_ptr()
+        self._adj = adj
+        self.forward_id = 0  # checkpoints belong to the latest forward
+        torch.cuda.synchronize(dev)
+        _hip.check(_hip.lib().sdt_adjoint_setup(sim._plan, adj, len(sim.shots)))
+
+    # ------------------------------------------------------------------
+    @property
+    def nshots(self) -> int:
+        return len(self.sim.shots)
+
+    def set_coefs(self, model: int, slots) -> None:
+        """Overwrite model ``model``'s 12 coefficient slots (FP64 tensors or
+        arrays of shape (nz, nx))."""
+        for q, v in enumerate(slots):
+            self.sim.coef_t[model][q].copy_(torch.as_tensor(v).to(self.device, self.dtype))
+
+    def set_increments(self, shot: int, inc) -> None:
+        """Overwrite shot ``shot``'s per-step source increments: ``inc`` of
+        shape (3, nt) (x, y, z; y unused in 2-D)."""
+        src = self.sim.shots[shot][2]["src"]
+        for c in (0, 2):
+            src[c][: self.nt].copy_(torch.as_tensor(inc[c]).to(self.device, self.dtype)[: self.nt])
+
+    def forward(self):
+        """Run all steps from the initial state, checkpointing every
+        ``every`` steps.  Returns the per-shot device trace buffers
+        ([nt, 2, nrec_max], plan precision)."""
+        sim = self.sim
+        self.forward_id += 1
+        for s, (_, _, b) in enumerate(sim.shots):
+            b["state"].copy_(self.init[s])
+            b["trace"].zero_()
+        sim.step = 0
+        for g in range(self.nseg):
+            for s, (_, _, b) in enumerate(sim.shots):
+                self.ckpt[s][g].copy_(b["state"])
+            n = min(self.every, self.nt - g * self.every)
+            sim.run(n, graph_block=self.every if (self.graph and n == self.every) else 0)
+        sim.check()
+        return [b["trace"] for _, _, b in sim.shots]
+
+    def backward(self, resid):
+        """Adjoint pass for the residuals ``resid`` (per shot, dJ/dtrace of
+        shape [nt, 2, nrec]).  Must follow :meth:`forward` with the same
+        coefficients.  Returns (grad, gsrc): per model the FP64
+        [12, nz, nx] coefficient-slot gradient, per shot the FP64 [nt, 3]
+        source-increment gradient."""
+        sim = self.sim
+        L = _hip.lib()
+        stream = torch.cuda.current_stream(self.device)
+        h = C.c_void_p(stream.cuda_stream)
+        for g in self.grad:
+            g.zero_()
+        for s in range(self.nshots):
+            self.lam[s].zero_()
+            self.gsrc[s].zero_()
+            r = torch.as_tensor(resid[s]).to(self.device, self.dtype)
+            self.resid[s].zero_()
+            self.resid[s][:, :, : r.shape[2]].copy_(r)
+        for g in range(self.nseg - 1, -1, -1):
+            t_lo = g * self.every
+            t_hi = min(self.nt, t_lo + self.every)
+            for s in range(self.nshots):
+                self.trail[s][0].copy_(self.ckpt[s][g])
+            _hip.check(L.sdt_plan_run_trail(sim._plan, t_lo, t_hi - t_lo, h))
+            _hip.check(L.sdt_adjoint_run(sim._plan, t_hi, t_lo, t_lo, h))
+        return self.grad, self.gsrc
+
+    def adjoint_state(self, shot: int) -> torch.Tensor:
+        """lambda_0 = dJ/d(initial state) of a shot after :meth:`backward`
+        (fields then psi, the layout of the state)."""
+        return self.lam[shot][0]
+
+    def close(self):
+        self.sim.close()
+
+
+class Elastic2Traces(torch.autograd.Function):
+    """traces = F(coefficient slots, source increments).
+
+    Inputs after the engine: per model one FP64 tensor [12, nz, nx] of
+    coefficient slots, then per shot one FP64 tensor [3, nt] of source
+    increments.  Output: FP64 traces [nshots, nt, 2, nrec_max].  The
+    backward pass is the engine's checkpointed adjoint.
+    """
+
+    @staticmethod
+    def forward(ctx, engine: Elastic2Engine, *inputs):
+        nm = len(engine.sim.problems)
+        for m in range(nm):
+            engine.set_coefs(m, inputs[m].detach())
+        for s in range(engine.nshots):
+            engine.set_increments(s, inputs[nm + s].detach())
+        tr = engine.forward()
+        nrec = max(t.shape[2] for t in tr)
+        out = torch.zeros((len(tr), engine.nt, 2, nrec), dtype=torch.float64, device=engine.device)
+        for s, t in enumerate(tr):
+            out[s, :, :, : t.shape[2]] = t.double()
+        ctx.engine = engine
+        ctx.forward_id = engine.forward_id
+        ctx.devs = [x.device for x in inputs]
+        return out.to(inputs[0].device)
+
+    @staticmethod
+    def backward(ctx, gout):
+        eng = ctx.engine
+        if eng.forward_id != ctx.forward_id:
+            raise RuntimeError("the engine ran another forward since this graph was built; "
+                               "backpropagate before the next forward")
+        gout = gout.to(eng.device)
+        grad, gsrc = eng.backward([gout[s, :, :, : eng.resid[s].shape[2]] for s in range(eng.nshots)])
+        outs = [g.clone().to(d) for g, d in zip(grad, ctx.devs)]
+        nm = len(grad)
+        outs += [gsrc[s].t().contiguous().to(ctx.devs[nm + s]) for s in range(eng.nshots)]
+        return (None, *outs)
+
+
+def elastic2_traces(engine: Elastic2Engine, params):
+    """Receiver traces of every shot as a differentiable function of the
+    per-cell model parameters.
+
+    ``params``: per model of the engine a dict of FP64 torch tensors (nz, nx)
+    with keys c11 c13 c15 c33 c35 c55 rho and optionally gamma_x gamma_z
+    (absent = the engine's problem values, held constant).  Returns FP64
+    [nshots, nt, 2, nrec_max]; backpropagating through it runs the adjoint.
+    """
+    sim = engine.sim
+    if len(params) != len(sim.problems):
+        raise ValueError("one parameter dict per model")
+    slots = []
+    rho_by_model = []
+    for m, (p, prm) in enumerate(zip(sim.problems, params)):
+        dev = prm["rho"].device
+        Cd = {k: prm[k] if k in prm else torch.as_tensor(p.coef[k], dtype=torch.float64, device=dev)
+              for k in ("c11", "c13", "c15", "c33", "c35", "c55", "rho")}
+        gam = {}
+        for a in ("x", "z"):
+            if "gamma_" + a in prm:
+                gam[a] = prm["gamma_" + a]
+            elif a in p.gamma:
+                gam[a] = torch.as_tensor(p.gamma[a], dtype=torch.float64, device=dev)
+        slots.append(torch.stack(elastic2_coefs(Cd, gam, p.dt, getattr(p, "density_method", "legacy"))))
+        rho_by_model.append(Cd["rho"])
+    incs = []
+    for m, s, _ in sim.shots:
+        p = sim.problems[m]
+        sh = p.shots[s]
+        i, _, k = sh.src
+        rho = rho_by_model[m]
+        comps = []
+        for c in range(3):
+            f = sh.force[c]
+            if f is None or c == 1:
+                comps.append(torch.zeros(engine.nt, dtype=torch.float64, device=rho.device))
+                continue
+            f = torch.as_tensor(np.pad(np.asarray(f, dtype=np.float64)[: p.nt], (0, engine.nt - p.nt)),
+                                device=rho.device)
+            # s(it) * dt / rho(src), seismicfdtd.f95:359-361
+            comps.append(f * p.dt / rho[k, i])
+        incs.append(torch.stack(comps))
+    return Elastic2Traces.apply(engine, *slots, *incs)

@@ -84,6 +84,18 @@ struct DShot {
   int* blowup;
 };
 
+// Adjoint-side buffers of one shot (elastic 2-D), see sdt_adj_shot.
+struct DAdj {
+  void* trail;      // forward states, slot k at trail + k * slot
+  long slot;        // elements of one state: fields then psi
+  int trail_len;    // slots available
+  void* lam[2];     // adjoint state ping-pong: lambda_t in lam[t & 1]
+  void* work;       // 12 scratch planes nx*nz
+  const void* resid;
+  double* grad;     // [12][nz][nx]
+  double* gsrc;     // [nt][3]
+};
+
 enum { PK = 0, PA, PB, PKH, PAH, PBH };
 
 // Array pointers live in descriptors in device memory, so the compiler would
@@ -185,20 +197,23 @@ __device__ __forceinline__ T lane_prev(T v) {  // value of lane l-1
   return __shfl_up(v, 1, 64);
 }
 
-template <typename T>
+template <typename T, bool TRAIL>
 __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
                                          const DShot* __restrict__ shots, const int4 wi, const int it,
-                                         const int lane);
+                                         const int lane, const DAdj* __restrict__ adj, const int t0);
 
 // Work: a block of EWB wavefronts takes one item (shot, xo, zo): wave w owns
 // columns xo + w*EW .. +EW-1 of rows zo .. zo+EH-1, so the block's row
 // accesses are EWB*EW contiguous cells of each array.  The grid is
 // persistent: block b takes items b, b + gridDim.x, ...; gridDim.x is a
 // multiple of 8, so item w always runs on XCD w % 8.
-template <typename T>
+// TRAIL: read the state of step it from trail slot it - t0 and write the
+// new state into the next slot (forward recomputation for the adjoint).
+template <typename T, bool TRAIL>
 __global__ void __launch_bounds__(64 * EWB, sizeof(T) == 4 ? SDT_EL2_WAVES_PER_SIMD : 3)
 k_el2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-           const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel) {
+           const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel,
+           const DAdj* __restrict__ adj, int t0) {
   // wave-uniform values kept in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int it = __builtin_amdgcn_readfirstlane(*base + rel);
@@ -207,14 +222,15 @@ k_el2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
     const int4 wi = make_int4(__builtin_amdgcn_readfirstlane(wv.x),
                               __builtin_amdgcn_readfirstlane(wv.y) + wave * EW,
                               __builtin_amdgcn_readfirstlane(wv.z), 0);
-    if (wi.x >= 0 && wi.y < models[shots[wi.x].model].nx) el2_tile<T>(models, shots, wi, it, threadIdx.x & 63);
+    if (wi.x >= 0 && wi.y < models[shots[wi.x].model].nx)
+      el2_tile<T, TRAIL>(models, shots, wi, it, threadIdx.x & 63, adj, t0);
   }
 }
 
-template <typename T>
+template <typename T, bool TRAIL>
 __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
                                          const DShot* __restrict__ shots, const int4 wi, const int it,
-                                         const int lane) {
+                                         const int lane, const DAdj* __restrict__ adj, const int t0) {
   const DShot& S = shots[wi.x];
   const DModel& M = models[S.model];
   const int nx = M.nx, nz = M.nz;
@@ -225,18 +241,26 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
   const bool own = incol && lane >= 1 && lane <= EW;
   const int zend = min(zo + EH, nz);
   const int par = it & 1;
-  const auto vxi = cp<T>(par ? S.g[0] : S.f[0]);
-  const auto vzi = cp<T>(par ? S.g[1] : S.f[1]);
-  const auto sxxi = cp<T>(par ? S.g[2] : S.f[2]);
-  const auto szzi = cp<T>(par ? S.g[3] : S.f[3]);
-  const auto sxzi = cp<T>(par ? S.g[4] : S.f[4]);
-  auto vxo = mp<T>(par ? S.f[0] : S.g[0]);
-  auto vzo = mp<T>(par ? S.f[1] : S.g[1]);
-  auto sxxo = mp<T>(par ? S.f[2] : S.g[2]);
-  auto szzo = mp<T>(par ? S.f[3] : S.g[3]);
-  auto sxzo = mp<T>(par ? S.f[4] : S.g[4]);
-  const auto pin = cp<T>(par ? S.psi2 : S.psi);
-  auto pout = mp<T>(par ? S.psi : S.psi2);
+  const long nn = (long)nx * nz;
+  const T* tin = nullptr;
+  T* tout = nullptr;
+  if (TRAIL) {
+    const DAdj& A = adj[wi.x];
+    tin = static_cast<const T*>(A.trail) + (long)(it - t0) * A.slot;
+    tout = static_cast<T*>(A.trail) + (long)(it - t0 + 1) * A.slot;
+  }
+  const auto vxi = cp<T>(TRAIL ? tin : (par ? S.g[0] : S.f[0]));
+  const auto vzi = cp<T>(TRAIL ? tin + nn : (par ? S.g[1] : S.f[1]));
+  const auto sxxi = cp<T>(TRAIL ? tin + 2 * nn : (par ? S.g[2] : S.f[2]));
+  const auto szzi = cp<T>(TRAIL ? tin + 3 * nn : (par ? S.g[3] : S.f[3]));
+  const auto sxzi = cp<T>(TRAIL ? tin + 4 * nn : (par ? S.g[4] : S.f[4]));
+  auto vxo = mp<T>(TRAIL ? tout : (par ? S.f[0] : S.g[0]));
+  auto vzo = mp<T>(TRAIL ? tout + nn : (par ? S.f[1] : S.g[1]));
+  auto sxxo = mp<T>(TRAIL ? tout + 2 * nn : (par ? S.f[2] : S.g[2]));
+  auto szzo = mp<T>(TRAIL ? tout + 3 * nn : (par ? S.f[3] : S.g[3]));
+  auto sxzo = mp<T>(TRAIL ? tout + 4 * nn : (par ? S.f[4] : S.g[4]));
+  const auto pin = cp<T>(TRAIL ? tin + 5 * nn : (par ? S.psi2 : S.psi));
+  auto pout = mp<T>(TRAIL ? tout + 5 * nn : (par ? S.psi : S.psi2));
   const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dt = T(M.dt);
   const int lx = M.lx, lz = M.lz;
   const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
@@ -425,6 +449,313 @@ __global__ void k_copy_back(const DShot* __restrict__ shots, int nfield, const l
       mp<T>(S.psi)[e] = cp<T>(S.psi2)[e];
     }
   }
+}
+
+// ===========================================================================
+// ELASTIC2 adjoint: the transpose of one k_el2_step, for gradients of a
+// trace misfit.  No counterpart in the reference (it has no adjoint).
+//
+// A step maps u_t = (v, sigma, psi) to u_{t+1} through the legacy sub-loops
+// N (normal stresses), S (shear stress), V (velocities), source and
+// Dirichlet edges, each linear in the state.  Its transpose runs them in
+// reverse: with lambda_{t+1} = dJ/du_{t+1} (residual already injected at
+// the receivers), adjoint step t produces lambda_t and adds
+//   dJ/dcoef += lambda' * (d new state / d coef)
+// where the partial products are recomputed from u_t and u_{t+1}, both
+// read from the trail of recomputed forward states.  Three launches:
+//   k_el2_adj_v  V^T : lambda of the new velocities -> partial lambda_v,
+//                psi_s adjoint, and per-cell stress-derivative adjoints
+//                (scratch planes 0..3);
+//   k_el2_adj_s  gathers those into lambda_sigma, then S^T and N^T with
+//                the shared velocity-derivative psi, writing per-cell
+//                velocity-derivative adjoints (planes 4..11);
+//   k_el2_adj_g  gathers those into lambda_v.
+// Gradient slots are the kernel coefficient slots of the model
+// (n11 n13 n15 n33 n35 s15 s35 s55 dt/rho_x dt/rho_z 1-gx 1-gz), FP64,
+// accumulated atomically so shots of one model may share one array.
+// ===========================================================================
+__device__ __forceinline__ void gadd(double* g, double v) {
+  if (v != 0.0) unsafeAtomicAdd(g, v);
+}
+
+// lambda_{t+1}(v) += dJ/dtrace(t) at the receivers (the trace of step t is
+// read from u_{t+1}).
+template <typename T>
+__global__ void k_el2_adj_inject(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+                                 const DAdj* __restrict__ adj, int nshots, int t) {
+  const int s = blockIdx.y;
+  if (s >= nshots) return;
+  const DShot& S = shots[s];
+  const DAdj& A = adj[s];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= S.nrec || A.resid == nullptr) return;
+  const DModel& M = models[S.model];
+  const long n = (long)M.nx * M.nz;
+  const auto q = cp<int>(S.rec) + 3 * r;
+  const long c = (long)q[2] * M.nx + q[0];
+  const auto res = cp<T>(A.resid) + (long)t * 2 * S.nrec + r;
+  T* L = static_cast<T*>(A.lam[(t + 1) & 1]);
+  atomicAdd(L + c, res[0]);
+  atomicAdd(L + n + c, res[S.nrec]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_el2_adj_v(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+            const DAdj* __restrict__ adj, int nshots, int t, int t0) {
+  const int s = blockIdx.z;
+  if (s >= nshots) return;
+  const DShot& S = shots[s];
+  const DModel& M = models[S.model];
+  const DAdj& A = adj[s];
+  const int nx = M.nx, nz = M.nz;
+  const int i = blockIdx.x * TX + threadIdx.x, j = blockIdx.y * TZ + threadIdx.y;
+  if (i >= nx || j >= nz) return;
+  const long n = (long)nx * nz, c = (long)j * nx + i;
+  const int lx = M.lx, lz = M.lz;
+  const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
+  const auto Lin = cp<T>(A.lam[(t + 1) & 1]);
+  auto Lout = mp<T>(A.lam[t & 1]);
+  const auto u1 = cp<T>(A.trail) + (long)(t + 1 - t0) * A.slot;
+  const auto u0 = cp<T>(A.trail) + (long)(t - t0) * A.slot;
+  auto W = mp<T>(A.work);
+  double* G = A.grad;
+  const Prof<T> P(&M.prof[0][0]);
+  const bool inx = in_slab(i, nx, lx), inz = in_slab(j, nz, lz);
+  const long ox = 5 * n + (inx ? (long)j * 2 * lx + slab_pos(i, nx, lx) : 0);
+  const long oz = 5 * n + 4 * SX + (inz ? (long)slab_pos(j, nz, lz) * nx + i : 0);
+  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz);
+  // Dirichlet edges come last in the step: nothing flows back through them
+  const bool edge = (i == 0 || i == nx - 1 || j == 0 || j == nz - 1);
+  const T lvx = edge ? T(0) : Lin[c], lvz = edge ? T(0) : Lin[n + c];
+  if (A.gsrc && i == S.isrc && j == S.ksrc) {  // d J / d source increment of step t
+    A.gsrc[3 * (long)t + 0] = (double)lvx;
+    A.gsrc[3 * (long)t + 2] = (double)lvz;
+  }
+  // vx(i,j) = vx * (1 - gx) + (dsxx_dx + dsxz_dz) * dt / rho, seismicfdtd.f95:317-337
+  T ax1 = 0, ax2 = 0;
+  if (i >= 1 && j >= 1) {
+    const T w = cp<T>(M.coef[8])[c] * lvx;
+    const T h1 = (u1[2 * n + c] - u1[2 * n + c - 1]) * idx;
+    const T h2 = (u1[4 * n + c] - u1[4 * n + c - nx]) * idz;
+    T b1 = h1, b2 = h2;
+    ax1 = w;
+    ax2 = w;
+    if (inx) {
+      const T lr = Lin[ox + 2 * SX] + w;
+      ax1 = P[PK][i] * w + P[PA][i] * lr;
+      Lout[ox + 2 * SX] = P[PB][i] * lr;
+      b1 = P[PK][i] * h1 + u1[ox + 2 * SX];
+    }
+    if (inz) {
+      const T lr = Lin[oz + 3 * SZ] + w;
+      ax2 = P[12 + PK][j] * w + P[12 + PA][j] * lr;
+      Lout[oz + 3 * SZ] = P[12 + PB][j] * lr;
+      b2 = P[12 + PK][j] * h2 + u1[oz + 3 * SZ];
+    }
+    gadd(G + 8 * n + c, (double)lvx * (double)(b1 + b2));
+    gadd(G + 10 * n + c, (double)lvx * (double)u0[c]);
+    Lout[c] = (M.coef[10] ? cp<T>(M.coef[10])[c] : T(1)) * lvx;
+  } else {
+    if (inx) Lout[ox + 2 * SX] = Lin[ox + 2 * SX];
+    if (inz) Lout[oz + 3 * SZ] = Lin[oz + 3 * SZ];
+    Lout[c] = lvx;
+  }
+  // vz(i,j), seismicfdtd.f95:339-357
+  T az1 = 0, az2 = 0;
+  if (i <= nx - 2 && j <= nz - 2) {
+    const T w = cp<T>(M.coef[9])[c] * lvz;
+    const T k1 = (u1[4 * n + c + 1] - u1[4 * n + c]) * idx;
+    const T k2 = (u1[3 * n + c + nx] - u1[3 * n + c]) * idz;
+    T b1 = k1, b2 = k2;
+    az1 = w;
+    az2 = w;
+    if (inx) {
+      const T lr = Lin[ox + 3 * SX] + w;
+      az1 = P[PKH][i] * w + P[PAH][i] * lr;
+      Lout[ox + 3 * SX] = P[PBH][i] * lr;
+      b1 = P[PKH][i] * k1 + u1[ox + 3 * SX];
+    }
+    if (inz) {
+      const T lr = Lin[oz + 2 * SZ] + w;
+      az2 = P[12 + PKH][j] * w + P[12 + PAH][j] * lr;
+      Lout[oz + 2 * SZ] = P[12 + PBH][j] * lr;
+      b2 = P[12 + PKH][j] * k2 + u1[oz + 2 * SZ];
+    }
+    gadd(G + 9 * n + c, (double)lvz * (double)(b1 + b2));
+    gadd(G + 11 * n + c, (double)lvz * (double)u0[n + c]);
+    Lout[n + c] = (M.coef[11] ? cp<T>(M.coef[11])[c] : T(1)) * lvz;
+  } else {
+    if (inx) Lout[ox + 3 * SX] = Lin[ox + 3 * SX];
+    if (inz) Lout[oz + 2 * SZ] = Lin[oz + 2 * SZ];
+    Lout[n + c] = lvz;
+  }
+  W[c] = ax1;          // adjoint of dsxx_dx at the vx node
+  W[n + c] = ax2;      // adjoint of dsxz_dz at the vx node
+  W[2 * n + c] = az1;  // adjoint of dsxz_dx at the vz node
+  W[3 * n + c] = az2;  // adjoint of dszz_dz at the vz node
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_el2_adj_s(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+            const DAdj* __restrict__ adj, int nshots, int t, int t0) {
+  const int s = blockIdx.z;
+  if (s >= nshots) return;
+  const DShot& S = shots[s];
+  const DModel& M = models[S.model];
+  const DAdj& A = adj[s];
+  const int nx = M.nx, nz = M.nz;
+  const int i = blockIdx.x * TX + threadIdx.x, j = blockIdx.y * TZ + threadIdx.y;
+  if (i >= nx || j >= nz) return;
+  const long n = (long)nx * nz, c = (long)j * nx + i;
+  const int lx = M.lx, lz = M.lz;
+  const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
+  const auto Lin = cp<T>(A.lam[(t + 1) & 1]);
+  auto Lout = mp<T>(A.lam[t & 1]);
+  const auto u1 = cp<T>(A.trail) + (long)(t + 1 - t0) * A.slot;
+  const auto u0 = cp<T>(A.trail) + (long)(t - t0) * A.slot;
+  auto W = mp<T>(A.work);
+  double* G = A.grad;
+  const Prof<T> P(&M.prof[0][0]);
+  const bool inx = in_slab(i, nx, lx), inz = in_slab(j, nz, lz);
+  const long ox = 5 * n + (inx ? (long)j * 2 * lx + slab_pos(i, nx, lx) : 0);
+  const long oz = 5 * n + 4 * SX + (inz ? (long)slab_pos(j, nz, lz) * nx + i : 0);
+  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dt = T(M.dt);
+  // stresses feed the velocity update through their differences
+  const T lsxx = Lin[2 * n + c] + idx * (W[c] - (i + 1 < nx ? W[c + 1] : T(0)));
+  const T lszz = Lin[3 * n + c] + idz * ((j >= 1 ? W[3 * n + c - nx] : T(0)) - W[3 * n + c]);
+  const T lsxz = Lin[4 * n + c] + idz * (W[n + c] - (j + 1 < nz ? W[n + c + nx] : T(0))) +
+                 idx * ((i >= 1 ? W[2 * n + c - 1] : T(0)) - W[2 * n + c]);
+  Lout[2 * n + c] = lsxx;
+  Lout[3 * n + c] = lszz;
+  Lout[4 * n + c] = lsxz;
+  // adjoints of the velocity-derivative psi (shared by both stress loops, A1)
+  T lm1 = inx ? Lin[ox] : T(0), lm3 = inx ? Lin[ox + SX] : T(0);
+  T lm2 = inz ? Lin[oz] : T(0), lm4 = inz ? Lin[oz + SZ] : T(0);
+  const T ikx = P[PK][i], ax = P[PA][i], bx = P[PB][i];
+  const T ikxh = P[PKH][i], axh = P[PAH][i], bxh = P[PBH][i];
+  const T ikz = P[12 + PK][j], az = P[12 + PA][j], bz = P[12 + PB][j];
+  const T ikzh = P[12 + PKH][j], azh = P[12 + PAH][j], bzh = P[12 + PBH][j];
+  // S^T: sigma_xz, seismicfdtd.f95:283-312 (psi after it is u_{t+1}'s)
+  T lf1 = 0, lf2 = 0, lf3 = 0, lf4 = 0;
+  if (j <= nz - 2 && i >= 1) {
+    const T mu = dt * lsxz;
+    const T f1 = (u0[c] - u0[c - 1]) * idx, f2 = (u0[n + c + nx] - u0[n + c]) * idz;
+    const T f3 = (u0[n + c] - u0[n + c - 1]) * idx, f4 = (u0[c + nx] - u0[c]) * idz;
+    const bool mono = M.coef[5] != nullptr;
+    const T s15 = mono ? cp<T>(M.coef[5])[c] : T(0), s35 = mono ? cp<T>(M.coef[6])[c] : T(0);
+    const T s55 = cp<T>(M.coef[7])[c];
+    const T lg1 = s15 * mu, lg2 = s35 * mu, lg3 = s55 * mu;
+    T g1 = f1, g2 = f2, g3 = f3, g4 = f4;
+    lf1 = lg1; lf2 = lg2; lf3 = lg3; lf4 = lg3;
+    if (inx) {
+      g1 = ikxh * f1 + u1[ox];
+      T l = lm1 + lg1;
+      lf1 = ikxh * lg1 + axh * l;
+      lm1 = bxh * l;
+      g3 = ikx * f3 + u1[ox + SX];
+      l = lm3 + lg3;
+      lf3 = ikx * lg3 + ax * l;
+      lm3 = bx * l;
+    }
+    if (inz) {
+      g2 = ikz * f2 + u1[oz];
+      T l = lm2 + lg2;
+      lf2 = ikz * lg2 + az * l;
+      lm2 = bz * l;
+      g4 = ikzh * f4 + u1[oz + SZ];
+      l = lm4 + lg3;
+      lf4 = ikzh * lg3 + azh * l;
+      lm4 = bzh * l;
+    }
+    const double m = (double)mu;
+    gadd(G + 5 * n + c, m * (double)g1);
+    gadd(G + 6 * n + c, m * (double)g2);
+    gadd(G + 7 * n + c, m * ((double)g3 + (double)g4));
+  }
+  // N^T: sigma_xx, sigma_zz, seismicfdtd.f95:247-281 (psi before it is u_t's)
+  T le1 = 0, le2 = 0, le3 = 0, le4 = 0;
+  if (j >= 1 && i <= nx - 2) {
+    const T mux = dt * lsxx, muz = dt * lszz;
+    const T e1 = (u0[c + 1] - u0[c]) * idx, e2 = (u0[n + c] - u0[n + c - nx]) * idz;
+    const T e3 = (u0[n + c + 1] - u0[n + c]) * idx, e4 = (u0[c] - u0[c - nx]) * idz;
+    const bool mono = M.coef[2] != nullptr;
+    const T n11 = cp<T>(M.coef[0])[c], n13 = cp<T>(M.coef[1])[c], n33 = cp<T>(M.coef[3])[c];
+    const T n15 = mono ? cp<T>(M.coef[2])[c] : T(0), n35 = mono ? cp<T>(M.coef[4])[c] : T(0);
+    const T ld1 = n11 * mux + n13 * muz, ld2 = n13 * mux + n33 * muz, lsh = n15 * mux + n35 * muz;
+    T d1 = e1, d2 = e2, d3 = e3, d4 = e4;
+    le1 = ld1; le2 = ld2; le3 = lsh; le4 = lsh;
+    if (inx) {
+      const T brow = cp<T>(M.aux)[j];  // b_x_half(j), :255
+      d1 = ikxh * e1 + (brow * u0[ox] + axh * e1);
+      T l = lm1 + ld1;
+      le1 = ikxh * ld1 + axh * l;
+      lm1 = brow * l;
+      d3 = ikx * e3 + (bx * u0[ox + SX] + ax * e3);
+      l = lm3 + lsh;
+      le3 = ikx * lsh + ax * l;
+      lm3 = bx * l;
+    }
+    if (inz) {
+      d2 = ikz * e2 + (bz * u0[oz] + az * e2);
+      T l = lm2 + ld2;
+      le2 = ikz * ld2 + az * l;
+      lm2 = bz * l;
+      d4 = ikzh * e4 + (bzh * u0[oz + SZ] + azh * e4);
+      l = lm4 + lsh;
+      le4 = ikzh * lsh + azh * l;
+      lm4 = bzh * l;
+    }
+    const double mx = (double)mux, mz = (double)muz, sh = (double)d3 + (double)d4;
+    gadd(G + c, mx * (double)d1);
+    gadd(G + n + c, mx * (double)d2 + mz * (double)d1);
+    gadd(G + 2 * n + c, mx * sh);
+    gadd(G + 3 * n + c, mz * (double)d2);
+    gadd(G + 4 * n + c, mz * sh);
+  }
+  if (inx) {
+    Lout[ox] = lm1;
+    Lout[ox + SX] = lm3;
+  }
+  if (inz) {
+    Lout[oz] = lm2;
+    Lout[oz + SZ] = lm4;
+  }
+  W[4 * n + c] = le1;   // dvx_dx at the normal-stress node
+  W[5 * n + c] = le2;   // dvz_dz
+  W[6 * n + c] = le3;   // dvz_dx
+  W[7 * n + c] = le4;   // dvx_dz
+  W[8 * n + c] = lf1;   // dvx_dx at the shear node
+  W[9 * n + c] = lf2;   // dvz_dz
+  W[10 * n + c] = lf3;  // dvz_dx
+  W[11 * n + c] = lf4;  // dvx_dz
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_el2_adj_g(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+            const DAdj* __restrict__ adj, int nshots, int t) {
+  const int s = blockIdx.z;
+  if (s >= nshots) return;
+  const DShot& S = shots[s];
+  const DModel& M = models[S.model];
+  const DAdj& A = adj[s];
+  const int nx = M.nx, nz = M.nz;
+  const int i = blockIdx.x * TX + threadIdx.x, j = blockIdx.y * TZ + threadIdx.y;
+  if (i >= nx || j >= nz) return;
+  const long n = (long)nx * nz, c = (long)j * nx + i;
+  auto Lout = mp<T>(A.lam[t & 1]);
+  const auto W = cp<T>(A.work);
+  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz);
+  const bool w_ = i >= 1, e_ = i + 1 < nx, n_ = j >= 1, s_ = j + 1 < nz;
+  auto at = [&](int q, long o, bool ok) { return ok ? W[q * n + c + o] : T(0); };
+  const T gx = idx * (at(4, -1, w_) - W[4 * n + c]) + idz * (W[7 * n + c] - at(7, nx, s_)) +
+               idx * (W[8 * n + c] - at(8, 1, e_)) + idz * (at(11, -nx, n_) - W[11 * n + c]);
+  const T gz = idz * (W[5 * n + c] - at(5, nx, s_)) + idx * (at(6, -1, w_) - W[6 * n + c]) +
+               idz * (at(9, -nx, n_) - W[9 * n + c]) + idx * (W[10 * n + c] - at(10, 1, e_));
+  Lout[c] = Lout[c] + gx;
+  Lout[n + c] = Lout[n + c] + gz;
 }
 
 // ===========================================================================
@@ -1060,6 +1391,9 @@ struct Plan {
   int graph_block = 0;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
+  DAdj* dadj = nullptr;          // adjoint descriptors (sdt_adjoint_setup)
+  std::vector<DAdj> hadj;
+  int maxnx = 0, maxnz = 0;
 };
 
 int kind_fields(int kind) {
@@ -1109,8 +1443,9 @@ void launch_one(Plan* p, hipStream_t st, int rel, int which) {
   }
   switch (p->kind * 2 + which) {
     case SDT_ELASTIC2 * 2:
-      hipLaunchKernelGGL(k_el2_step<T>, dim3(p->el2_grid), dim3(64 * EWB), 0, st, p->dmodels,
-                         p->dshots, p->dwork, p->nwork, p->dbase, rel);
+      hipLaunchKernelGGL((k_el2_step<T, false>), dim3(p->el2_grid), dim3(64 * EWB), 0, st,
+                         p->dmodels, p->dshots, p->dwork, p->nwork, p->dbase, rel,
+                         (const DAdj*)nullptr, 0);
       break;
     case SDT_ELASTIC2 * 2 + 1:
       break;  // the fused step kernel does both phases
@@ -1232,6 +1567,35 @@ void end_of_run(Plan* p, hipStream_t st, int next_step) {
   } else {
     launch_finalize<float>(p, st);
     if (b) launch_copy_back<float>(p, st);
+  }
+}
+
+long long state_elems(int kind, long long nx, long long ny, long long nz, long long lx,
+                      long long ly, long long lz) {
+  const long long np = psi_elems(kind, nx, ny, nz, lx, ly, lz);
+  return np < 0 ? -1 : kind_fields(kind) * nx * ny * nz + np;
+}
+
+template <typename T>
+void launch_trail_steps(Plan* p, hipStream_t st, int it0, int nsteps) {
+  for (int r = 0; r < nsteps; ++r)
+    hipLaunchKernelGGL((k_el2_step<T, true>), dim3(p->el2_grid), dim3(64 * EWB), 0, st,
+                       p->dmodels, p->dshots, p->dwork, p->nwork, p->dbase, r, p->dadj, it0);
+}
+
+template <typename T>
+void launch_adjoint_steps(Plan* p, hipStream_t st, int t_hi, int t_lo, int t0) {
+  const dim3 blk(TX, TZ);
+  const dim3 grd((p->maxnx + TX - 1) / TX, (p->maxnz + TZ - 1) / TZ, p->nshots);
+  for (int t = t_hi - 1; t >= t_lo; --t) {
+    if (p->maxrec > 0)
+      hipLaunchKernelGGL(k_el2_adj_inject<T>, dim3((p->maxrec + 63) / 64, p->nshots), dim3(64), 0,
+                         st, p->dmodels, p->dshots, p->dadj, p->nshots, t);
+    hipLaunchKernelGGL(k_el2_adj_v<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dadj, p->nshots,
+                       t, t0);
+    hipLaunchKernelGGL(k_el2_adj_s<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dadj, p->nshots,
+                       t, t0);
+    hipLaunchKernelGGL(k_el2_adj_g<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dadj, p->nshots, t);
   }
 }
 
@@ -1361,6 +1725,10 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
         }
   }
   p->nwork = (int)work.size();
+  for (int m = 0; m < nmodels; ++m) {
+    p->maxnx = std::max(p->maxnx, models[m].nx);
+    p->maxnz = std::max(p->maxnz, models[m].nz);
+  }
   {
     std::vector<long> nel(nshots), npsi(nshots);
     for (int s = 0; s < nshots; ++s) {
@@ -1477,6 +1845,94 @@ int sdt_plan_check(void* plan, int* first_step, int* per_shot) {
   return SDT_OK;
 }
 
+int sdt_state_elems(int kind, int nx, int ny, int nz, int lx, int ly, int lz, long long* out) {
+  if (!out) return fail(SDT_EINVAL, "out is null");
+  const long long n = state_elems(kind, nx, ny, nz, lx, ly, lz);
+  if (n < 0) return fail(SDT_EINVAL, "unknown kind");
+  *out = n;
+  return SDT_OK;
+}
+
+int sdt_adjoint_work_elems(int kind, int nx, int ny, int nz, int lx, int ly, int lz,
+                           long long* out) {
+  (void)ny; (void)lx; (void)ly; (void)lz;
+  if (!out) return fail(SDT_EINVAL, "out is null");
+  if (kind != SDT_ELASTIC2) return fail(SDT_EINVAL, "the adjoint is available for ELASTIC2 only");
+  *out = 12LL * nx * nz;
+  return SDT_OK;
+}
+
+int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p || !adj) return fail(SDT_EINVAL, "plan or adj is null");
+  if (p->kind != SDT_ELASTIC2) return fail(SDT_EINVAL, "the adjoint is available for ELASTIC2 only");
+  if (nshots != p->nshots) return fail(SDT_EINVAL, "adj must hold one entry per shot of the plan");
+  std::vector<DAdj> h(nshots);
+  for (int s = 0; s < nshots; ++s) {
+    const sdt_adj_shot& a = adj[s];
+    const std::string tag = "shot " + std::to_string(s) + ": ";
+    if (!a.trail || !a.lam[0] || !a.lam[1] || !a.work || !a.grad)
+      return fail(SDT_EINVAL, tag + "trail, lam, work and grad must be set");
+    if (a.trail_len < 2) return fail(SDT_EINVAL, tag + "trail_len must be at least 2");
+    const sdt_model& M = p->hmodels[p->hshots[s].model];
+    DAdj& D = h[s];
+    D.trail = a.trail;
+    D.slot = (long)state_elems(p->kind, M.nx, M.ny, M.nz, M.lx, kind_3d(p->kind) ? M.ly : 0, M.lz);
+    D.trail_len = a.trail_len;
+    D.lam[0] = a.lam[0];
+    D.lam[1] = a.lam[1];
+    D.work = a.work;
+    D.resid = a.resid;
+    D.grad = a.grad;
+    D.gsrc = a.gsrc;
+  }
+  // the previous descriptors may still be read by queued kernels
+  HIP_TRY(hipDeviceSynchronize());
+  if (!p->dadj) HIP_TRY(hipMalloc(&p->dadj, sizeof(DAdj) * nshots));
+  HIP_TRY(hipMemcpy(p->dadj, h.data(), sizeof(DAdj) * nshots, hipMemcpyHostToDevice));
+  p->hadj = h;
+  return SDT_OK;
+}
+
+int sdt_plan_run_trail(void* plan, int it0, int nsteps, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p) return fail(SDT_EINVAL, "plan is null");
+  if (!p->dadj) return fail(SDT_EINVAL, "call sdt_adjoint_setup first");
+  if (it0 < 0 || nsteps < 0 || (long long)it0 + nsteps > p->nt)
+    return fail(SDT_EINVAL, "step range outside [0, nt)");
+  for (const DAdj& a : p->hadj)
+    if (nsteps + 1 > a.trail_len) return fail(SDT_EINVAL, "trail too short for nsteps + 1 states");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  p->last = st;
+  if (nsteps == 0) return SDT_OK;
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, p->dbase, it0);
+  if (p->prec == SDT_FP64)
+    launch_trail_steps<double>(p, st, it0, nsteps);
+  else
+    launch_trail_steps<float>(p, st, it0, nsteps);
+  HIP_TRY(hipGetLastError());
+  return SDT_OK;
+}
+
+int sdt_adjoint_run(void* plan, int t_hi, int t_lo, int trail_t0, void* stream) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p) return fail(SDT_EINVAL, "plan is null");
+  if (!p->dadj) return fail(SDT_EINVAL, "call sdt_adjoint_setup first");
+  if (t_lo < 0 || t_hi < t_lo || t_hi > p->nt || trail_t0 < 0 || trail_t0 > t_lo)
+    return fail(SDT_EINVAL, "need 0 <= trail_t0 <= t_lo <= t_hi <= nt");
+  for (const DAdj& a : p->hadj)
+    if (t_hi - trail_t0 + 1 > a.trail_len)
+      return fail(SDT_EINVAL, "trail does not hold the states up to t_hi");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  p->last = st;
+  if (p->prec == SDT_FP64)
+    launch_adjoint_steps<double>(p, st, t_hi, t_lo, trail_t0);
+  else
+    launch_adjoint_steps<float>(p, st, t_hi, t_lo, trail_t0);
+  HIP_TRY(hipGetLastError());
+  return SDT_OK;
+}
+
 void sdt_plan_destroy(void* plan) {
   Plan* p = static_cast<Plan*>(plan);
   if (!p) return;
@@ -1490,6 +1946,7 @@ void sdt_plan_destroy(void* plan) {
   if (p->dbase) (void)hipFree(p->dbase);
   if (p->dnel) (void)hipFree(p->dnel);
   if (p->dnpsi) (void)hipFree(p->dnpsi);
+  if (p->dadj) (void)hipFree(p->dadj);
   delete p;
 }
 

@@ -26,7 +26,7 @@ from .cpml import EPS0, MU0, AxisProfile, slab_width
 
 __all__ = ["KINDS", "FIELDS", "STIFFNESS", "PERMITTIVITY", "CONDUCTIVITY", "Shot", "Problem",
            "kernel_coefs", "source_increments", "aux_b_x_half_by_row", "ncomp",
-           "DENSITY_METHODS", "node_density"]
+           "DENSITY_METHODS", "node_density", "elastic2_coefs"]
 
 # Density averaging at staggered velocity nodes.  The legacy solver has one
 # rule (weighted arithmetic mean); the reference binary takes a
@@ -47,11 +47,14 @@ def node_density(parts, weights, method: str = "legacy"):
             acc = acc + (w * x if w != 1 else x)
         return acc / tot
     if method == "geometric":
+        if _is_torch(parts[0]):
+            import torch
+            return torch.exp(sum(w * torch.log(x) for w, x in zip(weights, parts)) / tot)
         return np.exp(sum(w * np.log(x) for w, x in zip(weights, parts)) / tot)
     if method == "harmonic":
         return tot / sum(w / x for w, x in zip(weights, parts))
     if method == "none":
-        return np.array(parts[0], copy=True)
+        return parts[0].clone() if _is_torch(parts[0]) else np.array(parts[0], copy=True)
     raise ValueError(f"density_method must be one of {DENSITY_METHODS}")
 
 KINDS = {"elastic2": 0, "elastic25": 1, "em2": 2, "em25": 3}
@@ -157,10 +160,24 @@ class Problem:
                     raise ValueError(f"shot {s}: source series shorter than nt")
 
 
+# The elastic 2-D setup below accepts numpy arrays or torch tensors (the
+# adjoint path differentiates it with torch autograd); both go through the
+# same expressions in the same order.
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+def _zeros_like(a):
+    if _is_torch(a):
+        import torch
+        return torch.zeros_like(a)
+    return np.zeros_like(a)
+
+
 def _avg4_normal(c):
     """(c(i+1,j) + 2 c(i,j) + c(i,j-1)) / 4 at the sigma_xx node,
     seismicfdtd.f95:269-278; zero where the loop does not reach."""
-    out = np.zeros_like(c)
+    out = _zeros_like(c)
     out[1:, :-1] = (c[1:, 1:] + 2 * c[1:, :-1] + c[:-1, :-1]) / 4
     return out
 
@@ -168,7 +185,7 @@ def _avg4_normal(c):
 def _avg4_shear(c):
     """(c(i,j+1) + 2 c(i,j) + c(i-1,j)) / 4 at the sigma_xz node,
     seismicfdtd.f95:307-310."""
-    out = np.zeros_like(c)
+    out = _zeros_like(c)
     out[:-1, 1:] = (c[1:, 1:] + 2 * c[:-1, 1:] + c[:-1, :-1]) / 4
     return out
 
@@ -196,20 +213,7 @@ def kernel_coefs(p: Problem):
     C = p.coef
     dt = p.dt
     if p.kind == "elastic2":
-        rho = C["rho"]
-        gx = p.gamma.get("x", np.zeros_like(rho))
-        gz = p.gamma.get("z", np.zeros_like(rho))
-        dm = getattr(p, "density_method", "legacy")
-        dtr_z = np.zeros_like(rho)
-        # deltarho = (2 rho(i,j) + rho(i+1,j) + rho(i,j+1)) / 4, seismicfdtd.f95:342
-        dtr_z[:-1, :-1] = dt / node_density([rho[:-1, :-1], rho[:-1, 1:], rho[1:, :-1]],
-                                            [2, 1, 1], dm)
-        return [
-            _avg4_normal(C["c11"]), _avg4_normal(C["c13"]), _avg4_normal(C["c15"]),
-            _avg4_normal(C["c33"]), _avg4_normal(C["c35"]),
-            _avg4_shear(C["c15"]), _avg4_shear(C["c35"]), _avg4_shear(C["c55"]),
-            dt / rho, dtr_z, 1 - gx, 1 - gz,
-        ]
+        return elastic2_coefs(C, p.gamma, dt, getattr(p, "density_method", "legacy"))
     if p.kind == "elastic25":
         rho = C["rho"]
         # loop 1 stencil (i+1,k) (i,k) (i,k-1) (i+1,k-1): seismicfdtd.f95:651-673
@@ -265,6 +269,27 @@ def kernel_coefs(p: Problem):
         bz[:-1, 1:] = (4 * cb_z[:-1, 1:] + cb_z[:-1, :-1] + cb_z[1:, 1:]) / 6
         return [ca_x, cb_x, ay, by_, az, bz]
     raise ValueError(p.kind)
+
+
+def elastic2_coefs(C, gamma, dt, density_method="legacy"):
+    """ELASTIC2 coefficient slots from the per-cell fields C (c11 c13 c15
+    c33 c35 c55 rho) and gamma (x, z; absent = 0), as numpy arrays or torch
+    tensors (nz, nx)."""
+    rho = C["rho"]
+    gx = gamma.get("x")
+    gz = gamma.get("z")
+    gx = _zeros_like(rho) if gx is None else gx
+    gz = _zeros_like(rho) if gz is None else gz
+    dtr_z = _zeros_like(rho)
+    # deltarho = (2 rho(i,j) + rho(i+1,j) + rho(i,j+1)) / 4, seismicfdtd.f95:342
+    dtr_z[:-1, :-1] = dt / node_density([rho[:-1, :-1], rho[:-1, 1:], rho[1:, :-1]],
+                                        [2, 1, 1], density_method)
+    return [
+        _avg4_normal(C["c11"]), _avg4_normal(C["c13"]), _avg4_normal(C["c15"]),
+        _avg4_normal(C["c33"]), _avg4_normal(C["c35"]),
+        _avg4_shear(C["c15"]), _avg4_shear(C["c35"]), _avg4_shear(C["c55"]),
+        dt / rho, dtr_z, 1 - gx, 1 - gz,
+    ]
 
 
 def _cacb(sig, epsr_sum, dt):

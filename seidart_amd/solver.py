@@ -31,7 +31,11 @@ def _dev_ptr(t: torch.Tensor | None):
 class Simulation:
     """All shots of one or more Problems (same kind) on one GPU."""
 
-    def __init__(self, problems, precision: str = "fp64", device: str = "cuda:0"):
+    def __init__(self, problems, precision: str = "fp64", device: str = "cuda:0",
+                 all_slots: bool = False):
+        """``all_slots``: pass every coefficient slot and both source
+        components to the device even where they are zero (needed when the
+        coefficients are later overwritten, e.g. by the adjoint engine)."""
         if isinstance(problems, Problem):
             problems = [problems]
         if not problems:
@@ -54,6 +58,7 @@ class Simulation:
         self.ncomp = ncomp(self.kind)
         self.names = FIELDS[self.kind]
         self._keep = []  # every tensor a descriptor points to
+        self.coef_t = []  # per model: device coefficient tensor per slot (None = skipped)
         T = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=self.dtype, device=self.device)
 
         models = (_hip.SdtModel * len(problems))()
@@ -69,18 +74,21 @@ class Simulation:
                 M.scal[0] = em_dbh(p.dt)
             host = kernel_coefs(p)
             skip = set()
-            if self.kind == "elastic2":
+            if self.kind == "elastic2" and not all_slots:
                 # optional slots: c15/c35 couplings all zero, or no attenuation
                 if all(not np.any(host[q]) for q in (2, 4, 5, 6)):
                     skip |= {2, 4, 5, 6}
                 if all(np.all(host[q] == 1.0) for q in (10, 11)):
                     skip |= {10, 11}
+            self.coef_t.append([])
             for q, a in enumerate(host):
                 if q in skip:
                     M.coef[q] = None
+                    self.coef_t[-1].append(None)
                     continue
                 t = T(a)
                 self._keep.append(t)
+                self.coef_t[-1].append(t)
                 M.coef[q] = t.data_ptr()
             for ax, a in enumerate(("x", "y", "z")):
                 pr = p.prof.get(a)
@@ -97,20 +105,34 @@ class Simulation:
                 M.aux = aux.data_ptr()
             lx, ly, lz = p.slabs()
             npsi = _hip.psi_elems(self.kid, p.nx, p.ny, p.nz, lx, ly, lz)
+            nel = int(np.prod(p.field_shape))
             for s, sh in enumerate(p.shots):
                 init = (p.init[s] if p.init else None) or {}
                 bufs = {}
-                for n in self.names:
-                    if n in init:
-                        bufs[n] = T(np.asarray(init[n], dtype=np.float64).reshape(p.field_shape))
-                    else:
-                        bufs[n] = torch.zeros(p.field_shape, dtype=self.dtype, device=self.device)
-                bufs["psi"] = torch.zeros(max(npsi, 1), dtype=self.dtype, device=self.device)
-                if self.kind == "elastic2":  # second state buffer of the single-pass kernel
+                if self.kind == "elastic2":
+                    # one contiguous state (fields then psi) per buffer, the
+                    # layout of sdt_state_elems; the single-pass kernel
+                    # ping-pongs between state A and state B
+                    nst = _hip.state_elems(self.kid, p.nx, p.ny, p.nz, lx, ly, lz)
+                    for tag in ("", "_b"):
+                        st = torch.zeros(nst, dtype=self.dtype, device=self.device)
+                        bufs["state" + tag] = st
+                        for f, n in enumerate(self.names):
+                            bufs[n + tag] = st[f * nel:(f + 1) * nel].view(p.field_shape)
+                        bufs["psi" + tag] = st[len(self.names) * nel:] if npsi else st[:1]
                     for n in self.names:
-                        bufs[n + "_b"] = torch.empty_like(bufs[n])
-                    bufs["psi_b"] = torch.empty_like(bufs["psi"])
+                        if n in init:
+                            bufs[n].copy_(T(np.asarray(init[n], dtype=np.float64).reshape(p.field_shape)))
+                else:
+                    for n in self.names:
+                        if n in init:
+                            bufs[n] = T(np.asarray(init[n], dtype=np.float64).reshape(p.field_shape))
+                        else:
+                            bufs[n] = torch.zeros(p.field_shape, dtype=self.dtype, device=self.device)
+                    bufs["psi"] = torch.zeros(max(npsi, 1), dtype=self.dtype, device=self.device)
                 inc = source_increments(p, sh)
+                if all_slots and self.kind == "elastic2":
+                    inc = [np.zeros(p.nt) if (v is None and c != 1) else v for c, v in enumerate(inc)]
                 bufs["src"] = [None if v is None else T(np.pad(v, (0, self.nt - v.size))) for v in inc]
                 rec = np.asarray(sh.rec, dtype=np.int32).reshape(-1, 3)
                 bufs["rec"] = torch.as_tensor(rec, device=self.device) if rec.size else None

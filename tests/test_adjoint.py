@@ -1,0 +1,261 @@
+"""Adjoint (FWI gradient) of the elastic 2-D scheme.
+
+CPU: the pure-PyTorch FP64 restatement (oracle/torch_elastic2.py) is pinned
+against the C oracle's traces, and the torch form of the coefficient setup
+against the numpy form the forward uses.
+
+GPU (through the C ABI): the hand-written adjoint kernels are checked
+  * by the dot-product test <A x, y> = <x, A^T y> in FP64, where A maps an
+    initial state (fields and C-PML memory) to the receiver traces;
+  * against torch autograd of the FP64 restatement (gradients of an L2
+    trace misfit with respect to every per-cell parameter and the source
+    series);
+  * against central finite differences of the FP64 device forward at 20
+    seeded cells per parameter (relative step 1e-4);
+  * FP32 gradients against FP64 ones.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import rel_l2, tiny_problem
+from seidart_amd.problem import elastic2_coefs, kernel_coefs
+
+PHYS = ("c11", "c13", "c15", "c33", "c35", "c55", "rho")
+
+
+def _torch_ref_args(p, shot, params=None):
+    """Arguments of oracle.torch_elastic2.run_elastic2 for one shot."""
+    T = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64))
+    prm = params if params is not None else {k: T(p.coef[k]) for k in PHYS}
+    if p.gamma and params is None:
+        prm["gx"] = T(p.gamma["x"])
+        prm["gz"] = T(p.gamma["z"])
+    prof = {a: tuple(T(v) for v in (pr.kappa, pr.a, pr.b, pr.kappa_h, pr.a_h, pr.b_h))
+            for a, pr in p.prof.items()}
+    # b_x_half(j) for row j, rows past nx take the last entry
+    brow = T(p.prof["x"].b_h[np.minimum(np.arange(p.nz), p.nx - 1)])
+    sh = p.shots[shot]
+    force = tuple(None if f is None else T(f) for f in (sh.force[0], sh.force[2]))
+    rec = torch.as_tensor(np.asarray(sh.rec)[:, [0, 2]])
+    return dict(prm=prm, prof=prof, dx=p.dx, dz=p.dz, dt=p.dt, nt=p.nt, src=(sh.src[0], sh.src[2]),
+                force=force, rec=rec, brow=brow)
+
+
+def test_torch_reference_matches_oracle(oracle_lib):
+    from torch_elastic2 import run_elastic2
+    p = tiny_problem("elastic2", seed=41, nx=40, nz=34, npml=6, nt=70, gamma=1e-3, nrec=7)
+    with torch.no_grad():
+        tr = run_elastic2(**_torch_ref_args(p, 0)).numpy()
+    o = oracle_lib.OracleRun(p, shot=0).run()
+    assert np.abs(o.trace).max() > 0
+    assert rel_l2(tr, o.trace) < 1e-13
+
+
+def test_torch_reference_matches_oracle_no_attenuation(oracle_lib):
+    from torch_elastic2 import run_elastic2
+    p = tiny_problem("elastic2", seed=42, nx=36, nz=40, npml=5, nt=60, nrec=5, full21=False)
+    with torch.no_grad():
+        tr = run_elastic2(**_torch_ref_args(p, 0)).numpy()
+    o = oracle_lib.OracleRun(p, shot=0).run()
+    assert rel_l2(tr, o.trace) < 1e-13
+
+
+@pytest.mark.parametrize("dm", ["legacy", "geometric", "harmonic", "none"])
+def test_coefs_torch_matches_numpy(dm):
+    p = tiny_problem("elastic2", seed=43, nx=30, nz=26, npml=4, nt=10, gamma=1e-3)
+    p.density_method = dm
+    ref = kernel_coefs(p)
+    Ct = {k: torch.as_tensor(v) for k, v in p.coef.items()}
+    gt = {a: torch.as_tensor(v) for a, v in p.gamma.items()}
+    got = elastic2_coefs(Ct, gt, p.dt, dm)
+    # torch evaluates scalar / tensor as scalar * (1 / tensor): 1-ulp differences
+    for q, (a, b) in enumerate(zip(got, ref)):
+        assert np.max(np.abs(a.numpy() - b) / (np.abs(b) + 1e-300)) <= 1e-15, (dm, q)
+
+
+# ---------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------
+def _engine(p, precision="fp64", every=16):
+    from seidart_amd.adjoint import Elastic2Engine
+    return Elastic2Engine(p, precision=precision, every=every)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("every", [16, 25])
+def test_dot_product(every):
+    """<A x, y> = <x, A^T y> for A: initial state -> traces (no source)."""
+    p = tiny_problem("elastic2", seed=51, nx=52, nz=44, npml=7, nt=110, gamma=1e-3, nshots=2, nrec=9)
+    for sh in p.shots:
+        sh.force = (np.zeros(p.nt), None, np.zeros(p.nt))
+    eng = _engine(p, "fp64", every=every)
+    g = torch.Generator(device="cpu").manual_seed(5)
+    xs = []
+    for s in range(eng.nshots):
+        x = torch.randn(eng.init[s].numel(), generator=g, dtype=torch.float64)
+        eng.init[s].copy_(x.to(eng.device))
+        xs.append(x)
+    tr = eng.forward()
+    ys = [torch.randn(t.shape, generator=g, dtype=torch.float64) for t in tr]
+    lhs = sum(float((t.cpu() * y).sum()) for t, y in zip(tr, ys))
+    eng.backward(ys)
+    rhs = sum(float((x * eng.adjoint_state(s).cpu()).sum()) for s, x in enumerate(xs))
+    assert abs(lhs) > 0
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
+
+
+@pytest.mark.gpu
+def test_trail_recompute_is_bit_exact():
+    """The recomputed states equal the checkpoints of the forward run."""
+    p = tiny_problem("elastic2", seed=52, nx=48, nz=40, npml=6, nt=64, gamma=1e-3, nrec=4)
+    eng = _engine(p, "fp32", every=16)
+    eng.forward()
+    eng.backward([torch.zeros_like(eng.resid[0])])
+    # after the backward pass the trail holds segment 0's recomputed states
+    assert torch.equal(eng.trail[0][16], eng.ckpt[0][1])
+    assert torch.count_nonzero(eng.ckpt[0][1]) > 0
+
+
+def _gpu_grads(p, engine, d_obs):
+    """dJ/dparam of J = 1/2 sum (traces - d_obs)^2 through the autograd Function."""
+    from seidart_amd.adjoint import elastic2_traces
+    dev = engine.device
+    prm = {k: torch.tensor(p.coef[k], dtype=torch.float64, device=dev, requires_grad=True) for k in PHYS}
+    if p.gamma:
+        prm["gamma_x"] = torch.tensor(p.gamma["x"], device=dev, requires_grad=True)
+        prm["gamma_z"] = torch.tensor(p.gamma["z"], device=dev, requires_grad=True)
+    tr = elastic2_traces(engine, [prm])
+    J = 0.5 * ((tr - d_obs.to(dev)) ** 2).sum()
+    J.backward()
+    return float(J), {k: v.grad.cpu().numpy() for k, v in prm.items()}
+
+
+def _ref_grads(p, d_obs):
+    from torch_elastic2 import run_elastic2
+    prm = {k: torch.tensor(p.coef[k], dtype=torch.float64, requires_grad=True) for k in PHYS}
+    if p.gamma:
+        prm["gx"] = torch.tensor(p.gamma["x"], requires_grad=True)
+        prm["gz"] = torch.tensor(p.gamma["z"], requires_grad=True)
+    J = 0
+    for s in range(len(p.shots)):
+        tr = run_elastic2(**_torch_ref_args(p, s, params=prm))
+        nrec = tr.shape[2]
+        J = J + 0.5 * ((tr - d_obs[s, :, :, :nrec]) ** 2).sum()
+    J.backward()
+    out = {k: v.grad.numpy() for k, v in prm.items()}
+    if p.gamma:
+        out["gamma_x"] = out.pop("gx")
+        out["gamma_z"] = out.pop("gz")
+    return float(J), out
+
+
+def _observed(p, seed=7):
+    """Synthetic 'observed' data: the oracle traces of a perturbed model."""
+    from pyoracle import OracleRun
+    q = tiny_problem("elastic2", seed=p._seed, **p._kw)
+    rng = np.random.default_rng(seed)
+    for k in ("c11", "c33", "c55", "rho"):
+        q.coef[k] = q.coef[k] * (1 + 0.03 * rng.standard_normal(q.coef[k].shape).clip(-2, 2))
+    nrec = max(np.asarray(sh.rec).shape[0] for sh in q.shots)
+    d = np.zeros((len(q.shots), q.nt, 2, nrec))
+    for s in range(len(q.shots)):
+        t = OracleRun(q, shot=s).run().trace
+        d[s, :, :, : t.shape[2]] = t
+    return torch.as_tensor(d)
+
+
+def _grad_problem():
+    kw = dict(nx=64, nz=50, npml=8, nt=160, gamma=1e-3, nshots=2, nrec=10)
+    p = tiny_problem("elastic2", seed=61, **kw)
+    p._seed, p._kw = 61, kw
+    return p
+
+
+@pytest.mark.gpu
+def test_gradient_matches_torch_autograd_fp64():
+    p = _grad_problem()
+    d = _observed(p)
+    eng = _engine(p, "fp64", every=32)
+    Jg, gg = _gpu_grads(p, eng, d)
+    Jr, gr = _ref_grads(p, d)
+    assert abs(Jg - Jr) <= 1e-12 * abs(Jr)
+    for k in gr:
+        assert np.abs(gr[k]).max() > 0, k
+        assert rel_l2(gg[k], gr[k]) < 1e-9, (k, rel_l2(gg[k], gr[k]))
+
+
+@pytest.mark.gpu
+def test_gradient_fp32_against_fp64():
+    p = _grad_problem()
+    d = _observed(p)
+    _, g64 = _gpu_grads(p, _engine(p, "fp64", every=32), d)
+    _, g32 = _gpu_grads(p, _engine(p, "fp32", every=32), d)
+    for k in ("c11", "c13", "c33", "c55", "rho"):
+        assert rel_l2(g32[k], g64[k]) < 1e-3, (k, rel_l2(g32[k], g64[k]))
+
+
+@pytest.mark.gpu
+def test_source_series_gradient():
+    """dJ/d(force series) from the adjoint equals the torch autograd one."""
+    from seidart_amd.adjoint import Elastic2Traces
+    from seidart_amd.problem import elastic2_coefs
+    from torch_elastic2 import run_elastic2
+    p = tiny_problem("elastic2", seed=62, nx=48, nz=42, npml=6, nt=120, nrec=6)
+    eng = _engine(p, "fp64", every=40)
+    sh = p.shots[0]
+    dev = eng.device
+    fx = torch.tensor(sh.force[0], device=dev, requires_grad=True)
+    fz = torch.tensor(sh.force[2], device=dev, requires_grad=True)
+    rho_s = float(p.coef["rho"][sh.src[2], sh.src[0]])
+    inc = torch.stack([fx * p.dt / rho_s, torch.zeros_like(fx), fz * p.dt / rho_s])
+    slots = torch.stack([torch.as_tensor(v, device=dev) for v in kernel_coefs(p)])
+    w = torch.randn((p.nt, 2, 6), generator=torch.Generator().manual_seed(3), dtype=torch.float64)
+    tr = Elastic2Traces.apply(eng, slots, inc)
+    (tr[0] * w.to(dev)).sum().backward()
+    a = _torch_ref_args(p, 0)
+    rfx = a["force"][0].clone().requires_grad_(True)
+    rfz = a["force"][1].clone().requires_grad_(True)
+    a["force"] = (rfx, rfz)
+    (run_elastic2(**a) * w).sum().backward()
+    assert rel_l2(fx.grad.cpu().numpy(), rfx.grad.numpy()) < 1e-10
+    assert rel_l2(fz.grad.cpu().numpy(), rfz.grad.numpy()) < 1e-10
+
+
+@pytest.mark.gpu
+def test_gradient_finite_differences():
+    """Central differences of the FP64 device forward, relative step 1e-4,
+    at 20 seeded cells per parameter among the cells whose gradient is at
+    least 1% of the parameter's largest (smaller ones are round-off bound)."""
+    from seidart_amd.adjoint import elastic2_traces
+    p = _grad_problem()
+    d = _observed(p)
+    eng = _engine(p, "fp64", every=32)
+    _, gg = _gpu_grads(p, eng, d)
+    dev = eng.device
+    base = {k: torch.tensor(p.coef[k], dtype=torch.float64, device=dev) for k in PHYS}
+    base["gamma_x"] = torch.tensor(p.gamma["x"], device=dev)
+    base["gamma_z"] = torch.tensor(p.gamma["z"], device=dev)
+    dd = d.to(dev)
+
+    def J(prm):
+        with torch.no_grad():
+            return float(0.5 * ((elastic2_traces(eng, [prm]) - dd) ** 2).sum())
+
+    rng = np.random.default_rng(2024)
+    worst = {}
+    for k in ("c11", "c13", "c33", "c55", "rho"):
+        g = gg[k]
+        cand = np.argwhere(np.abs(g) >= 0.01 * np.abs(g).max())
+        pick = cand[rng.choice(len(cand), size=min(20, len(cand)), replace=False)]
+        errs = []
+        for j, i in pick:
+            h = 1e-4 * abs(float(p.coef[k][j, i]))
+            up = {q: v.clone() for q, v in base.items()}
+            dn = {q: v.clone() for q, v in base.items()}
+            up[k][j, i] += h
+            dn[k][j, i] -= h
+            fd = (J(up) - J(dn)) / (2 * h)
+            errs.append(abs(fd - g[j, i]) / abs(g[j, i]))
+        worst[k] = max(errs)
+    assert all(v < 1e-3 for v in worst.values()), worst
