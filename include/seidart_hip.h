@@ -44,6 +44,19 @@ extern "C" {
 #define SDT_EM2       2 /* electromag2  electromagfdtd.f95:10-326    */
 #define SDT_EM25      3 /* electromag25 electromagfdtd.f95:619-1092  */
 
+/* Flag OR'ed into an ELASTIC2 kind (sdt_plan_create, sdt_psi_elems,
+ * sdt_state_elems, sdt_adjoint_work_elems): the split C-PML variant.  The
+ * shear-stress loop keeps its own four velocity-derivative memory arrays
+ * instead of sharing the normal-stress loop's (seismicfdtd.f95:255-262 and
+ * :291-298 update the same arrays), and the normal loop's memory_dvx_dx
+ * decays with b_x_half(i) instead of b_x_half(j) (:255).  Not in the
+ * reference: with both legacy quirks the x and top C-PML grow without bound
+ * in long runs (a 6000-step VTI ice model with a snow surface blows up at
+ * step 4414 in the legacy arithmetic).  psi then holds 6 arrays per slab
+ * (x: dvx_dx dvz_dx dsxx_dx dsxz_dx, then the shear loop's dvx_dx dvz_dx;
+ * z: dvz_dz dvx_dz dszz_dz dsxz_dz, then dvz_dz dvx_dz) and aux is unused. */
+#define SDT_SPLIT_CPML 0x100
+
 /* arithmetic precision of fields, coefficients and traces */
 #define SDT_FP64 0
 #define SDT_FP32 1
@@ -175,7 +188,10 @@ int sdt_plan_profile(void* plan, int it0, int nsteps, void* stream, double* out_
  * ------------------------------------------------------------------------ */
 
 /* Per-shot adjoint buffers (device memory, plan precision unless noted).
- * trail:   trail_len states; slot k at trail + k * sdt_state_elems().
+ * trail[b]: trail_len states each; slot k at trail[b] + k * sdt_state_elems().
+ *          trail[1] is optional (NULL for every shot, or set for every
+ *          shot): with two buffers the recomputation of one segment can run
+ *          on another stream while the adjoint walks the previous one.
  * lam[2]:  adjoint state ping-pong, one state each; lambda_t (dJ/du_t) lives
  *          in lam[t & 1].  Zero both before the first (last-in-time) call.
  * work:    sdt_adjoint_work_elems() elements of scratch.
@@ -187,7 +203,7 @@ int sdt_plan_profile(void* plan, int it0, int nsteps, void* stream, double* out_
  * gsrc:    FP64 [nt][3] or NULL: dJ/d(source increment of component c at
  *          step t), written per step. */
 typedef struct sdt_adj_shot {
-  void* trail;
+  void* trail[2];
   int trail_len;
   void* lam[2];
   void* work;
@@ -205,19 +221,29 @@ int sdt_adjoint_work_elems(int kind, int nx, int ny, int nz, int lx, int ly, int
                            long long* out_elems);
 
 /* Hand the plan one sdt_adj_shot per shot (synchronises the device; call
- * again only to change buffers). */
-int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots);
+ * again only to change buffers or options).  grad_mask: bit q set =
+ * accumulate coefficient slot q (0xfff = all 12; slots nobody differentiates
+ * cost nothing when left out).  flags: 0, or SDT_ADJ_REFERENCE to run the
+ * unfused three-kernel adjoint (one thread per cell per phase, scratch
+ * planes in `work`), kept as a cross-check of the fused kernel. */
+#define SDT_ADJ_REFERENCE 1
+int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots, unsigned grad_mask,
+                      int flags);
 
-/* Forward recomputation: with trail slot 0 holding u_{it0}, run steps it0 ..
- * it0+nsteps-1 writing u_{it0+k} into slot k (no receiver gather).  Same
- * kernels and arithmetic as sdt_plan_run. */
-int sdt_plan_run_trail(void* plan, int it0, int nsteps, void* stream);
+/* Forward recomputation: with slot 0 of trail buffer `buf` holding u_{it0},
+ * run steps it0 .. it0+nsteps-1 writing u_{it0+k} into slot k (no receiver
+ * gather).  Same kernels and arithmetic as sdt_plan_run.  Segments of two or
+ * more steps run as a captured hipGraph. */
+int sdt_plan_run_trail(void* plan, int it0, int nsteps, int buf, void* stream);
 
-/* Adjoint steps t = t_hi-1 down to t_lo, with trail slot k holding
- * u_{trail_t0 + k} for trail_t0 + k in [t_lo, t_hi].  Step t injects
+/* Adjoint steps t = t_hi-1 down to t_lo, with slot k of trail buffer `buf`
+ * holding u_{trail_t0 + k} for trail_t0 + k in [t_lo, t_hi].  Step t injects
  * resid[t] at the receivers into lambda_{t+1}, then writes lambda_t and
- * accumulates grad (and gsrc). */
-int sdt_adjoint_run(void* plan, int t_hi, int t_lo, int trail_t0, void* stream);
+ * accumulates grad (and gsrc).  grad is complete when the call's work has
+ * run (FP32 plans stage it in FP32 within one call).  A recomputation
+ * (sdt_plan_run_trail) into the other buffer may run concurrently on
+ * another stream; calls of the same kind must not overlap. */
+int sdt_adjoint_run(void* plan, int t_hi, int t_lo, int trail_t0, int buf, void* stream);
 
 /* Release the plan's own device memory and graphs. */
 void sdt_plan_destroy(void* plan);

@@ -32,7 +32,7 @@ enum { PK = 0, PA, PB, PKH, PAH, PBH };
 long long or_psi_len(int kind, int nx, int ny, int nz) {
   long long n2 = (long long)nx * nz, n3 = n2 * ny;
   switch (kind) {
-    case 0: return 8 * n2;
+    case 0: return 12 * n2; /* 8 legacy arrays + 4 of the split variant */
     case 1: return 18 * n3;
     case 2: return 4 * n2;
     case 3: return 12 * n3;
@@ -74,6 +74,12 @@ static void elastic2(or_run_t* r) {
   double* m_dszz_dz = r->psi + 5 * n2;
   double* m_dsxz_dx = r->psi + 6 * n2;
   double* m_dsxz_dz = r->psi + 7 * n2;
+  /* shear-loop memory: the normal loop's arrays (A1), or their own (split) */
+  const int split = r->split;
+  double* s_dvx_dx = split ? r->psi + 8 * n2 : m_dvx_dx;
+  double* s_dvx_dz = split ? r->psi + 9 * n2 : m_dvx_dz;
+  double* s_dvz_dx = split ? r->psi + 10 * n2 : m_dvz_dx;
+  double* s_dvz_dz = split ? r->psi + 11 * n2 : m_dvz_dz;
 #define I2(i, j) ((long long)(i) + (long long)nx * (j))
 
   for (int s = 0; s < r->nsteps; ++s) {
@@ -88,7 +94,7 @@ static void elastic2(or_run_t* r) {
         double dvz_dz = (vz[c] - vz[I2(i, j - 1)]) / dz;
         double dvz_dx = (vz[I2(i + 1, j)] - vz[c]) / dx;
         double dvx_dz = (vx[c] - vx[I2(i, j - 1)]) / dz;
-        m_dvx_dx[c] = bxh[jb] * m_dvx_dx[c] + axh[i] * dvx_dx;
+        m_dvx_dx[c] = bxh[split ? i : jb] * m_dvx_dx[c] + axh[i] * dvx_dx;
         m_dvz_dz[c] = bz[j] * m_dvz_dz[c] + az[j] * dvz_dz;
         m_dvx_dz[c] = bzh[j] * m_dvx_dz[c] + azh[j] * dvx_dz;
         m_dvz_dx[c] = bx[i] * m_dvz_dx[c] + ax[i] * dvz_dx;
@@ -115,14 +121,14 @@ static void elastic2(or_run_t* r) {
         double dvz_dz = (vz[I2(i, j + 1)] - vz[c]) / dz;
         double dvz_dx = (vz[c] - vz[I2(i - 1, j)]) / dx;
         double dvx_dz = (vx[I2(i, j + 1)] - vx[c]) / dz;
-        m_dvx_dx[c] = bxh[i] * m_dvx_dx[c] + axh[i] * dvx_dx;
-        m_dvz_dz[c] = bz[j] * m_dvz_dz[c] + az[j] * dvz_dz;
-        m_dvx_dz[c] = bzh[j] * m_dvx_dz[c] + azh[j] * dvx_dz;
-        m_dvz_dx[c] = bx[i] * m_dvz_dx[c] + ax[i] * dvz_dx;
-        dvx_dx = dvx_dx / kxh[i] + m_dvx_dx[c];
-        dvz_dz = dvz_dz / kz[j] + m_dvz_dz[c];
-        dvz_dx = dvz_dx / kx[i] + m_dvz_dx[c];
-        dvx_dz = dvx_dz / kzh[j] + m_dvx_dz[c];
+        s_dvx_dx[c] = bxh[i] * s_dvx_dx[c] + axh[i] * dvx_dx;
+        s_dvz_dz[c] = bz[j] * s_dvz_dz[c] + az[j] * dvz_dz;
+        s_dvx_dz[c] = bzh[j] * s_dvx_dz[c] + azh[j] * dvx_dz;
+        s_dvz_dx[c] = bx[i] * s_dvz_dx[c] + ax[i] * dvz_dx;
+        dvx_dx = dvx_dx / kxh[i] + s_dvx_dx[c];
+        dvz_dz = dvz_dz / kz[j] + s_dvz_dz[c];
+        dvz_dx = dvz_dx / kx[i] + s_dvz_dx[c];
+        dvx_dz = dvx_dz / kzh[j] + s_dvx_dz[c];
         const long long n = I2(i, j + 1), w = I2(i - 1, j);
         const double a15 = (c15[n] + 2 * c15[c] + c15[w]) / 4;
         const double a35 = (c35[n] + 2 * c35[c] + c35[w]) / 4;

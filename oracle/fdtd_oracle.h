@@ -31,6 +31,12 @@
  *   2.5-D elastic multiplies each velocity by (1 - gamma) exactly as the 2-D
  *       code does (seismicfdtd.f95:334); gamma = 0 reproduces seismic25.
  *   dt is an input (the legacy recomputes it, :166-167, :511-512).
+ * Optional variant (split = 1, elastic2 only; not in the legacy): the shear
+ *   loop keeps its own four velocity-derivative memory arrays (undoing A1)
+ *   and the normal loop's memory_dvx_dx decays with b_x_half(i) (undoing
+ *   A2).  With both quirks the x and top C-PML can grow without bound in
+ *   long runs (a VTI ice model with a snow surface blows up after ~4400
+ *   steps); the variant is the stable scheme the gradient workloads use.
  */
 #ifndef SEIDART_FDTD_ORACLE_H
 #define SEIDART_FDTD_ORACLE_H
@@ -56,6 +62,7 @@ typedef struct {
   double* psi;             /* workspace (or NULL -> allocated per call)       */
   int blowup_step;         /* out: first step whose max |v| > 1e25 (or -1)    */
   int threads;             /* OpenMP threads (0 = runtime default)            */
+  int split;               /* elastic2 only: 1 = split C-PML variant (below)  */
 } or_run_t;
 
 /* Number of doubles of psi workspace a routine needs (kept across segments). */

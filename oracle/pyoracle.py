@@ -27,7 +27,7 @@ class OrRun(C.Structure):
         ("src", _P * 3),
         ("nrec", C.c_int), ("rec", C.POINTER(C.c_int)), ("trace", _P),
         ("f", _P * 9), ("psi", _P),
-        ("blowup_step", C.c_int), ("threads", C.c_int),
+        ("blowup_step", C.c_int), ("threads", C.c_int), ("split", C.c_int),
     ]
 
 
@@ -134,6 +134,7 @@ class OracleRun:
         r.psi = _ptr(self.psi)
         r.blowup_step = self.blowup
         r.threads = self.threads
+        r.split = 1 if getattr(p, "cpml_split", False) else 0
         rc = lib().or_run(self.kind, C.byref(r))
         if rc:
             raise RuntimeError(f"oracle failed with code {rc}")

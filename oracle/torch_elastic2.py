@@ -10,6 +10,7 @@ inside the step exactly as the legacy loops write them (so it shares no code
 with the device coefficient setup):
   normal stresses  :247-281  (A2: memory_dvx_dx decays with b_x_half(j))
   shear stress     :283-312  (A1: same memory arrays as the normal loop)
+(or, with split=True, the split C-PML variant of oracle/fdtd_oracle.h)
   vx               :317-337  (A6: dt / rho(i,j))
   vz               :339-357  (deltarho = (2 rho(i,j) + rho(i+1,j) + rho(i,j+1)) / 4)
   source           :359-361
@@ -44,7 +45,7 @@ def _put(old, reg, keep, new):
     return old * keep + F.pad(new, reg[1])
 
 
-def run_elastic2(prm, prof, dx, dz, dt, nt, src, force, rec, brow, state0=None):
+def run_elastic2(prm, prof, dx, dz, dt, nt, src, force, rec, brow, state0=None, split=False):
     """Run nt steps; return traces [nt, 2, nrec] (vx, vz after each step).
 
     prm:   dict of (nz, nx) FP64 tensors c11 c13 c15 c33 c35 c55 rho, and
@@ -54,6 +55,9 @@ def run_elastic2(prm, prof, dx, dz, dt, nt, src, force, rec, brow, state0=None):
     rec:   (nrec, 2) integer tensor of (i, k).
     brow:  b_x_half(min(j, nx-1)) per row j (seismicfdtd.f95:255).
     state0: optional dict of initial vx vz sxx szz sxz and m_* memory arrays.
+    split: the split C-PML variant of oracle/fdtd_oracle.h (the shear loop has
+           its own memory arrays s_*; memory_dvx_dx of the normal loop decays
+           with b_x_half(i)).
     """
     c11, c13, c15 = prm["c11"], prm["c13"], prm["c15"]
     c33, c35, c55, rho = prm["c33"], prm["c35"], prm["c55"], prm["rho"]
@@ -67,7 +71,8 @@ def run_elastic2(prm, prof, dx, dz, dt, nt, src, force, rec, brow, state0=None):
     DT = dt
     keep = {k: _keep((nz, nx), r, dtp, dev) for k, r in (("N", _N), ("S", _S), ("X", _X), ("Z", _Z))}
     names = ["vx", "vz", "sxx", "szz", "sxz", "m_dvx_dx", "m_dvx_dz", "m_dvz_dx", "m_dvz_dz",
-             "m_dsxx_dx", "m_dszz_dz", "m_dsxz_dx", "m_dsxz_dz"]
+             "m_dsxx_dx", "m_dszz_dz", "m_dsxz_dx", "m_dsxz_dz",
+             "s_dvx_dx", "s_dvx_dz", "s_dvz_dx", "s_dvz_dz"]
     st = {n: torch.zeros((nz, nx), dtype=dtp, device=dev) for n in names}
     if state0:
         for n, v in state0.items():
@@ -75,6 +80,8 @@ def run_elastic2(prm, prof, dx, dz, dt, nt, src, force, rec, brow, state0=None):
     vx, vz, sxx, szz, sxz = st["vx"], st["vz"], st["sxx"], st["szz"], st["sxz"]
     mvxx, mvxz, mvzx, mvzz = st["m_dvx_dx"], st["m_dvx_dz"], st["m_dvz_dx"], st["m_dvz_dz"]
     msxx, mszz, msxzx, msxzz = st["m_dsxx_dx"], st["m_dszz_dz"], st["m_dsxz_dx"], st["m_dsxz_dz"]
+    svxx, svxz, svzx, svzz = st["s_dvx_dx"], st["s_dvx_dz"], st["s_dvz_dx"], st["s_dvz_dz"]
+    bn = bxh[:, :-1] if split else brow[1:]  # decay of memory_dvx_dx in the normal loop
     interior = torch.zeros((nz, nx), dtype=dtp, device=dev)
     interior[1:-1, 1:-1] = 1.0
     isrc, ksrc = int(src[0]), int(src[1])
@@ -104,7 +111,7 @@ def run_elastic2(prm, prof, dx, dz, dt, nt, src, force, rec, brow, state0=None):
         e2 = (vz[R] - vz[:-1, :-1]) / dz
         e3 = (vz[1:, 1:] - vz[R]) / dx
         e4 = (vx[R] - vx[:-1, :-1]) / dz
-        p1 = brow[1:] * mvxx[R] + axh[:, :-1] * e1
+        p1 = bn * mvxx[R] + axh[:, :-1] * e1
         p2 = bz[1:] * mvzz[R] + az[1:] * e2
         p4 = bzh[1:] * mvxz[R] + azh[1:] * e4
         p3 = bx[:, :-1] * mvzx[R] + ax[:, :-1] * e3
@@ -122,12 +129,20 @@ def run_elastic2(prm, prof, dx, dz, dt, nt, src, force, rec, brow, state0=None):
         f2 = (vz[1:, 1:] - vz[R]) / dz
         f3 = (vz[R] - vz[:-1, :-1]) / dx
         f4 = (vx[1:, 1:] - vx[R]) / dz
-        q1 = bxh[:, 1:] * mvxx[R] + axh[:, 1:] * f1
-        q2 = bz[:-1] * mvzz[R] + az[:-1] * f2
-        q4 = bzh[:-1] * mvxz[R] + azh[:-1] * f4
-        q3 = bx[:, 1:] * mvzx[R] + ax[:, 1:] * f3
-        mvxx, mvzz = _put(mvxx, _S, keep["S"], q1), _put(mvzz, _S, keep["S"], q2)
-        mvxz, mvzx = _put(mvxz, _S, keep["S"], q4), _put(mvzx, _S, keep["S"], q3)
+        if split:
+            q1 = bxh[:, 1:] * svxx[R] + axh[:, 1:] * f1
+            q2 = bz[:-1] * svzz[R] + az[:-1] * f2
+            q4 = bzh[:-1] * svxz[R] + azh[:-1] * f4
+            q3 = bx[:, 1:] * svzx[R] + ax[:, 1:] * f3
+            svxx, svzz = _put(svxx, _S, keep["S"], q1), _put(svzz, _S, keep["S"], q2)
+            svxz, svzx = _put(svxz, _S, keep["S"], q4), _put(svzx, _S, keep["S"], q3)
+        else:
+            q1 = bxh[:, 1:] * mvxx[R] + axh[:, 1:] * f1
+            q2 = bz[:-1] * mvzz[R] + az[:-1] * f2
+            q4 = bzh[:-1] * mvxz[R] + azh[:-1] * f4
+            q3 = bx[:, 1:] * mvzx[R] + ax[:, 1:] * f3
+            mvxx, mvzz = _put(mvxx, _S, keep["S"], q1), _put(mvzz, _S, keep["S"], q2)
+            mvxz, mvzx = _put(mvxz, _S, keep["S"], q4), _put(mvzx, _S, keep["S"], q3)
         g1 = f1 / kxh[:, 1:] + q1
         g2 = f2 / kz[:-1] + q2
         g3 = f3 / kx[:, 1:] + q3

@@ -13,6 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libseidart_hip.so")
 
 SDT_ELASTIC2, SDT_ELASTIC25, SDT_EM2, SDT_EM25 = 0, 1, 2, 3
+SDT_SPLIT_CPML = 0x100
+SDT_ADJ_REFERENCE = 1
 SDT_FP64, SDT_FP32 = 0, 1
 SDT_OK, SDT_EINVAL, SDT_EDEVICE, SDT_EUNSTABLE = 0, 1, 2, 3
 
@@ -45,7 +47,7 @@ class SdtShot(C.Structure):
 
 class SdtAdjShot(C.Structure):
     _fields_ = [
-        ("trail", C.c_void_p),
+        ("trail", C.c_void_p * 2),
         ("trail_len", C.c_int),
         ("lam", C.c_void_p * 2),
         ("work", C.c_void_p),
@@ -105,11 +107,11 @@ def lib():
     L.sdt_state_elems.restype = C.c_int
     L.sdt_adjoint_work_elems.argtypes = [C.c_int] * 7 + [C.POINTER(C.c_longlong)]
     L.sdt_adjoint_work_elems.restype = C.c_int
-    L.sdt_adjoint_setup.argtypes = [C.c_void_p, C.POINTER(SdtAdjShot), C.c_int]
+    L.sdt_adjoint_setup.argtypes = [C.c_void_p, C.POINTER(SdtAdjShot), C.c_int, C.c_uint, C.c_int]
     L.sdt_adjoint_setup.restype = C.c_int
-    L.sdt_plan_run_trail.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    L.sdt_plan_run_trail.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
     L.sdt_plan_run_trail.restype = C.c_int
-    L.sdt_adjoint_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    L.sdt_adjoint_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
     L.sdt_adjoint_run.restype = C.c_int
     _lib = L
     return L

@@ -37,6 +37,10 @@ from .solver import Simulation
 __all__ = ["Elastic2Engine", "Elastic2Traces", "elastic2_traces", "PARAMS"]
 
 NSLOT = 12
+ALL_SLOTS = (1 << NSLOT) - 1
+# coefficient slots each per-cell parameter feeds (include/seidart_hip.h order)
+PARAM_SLOTS = {"c11": (0,), "c13": (1,), "c15": (2, 5), "c33": (3,), "c35": (4, 6), "c55": (7,),
+               "rho": (8, 9), "gamma_x": (10,), "gamma_z": (11,)}
 # per-cell parameters of elastic2_traces
 PARAMS = ("c11", "c13", "c15", "c33", "c35", "c55", "rho", "gamma_x", "gamma_z")
 
@@ -49,7 +53,13 @@ class Elastic2Engine:
     """
 
     def __init__(self, problems, precision: str = "fp32", every: int = 64,
-                 device: str = "cuda:0", graph: bool = True):
+                 device: str = "cuda:0", graph: bool = True, grad_mask: int = ALL_SLOTS,
+                 reference: bool = False, overlap: bool = True):
+        """``grad_mask``: bit q = accumulate coefficient slot q.  ``reference``:
+        run the unfused three-kernel adjoint (a cross-check of the fused
+        kernel; same results to rounding).  ``overlap``: two trail buffers,
+        the recomputation of the next segment running on a second stream
+        while the adjoint walks the current one."""
         self.sim = Simulation(problems, precision, device, all_slots=True)
         if self.sim.kind != "elastic2":
             raise ValueError("the adjoint engine handles elastic2 problems only")
@@ -74,14 +84,16 @@ class Elastic2Engine:
             nst = b["state"].numel()
             nwk = _hip.adjoint_work_elems(sim.kid, p.nx, p.ny, p.nz, lx, ly, lz)
             self.ckpt.append(torch.zeros((self.nseg, nst), dtype=dt, device=dev))
-            self.trail.append(torch.zeros((self.every + 1, nst), dtype=dt, device=dev))
+            self.trail.append(torch.zeros((2 if overlap else 1, self.every + 1, nst), dtype=dt,
+                                          device=dev))
             self.lam.append(torch.zeros((2, nst), dtype=dt, device=dev))
             self.work.append(torch.zeros(nwk, dtype=dt, device=dev))
             self.gsrc.append(torch.zeros((self.nt, 3), dtype=torch.float64, device=dev))
             self.resid.append(torch.zeros_like(b["trace"]))
             self.init.append(b["state"].clone())
             A = adj[s]
-            A.trail = self.trail[s].data_ptr()
+            A.trail[0] = self.trail[s][0].data_ptr()
+            A.trail[1] = self.trail[s][1].data_ptr() if overlap else None
             A.trail_len = self.every + 1
             A.lam[0] = self.lam[s][0].data_ptr()
             A.lam[1] = self.lam[s][1].data_ptr()
@@ -90,9 +102,23 @@ class Elastic2Engine:
             A.grad = self.grad[m].data_ptr()
             A.gsrc = self.gsrc[s].data_ptr()
         self._adj = adj
+        self.overlap = bool(overlap)
+        self._side = torch.cuda.Stream(device=dev) if overlap else None  # default priority
         self.forward_id = 0  # checkpoints belong to the latest forward
-        torch.cuda.synchronize(dev)
-        _hip.check(_hip.lib().sdt_adjoint_setup(sim._plan, adj, len(sim.shots)))
+        self.reference = bool(reference)
+        self.grad_mask = None
+        self.set_grad_mask(grad_mask)
+
+    def set_grad_mask(self, mask: int) -> None:
+        """Choose the coefficient slots the adjoint accumulates (bit q = slot q)."""
+        mask = int(mask) & ALL_SLOTS
+        if mask == self.grad_mask:
+            return
+        torch.cuda.synchronize(self.device)
+        flags = _hip.SDT_ADJ_REFERENCE if self.reference else 0
+        _hip.check(_hip.lib().sdt_adjoint_setup(self.sim._plan, self._adj, len(self.sim.shots),
+                                                mask, flags))
+        self.grad_mask = mask
 
     # ------------------------------------------------------------------
     @property
@@ -148,13 +174,46 @@ class Elastic2Engine:
             r = torch.as_tensor(resid[s]).to(self.device, self.dtype)
             self.resid[s].zero_()
             self.resid[s][:, :, : r.shape[2]].copy_(r)
+        seg = lambda g: (g * self.every, min(self.nt, (g + 1) * self.every))
+        if not self.overlap:
+            for g in range(self.nseg - 1, -1, -1):
+                t_lo, t_hi = seg(g)
+                for s in range(self.nshots):
+                    self.trail[s][0][0].copy_(self.ckpt[s][g])
+                _hip.check(L.sdt_plan_run_trail(sim._plan, t_lo, t_hi - t_lo, 0, h))
+                _hip.check(L.sdt_adjoint_run(sim._plan, t_hi, t_lo, t_lo, 0, h))
+            return self.grad, self.gsrc
+        # recompute segment g-1 into trail buffer (g-1)%2 on the side stream
+        # while the adjoint of segment g reads buffer g%2
+        side = self._side
+        hs = C.c_void_p(side.cuda_stream)
+        side.wait_stream(stream)
+        rec_done = {}
+        adj_done = {}
+
+        def recompute(g):
+            b = g % 2
+            if g + 2 in adj_done:  # the adjoint that read this buffer last
+                side.wait_event(adj_done[g + 2])
+            with torch.cuda.stream(side):
+                for s in range(self.nshots):
+                    self.trail[s][b][0].copy_(self.ckpt[s][g])
+            t_lo, t_hi = seg(g)
+            _hip.check(L.sdt_plan_run_trail(sim._plan, t_lo, t_hi - t_lo, b, hs))
+            ev = torch.cuda.Event()
+            ev.record(side)
+            rec_done[g] = ev
+
+        recompute(self.nseg - 1)
         for g in range(self.nseg - 1, -1, -1):
-            t_lo = g * self.every
-            t_hi = min(self.nt, t_lo + self.every)
-            for s in range(self.nshots):
-                self.trail[s][0].copy_(self.ckpt[s][g])
-            _hip.check(L.sdt_plan_run_trail(sim._plan, t_lo, t_hi - t_lo, h))
-            _hip.check(L.sdt_adjoint_run(sim._plan, t_hi, t_lo, t_lo, h))
+            stream.wait_event(rec_done[g])
+            if g > 0:
+                recompute(g - 1)
+            t_lo, t_hi = seg(g)
+            _hip.check(L.sdt_adjoint_run(sim._plan, t_hi, t_lo, t_lo, g % 2, h))
+            ev = torch.cuda.Event()
+            ev.record(stream)
+            adj_done[g] = ev
         return self.grad, self.gsrc
 
     def adjoint_state(self, shot: int) -> torch.Tensor:
@@ -218,6 +277,14 @@ def elastic2_traces(engine: Elastic2Engine, params):
     sim = engine.sim
     if len(params) != len(sim.problems):
         raise ValueError("one parameter dict per model")
+    mask = 0
+    for prm in params:
+        for k, v in prm.items():
+            if torch.is_tensor(v) and v.requires_grad:
+                for q in PARAM_SLOTS[k]:
+                    mask |= 1 << q
+    if mask:
+        engine.set_grad_mask(mask)
     slots = []
     rho_by_model = []
     for m, (p, prm) in enumerate(zip(sim.problems, params)):

@@ -10,6 +10,9 @@ measured on, built in FP64 on the host with fixed seeds.
                      attenuation, 100 Hz, 3000 steps.
 * ``batched_layered`` 2-D elastic, 3 layered models (512 x 256, 1024 x 512,
                      2048 x 768; dx = 0.5 m), 32 shots each, 150 Hz, 3000 steps.
+* ``fwi_vti``        2-D VTI elastic FWI gradient, 1024 x 512, dx = 1 m,
+                     16 shots, 80 Hz, 6000 steps, 256 receivers (true and
+                     smoothed start model).
 
 Cell indices given for sources and receivers are those of the model without
 the C-PML; the padded (device) index adds the C-PML width on each axis.
@@ -26,7 +29,7 @@ from .cpml import (CLIGHT, alpha_max_em, alpha_max_seismic, axis_profile, sigma_
 from .problem import STIFFNESS, Problem, Shot
 from .sources import point_force
 
-__all__ = ["plain_psv", "gpr", "aniso25", "batched_layered", "snow_firn_ice",
+__all__ = ["plain_psv", "gpr", "aniso25", "batched_layered", "fwi_vti", "snow_firn_ice",
            "rotate_stiffness", "voigt_to_tensor", "tensor_to_voigt", "CONFIGS"]
 
 SNOW = (400.0, 1800.0, 900.0)     # rho, Vp, Vs
@@ -152,6 +155,74 @@ def _elastic2_problem(rho, vp, vs, dx, dz, cpml, dt, nt, f0, srcs, recs, labels=
     rec = np.array([[r[0] + cpml, 0, r[1] + cpml] for r in recs], dtype=np.int32)
     shots = [Shot(src=(s[0] + cpml, 0, s[1] + cpml), force=(fx, None, fz), rec=rec) for s in srcs]
     return Problem("elastic2", nx, 1, nz, dx, 1.0, dz, dt, nt, coef, prof, shots)
+
+
+def _elastic2_problem_c(C0, dx, dz, cpml, dt, nt, f0, srcs, recs, vmax):
+    """Like _elastic2_problem, from stiffness fields (Pa) and density given
+    per cell on the unpadded grid."""
+    nz0, nx0 = C0["rho"].shape
+    C = _full_stiffness(nz0, nx0)
+    C.update(C0)
+    coef = {k: _pad_edge(v, cpml) for k, v in C.items()}
+    nz, nx = nz0 + 2 * cpml, nx0 + 2 * cpml
+    sm = sigma_max_seismic(vmax, cpml, 2, 0.001)
+    am = alpha_max_seismic(1.0, f0)
+    prof = {"x": axis_profile(nx, cpml, dx, sm, am, 5.0, 2, 2, dt, True),
+            "z": axis_profile(nz, cpml, dz, sm, am, 5.0, 2, 2, dt, True)}
+    fx, _, fz = point_force(nt, dt, f0, 1.0, "gaus1", xz_rotation=90.0)
+    rec = np.array([[r[0] + cpml, 0, r[1] + cpml] for r in recs], dtype=np.int32)
+    shots = [Shot(src=(s[0] + cpml, 0, s[1] + cpml), force=(fx, None, fz), rec=rec) for s in srcs]
+    return Problem("elastic2", nx, 1, nz, dx, 1.0, dz, dt, nt, coef, prof, shots)
+
+
+FWI_PARAMS = ("c11", "c13", "c33", "c55", "rho")
+
+
+def fwi_vti(nt: int = 6000, seed: int = 1005, nshots: int = 16, nx0: int = 1024, nz0: int = 512,
+            cpml: int = 20, nrec: int = 256):
+    """FWI gradient workload, 2-D VTI elastic: 1024 x 512 cells, dx = dz = 1 m,
+    C-PML 20, per-cell (C11, C13, C33, C55, rho).
+
+    True model: snow / firn / ice layers (snow to row 0.06 nz, firn to row
+    0.25 nz, isotropic from Vp / Vs), with the ice VTI (the ice single crystal
+    with a vertical c-axis: C11 13.93, C13 5.77, C33 15.01, C55 3.01 GPa),
+    plus 3 seeded Gaussian anomalies (sigma 20 cells) scaling all five
+    parameters by +-5%.  Start model: the true model Gaussian-smoothed with
+    sigma 30 cells.  16 vertical-force Ricker shots of 80 Hz evenly spaced
+    along x at z = 4, 256 receivers every 4 cells at z = 2, dt with CFL
+    1/sqrt(2) (classes.py:469-485), split C-PML variant.  Returns (true,
+    start) Problems that share sources, receivers, dt and C-PML."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    dx = 1.0
+    rho, vp, vs, labels = snow_firn_ice(nx0, nz0, dx, rng, 0.06 * nz0, 0.25 * nz0)
+    C = {k: v.copy() for k, v in _iso(rho, vp, vs).items()}
+    C["rho"] = rho.copy()
+    ice = labels == 2
+    for k, v in (("c11", ICE_C["c11"]), ("c13", ICE_C["c13"]), ("c33", ICE_C["c33"]),
+                 ("c55", ICE_C["c44"])):
+        C[k][ice] = v * 1e9
+    z, x = np.mgrid[0:nz0, 0:nx0].astype(np.float64)
+    pert = np.ones((nz0, nx0))
+    for _ in range(3):
+        cz = rng.uniform(0.3 * nz0, 0.9 * nz0)
+        cx = rng.uniform(0.1 * nx0, 0.9 * nx0)
+        amp = rng.choice([-0.05, 0.05])
+        pert += amp * np.exp(-((z - cz) ** 2 + (x - cx) ** 2) / (2 * 20.0 ** 2))
+    true = {k: C[k] * pert for k in FWI_PARAMS}
+    start = {k: gaussian_filter(v, 30.0, mode="nearest") for k, v in true.items()}
+    vmax = max(float(np.sqrt(np.max(np.maximum(M["c11"], M["c33"]) / M["rho"]))) for M in (true, start))
+    dt = (1 / np.sqrt(2.0)) / (vmax * np.sqrt(2.0 / dx ** 2))
+    srcs = [(int((k + 0.5) * nx0 / nshots), 4) for k in range(nshots)]
+    step = nx0 // nrec
+    recs = [(step // 2 + step * k, 2) for k in range(nrec)]
+    out = tuple(_elastic2_problem_c(M, dx, dx, cpml, dt, nt, 80.0, srcs, recs, vmax)
+                for M in (true, start))
+    for p in out:
+        # the legacy C-PML (shared memory arrays, b_x_half(j)) blows up on
+        # this model after ~4400 steps; gradients use the split variant
+        p.cpml_split = True
+    return out
 
 
 def plain_psv(nt: int = 2000, seed: int = 1001) -> Problem:
@@ -281,4 +352,5 @@ CONFIGS = {
     "gpr": gpr,
     "aniso25": aniso25,
     "batched_layered": batched_layered,
+    "fwi_vti": fwi_vti,
 }

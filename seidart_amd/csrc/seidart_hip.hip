@@ -86,7 +86,7 @@ struct DShot {
 
 // Adjoint-side buffers of one shot (elastic 2-D), see sdt_adj_shot.
 struct DAdj {
-  void* trail;      // forward states, slot k at trail + k * slot
+  void* trail[2];   // forward states, slot k at trail + k * slot (two buffers)
   long slot;        // elements of one state: fields then psi
   int trail_len;    // slots available
   void* lam[2];     // adjoint state ping-pong: lambda_t in lam[t & 1]
@@ -94,6 +94,14 @@ struct DAdj {
   const void* resid;
   double* grad;     // [12][nz][nx]
   double* gsrc;     // [nt][3]
+  float* g32;       // FP32 plans: per-call staging of grad (same layout)
+};
+
+// FP32 gradient staging of one distinct grad array, flushed into it.
+struct DFlush {
+  double* grad;
+  float* g32;
+  long n;           // cells of the model
 };
 
 enum { PK = 0, PA, PB, PKH, PAH, PBH };
@@ -150,8 +158,9 @@ __device__ __forceinline__ bool in_slab(int i, int n, int l) { return i < l || i
 // of neighbouring tiles on old values.  Each cell's fields, psi and
 // coefficients cross HBM once per step.
 //
-// psi slabs: x [nz][2lx]: dvx_dx dvz_dx dsxx_dx dsxz_dx
-//            z [2lz][nx]: dvz_dz dvx_dz dszz_dz dsxz_dz
+// psi slabs: x [nz][2lx]: dvx_dx dvz_dx dsxx_dx dsxz_dx (+ dvx_dx dvz_dx of
+//                          the shear loop in the split variant)
+//            z [2lz][nx]: dvz_dz dvx_dz dszz_dz dsxz_dz (+ dvz_dz dvx_dz)
 // ===========================================================================
 #ifndef SDT_EL2_ROWS
 #define SDT_EL2_ROWS 32
@@ -197,23 +206,26 @@ __device__ __forceinline__ T lane_prev(T v) {  // value of lane l-1
   return __shfl_up(v, 1, 64);
 }
 
-template <typename T, bool TRAIL>
+template <typename T, bool TRAIL, bool SPLIT>
 __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
                                          const DShot* __restrict__ shots, const int4 wi, const int it,
-                                         const int lane, const DAdj* __restrict__ adj, const int t0);
+                                         const int lane, const DAdj* __restrict__ adj, const int slot,
+                                         const int tb);
 
 // Work: a block of EWB wavefronts takes one item (shot, xo, zo): wave w owns
 // columns xo + w*EW .. +EW-1 of rows zo .. zo+EH-1, so the block's row
 // accesses are EWB*EW contiguous cells of each array.  The grid is
 // persistent: block b takes items b, b + gridDim.x, ...; gridDim.x is a
 // multiple of 8, so item w always runs on XCD w % 8.
-// TRAIL: read the state of step it from trail slot it - t0 and write the
-// new state into the next slot (forward recomputation for the adjoint).
-template <typename T, bool TRAIL>
+// TRAIL: read the state of step it from trail slot rel (the run starts at
+// slot 0) and write the new state into the next slot (forward recomputation
+// for the adjoint).
+// SPLIT: the split C-PML variant (see SDT_SPLIT_CPML).
+template <typename T, bool TRAIL, bool SPLIT>
 __global__ void __launch_bounds__(64 * EWB, sizeof(T) == 4 ? SDT_EL2_WAVES_PER_SIMD : 3)
 k_el2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
            const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel,
-           const DAdj* __restrict__ adj, int t0) {
+           const DAdj* __restrict__ adj, const int* __restrict__ tsel) {
   // wave-uniform values kept in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int it = __builtin_amdgcn_readfirstlane(*base + rel);
@@ -223,14 +235,16 @@ k_el2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
                               __builtin_amdgcn_readfirstlane(wv.y) + wave * EW,
                               __builtin_amdgcn_readfirstlane(wv.z), 0);
     if (wi.x >= 0 && wi.y < models[shots[wi.x].model].nx)
-      el2_tile<T, TRAIL>(models, shots, wi, it, threadIdx.x & 63, adj, t0);
+      el2_tile<T, TRAIL, SPLIT>(models, shots, wi, it, threadIdx.x & 63, adj, rel,
+                                TRAIL ? __builtin_amdgcn_readfirstlane(*tsel) : 0);
   }
 }
 
-template <typename T, bool TRAIL>
+template <typename T, bool TRAIL, bool SPLIT>
 __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
                                          const DShot* __restrict__ shots, const int4 wi, const int it,
-                                         const int lane, const DAdj* __restrict__ adj, const int t0) {
+                                         const int lane, const DAdj* __restrict__ adj, const int slot,
+                                         const int tb) {
   const DShot& S = shots[wi.x];
   const DModel& M = models[S.model];
   const int nx = M.nx, nz = M.nz;
@@ -246,8 +260,8 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
   T* tout = nullptr;
   if (TRAIL) {
     const DAdj& A = adj[wi.x];
-    tin = static_cast<const T*>(A.trail) + (long)(it - t0) * A.slot;
-    tout = static_cast<T*>(A.trail) + (long)(it - t0 + 1) * A.slot;
+    tin = static_cast<const T*>(A.trail[tb]) + (long)slot * A.slot;
+    tout = static_cast<T*>(A.trail[tb]) + (long)(slot + 1) * A.slot;
   }
   const auto vxi = cp<T>(TRAIL ? tin : (par ? S.g[0] : S.f[0]));
   const auto vzi = cp<T>(TRAIL ? tin + nn : (par ? S.g[1] : S.f[1]));
@@ -264,6 +278,7 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
   const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dt = T(M.dt);
   const int lx = M.lx, lz = M.lz;
   const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
+  const long ZB = (SPLIT ? 6 : 4) * SX;  // first z-slab array
   const bool inx = incol && in_slab(i, nx, lx);
   const int xs = inx ? slab_pos(i, nx, lx) : 0;
   const Prof<T> P(&M.prof[0][0]);
@@ -313,15 +328,25 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
     const int rvs = rr < 0 ? 0 : (rr >= nz ? nz - 1 : rr);
     const bool inz = in_slab(rs, nz, lz), inzv = in_slab(rvs, nz, lz);
     const long ox = (long)rs * 2 * lx + xs, oxv = (long)rvs * 2 * lx + xs;
-    const long oz = 4 * SX + (inz ? (long)slab_pos(rs, nz, lz) * nx + ic : 0);
-    const long ozv = 4 * SX + (inzv ? (long)slab_pos(rvs, nz, lz) * nx + ic : 0);
+    const long oz = ZB + (inz ? (long)slab_pos(rs, nz, lz) * nx + ic : 0);
+    const long ozv = ZB + (inzv ? (long)slab_pos(rvs, nz, lz) * nx + ic : 0);
     const long oxs = inx ? ox : 0, oxvs = inx ? oxv : 0;
-    T m_vxx = pin[oxs], m_vzx = pin[SX + oxs];            // x-slab psi of the stress row
-    T m_vzz = pin[oz], m_vxz = pin[SZ + oz];              // z-slab psi of the stress row
-    T m_sxx = pin[2 * SX + oxvs], m_sxzx = pin[3 * SX + oxvs];  // x-slab psi of the velocity row
-    T m_szz = pin[2 * SZ + ozv], m_sxzz = pin[3 * SZ + ozv];    // z-slab psi of the velocity row
+    // psi loads only where the cell has that memory (a zero-width slab has no storage)
+    const bool inzc = incol && inz, inzvc = incol && inzv;
+    T m_vxx = inx ? pin[oxs] : T(0), m_vzx = inx ? pin[SX + oxs] : T(0);        // x-slab, stress row
+    T m_vzz = inzc ? pin[oz] : T(0), m_vxz = inzc ? pin[SZ + oz] : T(0);        // z-slab, stress row
+    // split C-PML: the shear loop's own memory (x slab arrays 4, 5; z 4, 5)
+    T s_vxx = 0, s_vzx = 0, s_vzz = 0, s_vxz = 0;
+    if (SPLIT) {
+      s_vxx = inx ? pin[4 * SX + oxs] : T(0);
+      s_vzx = inx ? pin[5 * SX + oxs] : T(0);
+      s_vzz = inzc ? pin[4 * SZ + oz] : T(0);
+      s_vxz = inzc ? pin[5 * SZ + oz] : T(0);
+    }
+    T m_sxx = inx ? pin[2 * SX + oxvs] : T(0), m_sxzx = inx ? pin[3 * SX + oxvs] : T(0);  // velocity row
+    T m_szz = inzvc ? pin[2 * SZ + ozv] : T(0), m_sxzz = inzvc ? pin[3 * SZ + ozv] : T(0);
     const T bzr = bz[rs], azr = az[rs], ikzr = ikz[rs], bzhr = bzh[rs], azhr = azh[rs], ikzhr = ikzh[rs];
-    const T browr = brow[rs];
+    const T browr = SPLIT ? bxh : brow[rs];  // b_x_half(j) (A2), or b_x_half(i)
     const T bzv = bz[rvs], azv = az[rvs], ikzv = ikz[rvs], bzhv = bzh[rvs], azhv = azh[rvs], ikzhv = ikzh[rvs];
     const bool nxt = (r + 1 <= zend) && valid(r + 1);
     const T vx_n = nxt ? ld_vxn : T(0), vz_n = nxt ? ld_vzn : T(0);  // velocity row r+1
@@ -337,7 +362,7 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
       T dvz_dx = (vz_e - vz_c) * idx;
       T dvx_dz = (vx_c - vx_m) * idz;
       if (inx_r) {
-        m_vxx = browr * m_vxx + axh * dvx_dx;  // b_x_half(j), :255
+        m_vxx = browr * m_vxx + axh * dvx_dx;  // :255
         m_vzx = bx * m_vzx + ax * dvz_dx;
         dvx_dx = dvx_dx * ikxh + m_vxx;
         dvz_dx = dvz_dx * ikx + m_vzx;
@@ -352,22 +377,26 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
       sxx_c = sxx_c + (n11 * dvx_dx + n13 * dvz_dz + n15 * sh) * dt;
       szz_c = szz_c + (n13 * dvx_dx + n33 * dvz_dz + n35 * sh) * dt;
     }
-    if (rv && can_s && r <= nz - 2 && r < zend) {  // same psi arrays (A1)
+    if (rv && can_s && r <= nz - 2 && r < zend) {  // same psi arrays (A1) unless split
+      T& q_vxx = SPLIT ? s_vxx : m_vxx;
+      T& q_vzx = SPLIT ? s_vzx : m_vzx;
+      T& q_vzz = SPLIT ? s_vzz : m_vzz;
+      T& q_vxz = SPLIT ? s_vxz : m_vxz;
       T dvx_dx = (vx_c - vx_w) * idx;
       T dvz_dz = (vz_n - vz_c) * idz;
       T dvz_dx = (vz_c - vz_w) * idx;
       T dvx_dz = (vx_n - vx_c) * idz;
       if (inx_r) {
-        m_vxx = bxh * m_vxx + axh * dvx_dx;
-        m_vzx = bx * m_vzx + ax * dvz_dx;
-        dvx_dx = dvx_dx * ikxh + m_vxx;
-        dvz_dx = dvz_dx * ikx + m_vzx;
+        q_vxx = bxh * q_vxx + axh * dvx_dx;
+        q_vzx = bx * q_vzx + ax * dvz_dx;
+        dvx_dx = dvx_dx * ikxh + q_vxx;
+        dvz_dx = dvz_dx * ikx + q_vzx;
       }
       if (inz_r) {
-        m_vzz = bzr * m_vzz + azr * dvz_dz;
-        m_vxz = bzhr * m_vxz + azhr * dvx_dz;
-        dvz_dz = dvz_dz * ikzr + m_vzz;
-        dvx_dz = dvx_dz * ikzhr + m_vxz;
+        q_vzz = bzr * q_vzz + azr * dvz_dz;
+        q_vxz = bzhr * q_vxz + azhr * dvx_dz;
+        dvz_dz = dvz_dz * ikzr + q_vzz;
+        dvx_dz = dvx_dz * ikzhr + q_vxz;
       }
       sxz_c = sxz_c + (s15 * dvx_dx + s35 * dvz_dz + s55 * (dvz_dx + dvx_dz)) * dt;
     }
@@ -375,8 +404,22 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
       st_stream(sxxo + c, sxx_c);
       st_stream(szzo + c, szz_c);
       st_stream(sxzo + c, sxz_c);
-      if (inx) { pout[ox] = m_vxx; pout[SX + ox] = m_vzx; }
-      if (inz) { pout[oz] = m_vzz; pout[SZ + oz] = m_vxz; }
+      if (inx) {
+        pout[ox] = m_vxx;
+        pout[SX + ox] = m_vzx;
+        if (SPLIT) {
+          pout[4 * SX + ox] = s_vxx;
+          pout[5 * SX + ox] = s_vzx;
+        }
+      }
+      if (inz) {
+        pout[oz] = m_vzz;
+        pout[SZ + oz] = m_vxz;
+        if (SPLIT) {
+          pout[4 * SZ + oz] = s_vzz;
+          pout[5 * SZ + oz] = s_vxz;
+        }
+      }
     }
 
     // ---- velocities of row rr = r - 1 (seismicfdtd.f95:317-375) ----
@@ -477,12 +520,45 @@ __global__ void k_copy_back(const DShot* __restrict__ shots, int nfield, const l
 __device__ __forceinline__ void gadd(double* g, double v) {
   if (v != 0.0) unsafeAtomicAdd(g, v);
 }
+// FP32 plans stage the gradient in FP32 within one sdt_adjoint_run call (at
+// most a segment of steps) and flush it into the FP64 array at the end:
+// half the atomic bytes, so a model's staging stays in L2.
+__device__ __forceinline__ void gadd(float* g, double v) {
+  const float f = (float)v;
+  if (f != 0.0f) unsafeAtomicAdd(g, f);
+}
+template <typename T>
+struct GAcc;
+template <>
+struct GAcc<double> {
+  static __device__ __forceinline__ double* of(const DAdj& a) { return a.grad; }
+};
+template <>
+struct GAcc<float> {
+  static __device__ __forceinline__ float* of(const DAdj& a) { return a.g32; }
+};
+
+// grad += staging; staging = 0 (slots of the mask only)
+__global__ void k_grad_flush(const DFlush* __restrict__ fl, unsigned gmask) {
+  const DFlush& F = fl[blockIdx.y];
+  for (int q = 0; q < 12; ++q) {
+    if (!(gmask & (1u << q))) continue;
+    double* g = F.grad + (long)q * F.n;
+    float* h = F.g32 + (long)q * F.n;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < F.n; e += (long)gridDim.x * blockDim.x) {
+      g[e] += (double)h[e];
+      h[e] = 0.0f;
+    }
+  }
+}
 
 // lambda_{t+1}(v) += dJ/dtrace(t) at the receivers (the trace of step t is
 // read from u_{t+1}).
 template <typename T>
 __global__ void k_el2_adj_inject(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-                                 const DAdj* __restrict__ adj, int nshots, int t) {
+                                 const DAdj* __restrict__ adj, int nshots, const int* __restrict__ tbase,
+                                 int k) {
+  const int t = *tbase + k;
   const int s = blockIdx.y;
   if (s >= nshots) return;
   const DShot& S = shots[s];
@@ -499,10 +575,12 @@ __global__ void k_el2_adj_inject(const DModel* __restrict__ models, const DShot*
   atomicAdd(L + n + c, res[S.nrec]);
 }
 
-template <typename T>
+template <typename T, bool SPLIT>
 __global__ void __launch_bounds__(NT)
 k_el2_adj_v(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-            const DAdj* __restrict__ adj, int nshots, int t, int t0) {
+            const DAdj* __restrict__ adj, int nshots, const int* __restrict__ tbase, int k,
+            const int* __restrict__ tsel) {
+  const int t = *tbase + k, t0 = *tbase;
   const int s = blockIdx.z;
   if (s >= nshots) return;
   const DShot& S = shots[s];
@@ -516,14 +594,14 @@ k_el2_adj_v(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
   const auto Lin = cp<T>(A.lam[(t + 1) & 1]);
   auto Lout = mp<T>(A.lam[t & 1]);
-  const auto u1 = cp<T>(A.trail) + (long)(t + 1 - t0) * A.slot;
-  const auto u0 = cp<T>(A.trail) + (long)(t - t0) * A.slot;
+  const auto u1 = cp<T>(A.trail[*tsel]) + (long)(t + 1 - t0) * A.slot;
+  const auto u0 = cp<T>(A.trail[*tsel]) + (long)(t - t0) * A.slot;
   auto W = mp<T>(A.work);
   double* G = A.grad;
   const Prof<T> P(&M.prof[0][0]);
   const bool inx = in_slab(i, nx, lx), inz = in_slab(j, nz, lz);
   const long ox = 5 * n + (inx ? (long)j * 2 * lx + slab_pos(i, nx, lx) : 0);
-  const long oz = 5 * n + 4 * SX + (inz ? (long)slab_pos(j, nz, lz) * nx + i : 0);
+  const long oz = 5 * n + (SPLIT ? 6 : 4) * SX + (inz ? (long)slab_pos(j, nz, lz) * nx + i : 0);
   const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz);
   // Dirichlet edges come last in the step: nothing flows back through them
   const bool edge = (i == 0 || i == nx - 1 || j == 0 || j == nz - 1);
@@ -596,10 +674,12 @@ k_el2_adj_v(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   W[3 * n + c] = az2;  // adjoint of dszz_dz at the vz node
 }
 
-template <typename T>
+template <typename T, bool SPLIT>
 __global__ void __launch_bounds__(NT)
 k_el2_adj_s(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-            const DAdj* __restrict__ adj, int nshots, int t, int t0) {
+            const DAdj* __restrict__ adj, int nshots, const int* __restrict__ tbase, int k,
+            const int* __restrict__ tsel) {
+  const int t = *tbase + k, t0 = *tbase;
   const int s = blockIdx.z;
   if (s >= nshots) return;
   const DShot& S = shots[s];
@@ -613,14 +693,14 @@ k_el2_adj_s(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
   const auto Lin = cp<T>(A.lam[(t + 1) & 1]);
   auto Lout = mp<T>(A.lam[t & 1]);
-  const auto u1 = cp<T>(A.trail) + (long)(t + 1 - t0) * A.slot;
-  const auto u0 = cp<T>(A.trail) + (long)(t - t0) * A.slot;
+  const auto u1 = cp<T>(A.trail[*tsel]) + (long)(t + 1 - t0) * A.slot;
+  const auto u0 = cp<T>(A.trail[*tsel]) + (long)(t - t0) * A.slot;
   auto W = mp<T>(A.work);
   double* G = A.grad;
   const Prof<T> P(&M.prof[0][0]);
   const bool inx = in_slab(i, nx, lx), inz = in_slab(j, nz, lz);
   const long ox = 5 * n + (inx ? (long)j * 2 * lx + slab_pos(i, nx, lx) : 0);
-  const long oz = 5 * n + 4 * SX + (inz ? (long)slab_pos(j, nz, lz) * nx + i : 0);
+  const long oz = 5 * n + (SPLIT ? 6 : 4) * SX + (inz ? (long)slab_pos(j, nz, lz) * nx + i : 0);
   const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dt = T(M.dt);
   // stresses feed the velocity update through their differences
   const T lsxx = Lin[2 * n + c] + idx * (W[c] - (i + 1 < nx ? W[c + 1] : T(0)));
@@ -630,9 +710,22 @@ k_el2_adj_s(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   Lout[2 * n + c] = lsxx;
   Lout[3 * n + c] = lszz;
   Lout[4 * n + c] = lsxz;
-  // adjoints of the velocity-derivative psi (shared by both stress loops, A1)
+  // adjoints of the velocity-derivative psi: shared by both stress loops
+  // (A1), or the shear loop's own ones ls* in the split variant
   T lm1 = inx ? Lin[ox] : T(0), lm3 = inx ? Lin[ox + SX] : T(0);
   T lm2 = inz ? Lin[oz] : T(0), lm4 = inz ? Lin[oz + SZ] : T(0);
+  T ls1 = 0, ls3 = 0, ls2 = 0, ls4 = 0;
+  if (SPLIT) {
+    ls1 = inx ? Lin[ox + 4 * SX] : T(0);
+    ls3 = inx ? Lin[ox + 5 * SX] : T(0);
+    ls2 = inz ? Lin[oz + 4 * SZ] : T(0);
+    ls4 = inz ? Lin[oz + 5 * SZ] : T(0);
+  }
+  T& q1 = SPLIT ? ls1 : lm1;
+  T& q2 = SPLIT ? ls2 : lm2;
+  T& q3 = SPLIT ? ls3 : lm3;
+  T& q4 = SPLIT ? ls4 : lm4;
+  const long xq = SPLIT ? 4 * SX : 0, zq = SPLIT ? 4 * SZ : 0;  // shear-loop psi
   const T ikx = P[PK][i], ax = P[PA][i], bx = P[PB][i];
   const T ikxh = P[PKH][i], axh = P[PAH][i], bxh = P[PBH][i];
   const T ikz = P[12 + PK][j], az = P[12 + PA][j], bz = P[12 + PB][j];
@@ -650,24 +743,24 @@ k_el2_adj_s(const DModel* __restrict__ models, const DShot* __restrict__ shots,
     T g1 = f1, g2 = f2, g3 = f3, g4 = f4;
     lf1 = lg1; lf2 = lg2; lf3 = lg3; lf4 = lg3;
     if (inx) {
-      g1 = ikxh * f1 + u1[ox];
-      T l = lm1 + lg1;
+      g1 = ikxh * f1 + u1[ox + xq];
+      T l = q1 + lg1;
       lf1 = ikxh * lg1 + axh * l;
-      lm1 = bxh * l;
-      g3 = ikx * f3 + u1[ox + SX];
-      l = lm3 + lg3;
+      q1 = bxh * l;
+      g3 = ikx * f3 + u1[ox + xq + SX];
+      l = q3 + lg3;
       lf3 = ikx * lg3 + ax * l;
-      lm3 = bx * l;
+      q3 = bx * l;
     }
     if (inz) {
-      g2 = ikz * f2 + u1[oz];
-      T l = lm2 + lg2;
+      g2 = ikz * f2 + u1[oz + zq];
+      T l = q2 + lg2;
       lf2 = ikz * lg2 + az * l;
-      lm2 = bz * l;
-      g4 = ikzh * f4 + u1[oz + SZ];
-      l = lm4 + lg3;
+      q2 = bz * l;
+      g4 = ikzh * f4 + u1[oz + zq + SZ];
+      l = q4 + lg3;
       lf4 = ikzh * lg3 + azh * l;
-      lm4 = bzh * l;
+      q4 = bzh * l;
     }
     const double m = (double)mu;
     gadd(G + 5 * n + c, m * (double)g1);
@@ -687,7 +780,7 @@ k_el2_adj_s(const DModel* __restrict__ models, const DShot* __restrict__ shots,
     T d1 = e1, d2 = e2, d3 = e3, d4 = e4;
     le1 = ld1; le2 = ld2; le3 = lsh; le4 = lsh;
     if (inx) {
-      const T brow = cp<T>(M.aux)[j];  // b_x_half(j), :255
+      const T brow = SPLIT ? bxh : cp<T>(M.aux)[j];  // b_x_half(j), :255 (A2)
       d1 = ikxh * e1 + (brow * u0[ox] + axh * e1);
       T l = lm1 + ld1;
       le1 = ikxh * ld1 + axh * l;
@@ -717,10 +810,18 @@ k_el2_adj_s(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   if (inx) {
     Lout[ox] = lm1;
     Lout[ox + SX] = lm3;
+    if (SPLIT) {
+      Lout[ox + 4 * SX] = ls1;
+      Lout[ox + 5 * SX] = ls3;
+    }
   }
   if (inz) {
     Lout[oz] = lm2;
     Lout[oz + SZ] = lm4;
+    if (SPLIT) {
+      Lout[oz + 4 * SZ] = ls2;
+      Lout[oz + 5 * SZ] = ls4;
+    }
   }
   W[4 * n + c] = le1;   // dvx_dx at the normal-stress node
   W[5 * n + c] = le2;   // dvz_dz
@@ -735,7 +836,8 @@ k_el2_adj_s(const DModel* __restrict__ models, const DShot* __restrict__ shots,
 template <typename T>
 __global__ void __launch_bounds__(NT)
 k_el2_adj_g(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-            const DAdj* __restrict__ adj, int nshots, int t) {
+            const DAdj* __restrict__ adj, int nshots, const int* __restrict__ tbase, int k) {
+  const int t = *tbase + k;
   const int s = blockIdx.z;
   if (s >= nshots) return;
   const DShot& S = shots[s];
@@ -756,6 +858,338 @@ k_el2_adj_g(const DModel* __restrict__ models, const DShot* __restrict__ shots,
                idz * (at(9, -nx, n_) - W[9 * n + c]) + idx * (W[10 * n + c] - at(10, 1, e_));
   Lout[c] = Lout[c] + gx;
   Lout[n + c] = Lout[n + c] + gz;
+}
+
+// ---------------------------------------------------------------------------
+// Fused adjoint step: the same transpose as the three kernels above, in one
+// pass.  A wavefront owns AW columns (lanes 2..AW+1; two halo lanes on each
+// side) of AH rows of one shot and marches down them.  At march row r it
+// does V^T of row r, the stress gathers with S^T / N^T of row r-1 and the
+// velocity gather of row r-2, keeping the few rows each phase needs from
+// the previous ones in registers and taking x neighbours from the
+// neighbouring lanes.  The halo lanes and the two halo rows above and below
+// the tile are recomputed, so tiles are independent (lambda ping-pongs).
+// Only owned cells write lambda_t and add to the gradient.
+// ---------------------------------------------------------------------------
+#ifndef SDT_ADJ_WAVES_PER_SIMD
+#define SDT_ADJ_WAVES_PER_SIMD 3
+#endif
+constexpr int AW = 60;   // owned columns per wavefront
+constexpr int AWB = 4;   // wavefronts per block, side by side in x
+constexpr int AH = SDT_EL2_ROWS;
+
+template <typename T, bool SPLIT>
+__device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
+                                             const DShot* __restrict__ shots,
+                                             const DAdj* __restrict__ adj, const int s, const int xo,
+                                             const int zo, const int t, const int t0,
+                                             const unsigned gmask, const int lane, const int tb) {
+  const DShot& S = shots[s];
+  const DModel& M = models[S.model];
+  const DAdj& A = adj[s];
+  const int nx = M.nx, nz = M.nz;
+  const int i = xo - 2 + lane;
+  const bool incol = (i >= 0 && i < nx);
+  const int ic = incol ? i : (i < 0 ? 0 : nx - 1);
+  const bool own = incol && lane >= 2 && lane < 2 + AW;
+  const bool sl = incol && lane >= 1 && lane <= 2 + AW;  // lanes of the stress phase
+  const int zend = min(zo + AH, nz);
+  const long n = (long)nx * nz;
+  const int lx = M.lx, lz = M.lz;
+  const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
+  const long ZB = 5 * n + (SPLIT ? 6 : 4) * SX;
+  const auto Lin = cp<T>(A.lam[(t + 1) & 1]);
+  auto Lout = mp<T>(A.lam[t & 1]);
+  const auto u1 = cp<T>(A.trail[tb]) + (long)(t + 1 - t0) * A.slot;
+  const auto u0 = cp<T>(A.trail[tb]) + (long)(t - t0) * A.slot;
+  auto G = GAcc<T>::of(A);
+  const Prof<T> P(&M.prof[0][0]);
+  const bool inx = incol && in_slab(i, nx, lx);
+  const long xs = 5 * n + (inx ? slab_pos(i, nx, lx) : 0);
+  const T ikx = P[PK][ic], ax = P[PA][ic], bx = P[PB][ic];
+  const T ikxh = P[PKH][ic], axh = P[PAH][ic], bxh = P[PBH][ic];
+  const auto ikz = P[12 + PK], az = P[12 + PA], bz = P[12 + PB];
+  const auto ikzh = P[12 + PKH], azh = P[12 + PAH], bzh = P[12 + PBH];
+  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dt = T(M.dt);
+  const auto N11 = cp<T>(M.coef[0]), N13 = cp<T>(M.coef[1]), N15 = cp<T>(M.coef[2]);
+  const auto N33 = cp<T>(M.coef[3]), N35 = cp<T>(M.coef[4]);
+  const auto S15 = cp<T>(M.coef[5]), S35 = cp<T>(M.coef[6]), S55 = cp<T>(M.coef[7]);
+  const auto DTX = cp<T>(M.coef[8]), DTZ = cp<T>(M.coef[9]);
+  const auto OMX = cp<T>(M.coef[10]), OMZ = cp<T>(M.coef[11]);
+  const auto brow = cp<T>(M.aux);
+  const bool mono = M.coef[2] != nullptr, atten = M.coef[10] != nullptr;
+  auto row = [&](int r) { return (long)(r < 0 ? 0 : (r >= nz ? nz - 1 : r)) * nx + ic; };
+  auto valid = [&](int r) { return incol && r >= 0 && r < nz; };
+  auto clampr = [&](int r) { return r < 0 ? 0 : (r >= nz ? nz - 1 : r); };
+
+  // register windows (rows relative to the march row r)
+  T A1 = 0, B1 = 0, C1 = 0, D1 = 0, D2 = 0;      // A, B, C, D of row r-1; D of r-2
+  T pvx1 = 0, pvz1 = 0, pvx2 = 0, pvz2 = 0;      // partial lambda_v of rows r-1, r-2
+  T vx_m = 0, vz_m = 0, vx_c = 0, vz_c = 0;      // u_t velocities of rows r-2, r-1
+  T sxz_p = valid(zo - 3) ? u1[4 * n + row(zo - 3)] : T(0);  // u_{t+1} sxz of row r-1
+  T szz_c = valid(zo - 2) ? u1[3 * n + row(zo - 2)] : T(0);  // u_{t+1} szz of row r
+  T E1p = 0, E2p = 0, E3p = 0, E4p = 0, F1p = 0, F2p = 0, F3p = 0, F4p = 0;  // row r-2
+  T F2pp = 0, F4pp = 0;                                                      // row r-3
+
+  for (int r = zo - 2; r <= zend + 1; ++r) {
+    const int q = r - 1, pr = r - 2;
+    const long cr = row(r), cq = row(q), cn = row(r + 1);
+    const bool vr = valid(r), vq = valid(q);
+    const int rs = clampr(r), qs = clampr(q);
+    const bool inzr = incol && in_slab(rs, nz, lz), inzq = incol && in_slab(qs, nz, lz);
+    const long oxr = xs + (inx ? (long)rs * 2 * lx : 0), oxq = xs + (inx ? (long)qs * 2 * lx : 0);
+    const long ozr = ZB + (inzr ? (long)slab_pos(rs, nz, lz) * nx + ic : 0);
+    const long ozq = ZB + (inzq ? (long)slab_pos(qs, nz, lz) * nx + ic : 0);
+    // ---- loads ----
+    const T lvx0 = Lin[cr], lvz0 = Lin[n + cr];
+    const T dtx = DTX[cr], dtz = DTZ[cr];
+    const T omx = atten ? OMX[cr] : T(1), omz = atten ? OMZ[cr] : T(1);
+    const T sxx1 = u1[2 * n + cr], sxz1 = u1[4 * n + cr], szz1n = u1[3 * n + cn];
+    const T vx_n = vr ? u0[cr] : T(0), vz_n = vr ? u0[n + cr] : T(0);
+    const T lsxx0 = Lin[2 * n + cq], lszz0 = Lin[3 * n + cq], lsxz0 = Lin[4 * n + cq];
+    const T n11 = N11[cq], n13 = N13[cq], n33 = N33[cq], s55 = S55[cq];
+    const T n15 = mono ? N15[cq] : T(0), n35 = mono ? N35[cq] : T(0);
+    const T s15 = mono ? S15[cq] : T(0), s35 = mono ? S35[cq] : T(0);
+    // psi and lambda_psi of the velocity row r (stress-derivative memory)
+    // (slab loads only where the cell has that memory: a zero-width slab
+    // has no storage to read)
+    const T Lx2 = inx ? Lin[oxr + 2 * SX] : T(0), Lx3 = inx ? Lin[oxr + 3 * SX] : T(0);
+    const T Ux2 = inx ? u1[oxr + 2 * SX] : T(0), Ux3 = inx ? u1[oxr + 3 * SX] : T(0);
+    const T Lz2 = inzr ? Lin[ozr + 2 * SZ] : T(0), Lz3 = inzr ? Lin[ozr + 3 * SZ] : T(0);
+    const T Uz2 = inzr ? u1[ozr + 2 * SZ] : T(0), Uz3 = inzr ? u1[ozr + 3 * SZ] : T(0);
+    // of the stress row q (velocity-derivative memory)
+    const long xq2 = SPLIT ? 4 * SX : 0, zq2 = SPLIT ? 4 * SZ : 0;  // shear loop's arrays
+    const T Lx0 = inx ? Lin[oxq] : T(0), Lx1 = inx ? Lin[oxq + SX] : T(0);
+    const T Lz0 = inzq ? Lin[ozq] : T(0), Lz1 = inzq ? Lin[ozq + SZ] : T(0);
+    const T Ox0 = inx ? u0[oxq] : T(0), Ox1 = inx ? u0[oxq + SX] : T(0);
+    const T Oz0 = inzq ? u0[ozq] : T(0), Oz1 = inzq ? u0[ozq + SZ] : T(0);
+    const T Qx0 = inx ? u1[oxq + xq2] : T(0), Qx1 = inx ? u1[oxq + xq2 + SX] : T(0);
+    const T Qz0 = inzq ? u1[ozq + zq2] : T(0), Qz1 = inzq ? u1[ozq + zq2 + SZ] : T(0);
+    T Lsx0 = 0, Lsx1 = 0, Lsz0 = 0, Lsz1 = 0;
+    if (SPLIT) {
+      Lsx0 = inx ? Lin[oxq + 4 * SX] : T(0);
+      Lsx1 = inx ? Lin[oxq + 5 * SX] : T(0);
+      Lsz0 = inzq ? Lin[ozq + 4 * SZ] : T(0);
+      Lsz1 = inzq ? Lin[ozq + 5 * SZ] : T(0);
+    }
+    const T browq = SPLIT ? bxh : brow[qs];
+    const T bzr = bz[rs], azr = az[rs], ikzr = ikz[rs], bzhr = bzh[rs], azhr = azh[rs], ikzhr = ikzh[rs];
+    const T bzq = bz[qs], azq = az[qs], ikzq = ikz[qs], bzhq = bzh[qs], azhq = azh[qs], ikzhq = ikzh[qs];
+    // neighbour values (wave-wide shuffles, outside any branch)
+    const T sxx1_w = lane_prev(sxx1), sxz1_e = lane_next(sxz1);
+    const T A1_e = lane_next(A1), C1_w = lane_prev(C1);
+    const T vx_cw = lane_prev(vx_c), vz_cw = lane_prev(vz_c);
+    const T vx_ce = lane_next(vx_c), vz_ce = lane_next(vz_c);
+    const T E1p_w = lane_prev(E1p), E3p_w = lane_prev(E3p);
+    const T F1p_e = lane_next(F1p), F3p_e = lane_next(F3p);
+
+    // ---- V^T of row r: velocity updates, seismicfdtd.f95:317-375 ----
+    const bool ownr = own && r >= zo && r < zend;
+    const bool edge = (i == 0 || i == nx - 1 || r == 0 || r == nz - 1);
+    const T lvx = (vr && !edge) ? lvx0 : T(0), lvz = (vr && !edge) ? lvz0 : T(0);
+    T An = 0, Bn = 0, Cn = 0, Dn = 0, pvx = lvx, pvz = lvz;
+    if (ownr && i == S.isrc && r == S.ksrc && A.gsrc) {
+      A.gsrc[3 * (long)t + 0] = (double)lvx;
+      A.gsrc[3 * (long)t + 2] = (double)lvz;
+    }
+    if (vr && i >= 1 && r >= 1) {
+      const T w = dtx * lvx;
+      An = w;
+      Bn = w;
+      T b1 = (sxx1 - sxx1_w) * idx, b2 = (sxz1 - sxz_p) * idz;
+      if (inx) {
+        const T lr = Lx2 + w;
+        An = ikx * w + ax * lr;
+        if (ownr) Lout[oxr + 2 * SX] = bx * lr;
+        b1 = ikx * b1 + Ux2;
+      }
+      if (inzr) {
+        const T lr = Lz3 + w;
+        Bn = ikzr * w + azr * lr;
+        if (ownr) Lout[ozr + 3 * SZ] = bzr * lr;
+        b2 = ikzr * b2 + Uz3;
+      }
+      pvx = omx * lvx;
+      if (ownr) {
+        if (gmask & (1u << 8)) gadd(G + 8 * n + cr, (double)lvx * (double)(b1 + b2));
+        if (gmask & (1u << 10)) gadd(G + 10 * n + cr, (double)lvx * (double)vx_n);
+      }
+    } else if (ownr) {
+      if (inx) Lout[oxr + 2 * SX] = Lx2;
+      if (inzr) Lout[ozr + 3 * SZ] = Lz3;
+    }
+    if (vr && i <= nx - 2 && r <= nz - 2) {
+      const T w = dtz * lvz;
+      Cn = w;
+      Dn = w;
+      T b1 = (sxz1_e - sxz1) * idx, b2 = (szz1n - szz_c) * idz;
+      if (inx) {
+        const T lr = Lx3 + w;
+        Cn = ikxh * w + axh * lr;
+        if (ownr) Lout[oxr + 3 * SX] = bxh * lr;
+        b1 = ikxh * b1 + Ux3;
+      }
+      if (inzr) {
+        const T lr = Lz2 + w;
+        Dn = ikzhr * w + azhr * lr;
+        if (ownr) Lout[ozr + 2 * SZ] = bzhr * lr;
+        b2 = ikzhr * b2 + Uz2;
+      }
+      pvz = omz * lvz;
+      if (ownr) {
+        if (gmask & (1u << 9)) gadd(G + 9 * n + cr, (double)lvz * (double)(b1 + b2));
+        if (gmask & (1u << 11)) gadd(G + 11 * n + cr, (double)lvz * (double)vz_n);
+      }
+    } else if (ownr) {
+      if (inx) Lout[oxr + 3 * SX] = Lx3;
+      if (inzr) Lout[ozr + 2 * SZ] = Lz2;
+    }
+
+    // ---- stresses of row q: gathers, then S^T and N^T, :247-312 ----
+    T Ee1 = 0, Ee2 = 0, Ee3 = 0, Ee4 = 0, Ff1 = 0, Ff2 = 0, Ff3 = 0, Ff4 = 0;
+    if (sl && vq && q >= zo - 1) {
+      const bool ownq = own && q >= zo && q < zend;
+      const T lsxx = lsxx0 + idx * (A1 - A1_e);
+      const T lszz = lszz0 + idz * (D2 - D1);
+      const T lsxz = lsxz0 + idz * (B1 - Bn) + idx * (C1_w - C1);
+      if (ownq) {
+        Lout[2 * n + cq] = lsxx;
+        Lout[3 * n + cq] = lszz;
+        Lout[4 * n + cq] = lsxz;
+      }
+      T lm1 = Lx0, lm3 = Lx1, lm2 = Lz0, lm4 = Lz1;     // normal loop's psi adjoints
+      T ls1 = Lsx0, ls3 = Lsx1, ls2 = Lsz0, ls4 = Lsz1;  // shear loop's (split)
+      T& k1 = SPLIT ? ls1 : lm1;
+      T& k2 = SPLIT ? ls2 : lm2;
+      T& k3 = SPLIT ? ls3 : lm3;
+      T& k4 = SPLIT ? ls4 : lm4;
+      if (q <= nz - 2 && i >= 1) {  // S^T
+        const T mu = dt * lsxz;
+        const T f1 = (vx_c - vx_cw) * idx, f2 = (vz_n - vz_c) * idz;
+        const T f3 = (vz_c - vz_cw) * idx, f4 = (vx_n - vx_c) * idz;
+        const T lg1 = s15 * mu, lg2 = s35 * mu, lg3 = s55 * mu;
+        T g1 = f1, g2 = f2, g3 = f3, g4 = f4;
+        Ff1 = lg1; Ff2 = lg2; Ff3 = lg3; Ff4 = lg3;
+        if (inx) {
+          g1 = ikxh * f1 + Qx0;
+          T l = k1 + lg1;
+          Ff1 = ikxh * lg1 + axh * l;
+          k1 = bxh * l;
+          g3 = ikx * f3 + Qx1;
+          l = k3 + lg3;
+          Ff3 = ikx * lg3 + ax * l;
+          k3 = bx * l;
+        }
+        if (inzq) {
+          g2 = ikzq * f2 + Qz0;
+          T l = k2 + lg2;
+          Ff2 = ikzq * lg2 + azq * l;
+          k2 = bzq * l;
+          g4 = ikzhq * f4 + Qz1;
+          l = k4 + lg3;
+          Ff4 = ikzhq * lg3 + azhq * l;
+          k4 = bzhq * l;
+        }
+        if (ownq) {
+          const double m = (double)mu;
+          if (gmask & (1u << 5)) gadd(G + 5 * n + cq, m * (double)g1);
+          if (gmask & (1u << 6)) gadd(G + 6 * n + cq, m * (double)g2);
+          if (gmask & (1u << 7)) gadd(G + 7 * n + cq, m * ((double)g3 + (double)g4));
+        }
+      }
+      if (q >= 1 && i <= nx - 2) {  // N^T
+        const T mux = dt * lsxx, muz = dt * lszz;
+        const T e1 = (vx_ce - vx_c) * idx, e2 = (vz_c - vz_m) * idz;
+        const T e3 = (vz_ce - vz_c) * idx, e4 = (vx_c - vx_m) * idz;
+        const T ld1 = n11 * mux + n13 * muz, ld2 = n13 * mux + n33 * muz, lsh = n15 * mux + n35 * muz;
+        T d1 = e1, d2 = e2, d3 = e3, d4 = e4;
+        Ee1 = ld1; Ee2 = ld2; Ee3 = lsh; Ee4 = lsh;
+        if (inx) {
+          d1 = ikxh * e1 + (browq * Ox0 + axh * e1);
+          T l = lm1 + ld1;
+          Ee1 = ikxh * ld1 + axh * l;
+          lm1 = browq * l;
+          d3 = ikx * e3 + (bx * Ox1 + ax * e3);
+          l = lm3 + lsh;
+          Ee3 = ikx * lsh + ax * l;
+          lm3 = bx * l;
+        }
+        if (inzq) {
+          d2 = ikzq * e2 + (bzq * Oz0 + azq * e2);
+          T l = lm2 + ld2;
+          Ee2 = ikzq * ld2 + azq * l;
+          lm2 = bzq * l;
+          d4 = ikzhq * e4 + (bzhq * Oz1 + azhq * e4);
+          l = lm4 + lsh;
+          Ee4 = ikzhq * lsh + azhq * l;
+          lm4 = bzhq * l;
+        }
+        if (ownq) {
+          const double mx = (double)mux, mz = (double)muz, sh = (double)d3 + (double)d4;
+          if (gmask & (1u << 0)) gadd(G + cq, mx * (double)d1);
+          if (gmask & (1u << 1)) gadd(G + n + cq, mx * (double)d2 + mz * (double)d1);
+          if (gmask & (1u << 2)) gadd(G + 2 * n + cq, mx * sh);
+          if (gmask & (1u << 3)) gadd(G + 3 * n + cq, mz * (double)d2);
+          if (gmask & (1u << 4)) gadd(G + 4 * n + cq, mz * sh);
+        }
+      }
+      if (ownq) {
+        if (inx) {
+          Lout[oxq] = lm1;
+          Lout[oxq + SX] = lm3;
+          if (SPLIT) {
+            Lout[oxq + 4 * SX] = ls1;
+            Lout[oxq + 5 * SX] = ls3;
+          }
+        }
+        if (inzq) {
+          Lout[ozq] = lm2;
+          Lout[ozq + SZ] = lm4;
+          if (SPLIT) {
+            Lout[ozq + 4 * SZ] = ls2;
+            Lout[ozq + 5 * SZ] = ls4;
+          }
+        }
+      }
+    }
+
+    // ---- velocities of row pr: gather of the stress-phase adjoints ----
+    if (own && pr >= zo && pr < zend) {
+      const long cp_ = row(pr);
+      Lout[cp_] = pvx2 + idx * (E1p_w - E1p) + idz * (E4p - Ee4) + idx * (F1p - F1p_e) +
+                  idz * (F4pp - F4p);
+      Lout[n + cp_] = pvz2 + idz * (E2p - Ee2) + idx * (E3p_w - E3p) + idz * (F2pp - F2p) +
+                      idx * (F3p - F3p_e);
+    }
+
+    // ---- shift the windows ----
+    F2pp = F2p; F4pp = F4p;
+    E1p = Ee1; E2p = Ee2; E3p = Ee3; E4p = Ee4;
+    F1p = Ff1; F2p = Ff2; F3p = Ff3; F4p = Ff4;
+    pvx2 = pvx1; pvz2 = pvz1; pvx1 = pvx; pvz1 = pvz;
+    A1 = An; B1 = Bn; C1 = Cn; D2 = D1; D1 = Dn;
+    vx_m = vx_c; vz_m = vz_c; vx_c = vx_n; vz_c = vz_n;
+    sxz_p = sxz1; szz_c = szz1n;
+  }
+}
+
+template <typename T, bool SPLIT>
+__global__ void __launch_bounds__(64 * AWB, sizeof(T) == 4 ? SDT_ADJ_WAVES_PER_SIMD : 2)
+k_el2_adj(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+          const DAdj* __restrict__ adj, const int4* __restrict__ work, int nitems,
+          const int* __restrict__ tbase, int k, unsigned gmask, const int* __restrict__ tsel) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int t0 = __builtin_amdgcn_readfirstlane(*tbase);
+  const int tb = __builtin_amdgcn_readfirstlane(*tsel);
+  const int t = t0 + k;
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const int4 wv = work[w];
+    const int s = __builtin_amdgcn_readfirstlane(wv.x);
+    const int xo = __builtin_amdgcn_readfirstlane(wv.y) + wave * AW;
+    const int zo = __builtin_amdgcn_readfirstlane(wv.z);
+    if (s >= 0 && xo < models[shots[s].model].nx)
+      el2_adj_tile<T, SPLIT>(models, shots, adj, s, xo, zo, t, t0, gmask, threadIdx.x & 63, tb);
+  }
 }
 
 // ===========================================================================
@@ -1374,6 +1808,7 @@ __global__ void k_fill_int(int* p, int n, int v) {
 // ===========================================================================
 struct Plan {
   int kind = 0, prec = 0, nmodels = 0, nshots = 0, nt = 0, ncomp = 2;
+  bool split = false;  // SDT_SPLIT_CPML
   int maxrec = 0;
   std::vector<sdt_model> hmodels;
   std::vector<sdt_shot> hshots;
@@ -1393,6 +1828,25 @@ struct Plan {
   hipGraph_t graph = nullptr;
   DAdj* dadj = nullptr;          // adjoint descriptors (sdt_adjoint_setup)
   std::vector<DAdj> hadj;
+  int4* dadjwork = nullptr;      // work list of the fused adjoint kernel
+  int* dtbase = nullptr;         // first step of the trail (adjoint kernels)
+  int* dtsel_trail = nullptr;    // trail buffer the recomputation writes
+  int* dtsel_adj = nullptr;      // trail buffer the adjoint reads
+  // captured segments: (kind 0 trail / 1 adjoint, key) -> graph; kept until
+  // setup or destroy so a replay in flight is never freed
+  struct SegGraph {
+    int kind;
+    long long key;
+    hipGraph_t g;
+    hipGraphExec_t e;
+  };
+  std::vector<SegGraph> seg_graphs;
+  std::vector<float*> g32;       // FP32 gradient staging per distinct grad array
+  DFlush* dflush = nullptr;
+  int nflush = 0;
+  int nadjwork = 0;
+  unsigned gmask = 0xfffu;       // gradient slots to accumulate
+  int adj_flags = 0;
   int maxnx = 0, maxnz = 0;
 };
 
@@ -1416,8 +1870,13 @@ int kind_ncoef(int kind) {
 }
 bool kind_3d(int kind) { return kind == SDT_ELASTIC25 || kind == SDT_EM25; }
 
+// kind may carry SDT_SPLIT_CPML (ELASTIC2 only: two more memory arrays per slab)
 long long psi_elems(int kind, long long nx, long long ny, long long nz, long long lx,
                     long long ly, long long lz) {
+  if (kind & SDT_SPLIT_CPML) {
+    if ((kind & ~SDT_SPLIT_CPML) != SDT_ELASTIC2) return -1;
+    return 6 * nz * 2 * lx + 6 * 2 * lz * nx;
+  }
   switch (kind) {
     case SDT_ELASTIC2: return 4 * nz * 2 * lx + 4 * 2 * lz * nx;
     case SDT_EM2: return 2 * nz * 2 * lx + 2 * 2 * lz * nx;
@@ -1443,9 +1902,14 @@ void launch_one(Plan* p, hipStream_t st, int rel, int which) {
   }
   switch (p->kind * 2 + which) {
     case SDT_ELASTIC2 * 2:
-      hipLaunchKernelGGL((k_el2_step<T, false>), dim3(p->el2_grid), dim3(64 * EWB), 0, st,
-                         p->dmodels, p->dshots, p->dwork, p->nwork, p->dbase, rel,
-                         (const DAdj*)nullptr, 0);
+      if (p->split)
+        hipLaunchKernelGGL((k_el2_step<T, false, true>), dim3(p->el2_grid), dim3(64 * EWB), 0, st,
+                           p->dmodels, p->dshots, p->dwork, p->nwork, p->dbase, rel,
+                           (const DAdj*)nullptr, (const int*)nullptr);
+      else
+        hipLaunchKernelGGL((k_el2_step<T, false, false>), dim3(p->el2_grid), dim3(64 * EWB), 0, st,
+                           p->dmodels, p->dshots, p->dwork, p->nwork, p->dbase, rel,
+                           (const DAdj*)nullptr, (const int*)nullptr);
       break;
     case SDT_ELASTIC2 * 2 + 1:
       break;  // the fused step kernel does both phases
@@ -1497,8 +1961,9 @@ void launch_steps(Plan* p, hipStream_t st, int n) {
   }
 }
 
-int validate(int kind, int prec, const sdt_model* models, int nmodels, const sdt_shot* shots,
-             int nshots, int nt) {
+int validate(int kind, bool split, int prec, const sdt_model* models, int nmodels,
+             const sdt_shot* shots, int nshots, int nt) {
+  const int kpsi = kind | (split ? SDT_SPLIT_CPML : 0);
   if (kind < 0 || kind > 3) return fail(SDT_EINVAL, "unknown kind");
   if (prec != SDT_FP64 && prec != SDT_FP32) return fail(SDT_EINVAL, "unknown precision");
   if (!models || nmodels <= 0 || !shots || nshots <= 0) return fail(SDT_EINVAL, "no models or shots");
@@ -1531,7 +1996,8 @@ int validate(int kind, int prec, const sdt_model* models, int nmodels, const sdt
       for (int q = 0; q < 6; ++q)
         if (!M.prof[a][q]) return fail(SDT_EINVAL, tag + "C-PML profile missing");
     }
-    if (kind == SDT_ELASTIC2 && !M.aux) return fail(SDT_EINVAL, tag + "aux (b_x_half by row) missing");
+    if (kind == SDT_ELASTIC2 && !split && !M.aux)
+      return fail(SDT_EINVAL, tag + "aux (b_x_half by row) missing");
   }
   for (int s = 0; s < nshots; ++s) {
     const sdt_shot& S = shots[s];
@@ -1540,12 +2006,12 @@ int validate(int kind, int prec, const sdt_model* models, int nmodels, const sdt
     const sdt_model& M = models[S.model];
     for (int f = 0; f < kind_fields(kind); ++f)
       if (!S.field[f]) return fail(SDT_EINVAL, tag + "field pointer null");
-    if (!S.psi && psi_elems(kind, M.nx, M.ny, M.nz, M.lx, M.ly, M.lz) > 0)
+    if (!S.psi && psi_elems(kpsi, M.nx, M.ny, M.nz, M.lx, M.ly, M.lz) > 0)
       return fail(SDT_EINVAL, tag + "psi workspace null");
     if (kind == SDT_ELASTIC2) {
       for (int f = 0; f < kind_fields(kind); ++f)
         if (!S.field_b[f]) return fail(SDT_EINVAL, tag + "field_b pointer null (ELASTIC2 ping-pongs)");
-      if (!S.psi_b && psi_elems(kind, M.nx, M.ny, M.nz, M.lx, M.ly, M.lz) > 0)
+      if (!S.psi_b && psi_elems(kpsi, M.nx, M.ny, M.nz, M.lx, M.ly, M.lz) > 0)
         return fail(SDT_EINVAL, tag + "psi_b workspace null");
     }
     if (S.isrc < 0 || S.isrc >= M.nx || S.ksrc < 0 || S.ksrc >= M.nz ||
@@ -1573,30 +2039,94 @@ void end_of_run(Plan* p, hipStream_t st, int next_step) {
 long long state_elems(int kind, long long nx, long long ny, long long nz, long long lx,
                       long long ly, long long lz) {
   const long long np = psi_elems(kind, nx, ny, nz, lx, ly, lz);
-  return np < 0 ? -1 : kind_fields(kind) * nx * ny * nz + np;
+  return np < 0 ? -1 : kind_fields(kind & ~SDT_SPLIT_CPML) * nx * ny * nz + np;
 }
 
 template <typename T>
 void launch_trail_steps(Plan* p, hipStream_t st, int it0, int nsteps) {
-  for (int r = 0; r < nsteps; ++r)
-    hipLaunchKernelGGL((k_el2_step<T, true>), dim3(p->el2_grid), dim3(64 * EWB), 0, st,
-                       p->dmodels, p->dshots, p->dwork, p->nwork, p->dbase, r, p->dadj, it0);
+  for (int r = 0; r < nsteps; ++r) {
+    if (p->split)
+      hipLaunchKernelGGL((k_el2_step<T, true, true>), dim3(p->el2_grid), dim3(64 * EWB), 0, st,
+                         p->dmodels, p->dshots, p->dwork, p->nwork, p->dbase, r, p->dadj,
+                         p->dtsel_trail);
+    else
+      hipLaunchKernelGGL((k_el2_step<T, true, false>), dim3(p->el2_grid), dim3(64 * EWB), 0, st,
+                         p->dmodels, p->dshots, p->dwork, p->nwork, p->dbase, r, p->dadj,
+                         p->dtsel_trail);
+  }
 }
 
-template <typename T>
-void launch_adjoint_steps(Plan* p, hipStream_t st, int t_hi, int t_lo, int t0) {
+// Adjoint steps t = tbase + k for k = khi-1 down to klo (tbase = the trail's
+// first step, held on the device so a captured segment replays anywhere).
+template <typename T, bool SPLIT>
+void launch_adjoint_steps(Plan* p, hipStream_t st, int khi, int klo) {
   const dim3 blk(TX, TZ);
   const dim3 grd((p->maxnx + TX - 1) / TX, (p->maxnz + TZ - 1) / TZ, p->nshots);
-  for (int t = t_hi - 1; t >= t_lo; --t) {
+  const int* tb = p->dtbase;
+  for (int k = khi - 1; k >= klo; --k) {
     if (p->maxrec > 0)
       hipLaunchKernelGGL(k_el2_adj_inject<T>, dim3((p->maxrec + 63) / 64, p->nshots), dim3(64), 0,
-                         st, p->dmodels, p->dshots, p->dadj, p->nshots, t);
-    hipLaunchKernelGGL(k_el2_adj_v<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dadj, p->nshots,
-                       t, t0);
-    hipLaunchKernelGGL(k_el2_adj_s<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dadj, p->nshots,
-                       t, t0);
-    hipLaunchKernelGGL(k_el2_adj_g<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dadj, p->nshots, t);
+                         st, p->dmodels, p->dshots, p->dadj, p->nshots, tb, k);
+    if (!(p->adj_flags & SDT_ADJ_REFERENCE)) {
+      hipLaunchKernelGGL((k_el2_adj<T, SPLIT>), dim3(p->nadjwork), dim3(64 * AWB), 0, st,
+                         p->dmodels, p->dshots, p->dadj, p->dadjwork, p->nadjwork, tb, k, p->gmask,
+                         p->dtsel_adj);
+      continue;
+    }
+    hipLaunchKernelGGL((k_el2_adj_v<T, SPLIT>), grd, blk, 0, st, p->dmodels, p->dshots, p->dadj,
+                       p->nshots, tb, k, p->dtsel_adj);
+    hipLaunchKernelGGL((k_el2_adj_s<T, SPLIT>), grd, blk, 0, st, p->dmodels, p->dshots, p->dadj,
+                       p->nshots, tb, k, p->dtsel_adj);
+    hipLaunchKernelGGL(k_el2_adj_g<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dadj, p->nshots, tb, k);
   }
+}
+
+void launch_trail(Plan* p, hipStream_t st, int nsteps) {
+  if (p->prec == SDT_FP64)
+    launch_trail_steps<double>(p, st, 0, nsteps);
+  else
+    launch_trail_steps<float>(p, st, 0, nsteps);
+}
+
+void launch_adjoint(Plan* p, hipStream_t st, int khi, int klo) {
+  if (p->prec == SDT_FP64)
+    p->split ? launch_adjoint_steps<double, true>(p, st, khi, klo)
+             : launch_adjoint_steps<double, false>(p, st, khi, klo);
+  else
+    p->split ? launch_adjoint_steps<float, true>(p, st, khi, klo)
+             : launch_adjoint_steps<float, false>(p, st, khi, klo);
+  if (p->nflush > 0)
+    hipLaunchKernelGGL(k_grad_flush, dim3(256, p->nflush), dim3(256), 0, st, p->dflush, p->gmask);
+}
+
+void drop_adjoint_graphs(Plan* p) {
+  for (auto& x : p->seg_graphs) {
+    (void)hipGraphExecDestroy(x.e);
+    (void)hipGraphDestroy(x.g);
+  }
+  p->seg_graphs.clear();
+}
+
+// Replay (capturing on first use) the segment graph (kind, key) on st.
+template <typename F>
+int replay_segment(Plan* p, int kind, long long key, hipStream_t st, F&& enqueue) {
+  for (auto& x : p->seg_graphs)
+    if (x.kind == kind && x.key == key) {
+      HIP_TRY(hipGraphLaunch(x.e, st));
+      return SDT_OK;
+    }
+  if (p->seg_graphs.size() >= 8) {  // many segment shapes: free the old ones once idle
+    HIP_TRY(hipDeviceSynchronize());
+    drop_adjoint_graphs(p);
+  }
+  Plan::SegGraph x{kind, key, nullptr, nullptr};
+  HIP_TRY(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+  enqueue(p->cap);
+  HIP_TRY(hipStreamEndCapture(p->cap, &x.g));
+  HIP_TRY(hipGraphInstantiate(&x.e, x.g, nullptr, nullptr, 0));
+  p->seg_graphs.push_back(x);
+  HIP_TRY(hipGraphLaunch(x.e, st));
+  return SDT_OK;
 }
 
 }  // namespace
@@ -1622,12 +2152,16 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
                     const sdt_shot* shots, int nshots, int nt, void** out_plan) {
   if (!out_plan) return fail(SDT_EINVAL, "out_plan is null");
   *out_plan = nullptr;
-  int rc = validate(kind, prec, models, nmodels, shots, nshots, nt);
+  const bool split = (kind & SDT_SPLIT_CPML) != 0;
+  kind &= ~SDT_SPLIT_CPML;
+  if (split && kind != SDT_ELASTIC2) return fail(SDT_EINVAL, "SDT_SPLIT_CPML applies to ELASTIC2 only");
+  int rc = validate(kind, split, prec, models, nmodels, shots, nshots, nt);
   if (rc) return rc;
   // Receiver indices must be checked on the host before any kernel reads them.
   // They live in device memory, so copy each shot's list back once.
   Plan* p = new Plan();
   p->kind = kind;
+  p->split = split;
   p->prec = prec;
   p->nmodels = nmodels;
   p->nshots = nshots;
@@ -1734,7 +2268,8 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
     for (int s = 0; s < nshots; ++s) {
       const sdt_model& M = models[shots[s].model];
       nel[s] = (long)M.nx * M.ny * M.nz;
-      npsi[s] = (long)psi_elems(kind, M.nx, M.ny, M.nz, M.lx, kind_3d(kind) ? M.ly : 0, M.lz);
+      npsi[s] = (long)psi_elems(kind | (split ? SDT_SPLIT_CPML : 0), M.nx, M.ny, M.nz, M.lx,
+                                kind_3d(kind) ? M.ly : 0, M.lz);
     }
     if (hipMalloc(&p->dnel, sizeof(long) * nshots) != hipSuccess ||
         hipMalloc(&p->dnpsi, sizeof(long) * nshots) != hipSuccess ||
@@ -1857,27 +2392,35 @@ int sdt_adjoint_work_elems(int kind, int nx, int ny, int nz, int lx, int ly, int
                            long long* out) {
   (void)ny; (void)lx; (void)ly; (void)lz;
   if (!out) return fail(SDT_EINVAL, "out is null");
-  if (kind != SDT_ELASTIC2) return fail(SDT_EINVAL, "the adjoint is available for ELASTIC2 only");
+  if ((kind & ~SDT_SPLIT_CPML) != SDT_ELASTIC2)
+    return fail(SDT_EINVAL, "the adjoint is available for ELASTIC2 only");
   *out = 12LL * nx * nz;
   return SDT_OK;
 }
 
-int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots) {
+int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots, unsigned grad_mask,
+                      int flags) {
   Plan* p = static_cast<Plan*>(plan);
   if (!p || !adj) return fail(SDT_EINVAL, "plan or adj is null");
+  if (grad_mask & ~0xfffu) return fail(SDT_EINVAL, "grad_mask has bits beyond the 12 slots");
+  if (flags & ~SDT_ADJ_REFERENCE) return fail(SDT_EINVAL, "unknown adjoint flags");
   if (p->kind != SDT_ELASTIC2) return fail(SDT_EINVAL, "the adjoint is available for ELASTIC2 only");
   if (nshots != p->nshots) return fail(SDT_EINVAL, "adj must hold one entry per shot of the plan");
   std::vector<DAdj> h(nshots);
   for (int s = 0; s < nshots; ++s) {
     const sdt_adj_shot& a = adj[s];
     const std::string tag = "shot " + std::to_string(s) + ": ";
-    if (!a.trail || !a.lam[0] || !a.lam[1] || !a.work || !a.grad)
-      return fail(SDT_EINVAL, tag + "trail, lam, work and grad must be set");
+    if (!a.trail[0] || !a.lam[0] || !a.lam[1] || !a.work || !a.grad)
+      return fail(SDT_EINVAL, tag + "trail[0], lam, work and grad must be set");
+    if ((a.trail[1] != nullptr) != (adj[0].trail[1] != nullptr))
+      return fail(SDT_EINVAL, tag + "trail[1] must be set for every shot or for none");
     if (a.trail_len < 2) return fail(SDT_EINVAL, tag + "trail_len must be at least 2");
     const sdt_model& M = p->hmodels[p->hshots[s].model];
     DAdj& D = h[s];
-    D.trail = a.trail;
-    D.slot = (long)state_elems(p->kind, M.nx, M.ny, M.nz, M.lx, kind_3d(p->kind) ? M.ly : 0, M.lz);
+    D.trail[0] = a.trail[0];
+    D.trail[1] = a.trail[1];
+    D.slot = (long)state_elems(p->kind | (p->split ? SDT_SPLIT_CPML : 0), M.nx, M.ny, M.nz, M.lx,
+                               kind_3d(p->kind) ? M.ly : 0, M.lz);
     D.trail_len = a.trail_len;
     D.lam[0] = a.lam[0];
     D.lam[1] = a.lam[1];
@@ -1889,15 +2432,73 @@ int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots) {
   // the previous descriptors may still be read by queued kernels
   HIP_TRY(hipDeviceSynchronize());
   if (!p->dadj) HIP_TRY(hipMalloc(&p->dadj, sizeof(DAdj) * nshots));
+  p->gmask = grad_mask;
+  p->adj_flags = flags;
+  drop_adjoint_graphs(p);  // they bake in the options
+  // FP32 staging of the gradient, one per distinct grad array (fused kernel)
+  for (float* g : p->g32) (void)hipFree(g);
+  p->g32.clear();
+  if (p->dflush) { (void)hipFree(p->dflush); p->dflush = nullptr; }
+  p->nflush = 0;
+  if (p->prec == SDT_FP32 && !(flags & SDT_ADJ_REFERENCE)) {
+    std::vector<DFlush> fl;
+    for (int s = 0; s < nshots; ++s) {
+      const sdt_model& M = p->hmodels[p->hshots[s].model];
+      const long nel = (long)M.nx * M.nz;
+      int f = 0;
+      while (f < (int)fl.size() && fl[f].grad != h[s].grad) ++f;
+      if (f == (int)fl.size()) {
+        float* g = nullptr;
+        HIP_TRY(hipMalloc(&g, sizeof(float) * 12 * nel));
+        HIP_TRY(hipMemset(g, 0, sizeof(float) * 12 * nel));
+        p->g32.push_back(g);
+        fl.push_back(DFlush{h[s].grad, g, nel});
+      } else if (fl[f].n != nel) {
+        return fail(SDT_EINVAL, "shots sharing a grad array must have grids of one size");
+      }
+      h[s].g32 = fl[f].g32;
+    }
+    HIP_TRY(hipMalloc(&p->dflush, sizeof(DFlush) * fl.size()));
+    HIP_TRY(hipMemcpy(p->dflush, fl.data(), sizeof(DFlush) * fl.size(), hipMemcpyHostToDevice));
+    p->nflush = (int)fl.size();
+  }
   HIP_TRY(hipMemcpy(p->dadj, h.data(), sizeof(DAdj) * nshots, hipMemcpyHostToDevice));
   p->hadj = h;
+  if (!p->dtbase) HIP_TRY(hipMalloc(&p->dtbase, sizeof(int)));
+  if (!p->dtsel_trail) HIP_TRY(hipMalloc(&p->dtsel_trail, sizeof(int)));
+  if (!p->dtsel_adj) HIP_TRY(hipMalloc(&p->dtsel_adj, sizeof(int)));
+  if (!p->dadjwork) {
+    // (shot, x0, z0) items of AWB * AW columns x AH rows; the items of one
+    // tile over all shots of its model share blockIdx % 8 (one XCD's L2)
+    std::vector<int4> work;
+    for (int m = 0; m < p->nmodels; ++m) {
+      const sdt_model& M = p->hmodels[m];
+      const int bw = AW * AWB;
+      const int tx = (M.nx + bw - 1) / bw, tz = (M.nz + AH - 1) / AH, ntile = tx * tz;
+      std::vector<int> ms;
+      for (int s = 0; s < p->nshots; ++s)
+        if (p->hshots[s].model == m) ms.push_back(s);
+      if (ms.empty()) continue;
+      for (int tg = 0; tg < (ntile + 7) / 8; ++tg)
+        for (int s : ms)
+          for (int tt = 0; tt < 8; ++tt) {
+            const int t = tg * 8 + tt;
+            work.push_back(t < ntile ? make_int4(s, (t % tx) * bw, (t / tx) * AH, 0)
+                                     : make_int4(-1, 0, 0, 0));
+          }
+    }
+    HIP_TRY(hipMalloc(&p->dadjwork, sizeof(int4) * work.size()));
+    HIP_TRY(hipMemcpy(p->dadjwork, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice));
+    p->nadjwork = (int)work.size();
+  }
   return SDT_OK;
 }
 
-int sdt_plan_run_trail(void* plan, int it0, int nsteps, void* stream) {
+int sdt_plan_run_trail(void* plan, int it0, int nsteps, int buf, void* stream) {
   Plan* p = static_cast<Plan*>(plan);
   if (!p) return fail(SDT_EINVAL, "plan is null");
   if (!p->dadj) return fail(SDT_EINVAL, "call sdt_adjoint_setup first");
+  if (buf < 0 || buf > 1 || !p->hadj[0].trail[buf]) return fail(SDT_EINVAL, "no such trail buffer");
   if (it0 < 0 || nsteps < 0 || (long long)it0 + nsteps > p->nt)
     return fail(SDT_EINVAL, "step range outside [0, nt)");
   for (const DAdj& a : p->hadj)
@@ -1906,18 +2507,22 @@ int sdt_plan_run_trail(void* plan, int it0, int nsteps, void* stream) {
   p->last = st;
   if (nsteps == 0) return SDT_OK;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, p->dbase, it0);
-  if (p->prec == SDT_FP64)
-    launch_trail_steps<double>(p, st, it0, nsteps);
-  else
-    launch_trail_steps<float>(p, st, it0, nsteps);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, p->dtsel_trail, buf);
+  if (nsteps < 2) {
+    launch_trail(p, st, nsteps);
+  } else {
+    const int rc = replay_segment(p, 0, nsteps, st, [&](hipStream_t c) { launch_trail(p, c, nsteps); });
+    if (rc) return rc;
+  }
   HIP_TRY(hipGetLastError());
   return SDT_OK;
 }
 
-int sdt_adjoint_run(void* plan, int t_hi, int t_lo, int trail_t0, void* stream) {
+int sdt_adjoint_run(void* plan, int t_hi, int t_lo, int trail_t0, int buf, void* stream) {
   Plan* p = static_cast<Plan*>(plan);
   if (!p) return fail(SDT_EINVAL, "plan is null");
   if (!p->dadj) return fail(SDT_EINVAL, "call sdt_adjoint_setup first");
+  if (buf < 0 || buf > 1 || !p->hadj[0].trail[buf]) return fail(SDT_EINVAL, "no such trail buffer");
   if (t_lo < 0 || t_hi < t_lo || t_hi > p->nt || trail_t0 < 0 || trail_t0 > t_lo)
     return fail(SDT_EINVAL, "need 0 <= trail_t0 <= t_lo <= t_hi <= nt");
   for (const DAdj& a : p->hadj)
@@ -1925,10 +2530,17 @@ int sdt_adjoint_run(void* plan, int t_hi, int t_lo, int trail_t0, void* stream) 
       return fail(SDT_EINVAL, "trail does not hold the states up to t_hi");
   hipStream_t st = static_cast<hipStream_t>(stream);
   p->last = st;
-  if (p->prec == SDT_FP64)
-    launch_adjoint_steps<double>(p, st, t_hi, t_lo, trail_t0);
-  else
-    launch_adjoint_steps<float>(p, st, t_hi, t_lo, trail_t0);
+  if (t_hi == t_lo) return SDT_OK;
+  const int khi = t_hi - trail_t0, klo = t_lo - trail_t0;
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, p->dtbase, trail_t0);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, p->dtsel_adj, buf);
+  if (khi - klo < 2) {
+    launch_adjoint(p, st, khi, klo);
+  } else {
+    const long long key = (long long)khi << 32 | (unsigned)klo;
+    const int rc = replay_segment(p, 1, key, st, [&](hipStream_t c) { launch_adjoint(p, c, khi, klo); });
+    if (rc) return rc;
+  }
   HIP_TRY(hipGetLastError());
   return SDT_OK;
 }
@@ -1936,6 +2548,7 @@ int sdt_adjoint_run(void* plan, int t_hi, int t_lo, int trail_t0, void* stream) 
 void sdt_plan_destroy(void* plan) {
   Plan* p = static_cast<Plan*>(plan);
   if (!p) return;
+  (void)hipDeviceSynchronize();  // nothing of the plan may still be in flight
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->cap) (void)hipStreamDestroy(p->cap);
@@ -1947,6 +2560,13 @@ void sdt_plan_destroy(void* plan) {
   if (p->dnel) (void)hipFree(p->dnel);
   if (p->dnpsi) (void)hipFree(p->dnpsi);
   if (p->dadj) (void)hipFree(p->dadj);
+  if (p->dadjwork) (void)hipFree(p->dadjwork);
+  if (p->dtbase) (void)hipFree(p->dtbase);
+  if (p->dtsel_trail) (void)hipFree(p->dtsel_trail);
+  if (p->dtsel_adj) (void)hipFree(p->dtsel_adj);
+  drop_adjoint_graphs(p);
+  for (float* g : p->g32) (void)hipFree(g);
+  if (p->dflush) (void)hipFree(p->dflush);
   delete p;
 }
 

@@ -104,6 +104,11 @@ class Problem:
     shots: list
     gamma: dict = field(default_factory=dict)
     init: list | None = None
+    # elastic2 only: run the split C-PML variant (own velocity-derivative
+    # memory arrays for the shear loop, b_x_half(i) in the normal loop)
+    # instead of the legacy shared arrays and b_x_half(j).  Not in the
+    # reference; stable in long runs where the legacy C-PML grows.
+    cpml_split: bool = False
 
     @property
     def is3d(self) -> bool:

@@ -49,7 +49,14 @@ class Simulation:
             raise RuntimeError("no GPU visible: the HIP path needs a ROCm device")
         _hip.lib()  # fail early if the library is not built
         self.kind = problems[0].kind
-        self.kid = KINDS[self.kind]
+        splits = {bool(getattr(p, "cpml_split", False)) for p in problems}
+        if len(splits) != 1:
+            raise ValueError("all problems of a Simulation must agree on cpml_split")
+        self.split = splits.pop()
+        if self.split and self.kind != "elastic2":
+            raise ValueError("cpml_split applies to elastic2 only")
+        # kind as the C ABI takes it (with the split C-PML flag)
+        self.kid = KINDS[self.kind] | (_hip.SDT_SPLIT_CPML if self.split else 0)
         self.prec, self.dtype = PRECISIONS[precision]
         self.precision = precision
         self.device = torch.device(device)

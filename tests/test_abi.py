@@ -40,6 +40,7 @@ def test_struct_layout_matches_header():
     (_hip.SDT_EM2, (100, 1, 80, 10, 0, 12), 2 * 80 * 20 + 2 * 24 * 100),
     (_hip.SDT_ELASTIC25, (30, 20, 40, 5, 4, 6), 6 * (40 * 20 * 10 + 40 * 8 * 30 + 12 * 20 * 30)),
     (_hip.SDT_EM25, (30, 20, 40, 5, 4, 6), 4 * (40 * 20 * 10 + 40 * 8 * 30 + 12 * 20 * 30)),
+    (_hip.SDT_ELASTIC2 | _hip.SDT_SPLIT_CPML, (100, 1, 80, 10, 0, 12), 6 * 80 * 20 + 6 * 24 * 100),
 ])
 def test_psi_elems(kind, dims, expect):
     assert _hip.psi_elems(kind, *dims) == expect
@@ -48,6 +49,28 @@ def test_psi_elems(kind, dims, expect):
 def test_psi_elems_bad_kind():
     with pytest.raises(_hip.SeidartHipError, match="unknown kind"):
         _hip.psi_elems(9, 10, 1, 10, 1, 0, 1)
+    with pytest.raises(_hip.SeidartHipError, match="unknown kind"):
+        _hip.psi_elems(_hip.SDT_EM2 | _hip.SDT_SPLIT_CPML, 10, 1, 10, 1, 0, 1)
+
+
+def test_state_and_adjoint_sizes():
+    dims = (100, 1, 80, 10, 0, 12)
+    for k in (_hip.SDT_ELASTIC2, _hip.SDT_ELASTIC2 | _hip.SDT_SPLIT_CPML):
+        assert _hip.state_elems(k, *dims) == 5 * 8000 + _hip.psi_elems(k, *dims)
+        assert _hip.adjoint_work_elems(k, *dims) == 12 * 8000
+    assert _hip.state_elems(_hip.SDT_EM25, 30, 20, 40, 5, 4, 6) == \
+        6 * 30 * 20 * 40 + _hip.psi_elems(_hip.SDT_EM25, 30, 20, 40, 5, 4, 6)
+    with pytest.raises(_hip.SeidartHipError, match="ELASTIC2 only"):
+        _hip.adjoint_work_elems(_hip.SDT_EM2, *dims)
+    # trail[2], trail_len (+pad), lam[2], work, resid, grad, gsrc
+    assert C.sizeof(_hip.SdtAdjShot) == 16 + 8 + 16 + 8 * 4
+
+
+def test_adjoint_calls_reject_null_plan():
+    L = _hip.lib()
+    assert L.sdt_adjoint_setup(None, None, 0, 0xfff, 0) == _hip.SDT_EINVAL
+    assert L.sdt_plan_run_trail(None, 0, 1, 0, None) == _hip.SDT_EINVAL
+    assert L.sdt_adjoint_run(None, 1, 0, 0, 0, None) == _hip.SDT_EINVAL
 
 
 def _model(nx=20, nz=20):

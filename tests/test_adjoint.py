@@ -25,7 +25,8 @@ PHYS = ("c11", "c13", "c15", "c33", "c35", "c55", "rho")
 
 
 def _torch_ref_args(p, shot, params=None):
-    """Arguments of oracle.torch_elastic2.run_elastic2 for one shot."""
+    """Arguments of oracle.torch_elastic2.run_elastic2 for one shot (and the
+    C-PML variant of the problem)."""
     T = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64))
     prm = params if params is not None else {k: T(p.coef[k]) for k in PHYS}
     if p.gamma and params is None:
@@ -39,7 +40,7 @@ def _torch_ref_args(p, shot, params=None):
     force = tuple(None if f is None else T(f) for f in (sh.force[0], sh.force[2]))
     rec = torch.as_tensor(np.asarray(sh.rec)[:, [0, 2]])
     return dict(prm=prm, prof=prof, dx=p.dx, dz=p.dz, dt=p.dt, nt=p.nt, src=(sh.src[0], sh.src[2]),
-                force=force, rec=rec, brow=brow)
+                force=force, rec=rec, brow=brow, split=bool(getattr(p, "cpml_split", False)))
 
 
 def test_torch_reference_matches_oracle(oracle_lib):
@@ -61,6 +62,19 @@ def test_torch_reference_matches_oracle_no_attenuation(oracle_lib):
     assert rel_l2(tr, o.trace) < 1e-13
 
 
+def test_torch_reference_matches_oracle_split(oracle_lib):
+    from torch_elastic2 import run_elastic2
+    p = tiny_problem("elastic2", seed=44, nx=42, nz=38, npml=6, nt=70, gamma=1e-3, nrec=6)
+    p.cpml_split = True
+    with torch.no_grad():
+        tr = run_elastic2(**_torch_ref_args(p, 0)).numpy()
+    o = oracle_lib.OracleRun(p, shot=0).run()
+    assert rel_l2(tr, o.trace) < 1e-13
+    # the variant really differs from the legacy scheme in the C-PML
+    p.cpml_split = False
+    assert rel_l2(oracle_lib.OracleRun(p, shot=0).run().trace, o.trace) > 1e-8
+
+
 @pytest.mark.parametrize("dm", ["legacy", "geometric", "harmonic", "none"])
 def test_coefs_torch_matches_numpy(dm):
     p = tiny_problem("elastic2", seed=43, nx=30, nz=26, npml=4, nt=10, gamma=1e-3)
@@ -77,19 +91,21 @@ def test_coefs_torch_matches_numpy(dm):
 # ---------------------------------------------------------------------------
 # GPU
 # ---------------------------------------------------------------------------
-def _engine(p, precision="fp64", every=16):
+def _engine(p, precision="fp64", every=16, **kw):
     from seidart_amd.adjoint import Elastic2Engine
-    return Elastic2Engine(p, precision=precision, every=every)
+    return Elastic2Engine(p, precision=precision, every=every, **kw)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("every", [16, 25])
-def test_dot_product(every):
+@pytest.mark.parametrize("every,split,overlap", [(16, False, True), (25, False, True),
+                                                 (16, True, True), (16, False, False)])
+def test_dot_product(every, split, overlap):
     """<A x, y> = <x, A^T y> for A: initial state -> traces (no source)."""
     p = tiny_problem("elastic2", seed=51, nx=52, nz=44, npml=7, nt=110, gamma=1e-3, nshots=2, nrec=9)
+    p.cpml_split = split
     for sh in p.shots:
         sh.force = (np.zeros(p.nt), None, np.zeros(p.nt))
-    eng = _engine(p, "fp64", every=every)
+    eng = _engine(p, "fp64", every=every, overlap=overlap)
     g = torch.Generator(device="cpu").manual_seed(5)
     xs = []
     for s in range(eng.nshots):
@@ -106,14 +122,57 @@ def test_dot_product(every):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision,split", [("fp64", False), ("fp64", True), ("fp32", False)])
+def test_fused_adjoint_matches_reference_kernels(precision, split):
+    """The fused single-pass adjoint kernel against the three-kernel one:
+    same lambda_0 and gradients up to rounding (sums are reassociated),
+    on several models of uneven size with several shots each."""
+    probs = [tiny_problem("elastic2", seed=80 + m, nt=70, gamma=1e-3, nshots=2, nrec=7, **d)
+             for m, d in enumerate([dict(nx=300, nz=75, npml=6), dict(nx=57, nz=45, npml=5)])]
+    for p in probs:
+        p.cpml_split = split
+    out = []
+    for ref in (False, True):
+        eng = _engine(probs, precision, every=24, reference=ref)
+        g = torch.Generator().manual_seed(9)
+        for s in range(eng.nshots):
+            eng.init[s].copy_(torch.randn(eng.init[s].numel(), generator=g, dtype=torch.float64))
+        tr = eng.forward()
+        ys = [torch.randn(t.shape, generator=g, dtype=torch.float64) for t in tr]
+        grad, gsrc = eng.backward(ys)
+        out.append(([x.cpu().numpy() for x in grad], [x.cpu().numpy() for x in gsrc],
+                    [eng.adjoint_state(s).double().cpu().numpy() for s in range(eng.nshots)]))
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    (g0, s0, l0), (g1, s1, l1) = out
+    for a, b in zip(g0, g1):
+        for q in range(12):
+            assert rel_l2(a[q], b[q]) < tol, ("grad slot", q, rel_l2(a[q], b[q]))
+    for a, b in zip(s0, s1):
+        assert rel_l2(a, b) < tol
+    for a, b in zip(l0, l1):
+        assert rel_l2(a, b) < tol
+
+
+@pytest.mark.gpu
+def test_grad_mask_limits_accumulation():
+    p = tiny_problem("elastic2", seed=85, nx=48, nz=40, npml=6, nt=40, nrec=5)
+    eng = _engine(p, "fp64", every=20, grad_mask=(1 << 0) | (1 << 8))
+    tr = eng.forward()
+    grad, _ = eng.backward([torch.ones_like(t) for t in tr])
+    g = grad[0].cpu().numpy()
+    assert np.abs(g[0]).max() > 0 and np.abs(g[8]).max() > 0
+    assert all(np.abs(g[q]).max() == 0 for q in range(12) if q not in (0, 8))
+
+
+@pytest.mark.gpu
 def test_trail_recompute_is_bit_exact():
     """The recomputed states equal the checkpoints of the forward run."""
     p = tiny_problem("elastic2", seed=52, nx=48, nz=40, npml=6, nt=64, gamma=1e-3, nrec=4)
     eng = _engine(p, "fp32", every=16)
     eng.forward()
     eng.backward([torch.zeros_like(eng.resid[0])])
-    # after the backward pass the trail holds segment 0's recomputed states
-    assert torch.equal(eng.trail[0][16], eng.ckpt[0][1])
+    # after the backward pass trail buffer 0 holds segment 0's recomputed states
+    assert torch.equal(eng.trail[0][0][16], eng.ckpt[0][1])
     assert torch.count_nonzero(eng.ckpt[0][1]) > 0
 
 
@@ -154,6 +213,7 @@ def _observed(p, seed=7):
     """Synthetic 'observed' data: the oracle traces of a perturbed model."""
     from pyoracle import OracleRun
     q = tiny_problem("elastic2", seed=p._seed, **p._kw)
+    q.cpml_split = p.cpml_split
     rng = np.random.default_rng(seed)
     for k in ("c11", "c33", "c55", "rho"):
         q.coef[k] = q.coef[k] * (1 + 0.03 * rng.standard_normal(q.coef[k].shape).clip(-2, 2))
@@ -165,16 +225,18 @@ def _observed(p, seed=7):
     return torch.as_tensor(d)
 
 
-def _grad_problem():
+def _grad_problem(split=False):
     kw = dict(nx=64, nz=50, npml=8, nt=160, gamma=1e-3, nshots=2, nrec=10)
     p = tiny_problem("elastic2", seed=61, **kw)
     p._seed, p._kw = 61, kw
+    p.cpml_split = split
     return p
 
 
 @pytest.mark.gpu
-def test_gradient_matches_torch_autograd_fp64():
-    p = _grad_problem()
+@pytest.mark.parametrize("split", [False, True])
+def test_gradient_matches_torch_autograd_fp64(split):
+    p = _grad_problem(split)
     d = _observed(p)
     eng = _engine(p, "fp64", every=32)
     Jg, gg = _gpu_grads(p, eng, d)

@@ -128,3 +128,23 @@ def test_receivers_on_edges_and_source_on_boundary(oracle_lib):
     p.shots[0].src = (1, 0, 1)
     sim = _sim(p).run().check()
     _compare(sim, p, oracle_lib, FP64_TOL)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_split_cpml_variant_matches_oracle(oracle_lib, precision):
+    """The split C-PML variant (SDT_SPLIT_CPML) against the oracle's split
+    variant, several shots and models in one plan."""
+    probs = [tiny_problem("elastic2", seed=70 + m, nt=90, gamma=1e-3, nshots=2, **d)
+             for m, d in enumerate([dict(nx=48, nz=40, npml=6), dict(nx=61, nz=37, npml=5)])]
+    for p in probs:
+        p.cpml_split = True
+    sim = _sim(probs, precision).run(graph_block=16).check()
+    if precision == "fp64":
+        off = 0
+        for p in probs:
+            _compare(sim, p, oracle_lib, FP64_TOL, shot_offset=off)
+            off += len(p.shots)
+    else:
+        tr = sim.traces()
+        o = oracle_lib.OracleRun(probs[0], shot=0).run()
+        assert rel_l2(tr[0], o.trace) < FP32_TOL
