@@ -9,7 +9,12 @@ all shots advanced together in the same kernel launches, hipGraph blocks of
     each model, for the FP32-batched / FP64-plain speed ratio and the FP32
     trace error against FP64;
   * the oracle (FP64 C restatement of the legacy loops, OpenMP) on a bounded
-    sample of the same workload on the host cores (cpu_baseline).
+    sample of the same workload on the host cores (cpu_baseline);
+  * the FWI gradient workload (fwi_vti: 2-D VTI elastic 1024x512 + C-PML 20,
+    16 shots, 6000 steps, L2 trace misfit, gradients for C11 C13 C33 C55 rho,
+    FP32 with FP64 accumulation, checkpoints every 64 steps): seconds per
+    shot of one full gradient (forward with checkpoints + adjoint with
+    recomputation) and its ratio to a plain forward of the same shots.
 
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -91,6 +96,108 @@ def cpu_baseline(problems, budget_s):
             "value_1core": round(cells * n1 / el1 / 1e9, 4)}
 
 
+FWI_PARAMS = ("c11", "c13", "c33", "c55", "rho")
+
+
+def fwi_gradient(nt: int = 6000, nshots: int = 16, every: int = 64, fd_cells: int = 0):
+    """Time one full FWI gradient of the fwi_vti workload (FP32, FP64
+    accumulation) against a plain FP32 forward of the same shots; optionally
+    check it against central differences of the FP64 forward at fd_cells
+    seeded cells per parameter (relative step 1e-4)."""
+    import torch
+    from seidart_amd.adjoint import Elastic2Engine, elastic2_traces
+    from seidart_amd.configs import fwi_vti
+    from seidart_amd.solver import Simulation
+
+    true, start = fwi_vti(nt=nt, nshots=nshots)
+    sim = Simulation(true, precision="fp32")
+    sim.run(graph_block=every).check()
+    obs = torch.as_tensor(np.stack(sim.traces()), device="cuda")
+    sim.close()
+    sim = Simulation(start, precision="fp32")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.run(graph_block=every)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    sim.check()
+    sim.close()
+    del sim
+    eng = Elastic2Engine(start, "fp32", every=every)
+    prm = {k: torch.tensor(start.coef[k], device="cuda", requires_grad=k in FWI_PARAMS)
+           for k in ("c11", "c13", "c15", "c33", "c35", "c55", "rho")}
+
+    def gradient():
+        for v in prm.values():
+            v.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tr = elastic2_traces(eng, [prm])
+        J = 0.5 * ((tr - obs) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        J.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return float(J.detach()), t1 - t0, t2 - t1
+
+    gradient()  # captures the segment graphs
+    J, t_fwd, t_bwd = gradient()
+    mem = sum(x.numel() * x.element_size() for x in eng.ckpt + eng.trail)
+    out = {"workload": f"fwi_vti: 2-D VTI elastic {start.nx}x{start.nz} padded, {nshots} shots, "
+                       f"{nt} steps, 256 receivers, L2 misfit, grad of C11 C13 C33 C55 rho, "
+                       f"checkpoint every {every} steps, split C-PML",
+           "s_per_shot": round((t_fwd + t_bwd) / nshots, 4), "gradient_s": round(t_fwd + t_bwd, 3),
+           "forward_with_checkpoints_s": round(t_fwd, 3), "adjoint_s": round(t_bwd, 3),
+           "plain_forward_s": round(t_plain, 3),
+           "gradient_over_forward": round((t_fwd + t_bwd) / t_plain, 3),
+           "misfit": J, "dtype": "f32 (gradient accumulated in f64)",
+           "checkpoint_and_trail_GB": round(mem / 1e9, 2)}
+    g32 = {k: prm[k].grad.double().cpu().numpy() for k in FWI_PARAMS}
+    eng.close()
+    del eng
+    torch.cuda.empty_cache()
+    if fd_cells > 0:
+        # FD of the FP64 forward; gradients of the FP64 and FP32 adjoints
+        eng = Elastic2Engine(start, "fp64", every=every)
+        prm64 = {k: v.detach().double().clone().requires_grad_(k in FWI_PARAMS) for k, v in prm.items()}
+        tr = elastic2_traces(eng, [prm64])
+        (0.5 * ((tr - obs.double()) ** 2).sum()).backward()
+        g64 = {k: prm64[k].grad.cpu().numpy() for k in FWI_PARAMS}
+        base = {k: v.detach() for k, v in prm64.items()}
+
+        def Jof(p):
+            with torch.no_grad():
+                return float(0.5 * ((elastic2_traces(eng, [p]) - obs.double()) ** 2).sum())
+
+        rng = np.random.default_rng(2025)
+        fd = {}
+        for k in FWI_PARAMS:
+            g = g64[k]
+            cand = np.argwhere(np.abs(g) >= 0.05 * np.abs(g).max())
+            cells = cand[rng.choice(len(cand), size=min(fd_cells, len(cand)), replace=False)]
+            e64, e32 = [], []
+            for j, i in cells:
+                h = 1e-4 * abs(float(start.coef[k][j, i]))
+                up = dict(base)
+                dn = dict(base)
+                up[k] = base[k].clone()
+                dn[k] = base[k].clone()
+                up[k][j, i] += h
+                dn[k][j, i] -= h
+                d = (Jof(up) - Jof(dn)) / (2 * h)
+                e64.append(abs(g[j, i] - d) / abs(d))
+                e32.append(abs(g32[k][j, i] - d) / abs(d))
+            fd[k] = {"cells": len(cells), "max_rel_err_fp64_adjoint": float(f"{max(e64):.3e}"),
+                     "max_rel_err_fp32_adjoint": float(f"{max(e32):.3e}")}
+            print(f"fd check {k}: {fd[k]}", file=sys.stderr, flush=True)
+        out["fd_check"] = {"step": "1e-4 relative, central, FP64 forward",
+                           "cells": "seeded among those with |grad| >= 5% of the field's max",
+                           "per_param": fd}
+        eng.close()
+    return out
+
+
 def copy_calibration(gib: float) -> int:
     """Time torch's device-to-device copy of `gib` GiB (read gib + write gib):
     the achievable-bandwidth reference and the known-size kernel the HBM
@@ -124,6 +231,11 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fwi", action="store_true")
+    ap.add_argument("--fwi-steps", type=int, default=6000)
+    ap.add_argument("--fwi-fd", type=int, default=0,
+                    help="also check the FWI gradient against central differences at this "
+                         "many seeded cells per parameter (slow)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="only time a plain device copy of this many GiB (HBM calibration)")
     a = ap.parse_args(argv)
@@ -207,6 +319,7 @@ def main(argv=None):
                  "speedup_fp32_batched": round(value / pv, 3),
                  "fp32_trace_rel_l2_vs_fp64": [float(f"{e:.3e}") for e in errs]}
 
+    fwi = None if a.no_fwi else fwi_gradient(nt=a.fwi_steps, fd_cells=a.fwi_fd)
     cpu = None if a.no_cpu else cpu_baseline(problems, a.cpu_seconds)
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s",
@@ -222,6 +335,7 @@ def main(argv=None):
         "gpu_ms_events": round(gpu_ms, 3),
         "roofline": roofline,
         "fp64_plain": plain,
+        "fwi_grad": fwi,
         "cpu_baseline": cpu,
         "host_setup_s": round(setup_s, 2),
     }

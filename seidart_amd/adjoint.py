@@ -54,13 +54,17 @@ class Elastic2Engine:
 
     def __init__(self, problems, precision: str = "fp32", every: int = 64,
                  device: str = "cuda:0", graph: bool = True, grad_mask: int = ALL_SLOTS,
-                 reference: bool = False, overlap: bool = True):
+                 reference: bool = False, overlap: bool = True, all_slots: bool = False):
         """``grad_mask``: bit q = accumulate coefficient slot q.  ``reference``:
         run the unfused three-kernel adjoint (a cross-check of the fused
         kernel; same results to rounding).  ``overlap``: two trail buffers,
         the recomputation of the next segment running on a second stream
-        while the adjoint walks the current one."""
-        self.sim = Simulation(problems, precision, device, all_slots=True)
+        while the adjoint walks the current one.  ``all_slots``: keep the
+        optional coefficient slots (c15 / c35 couplings, attenuation) on the
+        device even where the problems leave them trivial, so that later
+        coefficients may make them non-trivial; otherwise the kernels skip
+        them (their gradients are still available)."""
+        self.sim = Simulation(problems, precision, device, all_slots=all_slots, all_sources=True)
         if self.sim.kind != "elastic2":
             raise ValueError("the adjoint engine handles elastic2 problems only")
         if every < 1:
@@ -127,9 +131,17 @@ class Elastic2Engine:
 
     def set_coefs(self, model: int, slots) -> None:
         """Overwrite model ``model``'s 12 coefficient slots (FP64 tensors or
-        arrays of shape (nz, nx))."""
+        arrays of shape (nz, nx)).  Slots the device skips must stay trivial
+        (0 for the couplings, 1 for 1 - gamma)."""
         for q, v in enumerate(slots):
-            self.sim.coef_t[model][q].copy_(torch.as_tensor(v).to(self.device, self.dtype))
+            dst = self.sim.coef_t[model][q]
+            v = torch.as_tensor(v)
+            if dst is None:
+                if not bool(torch.all(v == (1.0 if q >= 10 else 0.0))):
+                    raise ValueError(f"coefficient slot {q} is not on the device (trivial in the "
+                                     "problem); build the engine with all_slots=True")
+                continue
+            dst.copy_(v.to(self.device, self.dtype))
 
     def set_increments(self, shot: int, inc) -> None:
         """Overwrite shot ``shot``'s per-step source increments: ``inc`` of

@@ -32,10 +32,12 @@ class Simulation:
     """All shots of one or more Problems (same kind) on one GPU."""
 
     def __init__(self, problems, precision: str = "fp64", device: str = "cuda:0",
-                 all_slots: bool = False):
-        """``all_slots``: pass every coefficient slot and both source
-        components to the device even where they are zero (needed when the
-        coefficients are later overwritten, e.g. by the adjoint engine)."""
+                 all_slots: bool = False, all_sources: bool = False):
+        """``all_slots``: pass every coefficient slot to the device even where
+        the optional ones are trivial (zero c15 / c35 couplings, no
+        attenuation), so they can be overwritten later.  ``all_sources``:
+        allocate the x and z source series even where a component is
+        absent (the adjoint engine writes them)."""
         if isinstance(problems, Problem):
             problems = [problems]
         if not problems:
@@ -138,7 +140,7 @@ class Simulation:
                             bufs[n] = torch.zeros(p.field_shape, dtype=self.dtype, device=self.device)
                     bufs["psi"] = torch.zeros(max(npsi, 1), dtype=self.dtype, device=self.device)
                 inc = source_increments(p, sh)
-                if all_slots and self.kind == "elastic2":
+                if all_sources and self.kind == "elastic2":
                     inc = [np.zeros(p.nt) if (v is None and c != 1) else v for c, v in enumerate(inc)]
                 bufs["src"] = [None if v is None else T(np.pad(v, (0, self.nt - v.size))) for v in inc]
                 rec = np.asarray(sh.rec, dtype=np.int32).reshape(-1, 3)

@@ -154,6 +154,28 @@ def test_fused_adjoint_matches_reference_kernels(precision, split):
 
 
 @pytest.mark.gpu
+def test_skipped_trivial_slots_give_same_gradients():
+    """VTI without attenuation: the engine leaves the c15 / c35 and gamma
+    slots off the device, yet its gradients (including those slots') equal
+    the ones of an engine that keeps every slot."""
+    p = tiny_problem("elastic2", seed=86, nx=50, nz=42, npml=6, nt=80, nrec=6, full21=False)
+    for k in ("c15", "c35"):
+        p.coef[k] = np.zeros_like(p.coef[k])
+    res = []
+    for keep in (False, True):
+        eng = _engine(p, "fp64", every=32, all_slots=keep)
+        assert (eng.sim.coef_t[0][2] is None) == (not keep)
+        tr = eng.forward()
+        g, _ = eng.backward([torch.ones_like(t) for t in tr])
+        res.append(g[0].cpu().numpy())
+    for q in range(12):
+        assert rel_l2(res[0][q], res[1][q]) < 1e-13, q
+    with pytest.raises(ValueError, match="all_slots"):
+        eng0 = _engine(p, "fp64", every=32)
+        eng0.set_coefs(0, [np.ones((p.nz, p.nx))] * 12)
+
+
+@pytest.mark.gpu
 def test_grad_mask_limits_accumulation():
     p = tiny_problem("elastic2", seed=85, nx=48, nz=40, npml=6, nt=40, nrec=5)
     eng = _engine(p, "fp64", every=20, grad_mask=(1 << 0) | (1 << 8))
