@@ -115,6 +115,7 @@ def fwi_gradient(nt: int = 6000, nshots: int = 16, every: int = 64, fd_cells: in
     obs = torch.as_tensor(np.stack(sim.traces()), device="cuda")
     sim.close()
     sim = Simulation(start, precision="fp32")
+    sim.autotune(steps=10)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sim.run(graph_block=every)
@@ -124,6 +125,7 @@ def fwi_gradient(nt: int = 6000, nshots: int = 16, every: int = 64, fd_cells: in
     sim.close()
     del sim
     eng = Elastic2Engine(start, "fp32", every=every)
+    eng.sim.autotune(steps=10)
     prm = {k: torch.tensor(start.coef[k], device="cuda", requires_grad=k in FWI_PARAMS)
            for k in ("c11", "c13", "c15", "c33", "c35", "c55", "rho")}
 
@@ -198,6 +200,88 @@ def fwi_gradient(nt: int = 6000, nshots: int = 16, every: int = 64, fd_cells: in
     return out
 
 
+def alg_bytes_per_step(problems, itemsize):
+    """Compulsory HBM bytes of one time step of a plan over these problems,
+    for any kind: per shot every field and every slab memory array read and
+    written once; per model every coefficient array it passes to the device
+    read once (2.5-D coefficients are y-invariant planes)."""
+    from seidart_amd.problem import FIELDS, kernel_coefs
+    from seidart_amd import _hip
+    from seidart_amd.problem import KINDS
+    tot = 0
+    for p in problems:
+        lx, ly, lz = p.slabs()
+        kid = KINDS[p.kind] | (_hip.SDT_SPLIT_CPML if getattr(p, "cpml_split", False) else 0)
+        npsi = _hip.psi_elems(kid, p.nx, p.ny, p.nz, lx, ly, lz)
+        n = p.nx * p.ny * p.nz
+        K = kernel_coefs(p)
+        ncoef = len(K)
+        if p.kind == "elastic2":
+            ncoef -= (4 if all(not np.any(K[q]) for q in (2, 4, 5, 6)) else 0) + \
+                (2 if all(np.all(K[q] == 1.0) for q in (10, 11)) else 0)
+        tot += len(p.shots) * (2 * len(FIELDS[p.kind]) * n + 2 * npsi) + ncoef * p.nx * p.nz
+    return tot * itemsize
+
+
+def config_lines(steps: int = 400):
+    """Throughput of the other configurations at one GPU: bounded runs of
+    `steps` steps (after 50 warmup steps; plain_psv runs its full 2000), FP32
+    and FP64, each with its algorithmic bytes per cell-update, the fraction of
+    the HBM peak and the FP32 trace error against the FP64 run."""
+    import torch
+    from seidart_amd import configs
+    from seidart_amd.solver import Simulation
+    out = {}
+    cases = [("plain_psv", lambda nt: configs.plain_psv(nt=nt), 2000),
+             ("gpr", lambda nt: configs.gpr(nt=nt), steps),
+             ("aniso25", lambda nt: configs.aniso25(nt=nt), min(steps, 200))]
+    for name, make, K in cases:
+        W = 50
+        p = make(W + K)
+        cells = p.nx * p.ny * p.nz * len(p.shots)
+        rec = {"grid": f"{p.nx}x{p.ny}x{p.nz}" if p.is3d else f"{p.nx}x{p.nz}", "kind": p.kind,
+               "steps": K}
+        traces = {}
+        for prec, size in (("fp32", 4), ("fp64", 8)):
+            sim = Simulation(p, precision=prec)
+            if p.kind == "elastic2":
+                rec.setdefault("tiles", {})[prec] = list(min((r := sim.autotune(steps=10)), key=r.get))
+            sim.run(W, graph_block=50 if W >= 50 else 0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sim.run(K, graph_block=50)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            sim.check()
+            traces[prec] = sim.traces()[0][W:]
+            sim.close()
+            gcs = cells * K / el / 1e9
+            bpc = alg_bytes_per_step([p], size) / cells
+            rec[prec] = {"Gcell_updates_s": round(gcs, 3), "alg_B_per_cell_update": round(bpc, 2),
+                         "alg_GBs": round(gcs * bpc, 1), "frac_hbm": round(gcs * bpc / HBM_PEAK_GBS, 4)}
+        t32, t64 = traces["fp32"], traces["fp64"]
+        rec["fp32_trace_rel_l2_vs_fp64"] = float(f"{np.linalg.norm(t32 - t64) / np.linalg.norm(t64):.3e}")
+        out[name] = rec
+        torch.cuda.empty_cache()
+    return out
+
+
+def measured_traffic(alg_bytes: int):
+    """HBM bytes per launch of the headline kernel measured by rocprofv3 byte
+    counters (profiles/el2_step_batched_layered_fp32_hbm_bytes.md, scaled by
+    the copy calibration there), if that profile was taken on this workload
+    (same algorithmic bytes per launch); else None."""
+    path = os.path.join(ROOT, "profiles", "el2_step_batched_layered_fp32_hbm_bytes.md")
+    try:
+        kv = dict(line.split(":", 1) for line in open(path) if line.count(":") == 1
+                  and line.split(":")[0] in ("traffic_bytes_per_launch", "alg_bytes_per_launch"))
+        if int(kv["alg_bytes_per_launch"]) == int(alg_bytes):
+            return int(kv["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        pass
+    return None
+
+
 def copy_calibration(gib: float) -> int:
     """Time torch's device-to-device copy of `gib` GiB (read gib + write gib):
     the achievable-bandwidth reference and the known-size kernel the HBM
@@ -232,6 +316,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fwi", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--fwi-steps", type=int, default=6000)
     ap.add_argument("--fwi-fd", type=int, default=0,
                     help="also check the FWI gradient against central differences at this "
@@ -260,6 +345,7 @@ def main(argv=None):
     cells_per_step = sum(p.nx * p.nz * s for p, s in zip(problems, shots))
 
     sim = Simulation(problems, precision="fp32")
+    tune = sim.autotune(steps=10)   # tile shape of the step kernel (setup, untimed)
     gb = a.graph_block if W >= a.graph_block else 0
     sim.run(W, graph_block=gb)
     torch.cuda.synchronize()
@@ -281,7 +367,8 @@ def main(argv=None):
     dom_bytes = algorithmic_bytes(problems, shots, 4)
     achieved = dom_bytes / (k_ms[0] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(dom_bytes),
                 "kernel": "k_el2_step<float>", "kernel_ms": round(k_ms[0], 4),
                 "alg_bytes_per_launch": int(dom_bytes),
                 "alg_bytes_per_cell_update": round(dom_bytes / cells_per_step, 2),
@@ -320,6 +407,7 @@ def main(argv=None):
                  "fp32_trace_rel_l2_vs_fp64": [float(f"{e:.3e}") for e in errs]}
 
     fwi = None if a.no_fwi else fwi_gradient(nt=a.fwi_steps, fd_cells=a.fwi_fd)
+    others = None if a.no_configs else config_lines()
     cpu = None if a.no_cpu else cpu_baseline(problems, a.cpu_seconds)
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s",
@@ -331,11 +419,14 @@ def main(argv=None):
                                "(+C-PML 20), dx 0.5 m, 150 Hz, one launch set per step, "
                                "hipGraph per 50 steps",
                    "cells_per_step": cells_per_step, "shots": sum(shots),
-                   "graph_block": a.graph_block},
+                   "graph_block": a.graph_block,
+                   "tiles": {"chosen": list(min(tune, key=tune.get)),
+                             "ms_per_step": {f"{w}x{r}": round(v, 4) for (w, r), v in tune.items()}}},
         "gpu_ms_events": round(gpu_ms, 3),
         "roofline": roofline,
         "fp64_plain": plain,
         "fwi_grad": fwi,
+        "other_configs": others,
         "cpu_baseline": cpu,
         "host_setup_s": round(setup_s, 2),
     }

@@ -159,6 +159,14 @@ int sdt_plan_create(int kind, int precision, const sdt_model* models, int nmodel
  * and replays it; 0 or 1 launches kernels directly. */
 int sdt_plan_run(void* plan, int it0, int nsteps, int graph_block, void* stream);
 
+/* ELASTIC2 tile shape of the step kernel (no counterpart in the reference;
+ * for an autotuner, seidart_amd.solver.Simulation.autotune): a block of
+ * waves_per_block (1, 2 or 4) wavefronts side by side, each owning 61
+ * columns, walks `rows` rows (4..256).  Results do not depend on the shape.
+ * Synchronises the device when the shape changes. */
+int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows);
+int sdt_plan_get_tiles(void* plan, int* waves_per_block, int* rows);
+
 /* Synchronise the plan's last stream and report instability: *first_step
  * gets the first step at which any shot's field exceeded 1e25 in magnitude
  * or became non-finite (seismicfdtd.f95:378-379), or -1.  Returns

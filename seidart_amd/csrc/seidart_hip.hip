@@ -145,7 +145,7 @@ __device__ __forceinline__ bool in_slab(int i, int n, int l) { return i < l || i
 // ELASTIC2: seismic2, legacy/fortran/seismicfdtd.f95:12-385
 //
 // One fused kernel per time step.  A wavefront owns a tile of EW columns x
-// EH rows of one shot and walks it top to bottom.  Lane l holds column
+// eh rows of one shot and walks it top to bottom.  Lane l holds column
 // xo - 1 + l; lanes 1..EW own their column, lane 0 and lanes EW+1, EW+2 are
 // halo.  At row r the wave updates the stresses of row r (both legacy stress
 // loops, sharing the velocity-derivative psi as the legacy does) and then
@@ -175,8 +175,9 @@ __device__ __forceinline__ bool in_slab(int i, int n, int l) { return i < l || i
 #define SDT_EL2_WAVES_PER_BLOCK 4
 #endif
 constexpr int EW = 61;            // owned columns per wavefront
-constexpr int EWB = SDT_EL2_WAVES_PER_BLOCK;  // wavefronts per block (side by side in x)
-constexpr int EH = SDT_EL2_ROWS;  // owned rows per wavefront
+// Tile shape: wavefronts per block (side by side in x; a template
+// parameter, 1, 2 or 4) and rows per tile (run time); sdt_plan_set_tiles
+// changes both, and the autotuner picks them per plan.
 
 template <typename T>
 __device__ __forceinline__ T ld_stream(gcptr<T> p) {
@@ -210,22 +211,22 @@ template <typename T, bool TRAIL, bool SPLIT>
 __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
                                          const DShot* __restrict__ shots, const int4 wi, const int it,
                                          const int lane, const DAdj* __restrict__ adj, const int slot,
-                                         const int tb);
+                                         const int tb, const int eh);
 
-// Work: a block of EWB wavefronts takes one item (shot, xo, zo): wave w owns
-// columns xo + w*EW .. +EW-1 of rows zo .. zo+EH-1, so the block's row
-// accesses are EWB*EW contiguous cells of each array.  The grid is
+// Work: a block of WB wavefronts takes one item (shot, xo, zo): wave w owns
+// columns xo + w*EW .. +EW-1 of rows zo .. zo+eh-1, so the block's row
+// accesses are WB*EW contiguous cells of each array.  The grid is
 // persistent: block b takes items b, b + gridDim.x, ...; gridDim.x is a
 // multiple of 8, so item w always runs on XCD w % 8.
 // TRAIL: read the state of step it from trail slot rel (the run starts at
 // slot 0) and write the new state into the next slot (forward recomputation
 // for the adjoint).
 // SPLIT: the split C-PML variant (see SDT_SPLIT_CPML).
-template <typename T, bool TRAIL, bool SPLIT>
-__global__ void __launch_bounds__(64 * EWB, sizeof(T) == 4 ? SDT_EL2_WAVES_PER_SIMD : 3)
+template <typename T, bool TRAIL, bool SPLIT, int WB>
+__global__ void __launch_bounds__(64 * WB, sizeof(T) == 4 ? SDT_EL2_WAVES_PER_SIMD : 3)
 k_el2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
            const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel,
-           const DAdj* __restrict__ adj, const int* __restrict__ tsel) {
+           const DAdj* __restrict__ adj, const int* __restrict__ tsel, int eh) {
   // wave-uniform values kept in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int it = __builtin_amdgcn_readfirstlane(*base + rel);
@@ -236,7 +237,7 @@ k_el2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
                               __builtin_amdgcn_readfirstlane(wv.z), 0);
     if (wi.x >= 0 && wi.y < models[shots[wi.x].model].nx)
       el2_tile<T, TRAIL, SPLIT>(models, shots, wi, it, threadIdx.x & 63, adj, rel,
-                                TRAIL ? __builtin_amdgcn_readfirstlane(*tsel) : 0);
+                                TRAIL ? __builtin_amdgcn_readfirstlane(*tsel) : 0, eh);
   }
 }
 
@@ -244,7 +245,7 @@ template <typename T, bool TRAIL, bool SPLIT>
 __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
                                          const DShot* __restrict__ shots, const int4 wi, const int it,
                                          const int lane, const DAdj* __restrict__ adj, const int slot,
-                                         const int tb) {
+                                         const int tb, const int eh) {
   const DShot& S = shots[wi.x];
   const DModel& M = models[S.model];
   const int nx = M.nx, nz = M.nz;
@@ -253,7 +254,7 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
   const bool incol = (i >= 0 && i < nx);
   const int ic = incol ? i : (i < 0 ? 0 : nx - 1);  // clamped column: every load stays in bounds
   const bool own = incol && lane >= 1 && lane <= EW;
-  const int zend = min(zo + EH, nz);
+  const int zend = min(zo + eh, nz);
   const int par = it & 1;
   const long nn = (long)nx * nz;
   const T* tin = nullptr;
@@ -1817,6 +1818,8 @@ struct Plan {
   int4* dwork = nullptr;
   int nwork = 0;
   int el2_grid = 0;  // persistent grid of the ELASTIC2 step kernel
+  int el2_wb = SDT_EL2_WAVES_PER_BLOCK;  // tile: wavefronts per block
+  int el2_eh = SDT_EL2_ROWS;             // tile: rows
   int* dflags = nullptr;  // per shot first unstable step (INT_MAX = none)
   int* dbase = nullptr;   // step index of rel = 0
   long* dnel = nullptr;   // per shot: field elements (ping-pong copy-back)
@@ -1886,6 +1889,51 @@ long long psi_elems(int kind, long long nx, long long ny, long long nz, long lon
   return -1;
 }
 
+// One launch of the single-pass ELASTIC2 step with the plan's tile shape.
+template <typename T, bool TRAIL, bool SPLIT>
+void launch_el2(Plan* p, hipStream_t st, int rel, const int* tsel) {
+  const DAdj* adj = TRAIL ? p->dadj : nullptr;
+  const dim3 g(p->el2_grid);
+  switch (p->el2_wb) {
+    case 1:
+      hipLaunchKernelGGL((k_el2_step<T, TRAIL, SPLIT, 1>), g, dim3(64), 0, st, p->dmodels,
+                         p->dshots, p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_el2_step<T, TRAIL, SPLIT, 2>), g, dim3(128), 0, st, p->dmodels,
+                         p->dshots, p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh);
+      break;
+    default:
+      hipLaunchKernelGGL((k_el2_step<T, TRAIL, SPLIT, 4>), g, dim3(256), 0, st, p->dmodels,
+                         p->dshots, p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh);
+      break;
+  }
+}
+
+// Work list of the ELASTIC2 step kernel for the plan's tile shape: one item
+// (shot, x0, z0) per block of el2_wb adjacent x tiles; the items of one
+// block-tile over all shots of its model share blockIdx % 8 (one XCD's L2).
+std::vector<int4> el2_work(const Plan* p) {
+  std::vector<int4> work;
+  const int bw = EW * p->el2_wb, eh = p->el2_eh;
+  for (int m = 0; m < p->nmodels; ++m) {
+    const sdt_model& M = p->hmodels[m];
+    const int tx = (M.nx + bw - 1) / bw, tz = (M.nz + eh - 1) / eh, ntile = tx * tz;
+    std::vector<int> ms;
+    for (int s = 0; s < p->nshots; ++s)
+      if (p->hshots[s].model == m) ms.push_back(s);
+    if (ms.empty()) continue;
+    for (int tg = 0; tg < (ntile + 7) / 8; ++tg)
+      for (int s : ms)
+        for (int tt = 0; tt < 8; ++tt) {
+          const int t = tg * 8 + tt;
+          work.push_back(t < ntile ? make_int4(s, (t % tx) * bw, (t / tx) * eh, 0)
+                                   : make_int4(-1, 0, 0, 0));
+        }
+  }
+  return work;
+}
+
 // Launch kernel `which` (0: first field update, 1: second field update,
 // 2: receiver gather) of one step.
 template <typename T>
@@ -1903,13 +1951,9 @@ void launch_one(Plan* p, hipStream_t st, int rel, int which) {
   switch (p->kind * 2 + which) {
     case SDT_ELASTIC2 * 2:
       if (p->split)
-        hipLaunchKernelGGL((k_el2_step<T, false, true>), dim3(p->el2_grid), dim3(64 * EWB), 0, st,
-                           p->dmodels, p->dshots, p->dwork, p->nwork, p->dbase, rel,
-                           (const DAdj*)nullptr, (const int*)nullptr);
+        launch_el2<T, false, true>(p, st, rel, nullptr);
       else
-        hipLaunchKernelGGL((k_el2_step<T, false, false>), dim3(p->el2_grid), dim3(64 * EWB), 0, st,
-                           p->dmodels, p->dshots, p->dwork, p->nwork, p->dbase, rel,
-                           (const DAdj*)nullptr, (const int*)nullptr);
+        launch_el2<T, false, false>(p, st, rel, nullptr);
       break;
     case SDT_ELASTIC2 * 2 + 1:
       break;  // the fused step kernel does both phases
@@ -2046,13 +2090,9 @@ template <typename T>
 void launch_trail_steps(Plan* p, hipStream_t st, int it0, int nsteps) {
   for (int r = 0; r < nsteps; ++r) {
     if (p->split)
-      hipLaunchKernelGGL((k_el2_step<T, true, true>), dim3(p->el2_grid), dim3(64 * EWB), 0, st,
-                         p->dmodels, p->dshots, p->dwork, p->nwork, p->dbase, r, p->dadj,
-                         p->dtsel_trail);
+      launch_el2<T, true, true>(p, st, r, p->dtsel_trail);
     else
-      hipLaunchKernelGGL((k_el2_step<T, true, false>), dim3(p->el2_grid), dim3(64 * EWB), 0, st,
-                         p->dmodels, p->dshots, p->dwork, p->nwork, p->dbase, r, p->dadj,
-                         p->dtsel_trail);
+      launch_el2<T, true, false>(p, st, r, p->dtsel_trail);
   }
 }
 
@@ -2221,24 +2261,7 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
   // work list: (shot, x0, z0); all shots of one tile share blockIdx % 8 (XCD)
   std::vector<int4> work;
   if (kind == SDT_ELASTIC2) {
-    // one item per block: EWB adjacent x tiles of one shot; items of one
-    // block-tile over all shots of the model share blockIdx % 8 (XCD)
-    for (int m = 0; m < nmodels; ++m) {
-      const sdt_model& M = models[m];
-      const int bw = EW * EWB;
-      const int tx = (M.nx + bw - 1) / bw, tz = (M.nz + EH - 1) / EH, ntile = tx * tz;
-      std::vector<int> ms;
-      for (int s = 0; s < nshots; ++s)
-        if (shots[s].model == m) ms.push_back(s);
-      if (ms.empty()) continue;
-      for (int tg = 0; tg < (ntile + 7) / 8; ++tg)
-        for (int s : ms)
-          for (int tt = 0; tt < 8; ++tt) {
-            const int t = tg * 8 + tt;
-            work.push_back(t < ntile ? make_int4(s, (t % tx) * bw, (t / tx) * EH, 0)
-                                     : make_int4(-1, 0, 0, 0));
-          }
-    }
+    work = el2_work(p);
     const int nblk = (int)work.size();
     p->el2_grid = std::max(8, std::min(nblk, SDT_EL2_GRID) / 8 * 8);
   }
@@ -2377,6 +2400,43 @@ int sdt_plan_check(void* plan, int* first_step, int* per_shot) {
   if (first_step) *first_step = first == INT_MAX ? -1 : first;
   if (first != INT_MAX)
     return fail(SDT_EUNSTABLE, "code became unstable and blew up at step " + std::to_string(first));
+  return SDT_OK;
+}
+
+int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p) return fail(SDT_EINVAL, "plan is null");
+  if (p->kind != SDT_ELASTIC2) return fail(SDT_EINVAL, "tile shapes apply to ELASTIC2 plans");
+  if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
+    return fail(SDT_EINVAL, "waves_per_block must be 1, 2 or 4");
+  if (rows < 4 || rows > 256) return fail(SDT_EINVAL, "rows must be in [4, 256]");
+  if (waves_per_block == p->el2_wb && rows == p->el2_eh) return SDT_OK;
+  // the old work list and the graphs that bake in the launch shape may be in flight
+  HIP_TRY(hipDeviceSynchronize());
+  p->el2_wb = waves_per_block;
+  p->el2_eh = rows;
+  const std::vector<int4> work = el2_work(p);
+  int4* dw = nullptr;
+  HIP_TRY(hipMalloc(&dw, sizeof(int4) * work.size()));
+  if (hipMemcpy(dw, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(dw);
+    return fail(SDT_EDEVICE, "hipMemcpy failed (work list)");
+  }
+  (void)hipFree(p->dwork);
+  p->dwork = dw;
+  p->nwork = (int)work.size();
+  p->el2_grid = std::max(8, std::min(p->nwork, SDT_EL2_GRID) / 8 * 8);
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+  drop_adjoint_graphs(p);
+  return SDT_OK;
+}
+
+int sdt_plan_get_tiles(void* plan, int* waves_per_block, int* rows) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p || !waves_per_block || !rows) return fail(SDT_EINVAL, "null argument");
+  *waves_per_block = p->el2_wb;
+  *rows = p->el2_eh;
   return SDT_OK;
 }
 

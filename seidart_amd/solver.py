@@ -198,6 +198,51 @@ class Simulation:
         self.step += nsteps
         return tuple(float(v) for v in out)
 
+    # ------------------------------------------------------------------
+    TILE_CANDIDATES = tuple((wb, rows) for wb in (4, 2, 1) for rows in (8, 16, 32, 64))
+
+    def tiles(self):
+        """Current (waves per block, rows) of the ELASTIC2 step kernel."""
+        wb, rows = C.c_int(), C.c_int()
+        _hip.check(_hip.lib().sdt_plan_get_tiles(self._plan, C.byref(wb), C.byref(rows)))
+        return wb.value, rows.value
+
+    def set_tiles(self, waves_per_block: int, rows: int):
+        _hip.check(_hip.lib().sdt_plan_set_tiles(self._plan, int(waves_per_block), int(rows)))
+        return self
+
+    def autotune(self, steps: int = 8, candidates=None):
+        """Pick the ELASTIC2 tile shape that runs this plan's step fastest.
+
+        Each candidate runs ``steps`` steps (timed by HIP events around the
+        step kernel) from the current state; the state, traces and step
+        counter are restored afterwards, so the run continues as if nothing
+        had happened.  Returns {(wb, rows): ms per step} and keeps the best.
+        Other kinds have a single shape: returns {}."""
+        if self.kind != "elastic2":
+            return {}
+        steps = max(1, min(int(steps), self.nt - self.step))
+        keep = [(b["state"].clone(), b["state_b"].clone(), b["trace"].clone()) for _, _, b in self.shots]
+        step0 = self.step
+        res = {}
+        for wb, rows in (candidates or self.TILE_CANDIDATES):
+            self.set_tiles(wb, rows)
+            self.profile(1)                       # first launch of this shape
+            self.step = step0
+            for (_, _, b), (a0, b0, t0) in zip(self.shots, keep):
+                b["state"].copy_(a0)
+                b["state_b"].copy_(b0)
+            res[(wb, rows)] = self.profile(steps)[0]
+            self.step = step0
+            for (_, _, b), (a0, b0, t0) in zip(self.shots, keep):
+                b["state"].copy_(a0)
+                b["state_b"].copy_(b0)
+                b["trace"].copy_(t0)
+        best = min(res, key=res.get)
+        self.set_tiles(*best)
+        torch.cuda.synchronize(self.device)
+        return res
+
     def check(self):
         """Synchronise; raise UnstableError if any shot blew up."""
         first = C.c_int(-1)

@@ -71,6 +71,7 @@ def test_adjoint_calls_reject_null_plan():
     assert L.sdt_adjoint_setup(None, None, 0, 0xfff, 0) == _hip.SDT_EINVAL
     assert L.sdt_plan_run_trail(None, 0, 1, 0, None) == _hip.SDT_EINVAL
     assert L.sdt_adjoint_run(None, 1, 0, 0, 0, None) == _hip.SDT_EINVAL
+    assert L.sdt_plan_set_tiles(None, 4, 32) == _hip.SDT_EINVAL
 
 
 def _model(nx=20, nz=20):

@@ -148,3 +148,22 @@ def test_split_cpml_variant_matches_oracle(oracle_lib, precision):
         tr = sim.traces()
         o = oracle_lib.OracleRun(probs[0], shot=0).run()
         assert rel_l2(tr[0], o.trace) < FP32_TOL
+
+
+def test_tile_shapes_and_autotune_keep_results(oracle_lib):
+    """Every tile shape gives bit-identical results; autotune restores the
+    state it times from, so a run after it equals a run without it."""
+    probs = [tiny_problem("elastic2", seed=90 + m, nt=60, gamma=1e-3, nshots=2, **d)
+             for m, d in enumerate([dict(nx=300, nz=75, npml=6), dict(nx=57, nz=45, npml=5)])]
+    ref = _sim(probs, "fp32").run(graph_block=10).check().traces()
+    for wb, rows in ((1, 16), (2, 64), (4, 8)):
+        got = _sim(probs, "fp32").set_tiles(wb, rows).run(graph_block=10).check().traces()
+        for a, b in zip(ref, got):
+            np.testing.assert_array_equal(a, b)
+    s = _sim(probs, "fp32")
+    s.run(7)
+    res = s.autotune(steps=4)
+    assert len(res) == 12 and s.tiles() in res
+    got = s.run(graph_block=10).check().traces()
+    for a, b in zip(ref, got):
+        np.testing.assert_array_equal(a, b)
