@@ -185,7 +185,7 @@ int sdt_plan_profile(void* plan, int it0, int nsteps, void* stream, double* out_
  * Adjoint (gradients of a trace misfit).  No counterpart in the reference,
  * which has no adjoint or gradient code; these entry points serve an FWI
  * caller (a torch.autograd.Function in seidart_amd/adjoint.py).  ELASTIC2
- * only.
+ * and EM2.
  *
  * A "state" of one shot is one contiguous array of sdt_state_elems()
  * elements: the kind's fields (field order of sdt_shot, nx*nz each) followed
@@ -205,7 +205,8 @@ int sdt_plan_profile(void* plan, int it0, int nsteps, void* stream, double* out_
  * work:    sdt_adjoint_work_elems() elements of scratch.
  * resid:   dJ/dtrace, laid out like sdt_shot.trace ([nt][ncomp][nrec]);
  *          NULL = no residual.
- * grad:    FP64 [12][nz][nx]: dJ/d(kernel coefficient slot q at cell),
+ * grad:    FP64 [ncoef][nz][nx], ncoef = 12 (ELASTIC2) or 4 (EM2):
+ *          dJ/d(kernel coefficient slot q at cell),
  *          accumulated (+=) over steps; shots of one model may share it
  *          (atomic accumulation).
  * gsrc:    FP64 [nt][3] or NULL: dJ/d(source increment of component c at
@@ -230,8 +231,8 @@ int sdt_adjoint_work_elems(int kind, int nx, int ny, int nz, int lx, int ly, int
 
 /* Hand the plan one sdt_adj_shot per shot (synchronises the device; call
  * again only to change buffers or options).  grad_mask: bit q set =
- * accumulate coefficient slot q (0xfff = all 12; slots nobody differentiates
- * cost nothing when left out).  flags: 0, or SDT_ADJ_REFERENCE to run the
+ * accumulate coefficient slot q (0xfff = all 12 of ELASTIC2, 0xf = the 4 of
+ * EM2; slots nobody differentiates cost nothing when left out).  flags: 0, or SDT_ADJ_REFERENCE to run the
  * unfused three-kernel adjoint (one thread per cell per phase, scratch
  * planes in `work`), kept as a cross-check of the fused kernel. */
 #define SDT_ADJ_REFERENCE 1

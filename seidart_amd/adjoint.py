@@ -1,10 +1,11 @@
-"""Gradients of a receiver-trace misfit for the elastic 2-D scheme.
+"""Gradients of a receiver-trace misfit for the 2-D elastic and EM schemes.
 
 The reference has no adjoint or gradient code; this module adds full
-waveform inversion (FWI) gradients on top of the same forward scheme
-(legacy/fortran/seismicfdtd.f95:12-385, seismic2).  Pieces:
+waveform inversion (FWI) gradients on top of the same forward schemes
+(legacy/fortran/seismicfdtd.f95:12-385, seismic2; electromagfdtd.f95:10-326,
+electromag2).  Pieces:
 
-* :class:`Elastic2Engine` runs the forward of every shot of one
+* :class:`AdjointEngine` runs the forward of every shot of one
   :class:`~seidart_amd.solver.Simulation`, keeping a full state (fields and
   C-PML memory variables) every ``every`` steps.  Its backward pass walks
   the segments last to first: it copies the segment's checkpoint into slot 0
@@ -14,14 +15,17 @@ waveform inversion (FWI) gradients on top of the same forward scheme
   segment (``sdt_adjoint_run``).  Memory: one checkpoint per segment plus
   ``every + 1`` trail states per shot.
 * The adjoint yields dJ/d(kernel coefficient) per cell in FP64 (the slots of
-  ``include/seidart_hip.h``: n11 n13 n15 n33 n35 s15 s35 s55 dt/rho_x
-  dt/rho_z 1-gx 1-gz) and dJ/d(source increment) per step.
-* :class:`Elastic2Traces` is the ``torch.autograd.Function`` around it:
-  inputs are those coefficient slots and increments, output the traces.
-  :func:`elastic2_traces` composes it with the coefficient setup
-  (:func:`seidart_amd.problem.elastic2_coefs`, the averaging stencils and
-  dt / rho written with torch operations), so torch's autograd carries the
-  chain rule to the per-cell Cij, density and attenuation.
+  ``include/seidart_hip.h``: elastic n11 n13 n15 n33 n35 s15 s35 s55
+  dt/rho_x dt/rho_z 1-gx 1-gz; EM caEx cbEx caEz cbEz) and dJ/d(source
+  increment) per step.
+* :class:`Traces` is the ``torch.autograd.Function`` around it: inputs are
+  those coefficient slots and increments, output the traces.
+  :func:`elastic2_traces` and :func:`em2_traces` compose it with the
+  coefficient setup (:func:`seidart_amd.problem.elastic2_coefs`,
+  :func:`seidart_amd.problem.em2_coefs`: averaging stencils, dt / rho, the
+  ca / cb of permittivity and conductivity, written with torch operations),
+  so torch's autograd carries the chain rule to the per-cell Cij, density,
+  attenuation, permittivity and conductivity.
 """
 from __future__ import annotations
 
@@ -31,29 +35,34 @@ import numpy as np
 import torch
 
 from . import _hip
-from .problem import elastic2_coefs
+from .problem import elastic2_coefs, em2_coefs
 from .solver import Simulation
 
-__all__ = ["Elastic2Engine", "Elastic2Traces", "elastic2_traces", "PARAMS"]
+__all__ = ["AdjointEngine", "Elastic2Engine", "Traces", "Elastic2Traces", "elastic2_traces",
+           "em2_traces", "PARAMS", "PARAM_SLOTS", "NSLOTS"]
 
-NSLOT = 12
-ALL_SLOTS = (1 << NSLOT) - 1
+NSLOTS = {"elastic2": 12, "em2": 4}
 # coefficient slots each per-cell parameter feeds (include/seidart_hip.h order)
-PARAM_SLOTS = {"c11": (0,), "c13": (1,), "c15": (2, 5), "c33": (3,), "c35": (4, 6), "c55": (7,),
-               "rho": (8, 9), "gamma_x": (10,), "gamma_z": (11,)}
-# per-cell parameters of elastic2_traces
-PARAMS = ("c11", "c13", "c15", "c33", "c35", "c55", "rho", "gamma_x", "gamma_z")
+PARAM_SLOTS = {
+    "elastic2": {"c11": (0,), "c13": (1,), "c15": (2, 5), "c33": (3,), "c35": (4, 6), "c55": (7,),
+                 "rho": (8, 9), "gamma_x": (10,), "gamma_z": (11,)},
+    "em2": {"e11": (0, 1), "e13": (0, 1, 2, 3), "e33": (2, 3),
+            "s11": (0, 1), "s13": (0, 1, 2, 3), "s33": (2, 3)},
+}
+# per-cell parameters of elastic2_traces / em2_traces
+PARAMS = {k: tuple(v) for k, v in PARAM_SLOTS.items()}
 
 
-class Elastic2Engine:
-    """Checkpointed forward and adjoint of every shot of elastic 2-D problems.
+class AdjointEngine:
+    """Checkpointed forward and adjoint of every shot of 2-D elastic or EM
+    problems (one kind per engine).
 
     ``problems``: one Problem or a list (one plan; shots of all models run in
     the same launches).  ``every``: steps per checkpoint segment.
     """
 
     def __init__(self, problems, precision: str = "fp32", every: int = 64,
-                 device: str = "cuda:0", graph: bool = True, grad_mask: int = ALL_SLOTS,
+                 device: str = "cuda:0", graph: bool = True, grad_mask: int | None = None,
                  reference: bool = False, overlap: bool = True, all_slots: bool = False):
         """``grad_mask``: bit q = accumulate coefficient slot q.  ``reference``:
         run the unfused three-kernel adjoint (a cross-check of the fused
@@ -65,8 +74,13 @@ class Elastic2Engine:
         coefficients may make them non-trivial; otherwise the kernels skip
         them (their gradients are still available)."""
         self.sim = Simulation(problems, precision, device, all_slots=all_slots, all_sources=True)
-        if self.sim.kind != "elastic2":
-            raise ValueError("the adjoint engine handles elastic2 problems only")
+        if self.sim.kind not in NSLOTS:
+            raise ValueError("the adjoint engine handles elastic2 and em2 problems")
+        self.kind = self.sim.kind
+        self.nslot = NSLOTS[self.kind]
+        self.all_slots_mask = (1 << self.nslot) - 1
+        if grad_mask is None:
+            grad_mask = self.all_slots_mask
         if every < 1:
             raise ValueError("every must be >= 1")
         sim = self.sim
@@ -79,7 +93,7 @@ class Elastic2Engine:
         dev, dt = self.device, self.dtype
         self.grad = []
         for p in sim.problems:
-            self.grad.append(torch.zeros((NSLOT, p.nz, p.nx), dtype=torch.float64, device=dev))
+            self.grad.append(torch.zeros((self.nslot, p.nz, p.nx), dtype=torch.float64, device=dev))
         self.ckpt, self.trail, self.lam, self.work, self.gsrc, self.resid, self.init = ([] for _ in range(7))
         adj = (_hip.SdtAdjShot * len(sim.shots))()
         for s, (m, _, b) in enumerate(sim.shots):
@@ -115,7 +129,7 @@ class Elastic2Engine:
 
     def set_grad_mask(self, mask: int) -> None:
         """Choose the coefficient slots the adjoint accumulates (bit q = slot q)."""
-        mask = int(mask) & ALL_SLOTS
+        mask = int(mask) & self.all_slots_mask
         if mask == self.grad_mask:
             return
         torch.cuda.synchronize(self.device)
@@ -130,9 +144,9 @@ class Elastic2Engine:
         return len(self.sim.shots)
 
     def set_coefs(self, model: int, slots) -> None:
-        """Overwrite model ``model``'s 12 coefficient slots (FP64 tensors or
+        """Overwrite model ``model``'s coefficient slots (FP64 tensors or
         arrays of shape (nz, nx)).  Slots the device skips must stay trivial
-        (0 for the couplings, 1 for 1 - gamma)."""
+        (elastic: 0 for the couplings, 1 for 1 - gamma)."""
         for q, v in enumerate(slots):
             dst = self.sim.coef_t[model][q]
             v = torch.as_tensor(v)
@@ -172,7 +186,7 @@ class Elastic2Engine:
         """Adjoint pass for the residuals ``resid`` (per shot, dJ/dtrace of
         shape [nt, 2, nrec]).  Must follow :meth:`forward` with the same
         coefficients.  Returns (grad, gsrc): per model the FP64
-        [12, nz, nx] coefficient-slot gradient, per shot the FP64 [nt, 3]
+        [nslot, nz, nx] coefficient-slot gradient, per shot the FP64 [nt, 3]
         source-increment gradient."""
         sim = self.sim
         L = _hip.lib()
@@ -237,10 +251,13 @@ class Elastic2Engine:
         self.sim.close()
 
 
-class Elastic2Traces(torch.autograd.Function):
+Elastic2Engine = AdjointEngine
+
+
+class Traces(torch.autograd.Function):
     """traces = F(coefficient slots, source increments).
 
-    Inputs after the engine: per model one FP64 tensor [12, nz, nx] of
+    Inputs after the engine: per model one FP64 tensor [nslot, nz, nx] of
     coefficient slots, then per shot one FP64 tensor [3, nt] of source
     increments.  Output: FP64 traces [nshots, nt, 2, nrec_max].  The
     backward pass is the engine's checkpointed adjoint.
@@ -277,7 +294,68 @@ class Elastic2Traces(torch.autograd.Function):
         return (None, *outs)
 
 
-def elastic2_traces(engine: Elastic2Engine, params):
+def _mask_from(engine, params, kind):
+    """Accumulate only the coefficient slots the parameters that need a
+    gradient feed."""
+    if engine.kind != kind:
+        raise ValueError(f"engine runs {engine.kind} problems, not {kind}")
+    mask = 0
+    for prm in params:
+        for k, v in prm.items():
+            if torch.is_tensor(v) and v.requires_grad:
+                for q in PARAM_SLOTS[kind][k]:
+                    mask |= 1 << q
+    if mask:
+        engine.set_grad_mask(mask)
+
+
+def _increments(engine, divisor_of):
+    """Per shot [3, nt] source increments s(it) dt / divisor(src) (torch)."""
+    sim = engine.sim
+    incs = []
+    for m, s, _ in sim.shots:
+        p = sim.problems[m]
+        sh = p.shots[s]
+        i, _, k = sh.src
+        comps = []
+        for c in range(3):
+            f = sh.force[c]
+            div = divisor_of(m, c)
+            if f is None or c == 1:
+                comps.append(torch.zeros(engine.nt, dtype=torch.float64, device=div.device))
+                continue
+            f = torch.as_tensor(np.pad(np.asarray(f, dtype=np.float64)[: p.nt], (0, engine.nt - p.nt)),
+                                device=div.device)
+            comps.append(f * p.dt / div[k, i])
+        incs.append(torch.stack(comps))
+    return incs
+
+
+def em2_traces(engine: AdjointEngine, params):
+    """EM 2-D receiver traces (Ex, Ez) of every shot as a differentiable
+    function of the per-cell relative permittivity and conductivity.
+
+    ``params``: per model a dict of FP64 torch tensors (nz, nx) with keys
+    among e11 e13 e33 s11 s13 s33 (absent = the engine's problem values, held
+    constant).  Returns FP64 [nshots, nt, 2, nrec_max]; backpropagating runs
+    the adjoint.  The source increment divides by e11 / e33 at the source
+    cell (electromagfdtd.f95:300-301), so gradients include that term."""
+    sim = engine.sim
+    if len(params) != len(sim.problems):
+        raise ValueError("one parameter dict per model")
+    _mask_from(engine, params, "em2")
+    slots, Cs = [], []
+    for p, prm in zip(sim.problems, params):
+        dev = next(iter(prm.values())).device if prm else engine.device
+        Cd = {k: prm[k] if k in prm else torch.as_tensor(p.coef[k], dtype=torch.float64, device=dev)
+              for k in ("e11", "e13", "e33", "s11", "s13", "s33")}
+        slots.append(torch.stack(em2_coefs(Cd, p.dt)))
+        Cs.append(Cd)
+    incs = _increments(engine, lambda m, c: Cs[m]["e11" if c == 0 else "e33"])
+    return Traces.apply(engine, *slots, *incs)
+
+
+def elastic2_traces(engine: AdjointEngine, params):
     """Receiver traces of every shot as a differentiable function of the
     per-cell model parameters.
 
@@ -289,14 +367,7 @@ def elastic2_traces(engine: Elastic2Engine, params):
     sim = engine.sim
     if len(params) != len(sim.problems):
         raise ValueError("one parameter dict per model")
-    mask = 0
-    for prm in params:
-        for k, v in prm.items():
-            if torch.is_tensor(v) and v.requires_grad:
-                for q in PARAM_SLOTS[k]:
-                    mask |= 1 << q
-    if mask:
-        engine.set_grad_mask(mask)
+    _mask_from(engine, params, "elastic2")
     slots = []
     rho_by_model = []
     for m, (p, prm) in enumerate(zip(sim.problems, params)):
@@ -311,21 +382,9 @@ def elastic2_traces(engine: Elastic2Engine, params):
                 gam[a] = torch.as_tensor(p.gamma[a], dtype=torch.float64, device=dev)
         slots.append(torch.stack(elastic2_coefs(Cd, gam, p.dt, getattr(p, "density_method", "legacy"))))
         rho_by_model.append(Cd["rho"])
-    incs = []
-    for m, s, _ in sim.shots:
-        p = sim.problems[m]
-        sh = p.shots[s]
-        i, _, k = sh.src
-        rho = rho_by_model[m]
-        comps = []
-        for c in range(3):
-            f = sh.force[c]
-            if f is None or c == 1:
-                comps.append(torch.zeros(engine.nt, dtype=torch.float64, device=rho.device))
-                continue
-            f = torch.as_tensor(np.pad(np.asarray(f, dtype=np.float64)[: p.nt], (0, engine.nt - p.nt)),
-                                device=rho.device)
-            # s(it) * dt / rho(src), seismicfdtd.f95:359-361
-            comps.append(f * p.dt / rho[k, i])
-        incs.append(torch.stack(comps))
-    return Elastic2Traces.apply(engine, *slots, *incs)
+    # s(it) * dt / rho(src), seismicfdtd.f95:359-361
+    incs = _increments(engine, lambda m, c: rho_by_model[m])
+    return Traces.apply(engine, *slots, *incs)
+
+
+Elastic2Traces = Traces

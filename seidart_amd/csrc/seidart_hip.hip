@@ -1569,6 +1569,192 @@ k_el25_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
 }
 
 // ===========================================================================
+// EM2 adjoint: the transpose of one k_em2_h + k_em2_e step, for gradients
+// of a trace misfit with respect to the E-update coefficients (caEx cbEx
+// caEz cbEz, from permittivity and conductivity).  No counterpart in the
+// reference.  The step as the device runs it: Hy' = m * Hy + dbH (dEz_dx +
+// dEx_dz) with m = 0 on the Hy Dirichlet ring (the legacy zeroes it at the
+// end of the previous step, electromagfdtd.f95:312-315), then Ex, Ez from
+// Hy' (:273-295), source, Dirichlet (:304-311).  Three launches, like the
+// elastic reference kernels: E^T (scratch planes 0, 1: adjoints of dHy_dz
+// at Ex nodes and of dHy_dx at Ez nodes), H^T (gathers them into lambda_Hy;
+// planes 2, 3: adjoints of dEx_dz and dEz_dx), and the E gather.
+// psi slabs: x [nz][2lx]: dEz_dx dHy_dx ; z [2lz][nx]: dEx_dz dHy_dz.
+// ===========================================================================
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_em2_adj_e(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+            const DAdj* __restrict__ adj, int nshots, const int* __restrict__ tbase, int k,
+            const int* __restrict__ tsel, unsigned gmask) {
+  const int s = blockIdx.z;
+  if (s >= nshots) return;
+  const DShot& S = shots[s];
+  const DModel& M = models[S.model];
+  const DAdj& A = adj[s];
+  const int nx = M.nx, nz = M.nz;
+  const int i = blockIdx.x * TX + threadIdx.x, j = blockIdx.y * TZ + threadIdx.y;
+  if (i >= nx || j >= nz) return;
+  const int t = *tbase + k;
+  const long n = (long)nx * nz, c = (long)j * nx + i;
+  const int lx = M.lx, lz = M.lz;
+  const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
+  const auto Lin = cp<T>(A.lam[(t + 1) & 1]);
+  auto Lout = mp<T>(A.lam[t & 1]);
+  const auto u1 = cp<T>(A.trail[*tsel]) + (long)(k + 1) * A.slot;
+  const auto u0 = cp<T>(A.trail[*tsel]) + (long)k * A.slot;
+  auto W = mp<T>(A.work);
+  auto G = GAcc<T>::of(A);
+  const Prof<T> P(&M.prof[0][0]);
+  const bool inx = in_slab(i, nx, lx), inz = in_slab(j, nz, lz);
+  const long ox = 3 * n + (inx ? (long)j * 2 * lx + slab_pos(i, nx, lx) : 0);
+  const long oz = 3 * n + 2 * SX + (inz ? (long)slab_pos(j, nz, lz) * nx + i : 0);
+  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz);
+  // Dirichlet ring of Ex and Ez (and cells outside their Fortran extent)
+  const bool edx = (i == 0 || i >= nx - 2 || j == 0 || j == nz - 1);
+  const bool edz = (i == 0 || i == nx - 1 || j == 0 || j >= nz - 2);
+  const T lex = edx ? T(0) : Lin[c], lez = edz ? T(0) : Lin[n + c];
+  if (A.gsrc && i == S.isrc && j == S.ksrc) {
+    A.gsrc[3 * (long)t + 0] = (double)lex;
+    A.gsrc[3 * (long)t + 2] = (double)lez;
+  }
+  T wx = 0, wz = 0;
+  if (j >= 1 && j <= nz - 2 && i <= nx - 2) {  // Ex node, :273-283
+    const T ld = cp<T>(M.coef[1])[c] * lex;
+    const T d = (u1[2 * n + c] - u1[2 * n + c - nx]) * idz;
+    T D = d;
+    wx = ld;
+    if (inz) {
+      const T lr = Lin[oz + SZ] + ld;
+      wx = P[12 + PKH][j] * ld + P[12 + PAH][j] * lr;
+      Lout[oz + SZ] = P[12 + PBH][j] * lr;
+      D = P[12 + PKH][j] * d + u1[oz + SZ];
+    }
+    if (gmask & 1u) gadd(G + c, (double)lex * (double)u0[c]);
+    if (gmask & 2u) gadd(G + n + c, (double)lex * (double)D);
+    Lout[c] = cp<T>(M.coef[0])[c] * lex;
+  } else {
+    if (inz) Lout[oz + SZ] = Lin[oz + SZ];
+    Lout[c] = lex;
+  }
+  if (j <= nz - 2 && i >= 1 && i <= nx - 2) {  // Ez node, :285-295
+    const T le = cp<T>(M.coef[3])[c] * lez;
+    const T e = (u1[2 * n + c] - u1[2 * n + c - 1]) * idx;
+    T E = e;
+    wz = le;
+    if (inx) {
+      const T lr = Lin[ox + SX] + le;
+      wz = P[PKH][i] * le + P[PAH][i] * lr;
+      Lout[ox + SX] = P[PBH][i] * lr;
+      E = P[PKH][i] * e + u1[ox + SX];
+    }
+    if (gmask & 4u) gadd(G + 2 * n + c, (double)lez * (double)u0[n + c]);
+    if (gmask & 8u) gadd(G + 3 * n + c, (double)lez * (double)E);
+    Lout[n + c] = cp<T>(M.coef[2])[c] * lez;
+  } else {
+    if (inx) Lout[ox + SX] = Lin[ox + SX];
+    Lout[n + c] = lez;
+  }
+  W[c] = wx;
+  W[n + c] = wz;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_em2_adj_h(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+            const DAdj* __restrict__ adj, int nshots, const int* __restrict__ tbase, int k) {
+  const int s = blockIdx.z;
+  if (s >= nshots) return;
+  const DShot& S = shots[s];
+  const DModel& M = models[S.model];
+  const DAdj& A = adj[s];
+  const int nx = M.nx, nz = M.nz;
+  const int i = blockIdx.x * TX + threadIdx.x, j = blockIdx.y * TZ + threadIdx.y;
+  if (i >= nx || j >= nz) return;
+  const int t = *tbase + k;
+  const long n = (long)nx * nz, c = (long)j * nx + i;
+  const int lx = M.lx, lz = M.lz;
+  const long SX = (long)nz * 2 * lx;
+  const auto Lin = cp<T>(A.lam[(t + 1) & 1]);
+  auto Lout = mp<T>(A.lam[t & 1]);
+  auto W = mp<T>(A.work);
+  const Prof<T> P(&M.prof[0][0]);
+  const bool inx = in_slab(i, nx, lx), inz = in_slab(j, nz, lz);
+  const long ox = 3 * n + (inx ? (long)j * 2 * lx + slab_pos(i, nx, lx) : 0);
+  const long oz = 3 * n + 2 * SX + (inz ? (long)slab_pos(j, nz, lz) * nx + i : 0);
+  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz);
+  // Hy' feeds the E updates through dHy_dz (Ex node c, c+nx) and dHy_dx (Ez node c, c+1)
+  const T lh = Lin[2 * n + c] + idz * (W[c] - (j + 1 < nz ? W[c + nx] : T(0))) +
+               idx * (W[n + c] - (i + 1 < nx ? W[n + c + 1] : T(0)));
+  T la = 0, lb = 0;
+  if (i <= nx - 2 && j <= nz - 2) {  // Hy node, :249-266
+    const bool ring = (i == 0 || i == nx - 2 || j == 0 || j == nz - 2);
+    Lout[2 * n + c] = ring ? T(0) : lh;
+    const T w = T(M.scal[0]) * lh;
+    la = w;
+    lb = w;
+    if (inz) {
+      const T lr = Lin[oz] + w;
+      la = P[12 + PK][j] * w + P[12 + PA][j] * lr;
+      Lout[oz] = P[12 + PB][j] * lr;
+    }
+    if (inx) {
+      const T lr = Lin[ox] + w;
+      lb = P[PK][i] * w + P[PA][i] * lr;
+      Lout[ox] = P[PB][i] * lr;
+    }
+  } else {
+    Lout[2 * n + c] = lh;
+    if (inz) Lout[oz] = Lin[oz];
+    if (inx) Lout[ox] = Lin[ox];
+  }
+  W[2 * n + c] = la;  // adjoint of dEx_dz at the Hy node
+  W[3 * n + c] = lb;  // adjoint of dEz_dx at the Hy node
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT)
+k_em2_adj_g(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+            const DAdj* __restrict__ adj, int nshots, const int* __restrict__ tbase, int k) {
+  const int s = blockIdx.z;
+  if (s >= nshots) return;
+  const DShot& S = shots[s];
+  const DModel& M = models[S.model];
+  const DAdj& A = adj[s];
+  const int nx = M.nx, nz = M.nz;
+  const int i = blockIdx.x * TX + threadIdx.x, j = blockIdx.y * TZ + threadIdx.y;
+  if (i >= nx || j >= nz) return;
+  const int t = *tbase + k;
+  const long n = (long)nx * nz, c = (long)j * nx + i;
+  auto Lout = mp<T>(A.lam[t & 1]);
+  const auto W = cp<T>(A.work);
+  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz);
+  Lout[c] = Lout[c] + idz * ((j >= 1 ? W[2 * n + c - nx] : T(0)) - W[2 * n + c]);
+  Lout[n + c] = Lout[n + c] + idx * ((i >= 1 ? W[3 * n + c - 1] : T(0)) - W[3 * n + c]);
+}
+
+// Copy a shot's state (fields then psi) between its own buffers and trail
+// slot `slot` (to_trail = 1: state -> trail).  Kinds that update in place
+// recompute through their own buffers and record each state this way.
+template <typename T>
+__global__ void k_state_copy(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+                             const DAdj* __restrict__ adj, int nfield, const int* __restrict__ tsel,
+                             int slot, int to_trail) {
+  const DShot& S = shots[blockIdx.y];
+  const DModel& M = models[S.model];
+  const DAdj& A = adj[blockIdx.y];
+  const long n = (long)M.nx * M.ny * M.nz;
+  T* tr = static_cast<T*>(A.trail[*tsel]) + (long)slot * A.slot;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < A.slot;
+       q += (long)gridDim.x * blockDim.x) {
+    T* f = q < nfield * n ? static_cast<T*>(S.f[q / n]) + q % n : static_cast<T*>(S.psi) + (q - nfield * n);
+    if (to_trail)
+      tr[q] = *f;
+    else
+      *f = tr[q];
+  }
+}
+
+// ===========================================================================
 // EM25: electromag25, legacy/fortran/electromagfdtd.f95:619-1092
 // psi slabs: x [nz][ny][2lx]: dEz_dx dEy_dx dHz_dx dHy_dx
 //            y [nz][2ly][nx]: dEz_dy dEx_dy dHz_dy dHx_dy
@@ -2121,15 +2307,60 @@ void launch_adjoint_steps(Plan* p, hipStream_t st, int khi, int klo) {
   }
 }
 
+// EM2 updates in place: recompute through the shot's own buffers, loading
+// slot 0 first and recording every new state.
+template <typename T>
+void launch_trail_em2(Plan* p, hipStream_t st, int nsteps) {
+  const dim3 g(256, p->nshots);
+  hipLaunchKernelGGL(k_state_copy<T>, g, dim3(256), 0, st, p->dmodels, p->dshots, p->dadj, 3,
+                     p->dtsel_trail, 0, 0);
+  for (int r = 0; r < nsteps; ++r) {
+    launch_one<T>(p, st, r, 0);
+    launch_one<T>(p, st, r, 1);
+    hipLaunchKernelGGL(k_state_copy<T>, g, dim3(256), 0, st, p->dmodels, p->dshots, p->dadj, 3,
+                       p->dtsel_trail, r + 1, 1);
+  }
+}
+
 void launch_trail(Plan* p, hipStream_t st, int nsteps) {
+  if (p->kind == SDT_EM2) {
+    if (p->prec == SDT_FP64)
+      launch_trail_em2<double>(p, st, nsteps);
+    else
+      launch_trail_em2<float>(p, st, nsteps);
+    return;
+  }
   if (p->prec == SDT_FP64)
     launch_trail_steps<double>(p, st, 0, nsteps);
   else
     launch_trail_steps<float>(p, st, 0, nsteps);
 }
 
+template <typename T>
+void launch_adjoint_em2(Plan* p, hipStream_t st, int khi, int klo) {
+  const dim3 blk(TX, TZ);
+  const dim3 grd((p->maxnx + TX - 1) / TX, (p->maxnz + TZ - 1) / TZ, p->nshots);
+  const int* tb = p->dtbase;
+  for (int k = khi - 1; k >= klo; --k) {
+    if (p->maxrec > 0)
+      hipLaunchKernelGGL(k_el2_adj_inject<T>, dim3((p->maxrec + 63) / 64, p->nshots), dim3(64), 0,
+                         st, p->dmodels, p->dshots, p->dadj, p->nshots, tb, k);
+    hipLaunchKernelGGL(k_em2_adj_e<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dadj, p->nshots,
+                       tb, k, p->dtsel_adj, p->gmask);
+    hipLaunchKernelGGL(k_em2_adj_h<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dadj, p->nshots,
+                       tb, k);
+    hipLaunchKernelGGL(k_em2_adj_g<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dadj, p->nshots,
+                       tb, k);
+  }
+}
+
 void launch_adjoint(Plan* p, hipStream_t st, int khi, int klo) {
-  if (p->prec == SDT_FP64)
+  if (p->kind == SDT_EM2) {
+    if (p->prec == SDT_FP64)
+      launch_adjoint_em2<double>(p, st, khi, klo);
+    else
+      launch_adjoint_em2<float>(p, st, khi, klo);
+  } else if (p->prec == SDT_FP64)
     p->split ? launch_adjoint_steps<double, true>(p, st, khi, klo)
              : launch_adjoint_steps<double, false>(p, st, khi, klo);
   else
@@ -2452,9 +2683,10 @@ int sdt_adjoint_work_elems(int kind, int nx, int ny, int nz, int lx, int ly, int
                            long long* out) {
   (void)ny; (void)lx; (void)ly; (void)lz;
   if (!out) return fail(SDT_EINVAL, "out is null");
-  if ((kind & ~SDT_SPLIT_CPML) != SDT_ELASTIC2)
-    return fail(SDT_EINVAL, "the adjoint is available for ELASTIC2 only");
-  *out = 12LL * nx * nz;
+  const int base = kind & ~SDT_SPLIT_CPML;
+  if (base != SDT_ELASTIC2 && base != SDT_EM2)
+    return fail(SDT_EINVAL, "the adjoint is available for ELASTIC2 and EM2 only");
+  *out = (base == SDT_ELASTIC2 ? 12LL : 4LL) * nx * nz;
   return SDT_OK;
 }
 
@@ -2464,8 +2696,13 @@ int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots, unsigned 
   if (!p || !adj) return fail(SDT_EINVAL, "plan or adj is null");
   if (grad_mask & ~0xfffu) return fail(SDT_EINVAL, "grad_mask has bits beyond the 12 slots");
   if (flags & ~SDT_ADJ_REFERENCE) return fail(SDT_EINVAL, "unknown adjoint flags");
-  if (p->kind != SDT_ELASTIC2) return fail(SDT_EINVAL, "the adjoint is available for ELASTIC2 only");
+  if (p->kind != SDT_ELASTIC2 && p->kind != SDT_EM2)
+    return fail(SDT_EINVAL, "the adjoint is available for ELASTIC2 and EM2 only");
   if (nshots != p->nshots) return fail(SDT_EINVAL, "adj must hold one entry per shot of the plan");
+  if (p->kind == SDT_EM2 && (grad_mask & ~0xfu))
+    return fail(SDT_EINVAL, "EM2 has 4 coefficient slots (grad_mask <= 0xf)");
+  if (p->kind == SDT_EM2 && (flags & SDT_ADJ_REFERENCE))
+    return fail(SDT_EINVAL, "SDT_ADJ_REFERENCE applies to ELASTIC2");
   std::vector<DAdj> h(nshots);
   for (int s = 0; s < nshots; ++s) {
     const sdt_adj_shot& a = adj[s];
@@ -2527,7 +2764,7 @@ int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots, unsigned 
   if (!p->dtbase) HIP_TRY(hipMalloc(&p->dtbase, sizeof(int)));
   if (!p->dtsel_trail) HIP_TRY(hipMalloc(&p->dtsel_trail, sizeof(int)));
   if (!p->dtsel_adj) HIP_TRY(hipMalloc(&p->dtsel_adj, sizeof(int)));
-  if (!p->dadjwork) {
+  if (!p->dadjwork && p->kind == SDT_ELASTIC2) {
     // (shot, x0, z0) items of AWB * AW columns x AH rows; the items of one
     // tile over all shots of its model share blockIdx % 8 (one XCD's L2)
     std::vector<int4> work;

@@ -26,7 +26,7 @@ from .cpml import EPS0, MU0, AxisProfile, slab_width
 
 __all__ = ["KINDS", "FIELDS", "STIFFNESS", "PERMITTIVITY", "CONDUCTIVITY", "Shot", "Problem",
            "kernel_coefs", "source_increments", "aux_b_x_half_by_row", "ncomp",
-           "DENSITY_METHODS", "node_density", "elastic2_coefs"]
+           "DENSITY_METHODS", "node_density", "elastic2_coefs", "em2_coefs"]
 
 # Density averaging at staggered velocity nodes.  The legacy solver has one
 # rule (weighted arithmetic mean); the reference binary takes a
@@ -250,16 +250,7 @@ def kernel_coefs(p: Problem):
         om = [1 - p.gamma.get(a, z) for a in ("x", "y", "z")]
         return L1 + L2 + L3 + L4 + [dx_, dy_, dz_] + om
     if p.kind == "em2":
-        # electromagfdtd.f95:164-177
-        ca_x, cb_x = _cacb(C["s11"] + C["s13"], C["e11"] + C["e13"], dt)
-        ca_z, cb_z = _cacb(C["s13"] + C["s33"], C["e13"] + C["e33"], dt)
-        o = [np.zeros_like(ca_x) for _ in range(4)]
-        # (ca(i,j) + ca(i,j-1)) / 2 for Ex, :281-282; (ca(i,j) + ca(i-1,j)) / 2 for Ez, :293-294
-        o[0][1:, :] = (ca_x[1:, :] + ca_x[:-1, :]) / 2
-        o[1][1:, :] = (cb_x[1:, :] + cb_x[:-1, :]) / 2
-        o[2][:, 1:] = (ca_z[:, 1:] + ca_z[:, :-1]) / 2
-        o[3][:, 1:] = (cb_z[:, 1:] + cb_z[:, :-1]) / 2
-        return o
+        return em2_coefs(C, dt)
     if p.kind == "em25":
         # electromagfdtd.f95:760-781
         ca_x, cb_x = _cacb(C["s11"] + C["s12"] + C["s13"], C["e11"] + C["e12"] + C["e13"], dt)
@@ -295,6 +286,22 @@ def elastic2_coefs(C, gamma, dt, density_method="legacy"):
         _avg4_shear(C["c15"]), _avg4_shear(C["c35"]), _avg4_shear(C["c55"]),
         dt / rho, dtr_z, 1 - gx, 1 - gz,
     ]
+
+
+def em2_coefs(C, dt):
+    """EM2 coefficient slots (caEx cbEx caEz cbEz, 2-point averaged) from the
+    per-cell relative permittivity e11 e13 e33 and conductivity s11 s13 s33,
+    as numpy arrays or torch tensors (nz, nx)."""
+    # electromagfdtd.f95:164-177
+    ca_x, cb_x = _cacb(C["s11"] + C["s13"], C["e11"] + C["e13"], dt)
+    ca_z, cb_z = _cacb(C["s13"] + C["s33"], C["e13"] + C["e33"], dt)
+    o = [_zeros_like(ca_x) for _ in range(4)]
+    # (ca(i,j) + ca(i,j-1)) / 2 for Ex, :281-282; (ca(i,j) + ca(i-1,j)) / 2 for Ez, :293-294
+    o[0][1:, :] = (ca_x[1:, :] + ca_x[:-1, :]) / 2
+    o[1][1:, :] = (cb_x[1:, :] + cb_x[:-1, :]) / 2
+    o[2][:, 1:] = (ca_z[:, 1:] + ca_z[:, :-1]) / 2
+    o[3][:, 1:] = (cb_z[:, 1:] + cb_z[:, :-1]) / 2
+    return o
 
 
 def _cacb(sig, epsr_sum, dt):

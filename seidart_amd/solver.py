@@ -118,29 +118,21 @@ class Simulation:
             for s, sh in enumerate(p.shots):
                 init = (p.init[s] if p.init else None) or {}
                 bufs = {}
-                if self.kind == "elastic2":
-                    # one contiguous state (fields then psi) per buffer, the
-                    # layout of sdt_state_elems; the single-pass kernel
-                    # ping-pongs between state A and state B
-                    nst = _hip.state_elems(self.kid, p.nx, p.ny, p.nz, lx, ly, lz)
-                    for tag in ("", "_b"):
-                        st = torch.zeros(nst, dtype=self.dtype, device=self.device)
-                        bufs["state" + tag] = st
-                        for f, n in enumerate(self.names):
-                            bufs[n + tag] = st[f * nel:(f + 1) * nel].view(p.field_shape)
-                        bufs["psi" + tag] = st[len(self.names) * nel:] if npsi else st[:1]
-                    for n in self.names:
-                        if n in init:
-                            bufs[n].copy_(T(np.asarray(init[n], dtype=np.float64).reshape(p.field_shape)))
-                else:
-                    for n in self.names:
-                        if n in init:
-                            bufs[n] = T(np.asarray(init[n], dtype=np.float64).reshape(p.field_shape))
-                        else:
-                            bufs[n] = torch.zeros(p.field_shape, dtype=self.dtype, device=self.device)
-                    bufs["psi"] = torch.zeros(max(npsi, 1), dtype=self.dtype, device=self.device)
+                # one contiguous state (fields then psi), the layout of
+                # sdt_state_elems; the single-pass ELASTIC2 kernel ping-pongs
+                # between state A and state B
+                nst = _hip.state_elems(self.kid, p.nx, p.ny, p.nz, lx, ly, lz)
+                for tag in (("", "_b") if self.kind == "elastic2" else ("",)):
+                    st = torch.zeros(nst, dtype=self.dtype, device=self.device)
+                    bufs["state" + tag] = st
+                    for f, n in enumerate(self.names):
+                        bufs[n + tag] = st[f * nel:(f + 1) * nel].view(p.field_shape)
+                    bufs["psi" + tag] = st[len(self.names) * nel:] if npsi else st[:1]
+                for n in self.names:
+                    if n in init:
+                        bufs[n].copy_(T(np.asarray(init[n], dtype=np.float64).reshape(p.field_shape)))
                 inc = source_increments(p, sh)
-                if all_sources and self.kind == "elastic2":
+                if all_sources and self.kind in ("elastic2", "em2"):
                     inc = [np.zeros(p.nt) if (v is None and c != 1) else v for c, v in enumerate(inc)]
                 bufs["src"] = [None if v is None else T(np.pad(v, (0, self.nt - v.size))) for v in inc]
                 rec = np.asarray(sh.rec, dtype=np.int32).reshape(-1, 3)

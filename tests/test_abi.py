@@ -60,8 +60,9 @@ def test_state_and_adjoint_sizes():
         assert _hip.adjoint_work_elems(k, *dims) == 12 * 8000
     assert _hip.state_elems(_hip.SDT_EM25, 30, 20, 40, 5, 4, 6) == \
         6 * 30 * 20 * 40 + _hip.psi_elems(_hip.SDT_EM25, 30, 20, 40, 5, 4, 6)
-    with pytest.raises(_hip.SeidartHipError, match="ELASTIC2 only"):
-        _hip.adjoint_work_elems(_hip.SDT_EM2, *dims)
+    assert _hip.adjoint_work_elems(_hip.SDT_EM2, *dims) == 4 * 8000
+    with pytest.raises(_hip.SeidartHipError, match="ELASTIC2 and EM2 only"):
+        _hip.adjoint_work_elems(_hip.SDT_EM25, 30, 20, 40, 5, 4, 6)
     # trail[2], trail_len (+pad), lam[2], work, resid, grad, gsrc
     assert C.sizeof(_hip.SdtAdjShot) == 16 + 8 + 16 + 8 * 4
 

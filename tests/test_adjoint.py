@@ -343,3 +343,147 @@ def test_gradient_finite_differences():
             errs.append(abs(fd - g[j, i]) / abs(g[j, i]))
         worst[k] = max(errs)
     assert all(v < 1e-3 for v in worst.values()), worst
+
+
+# ---------------------------------------------------------------------------
+# EM 2-D (permittivity and conductivity gradients)
+# ---------------------------------------------------------------------------
+EMP = ("e11", "e13", "e33", "s11", "s13", "s33")
+
+
+def _em_ref_args(p, shot, params=None):
+    T = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64))
+    prm = params if params is not None else {k: T(p.coef[k]) for k in EMP}
+    prof = {a: tuple(T(v) for v in (pr.kappa, pr.a, pr.b, pr.kappa_h, pr.a_h, pr.b_h))
+            for a, pr in p.prof.items()}
+    sh = p.shots[shot]
+    force = tuple(None if f is None else T(f) for f in (sh.force[0], sh.force[2]))
+    rec = torch.as_tensor(np.asarray(sh.rec)[:, [0, 2]])
+    return dict(prm=prm, prof=prof, dx=p.dx, dz=p.dz, dt=p.dt, nt=p.nt, src=(sh.src[0], sh.src[2]),
+                force=force, rec=rec)
+
+
+def test_torch_em_reference_matches_oracle(oracle_lib):
+    from torch_em2 import run_em2
+    p = tiny_problem("em2", seed=45, nx=40, nz=36, npml=6, nt=80, nrec=7)
+    with torch.no_grad():
+        tr = run_em2(**_em_ref_args(p, 0)).numpy()
+    o = oracle_lib.OracleRun(p, shot=0).run()
+    assert np.abs(o.trace).max() > 0
+    assert rel_l2(tr, o.trace) < 1e-13
+
+
+def test_em_coefs_torch_matches_numpy():
+    from seidart_amd.problem import em2_coefs
+    p = tiny_problem("em2", seed=46, nx=30, nz=26, npml=4, nt=10)
+    ref = kernel_coefs(p)
+    got = em2_coefs({k: torch.as_tensor(v) for k, v in p.coef.items()}, p.dt)
+    for a, b in zip(got, ref):
+        assert np.max(np.abs(a.numpy() - b) / (np.abs(b) + 1e-300)) <= 1e-15
+
+
+@pytest.mark.gpu
+def test_em_dot_product():
+    p = tiny_problem("em2", seed=53, nx=52, nz=44, npml=7, nt=110, nshots=2, nrec=9)
+    for sh in p.shots:
+        sh.force = (np.zeros(p.nt), None, np.zeros(p.nt))
+    eng = _engine(p, "fp64", every=16)
+    g = torch.Generator(device="cpu").manual_seed(6)
+    xs = []
+    for s in range(eng.nshots):
+        x = torch.randn(eng.init[s].numel(), generator=g, dtype=torch.float64)
+        eng.init[s].copy_(x.to(eng.device))
+        xs.append(x)
+    tr = eng.forward()
+    ys = [torch.randn(t.shape, generator=g, dtype=torch.float64) for t in tr]
+    lhs = sum(float((t.cpu() * y).sum()) for t, y in zip(tr, ys))
+    eng.backward(ys)
+    rhs = sum(float((x * eng.adjoint_state(s).cpu()).sum()) for s, x in enumerate(xs))
+    assert abs(lhs) > 0
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
+
+
+def _em_problem():
+    kw = dict(nx=60, nz=50, npml=8, nt=160, nshots=2, nrec=10)
+    p = tiny_problem("em2", seed=63, **kw)
+    p._seed, p._kw = 63, kw
+    return p
+
+
+def _em_observed(p, seed=8):
+    from pyoracle import OracleRun
+    q = tiny_problem("em2", seed=p._seed, **p._kw)
+    rng = np.random.default_rng(seed)
+    for k in ("e11", "e33", "s11", "s33"):
+        q.coef[k] = q.coef[k] * (1 + 0.03 * rng.standard_normal(q.coef[k].shape).clip(-2, 2))
+    nrec = max(np.asarray(sh.rec).shape[0] for sh in q.shots)
+    d = np.zeros((len(q.shots), q.nt, 2, nrec))
+    for s in range(len(q.shots)):
+        t = OracleRun(q, shot=s).run().trace
+        d[s, :, :, : t.shape[2]] = t
+    return torch.as_tensor(d)
+
+
+def _em_gpu_grads(p, engine, d_obs):
+    from seidart_amd.adjoint import em2_traces
+    dev = engine.device
+    prm = {k: torch.tensor(p.coef[k], dtype=torch.float64, device=dev, requires_grad=True) for k in EMP}
+    tr = em2_traces(engine, [prm])
+    J = 0.5 * ((tr - d_obs.to(dev)) ** 2).sum()
+    J.backward()
+    return float(J.detach()), {k: v.grad.cpu().numpy() for k, v in prm.items()}
+
+
+@pytest.mark.gpu
+def test_em_gradient_matches_torch_autograd():
+    from torch_em2 import run_em2
+    p = _em_problem()
+    d = _em_observed(p)
+    Jg, gg = _em_gpu_grads(p, _engine(p, "fp64", every=32), d)
+    prm = {k: torch.tensor(p.coef[k], dtype=torch.float64, requires_grad=True) for k in EMP}
+    J = 0
+    for s in range(len(p.shots)):
+        tr = run_em2(**_em_ref_args(p, s, params=prm))
+        J = J + 0.5 * ((tr - d[s, :, :, : tr.shape[2]]) ** 2).sum()
+    J.backward()
+    assert abs(Jg - float(J)) <= 1e-12 * abs(float(J))
+    for k in EMP:
+        ref = prm[k].grad.numpy()
+        assert np.abs(ref).max() > 0, k
+        assert rel_l2(gg[k], ref) < 1e-9, (k, rel_l2(gg[k], ref))
+    _, g32 = _em_gpu_grads(p, _engine(p, "fp32", every=32), d)
+    for k in ("e11", "e33", "s11", "s33"):
+        assert rel_l2(g32[k], gg[k]) < 1e-3, (k, rel_l2(g32[k], gg[k]))
+
+
+@pytest.mark.gpu
+def test_em_gradient_finite_differences():
+    from seidart_amd.adjoint import em2_traces
+    p = _em_problem()
+    d = _em_observed(p)
+    eng = _engine(p, "fp64", every=32)
+    _, gg = _em_gpu_grads(p, eng, d)
+    dev = eng.device
+    base = {k: torch.tensor(p.coef[k], dtype=torch.float64, device=dev) for k in EMP}
+    dd = d.to(dev)
+
+    def J(prm):
+        with torch.no_grad():
+            return float(0.5 * ((em2_traces(eng, [prm]) - dd) ** 2).sum())
+
+    rng = np.random.default_rng(2026)
+    worst = {}
+    for k in ("e11", "e33", "s11", "s33"):
+        g = gg[k]
+        cand = np.argwhere(np.abs(g) >= 0.01 * np.abs(g).max())
+        pick = cand[rng.choice(len(cand), size=min(20, len(cand)), replace=False)]
+        errs = []
+        for j, i in pick:
+            h = 1e-4 * abs(float(p.coef[k][j, i]))
+            up = {q: v.clone() for q, v in base.items()}
+            dn = {q: v.clone() for q, v in base.items()}
+            up[k][j, i] += h
+            dn[k][j, i] -= h
+            errs.append(abs((J(up) - J(dn)) / (2 * h) - g[j, i]) / abs(g[j, i]))
+        worst[k] = max(errs)
+    assert all(v < 1e-3 for v in worst.values()), worst
