@@ -15,7 +15,7 @@
  * (nx, nz); 3-D fields are (nx, ny, nz).  CPML coefficients are the 1-D
  * profiles the legacy solver streams in (readwrite_routines.f95:26-37).
  *
- * Kept exactly as the legacy code does them (documented in DESIGN.md):
+ * Kept exactly as the legacy code does them:
  *   A1  one psi array per derivative, updated by every stress loop that
  *       computes that derivative (seismicfdtd.f95:255-262 and :291-298);
  *   A2  memory_dvx_dx in the normal-stress loop decays with b_x_half(j)

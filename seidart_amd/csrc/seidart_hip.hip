@@ -16,12 +16,14 @@
 //                 interior carries no C-PML state at all.
 //   traces        [it][comp][rec] per shot
 //
-// Execution: every time step is two field-update launches (stress then
-// velocity, or H then E) over one (shot, tile) work list covering every
-// shot of every model, plus one receiver-gather launch.  Work items are
-// ordered so that all shots of one tile land on the same XCD (blocks are
-// dealt to the 8 XCDs round-robin), so a model's coefficient tile is read
-// from HBM once and served from that XCD's L2 to the other shots.
+// Execution: every time step is one field-update launch (2-D elastic, a
+// single pass) or two (stress then velocity, or H then E) over one
+// (shot, tile) work list covering every shot of every model, plus one
+// receiver-gather launch.  Work items are ordered so that all shots of one
+// tile land on the same XCD (blocks are dealt to the 8 XCDs round-robin),
+// so a model's coefficient tile is read from HBM once and served from that
+// XCD's L2 to the other shots.  The adjoint kernels (2-D elastic and EM)
+// sit after the forward kernels of their scheme.
 
 #include <hip/hip_runtime.h>
 
