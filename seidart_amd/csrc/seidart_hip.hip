@@ -32,6 +32,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/seidart_hip.h"
@@ -173,6 +174,9 @@ __device__ __forceinline__ bool in_slab(int i, int n, int l) { return i < l || i
 #ifndef SDT_EL2_WAVES_PER_SIMD
 #define SDT_EL2_WAVES_PER_SIMD 4
 #endif
+#ifndef SDT_EL2_PREFETCH
+#define SDT_EL2_PREFETCH 0
+#endif
 #ifndef SDT_EL2_WAVES_PER_BLOCK
 #define SDT_EL2_WAVES_PER_BLOCK 4
 #endif
@@ -181,24 +185,19 @@ constexpr int EW = 61;            // owned columns per wavefront
 // parameter, 1, 2 or 4) and rows per tile (run time); sdt_plan_set_tiles
 // changes both, and the autotuner picks them per plan.
 
+// Array addressed by 32-bit element offsets: the scalar base plus a 32-bit
+// vector byte offset maps onto the global load/store form with an SGPR base
+// and a VGPR offset, so no per-access 64-bit address arithmetic.  Every
+// array it indexes is smaller than 4 GiB (checked at plan creation).
 template <typename T>
-__device__ __forceinline__ T ld_stream(gcptr<T> p) {
-#if SDT_NT_LOAD
-  return __builtin_nontemporal_load(p);
-#else
-  return *p;
-#endif
-}
-
-// Outputs of a step are not read again within it: stream them past L2.
-template <typename T>
-__device__ __forceinline__ void st_stream(gptr<T> p, T v) {
-#if SDT_NT_STORE
-  __builtin_nontemporal_store(v, p);
-#else
-  *p = v;
-#endif
-}
+struct A32 {
+  using PT = __attribute__((address_space(1))) T*;
+  PT p;
+  __device__ explicit A32(const void* q) : p((PT)q) {}
+  __device__ __forceinline__ __attribute__((address_space(1))) T& operator[](uint32_t e) const {
+    return *(PT)((__attribute__((address_space(1))) char*)p + e * (uint32_t)sizeof(T));
+  }
+};
 
 template <typename T>
 __device__ __forceinline__ T lane_next(T v) {  // value of lane l+1
@@ -266,22 +265,22 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
     tin = static_cast<const T*>(A.trail[tb]) + (long)slot * A.slot;
     tout = static_cast<T*>(A.trail[tb]) + (long)(slot + 1) * A.slot;
   }
-  const auto vxi = cp<T>(TRAIL ? tin : (par ? S.g[0] : S.f[0]));
-  const auto vzi = cp<T>(TRAIL ? tin + nn : (par ? S.g[1] : S.f[1]));
-  const auto sxxi = cp<T>(TRAIL ? tin + 2 * nn : (par ? S.g[2] : S.f[2]));
-  const auto szzi = cp<T>(TRAIL ? tin + 3 * nn : (par ? S.g[3] : S.f[3]));
-  const auto sxzi = cp<T>(TRAIL ? tin + 4 * nn : (par ? S.g[4] : S.f[4]));
-  auto vxo = mp<T>(TRAIL ? tout : (par ? S.f[0] : S.g[0]));
-  auto vzo = mp<T>(TRAIL ? tout + nn : (par ? S.f[1] : S.g[1]));
-  auto sxxo = mp<T>(TRAIL ? tout + 2 * nn : (par ? S.f[2] : S.g[2]));
-  auto szzo = mp<T>(TRAIL ? tout + 3 * nn : (par ? S.f[3] : S.g[3]));
-  auto sxzo = mp<T>(TRAIL ? tout + 4 * nn : (par ? S.f[4] : S.g[4]));
-  const auto pin = cp<T>(TRAIL ? tin + 5 * nn : (par ? S.psi2 : S.psi));
-  auto pout = mp<T>(TRAIL ? tout + 5 * nn : (par ? S.psi : S.psi2));
+  const A32<const T> vxi(TRAIL ? tin : (par ? S.g[0] : S.f[0]));
+  const A32<const T> vzi(TRAIL ? tin + nn : (par ? S.g[1] : S.f[1]));
+  const A32<const T> sxxi(TRAIL ? tin + 2 * nn : (par ? S.g[2] : S.f[2]));
+  const A32<const T> szzi(TRAIL ? tin + 3 * nn : (par ? S.g[3] : S.f[3]));
+  const A32<const T> sxzi(TRAIL ? tin + 4 * nn : (par ? S.g[4] : S.f[4]));
+  const A32<T> vxo(TRAIL ? tout : (par ? S.f[0] : S.g[0]));
+  const A32<T> vzo(TRAIL ? tout + nn : (par ? S.f[1] : S.g[1]));
+  const A32<T> sxxo(TRAIL ? tout + 2 * nn : (par ? S.f[2] : S.g[2]));
+  const A32<T> szzo(TRAIL ? tout + 3 * nn : (par ? S.f[3] : S.g[3]));
+  const A32<T> sxzo(TRAIL ? tout + 4 * nn : (par ? S.f[4] : S.g[4]));
+  const A32<const T> pin(TRAIL ? tin + 5 * nn : (par ? S.psi2 : S.psi));
+  const A32<T> pout(TRAIL ? tout + 5 * nn : (par ? S.psi : S.psi2));
   const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dt = T(M.dt);
   const int lx = M.lx, lz = M.lz;
-  const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
-  const long ZB = (SPLIT ? 6 : 4) * SX;  // first z-slab array
+  const uint32_t SX = (uint32_t)nz * 2 * lx, SZ = (uint32_t)2 * lz * nx;
+  const uint32_t ZB = (SPLIT ? 6 : 4) * SX;  // first z-slab array
   const bool inx = incol && in_slab(i, nx, lx);
   const int xs = inx ? slab_pos(i, nx, lx) : 0;
   const Prof<T> P(&M.prof[0][0]);
@@ -290,11 +289,11 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
   const auto ikz = P[12 + PK], az = P[12 + PA], bz = P[12 + PB];
   const auto ikzh = P[12 + PKH], azh = P[12 + PAH], bzh = P[12 + PBH];
   const auto brow = cp<T>(M.aux);
-  const auto N11 = cp<T>(M.coef[0]), N13 = cp<T>(M.coef[1]), N15 = cp<T>(M.coef[2]);
-  const auto N33 = cp<T>(M.coef[3]), N35 = cp<T>(M.coef[4]);
-  const auto S15 = cp<T>(M.coef[5]), S35 = cp<T>(M.coef[6]), S55 = cp<T>(M.coef[7]);
-  const auto DTX = cp<T>(M.coef[8]), DTZ = cp<T>(M.coef[9]);
-  const auto OMX = cp<T>(M.coef[10]), OMZ = cp<T>(M.coef[11]);
+  const A32<const T> N11(M.coef[0]), N13(M.coef[1]), N15(M.coef[2]);
+  const A32<const T> N33(M.coef[3]), N35(M.coef[4]);
+  const A32<const T> S15(M.coef[5]), S35(M.coef[6]), S55(M.coef[7]);
+  const A32<const T> DTX(M.coef[8]), DTZ(M.coef[9]);
+  const A32<const T> OMX(M.coef[10]), OMZ(M.coef[11]);
   const bool mono = M.coef[2] != nullptr;   // c15 / c35 terms present
   const bool atten = M.coef[10] != nullptr; // gamma present
   const bool can_n = incol && lane <= EW + 1 && i <= nx - 2;          // normal stress (needs i+1)
@@ -306,7 +305,7 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
   const T incx = (srcx != nullptr) ? srcx[it] : T(0);
   const T incz = (srcz != nullptr) ? srcz[it] : T(0);
 
-  auto row = [&](int r) { return (long)(r < 0 ? 0 : (r >= nz ? nz - 1 : r)) * nx + ic; };
+  auto row = [&](int r) { return (uint32_t)(r < 0 ? 0 : (r >= nz ? nz - 1 : r)) * nx + ic; };
   auto valid = [&](int r) { return incol && r >= 0 && r < nz; };
   T vx_m = valid(zo - 2) ? vxi[row(zo - 2)] : T(0);  // velocity row r-1
   T vz_m = valid(zo - 2) ? vzi[row(zo - 2)] : T(0);
@@ -314,40 +313,87 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
   T vz_c = valid(zo - 1) ? vzi[row(zo - 1)] : T(0);
   T sxx_p = 0, szz_p = 0, sxz_p = 0, sxz_pp = 0;      // new stresses of rows r-1, r-2
 
-  for (int r = zo - 1; r <= zend; ++r) {
-    // ---- all loads of this row first (addresses clamped in bounds) ----
-    const int rr = r - 1;                             // velocity row
-    const long c = row(r), cv = row(rr), cn = row(r + 1);
-    const bool rv = valid(r);
-    const T ld_vxn = ld_stream(vxi + cn), ld_vzn = ld_stream(vzi + cn);
-    const T ld_sxx = ld_stream(sxxi + c), ld_szz = ld_stream(szzi + c), ld_sxz = ld_stream(sxzi + c);
-    const T n11 = N11[c], n13 = N13[c], n33 = N33[c], s55 = S55[c];
+  // Every load a row needs (addresses clamped in bounds).  With
+  // SDT_EL2_PREFETCH the loads of row r+1 are issued before row r is
+  // computed, so each wave keeps two rows of loads in flight.
+  struct Row {
+    T vxn, vzn, sxx, szz, sxz, n11, n13, n33, s55, n15, n35, s15, s35, dtx, dtz, omx, omz;
+    T m_vxx, m_vzx, m_vzz, m_vxz, s_vxx, s_vzx, s_vzz, s_vxz, m_sxx, m_sxzx, m_szz, m_sxzz;
+  };
+  auto load_row = [&](int r) {
+    Row L;
+    const int rr = r - 1;
+    const uint32_t c = row(r), cv = row(rr), cn = row(r + 1);
+    L.vxn = vxi[cn];
+    L.vzn = vzi[cn];
+    L.sxx = sxxi[c];
+    L.szz = szzi[c];
+    L.sxz = sxzi[c];
+    L.n11 = N11[c];
+    L.n13 = N13[c];
+    L.n33 = N33[c];
+    L.s55 = S55[c];
     // monoclinic couplings and attenuation are optional (null = 0 and 1)
-    const T n15 = mono ? N15[c] : T(0), n35 = mono ? N35[c] : T(0);
-    const T s15 = mono ? S15[c] : T(0), s35 = mono ? S35[c] : T(0);
-    const T dtx = DTX[cv], dtz = DTZ[cv];
-    const T omx = atten ? OMX[cv] : T(1), omz = atten ? OMZ[cv] : T(1);
+    L.n15 = mono ? N15[c] : T(0);
+    L.n35 = mono ? N35[c] : T(0);
+    L.s15 = mono ? S15[c] : T(0);
+    L.s35 = mono ? S35[c] : T(0);
+    L.dtx = DTX[cv];
+    L.dtz = DTZ[cv];
+    L.omx = atten ? OMX[cv] : T(1);
+    L.omz = atten ? OMZ[cv] : T(1);
     const int rs = r < 0 ? 0 : (r >= nz ? nz - 1 : r);
     const int rvs = rr < 0 ? 0 : (rr >= nz ? nz - 1 : rr);
     const bool inz = in_slab(rs, nz, lz), inzv = in_slab(rvs, nz, lz);
-    const long ox = (long)rs * 2 * lx + xs, oxv = (long)rvs * 2 * lx + xs;
-    const long oz = ZB + (inz ? (long)slab_pos(rs, nz, lz) * nx + ic : 0);
-    const long ozv = ZB + (inzv ? (long)slab_pos(rvs, nz, lz) * nx + ic : 0);
-    const long oxs = inx ? ox : 0, oxvs = inx ? oxv : 0;
+    const uint32_t oxs = inx ? (uint32_t)rs * 2 * lx + xs : 0, oxvs = inx ? (uint32_t)rvs * 2 * lx + xs : 0;
+    const uint32_t oz = ZB + (inz ? (uint32_t)slab_pos(rs, nz, lz) * nx + ic : 0);
+    const uint32_t ozv = ZB + (inzv ? (uint32_t)slab_pos(rvs, nz, lz) * nx + ic : 0);
     // psi loads only where the cell has that memory (a zero-width slab has no storage)
     const bool inzc = incol && inz, inzvc = incol && inzv;
-    T m_vxx = inx ? pin[oxs] : T(0), m_vzx = inx ? pin[SX + oxs] : T(0);        // x-slab, stress row
-    T m_vzz = inzc ? pin[oz] : T(0), m_vxz = inzc ? pin[SZ + oz] : T(0);        // z-slab, stress row
+    L.m_vxx = inx ? pin[oxs] : T(0);  // x-slab, stress row
+    L.m_vzx = inx ? pin[SX + oxs] : T(0);
+    L.m_vzz = inzc ? pin[oz] : T(0);  // z-slab, stress row
+    L.m_vxz = inzc ? pin[SZ + oz] : T(0);
     // split C-PML: the shear loop's own memory (x slab arrays 4, 5; z 4, 5)
-    T s_vxx = 0, s_vzx = 0, s_vzz = 0, s_vxz = 0;
-    if (SPLIT) {
-      s_vxx = inx ? pin[4 * SX + oxs] : T(0);
-      s_vzx = inx ? pin[5 * SX + oxs] : T(0);
-      s_vzz = inzc ? pin[4 * SZ + oz] : T(0);
-      s_vxz = inzc ? pin[5 * SZ + oz] : T(0);
+    L.s_vxx = (SPLIT && inx) ? pin[4 * SX + oxs] : T(0);
+    L.s_vzx = (SPLIT && inx) ? pin[5 * SX + oxs] : T(0);
+    L.s_vzz = (SPLIT && inzc) ? pin[4 * SZ + oz] : T(0);
+    L.s_vxz = (SPLIT && inzc) ? pin[5 * SZ + oz] : T(0);
+    L.m_sxx = inx ? pin[2 * SX + oxvs] : T(0);  // velocity row
+    L.m_sxzx = inx ? pin[3 * SX + oxvs] : T(0);
+    L.m_szz = inzvc ? pin[2 * SZ + ozv] : T(0);
+    L.m_sxzz = inzvc ? pin[3 * SZ + ozv] : T(0);
+    return L;
+  };
+  // (FP64 keeps one row in flight: two would spill at its register budget)
+  constexpr bool PF = SDT_EL2_PREFETCH && sizeof(T) == 4;
+  Row ahead{};
+  if constexpr (PF) ahead = load_row(zo - 1);
+
+  for (int r = zo - 1; r <= zend; ++r) {
+    Row L;
+    if constexpr (PF) {
+      L = ahead;
+      ahead = load_row(r + 1);
+    } else {
+      L = load_row(r);
     }
-    T m_sxx = inx ? pin[2 * SX + oxvs] : T(0), m_sxzx = inx ? pin[3 * SX + oxvs] : T(0);  // velocity row
-    T m_szz = inzvc ? pin[2 * SZ + ozv] : T(0), m_sxzz = inzvc ? pin[3 * SZ + ozv] : T(0);
+    const int rr = r - 1;                             // velocity row
+    const uint32_t c = row(r), cv = row(rr);
+    const bool rv = valid(r);
+    const T ld_vxn = L.vxn, ld_vzn = L.vzn, ld_sxx = L.sxx, ld_szz = L.szz, ld_sxz = L.sxz;
+    const T n11 = L.n11, n13 = L.n13, n33 = L.n33, s55 = L.s55;
+    const T n15 = L.n15, n35 = L.n35, s15 = L.s15, s35 = L.s35;
+    const T dtx = L.dtx, dtz = L.dtz, omx = L.omx, omz = L.omz;
+    const int rs = r < 0 ? 0 : (r >= nz ? nz - 1 : r);
+    const int rvs = rr < 0 ? 0 : (rr >= nz ? nz - 1 : rr);
+    const bool inz = in_slab(rs, nz, lz), inzv = in_slab(rvs, nz, lz);
+    const uint32_t ox = (uint32_t)rs * 2 * lx + xs, oxv = (uint32_t)rvs * 2 * lx + xs;
+    const uint32_t oz = ZB + (inz ? (uint32_t)slab_pos(rs, nz, lz) * nx + ic : 0);
+    const uint32_t ozv = ZB + (inzv ? (uint32_t)slab_pos(rvs, nz, lz) * nx + ic : 0);
+    T m_vxx = L.m_vxx, m_vzx = L.m_vzx, m_vzz = L.m_vzz, m_vxz = L.m_vxz;
+    T s_vxx = L.s_vxx, s_vzx = L.s_vzx, s_vzz = L.s_vzz, s_vxz = L.s_vxz;
+    T m_sxx = L.m_sxx, m_sxzx = L.m_sxzx, m_szz = L.m_szz, m_sxzz = L.m_sxzz;
     const T bzr = bz[rs], azr = az[rs], ikzr = ikz[rs], bzhr = bzh[rs], azhr = azh[rs], ikzhr = ikzh[rs];
     const T browr = SPLIT ? bxh : brow[rs];  // b_x_half(j) (A2), or b_x_half(i)
     const T bzv = bz[rvs], azv = az[rvs], ikzv = ikz[rvs], bzhv = bzh[rvs], azhv = azh[rvs], ikzhv = ikzh[rvs];
@@ -404,9 +450,9 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
       sxz_c = sxz_c + (s15 * dvx_dx + s35 * dvz_dz + s55 * (dvz_dx + dvx_dz)) * dt;
     }
     if (own && rv && r >= zo && r < zend) {
-      st_stream(sxxo + c, sxx_c);
-      st_stream(szzo + c, szz_c);
-      st_stream(sxzo + c, sxz_c);
+      sxxo[c] = sxx_c;
+      szzo[c] = szz_c;
+      sxzo[c] = sxz_c;
       if (inx) {
         pout[ox] = m_vxx;
         pout[SX + ox] = m_vzx;
@@ -461,8 +507,8 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
       if (is_src_col && rr == ksrc) { vxn += incx; vzn += incz; }   // source, :359-361
       if (i == 0 || i == nx - 1 || rr == 0 || rr == nz - 1) { vxn = 0; vzn = 0; }  // :365-375
       check_blowup((double)vxn * vxn + (double)vzn * vzn, S.blowup, it);
-      st_stream(vxo + cv, vxn);
-      st_stream(vzo + cv, vzn);
+      vxo[cv] = vxn;
+      vzo[cv] = vzn;
     }
     sxz_pp = sxz_p;
     sxx_p = sxx_c;
@@ -529,6 +575,12 @@ __device__ __forceinline__ void gadd(double* g, double v) {
 __device__ __forceinline__ void gadd(float* g, double v) {
   const float f = (float)v;
   if (f != 0.0f) unsafeAtomicAdd(g, f);
+}
+__device__ __forceinline__ void gadd(__attribute__((address_space(1))) double* g, double v) {
+  gadd((double*)g, v);
+}
+__device__ __forceinline__ void gadd(__attribute__((address_space(1))) float* g, double v) {
+  gadd((float*)g, v);
 }
 template <typename T>
 struct GAcc;
@@ -897,31 +949,32 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
   const bool own = incol && lane >= 2 && lane < 2 + AW;
   const bool sl = incol && lane >= 1 && lane <= 2 + AW;  // lanes of the stress phase
   const int zend = min(zo + AH, nz);
-  const long n = (long)nx * nz;
+  // 32-bit element offsets within one state / gradient array (checked at setup)
+  const uint32_t n = (uint32_t)nx * nz;
   const int lx = M.lx, lz = M.lz;
-  const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
-  const long ZB = 5 * n + (SPLIT ? 6 : 4) * SX;
-  const auto Lin = cp<T>(A.lam[(t + 1) & 1]);
-  auto Lout = mp<T>(A.lam[t & 1]);
-  const auto u1 = cp<T>(A.trail[tb]) + (long)(t + 1 - t0) * A.slot;
-  const auto u0 = cp<T>(A.trail[tb]) + (long)(t - t0) * A.slot;
-  auto G = GAcc<T>::of(A);
+  const uint32_t SX = (uint32_t)nz * 2 * lx, SZ = (uint32_t)2 * lz * nx;
+  const uint32_t ZB = 5 * n + (SPLIT ? 6 : 4) * SX;
+  const A32<const T> Lin(A.lam[(t + 1) & 1]);
+  const A32<T> Lout(A.lam[t & 1]);
+  const A32<const T> u1(static_cast<const T*>(A.trail[tb]) + (long)(t + 1 - t0) * A.slot);
+  const A32<const T> u0(static_cast<const T*>(A.trail[tb]) + (long)(t - t0) * A.slot);
+  const A32<typename std::remove_pointer<decltype(GAcc<T>::of(A))>::type> G(GAcc<T>::of(A));
   const Prof<T> P(&M.prof[0][0]);
   const bool inx = incol && in_slab(i, nx, lx);
-  const long xs = 5 * n + (inx ? slab_pos(i, nx, lx) : 0);
+  const uint32_t xs = 5 * n + (inx ? slab_pos(i, nx, lx) : 0);
   const T ikx = P[PK][ic], ax = P[PA][ic], bx = P[PB][ic];
   const T ikxh = P[PKH][ic], axh = P[PAH][ic], bxh = P[PBH][ic];
   const auto ikz = P[12 + PK], az = P[12 + PA], bz = P[12 + PB];
   const auto ikzh = P[12 + PKH], azh = P[12 + PAH], bzh = P[12 + PBH];
   const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dt = T(M.dt);
-  const auto N11 = cp<T>(M.coef[0]), N13 = cp<T>(M.coef[1]), N15 = cp<T>(M.coef[2]);
-  const auto N33 = cp<T>(M.coef[3]), N35 = cp<T>(M.coef[4]);
-  const auto S15 = cp<T>(M.coef[5]), S35 = cp<T>(M.coef[6]), S55 = cp<T>(M.coef[7]);
-  const auto DTX = cp<T>(M.coef[8]), DTZ = cp<T>(M.coef[9]);
-  const auto OMX = cp<T>(M.coef[10]), OMZ = cp<T>(M.coef[11]);
+  const A32<const T> N11(M.coef[0]), N13(M.coef[1]), N15(M.coef[2]);
+  const A32<const T> N33(M.coef[3]), N35(M.coef[4]);
+  const A32<const T> S15(M.coef[5]), S35(M.coef[6]), S55(M.coef[7]);
+  const A32<const T> DTX(M.coef[8]), DTZ(M.coef[9]);
+  const A32<const T> OMX(M.coef[10]), OMZ(M.coef[11]);
   const auto brow = cp<T>(M.aux);
   const bool mono = M.coef[2] != nullptr, atten = M.coef[10] != nullptr;
-  auto row = [&](int r) { return (long)(r < 0 ? 0 : (r >= nz ? nz - 1 : r)) * nx + ic; };
+  auto row = [&](int r) { return (uint32_t)(r < 0 ? 0 : (r >= nz ? nz - 1 : r)) * nx + ic; };
   auto valid = [&](int r) { return incol && r >= 0 && r < nz; };
   auto clampr = [&](int r) { return r < 0 ? 0 : (r >= nz ? nz - 1 : r); };
 
@@ -936,13 +989,13 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
 
   for (int r = zo - 2; r <= zend + 1; ++r) {
     const int q = r - 1, pr = r - 2;
-    const long cr = row(r), cq = row(q), cn = row(r + 1);
+    const uint32_t cr = row(r), cq = row(q), cn = row(r + 1);
     const bool vr = valid(r), vq = valid(q);
     const int rs = clampr(r), qs = clampr(q);
     const bool inzr = incol && in_slab(rs, nz, lz), inzq = incol && in_slab(qs, nz, lz);
-    const long oxr = xs + (inx ? (long)rs * 2 * lx : 0), oxq = xs + (inx ? (long)qs * 2 * lx : 0);
-    const long ozr = ZB + (inzr ? (long)slab_pos(rs, nz, lz) * nx + ic : 0);
-    const long ozq = ZB + (inzq ? (long)slab_pos(qs, nz, lz) * nx + ic : 0);
+    const uint32_t oxr = xs + (inx ? (uint32_t)rs * 2 * lx : 0), oxq = xs + (inx ? (uint32_t)qs * 2 * lx : 0);
+    const uint32_t ozr = ZB + (inzr ? (uint32_t)slab_pos(rs, nz, lz) * nx + ic : 0);
+    const uint32_t ozq = ZB + (inzq ? (uint32_t)slab_pos(qs, nz, lz) * nx + ic : 0);
     // ---- loads ----
     const T lvx0 = Lin[cr], lvz0 = Lin[n + cr];
     const T dtx = DTX[cr], dtz = DTZ[cr];
@@ -961,7 +1014,7 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
     const T Lz2 = inzr ? Lin[ozr + 2 * SZ] : T(0), Lz3 = inzr ? Lin[ozr + 3 * SZ] : T(0);
     const T Uz2 = inzr ? u1[ozr + 2 * SZ] : T(0), Uz3 = inzr ? u1[ozr + 3 * SZ] : T(0);
     // of the stress row q (velocity-derivative memory)
-    const long xq2 = SPLIT ? 4 * SX : 0, zq2 = SPLIT ? 4 * SZ : 0;  // shear loop's arrays
+    const uint32_t xq2 = SPLIT ? 4 * SX : 0, zq2 = SPLIT ? 4 * SZ : 0;  // shear loop's arrays
     const T Lx0 = inx ? Lin[oxq] : T(0), Lx1 = inx ? Lin[oxq + SX] : T(0);
     const T Lz0 = inzq ? Lin[ozq] : T(0), Lz1 = inzq ? Lin[ozq + SZ] : T(0);
     const T Ox0 = inx ? u0[oxq] : T(0), Ox1 = inx ? u0[oxq + SX] : T(0);
@@ -1014,8 +1067,8 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
       }
       pvx = omx * lvx;
       if (ownr) {
-        if (gmask & (1u << 8)) gadd(G + 8 * n + cr, (double)lvx * (double)(b1 + b2));
-        if (gmask & (1u << 10)) gadd(G + 10 * n + cr, (double)lvx * (double)vx_n);
+        if (gmask & (1u << 8)) gadd(&G[8 * n + cr], (double)lvx * (double)(b1 + b2));
+        if (gmask & (1u << 10)) gadd(&G[10 * n + cr], (double)lvx * (double)vx_n);
       }
     } else if (ownr) {
       if (inx) Lout[oxr + 2 * SX] = Lx2;
@@ -1040,8 +1093,8 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
       }
       pvz = omz * lvz;
       if (ownr) {
-        if (gmask & (1u << 9)) gadd(G + 9 * n + cr, (double)lvz * (double)(b1 + b2));
-        if (gmask & (1u << 11)) gadd(G + 11 * n + cr, (double)lvz * (double)vz_n);
+        if (gmask & (1u << 9)) gadd(&G[9 * n + cr], (double)lvz * (double)(b1 + b2));
+        if (gmask & (1u << 11)) gadd(&G[11 * n + cr], (double)lvz * (double)vz_n);
       }
     } else if (ownr) {
       if (inx) Lout[oxr + 3 * SX] = Lx3;
@@ -1095,9 +1148,9 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
         }
         if (ownq) {
           const double m = (double)mu;
-          if (gmask & (1u << 5)) gadd(G + 5 * n + cq, m * (double)g1);
-          if (gmask & (1u << 6)) gadd(G + 6 * n + cq, m * (double)g2);
-          if (gmask & (1u << 7)) gadd(G + 7 * n + cq, m * ((double)g3 + (double)g4));
+          if (gmask & (1u << 5)) gadd(&G[5 * n + cq], m * (double)g1);
+          if (gmask & (1u << 6)) gadd(&G[6 * n + cq], m * (double)g2);
+          if (gmask & (1u << 7)) gadd(&G[7 * n + cq], m * ((double)g3 + (double)g4));
         }
       }
       if (q >= 1 && i <= nx - 2) {  // N^T
@@ -1129,11 +1182,11 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
         }
         if (ownq) {
           const double mx = (double)mux, mz = (double)muz, sh = (double)d3 + (double)d4;
-          if (gmask & (1u << 0)) gadd(G + cq, mx * (double)d1);
-          if (gmask & (1u << 1)) gadd(G + n + cq, mx * (double)d2 + mz * (double)d1);
-          if (gmask & (1u << 2)) gadd(G + 2 * n + cq, mx * sh);
-          if (gmask & (1u << 3)) gadd(G + 3 * n + cq, mz * (double)d2);
-          if (gmask & (1u << 4)) gadd(G + 4 * n + cq, mz * sh);
+          if (gmask & (1u << 0)) gadd(&G[cq], mx * (double)d1);
+          if (gmask & (1u << 1)) gadd(&G[n + cq], mx * (double)d2 + mz * (double)d1);
+          if (gmask & (1u << 2)) gadd(&G[2 * n + cq], mx * sh);
+          if (gmask & (1u << 3)) gadd(&G[3 * n + cq], mz * (double)d2);
+          if (gmask & (1u << 4)) gadd(&G[4 * n + cq], mz * sh);
         }
       }
       if (ownq) {
@@ -1158,7 +1211,7 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
 
     // ---- velocities of row pr: gather of the stress-phase adjoints ----
     if (own && pr >= zo && pr < zend) {
-      const long cp_ = row(pr);
+      const uint32_t cp_ = row(pr);
       Lout[cp_] = pvx2 + idx * (E1p_w - E1p) + idz * (E4p - Ee4) + idx * (F1p - F1p_e) +
                   idz * (F4pp - F4p);
       Lout[n + cp_] = pvz2 + idz * (E2p - Ee2) + idx * (E3p_w - E3p) + idz * (F2pp - F2p) +
@@ -2230,6 +2283,13 @@ int validate(int kind, bool split, int prec, const sdt_model* models, int nmodel
     }
     if (kind == SDT_ELASTIC2 && !split && !M.aux)
       return fail(SDT_EINVAL, tag + "aux (b_x_half by row) missing");
+    if (kind == SDT_ELASTIC2) {
+      // the step kernel addresses each array with 32-bit byte offsets
+      const long long es = prec == SDT_FP64 ? 8 : 4;
+      const long long np = psi_elems(kpsi, M.nx, M.ny, M.nz, M.lx, M.ly, M.lz);
+      if ((long long)M.nx * M.nz * es >= (1LL << 32) || np * es >= (1LL << 32))
+        return fail(SDT_EINVAL, tag + "grid too large (every array must stay under 4 GiB)");
+    }
   }
   for (int s = 0; s < nshots; ++s) {
     const sdt_shot& S = shots[s];
@@ -2716,6 +2776,14 @@ int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots, unsigned 
     if (a.trail_len < 2) return fail(SDT_EINVAL, tag + "trail_len must be at least 2");
     const sdt_model& M = p->hmodels[p->hshots[s].model];
     DAdj& D = h[s];
+    if (p->kind == SDT_ELASTIC2) {
+      // the fused adjoint addresses a state and a gradient with 32-bit byte offsets
+      const long long es = p->prec == SDT_FP64 ? 8 : 4;
+      const long long st = state_elems(p->kind | (p->split ? SDT_SPLIT_CPML : 0), M.nx, M.ny, M.nz,
+                                       M.lx, M.ly, M.lz);
+      if (st * es >= (1LL << 32) || 12LL * M.nx * M.nz * es >= (1LL << 32))
+        return fail(SDT_EINVAL, tag + "grid too large for the adjoint (state or gradient over 4 GiB)");
+    }
     D.trail[0] = a.trail[0];
     D.trail[1] = a.trail[1];
     D.slot = (long)state_elems(p->kind | (p->split ? SDT_SPLIT_CPML : 0), M.nx, M.ny, M.nz, M.lx,
