@@ -208,7 +208,7 @@ __device__ __forceinline__ T lane_prev(T v) {  // value of lane l-1
   return __shfl_up(v, 1, 64);
 }
 
-template <typename T, bool TRAIL, bool SPLIT>
+template <typename T, bool TRAIL, bool SPLIT, bool PML>
 __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
                                          const DShot* __restrict__ shots, const int4 wi, const int it,
                                          const int lane, const DAdj* __restrict__ adj, const int slot,
@@ -236,17 +236,29 @@ k_el2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
     const int4 wi = make_int4(__builtin_amdgcn_readfirstlane(wv.x),
                               __builtin_amdgcn_readfirstlane(wv.y) + wave * EW,
                               __builtin_amdgcn_readfirstlane(wv.z), 0);
-    if (wi.x >= 0 && wi.y < models[shots[wi.x].model].nx)
-      el2_tile<T, TRAIL, SPLIT>(models, shots, wi, it, threadIdx.x & 63, adj, rel,
-                                TRAIL ? __builtin_amdgcn_readfirstlane(*tsel) : 0, eh);
+    if (wi.x < 0) continue;
+    const DModel& M = models[shots[wi.x].model];
+    if (wi.y >= M.nx) continue;
+    // a wave whose columns and rows (halo included) avoid every C-PML slab
+    // runs the body without memory variables
+    const bool interior = wi.y - 1 >= M.lx && wi.y + 62 < M.nx - M.lx && wi.z - 2 >= M.lz &&
+                          wi.z + eh + 1 < M.nz - M.lz;
+    const int tb = TRAIL ? __builtin_amdgcn_readfirstlane(*tsel) : 0;
+    if (interior)
+      el2_tile<T, TRAIL, SPLIT, false>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh);
+    else
+      el2_tile<T, TRAIL, SPLIT, true>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh);
   }
 }
 
-template <typename T, bool TRAIL, bool SPLIT>
+template <typename T, bool TRAIL, bool SPLIT, bool PML>
 __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
                                          const DShot* __restrict__ shots, const int4 wi, const int it,
                                          const int lane, const DAdj* __restrict__ adj, const int slot,
                                          const int tb, const int eh) {
+  // Every fused multiply-add is written out; nothing else is contracted, so
+  // the interior and C-PML bodies (and every tile shape) round identically.
+#pragma clang fp contract(off)
   const DShot& S = shots[wi.x];
   const DModel& M = models[S.model];
   const int nx = M.nx, nz = M.nz;
@@ -281,7 +293,7 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
   const int lx = M.lx, lz = M.lz;
   const uint32_t SX = (uint32_t)nz * 2 * lx, SZ = (uint32_t)2 * lz * nx;
   const uint32_t ZB = (SPLIT ? 6 : 4) * SX;  // first z-slab array
-  const bool inx = incol && in_slab(i, nx, lx);
+  const bool inx = PML && incol && in_slab(i, nx, lx);
   const int xs = inx ? slab_pos(i, nx, lx) : 0;
   const Prof<T> P(&M.prof[0][0]);
   const T ikx = P[PK][ic], ax = P[PA][ic], bx = P[PB][ic];
@@ -344,7 +356,7 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
     L.omz = atten ? OMZ[cv] : T(1);
     const int rs = r < 0 ? 0 : (r >= nz ? nz - 1 : r);
     const int rvs = rr < 0 ? 0 : (rr >= nz ? nz - 1 : rr);
-    const bool inz = in_slab(rs, nz, lz), inzv = in_slab(rvs, nz, lz);
+    const bool inz = PML && in_slab(rs, nz, lz), inzv = PML && in_slab(rvs, nz, lz);
     const uint32_t oxs = inx ? (uint32_t)rs * 2 * lx + xs : 0, oxvs = inx ? (uint32_t)rvs * 2 * lx + xs : 0;
     const uint32_t oz = ZB + (inz ? (uint32_t)slab_pos(rs, nz, lz) * nx + ic : 0);
     const uint32_t ozv = ZB + (inzv ? (uint32_t)slab_pos(rvs, nz, lz) * nx + ic : 0);
@@ -387,7 +399,7 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
     const T dtx = L.dtx, dtz = L.dtz, omx = L.omx, omz = L.omz;
     const int rs = r < 0 ? 0 : (r >= nz ? nz - 1 : r);
     const int rvs = rr < 0 ? 0 : (rr >= nz ? nz - 1 : rr);
-    const bool inz = in_slab(rs, nz, lz), inzv = in_slab(rvs, nz, lz);
+    const bool inz = PML && in_slab(rs, nz, lz), inzv = PML && in_slab(rvs, nz, lz);
     const uint32_t ox = (uint32_t)rs * 2 * lx + xs, oxv = (uint32_t)rvs * 2 * lx + xs;
     const uint32_t oz = ZB + (inz ? (uint32_t)slab_pos(rs, nz, lz) * nx + ic : 0);
     const uint32_t ozv = ZB + (inzv ? (uint32_t)slab_pos(rvs, nz, lz) * nx + ic : 0);
@@ -411,20 +423,20 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
       T dvz_dx = (vz_e - vz_c) * idx;
       T dvx_dz = (vx_c - vx_m) * idz;
       if (inx_r) {
-        m_vxx = browr * m_vxx + axh * dvx_dx;  // :255
-        m_vzx = bx * m_vzx + ax * dvz_dx;
-        dvx_dx = dvx_dx * ikxh + m_vxx;
-        dvz_dx = dvz_dx * ikx + m_vzx;
+        m_vxx = fma(browr, m_vxx, axh * dvx_dx);  // :255
+        m_vzx = fma(bx, m_vzx, ax * dvz_dx);
+        dvx_dx = fma(dvx_dx, ikxh, m_vxx);
+        dvz_dx = fma(dvz_dx, ikx, m_vzx);
       }
       if (inz_r) {
-        m_vzz = bzr * m_vzz + azr * dvz_dz;
-        m_vxz = bzhr * m_vxz + azhr * dvx_dz;
-        dvz_dz = dvz_dz * ikzr + m_vzz;
-        dvx_dz = dvx_dz * ikzhr + m_vxz;
+        m_vzz = fma(bzr, m_vzz, azr * dvz_dz);
+        m_vxz = fma(bzhr, m_vxz, azhr * dvx_dz);
+        dvz_dz = fma(dvz_dz, ikzr, m_vzz);
+        dvx_dz = fma(dvx_dz, ikzhr, m_vxz);
       }
       const T sh = dvz_dx + dvx_dz;
-      sxx_c = sxx_c + (n11 * dvx_dx + n13 * dvz_dz + n15 * sh) * dt;
-      szz_c = szz_c + (n13 * dvx_dx + n33 * dvz_dz + n35 * sh) * dt;
+      sxx_c = fma(fma(n15, sh, fma(n13, dvz_dz, n11 * dvx_dx)), dt, sxx_c);
+      szz_c = fma(fma(n35, sh, fma(n33, dvz_dz, n13 * dvx_dx)), dt, szz_c);
     }
     if (rv && can_s && r <= nz - 2 && r < zend) {  // same psi arrays (A1) unless split
       T& q_vxx = SPLIT ? s_vxx : m_vxx;
@@ -436,18 +448,18 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
       T dvz_dx = (vz_c - vz_w) * idx;
       T dvx_dz = (vx_n - vx_c) * idz;
       if (inx_r) {
-        q_vxx = bxh * q_vxx + axh * dvx_dx;
-        q_vzx = bx * q_vzx + ax * dvz_dx;
-        dvx_dx = dvx_dx * ikxh + q_vxx;
-        dvz_dx = dvz_dx * ikx + q_vzx;
+        q_vxx = fma(bxh, q_vxx, axh * dvx_dx);
+        q_vzx = fma(bx, q_vzx, ax * dvz_dx);
+        dvx_dx = fma(dvx_dx, ikxh, q_vxx);
+        dvz_dx = fma(dvz_dx, ikx, q_vzx);
       }
       if (inz_r) {
-        q_vzz = bzr * q_vzz + azr * dvz_dz;
-        q_vxz = bzhr * q_vxz + azhr * dvx_dz;
-        dvz_dz = dvz_dz * ikzr + q_vzz;
-        dvx_dz = dvx_dz * ikzhr + q_vxz;
+        q_vzz = fma(bzr, q_vzz, azr * dvz_dz);
+        q_vxz = fma(bzhr, q_vxz, azhr * dvx_dz);
+        dvz_dz = fma(dvz_dz, ikzr, q_vzz);
+        dvx_dz = fma(dvx_dz, ikzhr, q_vxz);
       }
-      sxz_c = sxz_c + (s15 * dvx_dx + s35 * dvz_dz + s55 * (dvz_dx + dvx_dz)) * dt;
+      sxz_c = fma(fma(s55, dvz_dx + dvx_dz, fma(s35, dvz_dz, s15 * dvx_dx)), dt, sxz_c);
     }
     if (own && rv && r >= zo && r < zend) {
       sxxo[c] = sxx_c;
@@ -480,27 +492,27 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
         T a = (sxx_p - sxx_w) * idx;
         T b = (sxz_p - sxz_pp) * idz;
         if (inx) {
-          m_sxx = bx * m_sxx + ax * a;
-          a = a * ikx + m_sxx;
+          m_sxx = fma(bx, m_sxx, ax * a);
+          a = fma(a, ikx, m_sxx);
         }
         if (inzv) {
-          m_sxzz = bzv * m_sxzz + azv * b;
-          b = b * ikzv + m_sxzz;
+          m_sxzz = fma(bzv, m_sxzz, azv * b);
+          b = fma(b, ikzv, m_sxzz);
         }
-        vxn = vxn * omx + (a + b) * dtx;
+        vxn = fma(vxn, omx, (a + b) * dtx);
       }
       if (uz && rr <= nz - 2) {
         T a = (sxz_e - sxz_p) * idx;
         T b = (szz_c - szz_p) * idz;
         if (inx) {
-          m_sxzx = bxh * m_sxzx + axh * a;
-          a = a * ikxh + m_sxzx;
+          m_sxzx = fma(bxh, m_sxzx, axh * a);
+          a = fma(a, ikxh, m_sxzx);
         }
         if (inzv) {
-          m_szz = bzhv * m_szz + azhv * b;
-          b = b * ikzhv + m_szz;
+          m_szz = fma(bzhv, m_szz, azhv * b);
+          b = fma(b, ikzhv, m_szz);
         }
-        vzn = vzn * omz + (a + b) * dtz;
+        vzn = fma(vzn, omz, (a + b) * dtz);
       }
       if (inx) { pout[2 * SX + oxv] = m_sxx; pout[3 * SX + oxv] = m_sxzx; }
       if (inzv) { pout[2 * SZ + ozv] = m_szz; pout[3 * SZ + ozv] = m_sxzz; }
