@@ -945,7 +945,7 @@ constexpr int AW = 60;   // owned columns per wavefront
 constexpr int AWB = 4;   // wavefronts per block, side by side in x
 constexpr int AH = SDT_EL2_ROWS;
 
-template <typename T, bool SPLIT>
+template <typename T, bool SPLIT, bool PML>
 __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
                                              const DShot* __restrict__ shots,
                                              const DAdj* __restrict__ adj, const int s, const int xo,
@@ -972,7 +972,7 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
   const A32<const T> u0(static_cast<const T*>(A.trail[tb]) + (long)(t - t0) * A.slot);
   const A32<typename std::remove_pointer<decltype(GAcc<T>::of(A))>::type> G(GAcc<T>::of(A));
   const Prof<T> P(&M.prof[0][0]);
-  const bool inx = incol && in_slab(i, nx, lx);
+  const bool inx = PML && incol && in_slab(i, nx, lx);
   const uint32_t xs = 5 * n + (inx ? slab_pos(i, nx, lx) : 0);
   const T ikx = P[PK][ic], ax = P[PA][ic], bx = P[PB][ic];
   const T ikxh = P[PKH][ic], axh = P[PAH][ic], bxh = P[PBH][ic];
@@ -1004,7 +1004,7 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
     const uint32_t cr = row(r), cq = row(q), cn = row(r + 1);
     const bool vr = valid(r), vq = valid(q);
     const int rs = clampr(r), qs = clampr(q);
-    const bool inzr = incol && in_slab(rs, nz, lz), inzq = incol && in_slab(qs, nz, lz);
+    const bool inzr = PML && incol && in_slab(rs, nz, lz), inzq = PML && incol && in_slab(qs, nz, lz);
     const uint32_t oxr = xs + (inx ? (uint32_t)rs * 2 * lx : 0), oxq = xs + (inx ? (uint32_t)qs * 2 * lx : 0);
     const uint32_t ozr = ZB + (inzr ? (uint32_t)slab_pos(rs, nz, lz) * nx + ic : 0);
     const uint32_t ozq = ZB + (inzq ? (uint32_t)slab_pos(qs, nz, lz) * nx + ic : 0);
@@ -1255,8 +1255,17 @@ k_el2_adj(const DModel* __restrict__ models, const DShot* __restrict__ shots,
     const int s = __builtin_amdgcn_readfirstlane(wv.x);
     const int xo = __builtin_amdgcn_readfirstlane(wv.y) + wave * AW;
     const int zo = __builtin_amdgcn_readfirstlane(wv.z);
-    if (s >= 0 && xo < models[shots[s].model].nx)
-      el2_adj_tile<T, SPLIT>(models, shots, adj, s, xo, zo, t, t0, gmask, threadIdx.x & 63, tb);
+    if (s < 0) continue;
+    const DModel& M = models[shots[s].model];
+    if (xo >= M.nx) continue;
+    // a wave whose columns and rows (halo included) avoid every C-PML slab
+    // runs the body without memory variables
+    const bool interior = xo - 2 >= M.lx && xo + 61 < M.nx - M.lx && zo - 3 >= M.lz &&
+                          zo + AH + 2 < M.nz - M.lz;
+    if (interior)
+      el2_adj_tile<T, SPLIT, false>(models, shots, adj, s, xo, zo, t, t0, gmask, threadIdx.x & 63, tb);
+    else
+      el2_adj_tile<T, SPLIT, true>(models, shots, adj, s, xo, zo, t, t0, gmask, threadIdx.x & 63, tb);
   }
 }
 
