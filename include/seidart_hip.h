@@ -162,7 +162,7 @@ int sdt_plan_run(void* plan, int it0, int nsteps, int graph_block, void* stream)
 /* ELASTIC2 tile shape of the step kernel (no counterpart in the reference;
  * for an autotuner, seidart_amd.solver.Simulation.autotune): a block of
  * waves_per_block (1, 2 or 4) wavefronts side by side, each owning 61
- * columns, walks `rows` rows (4..256).  Results do not depend on the shape.
+ * columns, walks `rows` rows (2..256).  Results do not depend on the shape.
  * Synchronises the device when the shape changes. */
 int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows);
 int sdt_plan_get_tiles(void* plan, int* waves_per_block, int* rows);

@@ -1375,13 +1375,31 @@ k_em2_e(const DModel* __restrict__ models, const DShot* __restrict__ shots,
 
 // ===========================================================================
 // ELASTIC25: seismic25, legacy/fortran/seismicfdtd.f95:388-949
-// One thread per (x, z) column walks the y axis: the 2-D (y-invariant)
-// coefficients are loaded once into registers and reused for all ny cells.
+// Two launches per step: stresses, then velocities.  A wavefront takes one
+// z plane k, 64 adjacent columns (lanes 1..EW25 are owned; lanes 0 and 63
+// only supply x neighbours, which travel by lane shuffles) and a chunk of Y25
+// rows in y, which it walks with its y neighbours kept in registers.  The 2-D
+// (y-invariant) coefficients of its columns are loaded once per chunk.  A
+// block stacks KB25 wavefronts on consecutive planes, so the z-neighbour rows
+// one of them reads are rows its neighbours read as well.
 // psi slabs (per shot):
 //   x [nz][ny][2lx]: dvx_dx dvy_dx dvz_dx dsxx_dx dsxy_dx dsxz_dx
 //   y [nz][2ly][nx]: dvx_dy dvy_dy dvz_dy dsyy_dy dsxy_dy dsyz_dy
 //   z [2lz][ny][nx]: dvx_dz dvy_dz dvz_dz dszz_dz dsxz_dz dsyz_dz
 // ===========================================================================
+#ifndef SDT_EL25_PLANES
+#define SDT_EL25_PLANES 4
+#endif
+#ifndef SDT_EL25_ROWS
+#define SDT_EL25_ROWS 32
+#endif
+#ifndef SDT_EL25_SYNC
+#define SDT_EL25_SYNC 0
+#endif
+constexpr int EW25 = 62;               // owned columns per wavefront
+constexpr int KB25 = SDT_EL25_PLANES;  // z planes (wavefronts) per block
+constexpr int Y25 = SDT_EL25_ROWS;     // y rows per work item
+
 struct Slab3 {
   long SX, SY, SZ;       // elements per memory-variable array on each axis
   long bx, by, bz;       // base offsets of the x, y, z groups
@@ -1395,74 +1413,113 @@ struct Slab3 {
   }
 };
 
+// One z plane of a 3-D field: wave-uniform base, 32-bit offsets j * nx + i.
 template <typename T>
-__global__ void __launch_bounds__(NT)
+__device__ __forceinline__ A32<T> plane(void* f, long nxy, int k) {
+  return A32<T>(static_cast<T*>(f) + (long)k * nxy);
+}
+template <typename T>
+__device__ __forceinline__ A32<const T> cplane(const void* f, long nxy, int k) {
+  return A32<const T>(static_cast<const T*>(f) + (long)k * nxy);
+}
+
+// waves per SIMD the register budget is sized for (FP32; FP64 takes 2)
+#ifndef SDT_EL25S_WAVES
+#define SDT_EL25S_WAVES 3
+#endif
+#ifndef SDT_EL25V_WAVES
+#define SDT_EL25V_WAVES 4
+#endif
+
+template <typename T>
+__global__ void __launch_bounds__(64 * KB25, sizeof(T) == 4 ? SDT_EL25S_WAVES : 2)
 k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots,
               const int4* __restrict__ work) {
-  const int4 wi = work[blockIdx.x];
-  if (wi.x < 0) return;
-  const DShot& S = shots[wi.x];
+  const int4 wv = work[blockIdx.x];
+  const int s = __builtin_amdgcn_readfirstlane(wv.x);
+  if (s < 0) return;
+  const DShot& S = shots[s];
   const DModel& M = models[S.model];
   const int nx = M.nx, ny = M.ny, nz = M.nz;
-  const int i = wi.y + threadIdx.x, k = wi.z + threadIdx.y;
-  if (i >= nx || k >= nz) return;
+  // a wavefront past the last plane stays (clamped, storing nothing) so that
+  // every wavefront of the block takes the same barriers
+  const int kw = __builtin_amdgcn_readfirstlane(wv.z) + (int)(threadIdx.x >> 6);
+  const bool live = kw < nz;
+  const int k = live ? kw : nz - 1;
+  const int lane = threadIdx.x & 63;
+  const int i = __builtin_amdgcn_readfirstlane(wv.y) - 1 + lane;
+  const bool incol = (i >= 0 && i < nx);
+  const int ic = incol ? i : (i < 0 ? 0 : nx - 1);  // clamped column: loads stay in bounds
+  const bool own = live && incol && lane >= 1 && lane <= EW25;
+  const int j0 = __builtin_amdgcn_readfirstlane(wv.w), j1 = min(j0 + Y25, ny);
   const long nxy = (long)nx * ny;
-  const long p = (long)k * nx + i;
+  const uint32_t p = (uint32_t)k * nx + ic;
   const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz), dt = T(M.dt);
   const int lx = M.lx, ly = M.ly, lz = M.lz;
-  const bool inx = in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
+  const bool inx = incol && in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
   const Prof<T> P(&M.prof[0][0]);
-  gcptr<T> ikx = P[PK], ax = P[PA], bx = P[PB], ikxh = P[PKH], axh = P[PAH], bxh = P[PBH];
   gcptr<T> iky = P[6 + PK], ay = P[6 + PA], by = P[6 + PB];
   gcptr<T> ikyh = P[6 + PKH], ayh = P[6 + PAH], byh = P[6 + PBH];
-  gcptr<T> ikz = P[12 + PK], az = P[12 + PA], bz = P[12 + PB];
-  gcptr<T> ikzh = P[12 + PKH], azh = P[12 + PAH], bzh = P[12 + PBH];
-  // per-column profile values (x and z are fixed for the thread)
-  const T bx_i = bx[i], ax_i = ax[i], ikx_i = ikx[i], bxh_i = bxh[i], axh_i = axh[i], ikxh_i = ikxh[i];
-  const T bz_k = bz[k], az_k = az[k], ikz_k = ikz[k], bzh_k = bzh[k], azh_k = azh[k], ikzh_k = ikzh[k];
-  const bool p1 = (k >= 1 && i <= nx - 2), p2 = (k >= 1 && i >= 1);
-  const bool p3 = (k <= nz - 2 && i >= 1), p4 = (k <= nz - 2 && i <= nx - 2);
+  const T bx_i = P[PB][ic], ax_i = P[PA][ic], ikx_i = P[PK][ic];
+  const T bxh_i = P[PBH][ic], axh_i = P[PAH][ic], ikxh_i = P[PKH][ic];
+  const T bz_k = P[12 + PB][k], az_k = P[12 + PA][k], ikz_k = P[12 + PK][k];
+  const T bzh_k = P[12 + PBH][k], azh_k = P[12 + PAH][k], ikzh_k = P[12 + PKH][k];
+  // loop ranges of the four legacy stress loops (seismicfdtd.f95:617, :677, :728, :771)
+  const bool p1 = own && k >= 1 && i <= nx - 2, p2 = own && k >= 1 && i >= 1;
+  const bool p3 = own && k <= nz - 2 && i >= 1, p4 = own && k <= nz - 2 && i <= nx - 2;
   T C1[15], C2[6], C3[6], C4[6];
 #pragma unroll
-  for (int q = 0; q < 15; ++q) C1[q] = p1 ? cp<T>(M.coef[q])[p] : T(0);
+  for (int q = 0; q < 15; ++q) C1[q] = p1 ? A32<const T>(M.coef[q])[p] : T(0);
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
-    C2[q] = p2 ? cp<T>(M.coef[15 + q])[p] : T(0);
-    C3[q] = p3 ? cp<T>(M.coef[21 + q])[p] : T(0);
-    C4[q] = p4 ? cp<T>(M.coef[27 + q])[p] : T(0);
+    C2[q] = p2 ? A32<const T>(M.coef[15 + q])[p] : T(0);
+    C3[q] = p3 ? A32<const T>(M.coef[21 + q])[p] : T(0);
+    C4[q] = p4 ? A32<const T>(M.coef[27 + q])[p] : T(0);
   }
-  const auto vx = cp<T>(S.f[0]);
-  const auto vy = cp<T>(S.f[1]);
-  const auto vz = cp<T>(S.f[2]);
-  auto sxx = mp<T>(S.f[3]);
-  auto syy = mp<T>(S.f[4]);
-  auto szz = mp<T>(S.f[5]);
-  auto sxy = mp<T>(S.f[6]);
-  auto sxz = mp<T>(S.f[7]);
-  auto syz = mp<T>(S.f[8]);
+  const int kd = k >= 1 ? k - 1 : 0, ku = k <= nz - 2 ? k + 1 : nz - 1;
+  const A32<const T> VX = cplane<T>(S.f[0], nxy, k), VY = cplane<T>(S.f[1], nxy, k);
+  const A32<const T> VZ = cplane<T>(S.f[2], nxy, k);
+  const A32<const T> VXD = cplane<T>(S.f[0], nxy, kd), VYD = cplane<T>(S.f[1], nxy, kd);
+  const A32<const T> VZD = cplane<T>(S.f[2], nxy, kd);
+  const A32<const T> VXU = cplane<T>(S.f[0], nxy, ku), VYU = cplane<T>(S.f[1], nxy, ku);
+  const A32<const T> VZU = cplane<T>(S.f[2], nxy, ku);
+  const A32<T> SXX = plane<T>(S.f[3], nxy, k), SYY = plane<T>(S.f[4], nxy, k);
+  const A32<T> SZZ = plane<T>(S.f[5], nxy, k), SXY = plane<T>(S.f[6], nxy, k);
+  const A32<T> SXZ = plane<T>(S.f[7], nxy, k), SYZ = plane<T>(S.f[8], nxy, k);
   auto psi = mp<T>(S.psi);
   const Slab3 L(nx, ny, nz, lx, ly, lz);
-  const int xs = slab_pos(i, nx, lx), zs = slab_pos(k, nz, lz);
+  const int xs = inx ? slab_pos(i, nx, lx) : 0, zs = inz ? slab_pos(k, nz, lz) : 0;
 
-  for (int j = 0; j < ny; ++j) {
-    const long c = (long)k * nxy + (long)j * nx + i;
+  // y window of the velocities: rows j-1 (m), j (c)
+  const uint32_t r0 = (uint32_t)j0 * nx + ic;
+  T vxm = 0, vym = 0, vzm = 0;
+  if (j0 >= 1) { vxm = VX[r0 - nx]; vym = VY[r0 - nx]; vzm = VZ[r0 - nx]; }
+  T vxc = VX[r0], vyc = VY[r0], vzc = VZ[r0];
+#pragma unroll 1
+  for (int j = j0; j < j1; ++j) {
+    const uint32_t c = (uint32_t)j * nx + ic;
+    const uint32_t cn = j + 1 < ny ? c + nx : c;
+    const T vxn = VX[cn], vyn = VY[cn], vzn = VZ[cn];      // row j+1 (n)
+    const T vxd = VXD[c], vyd = VYD[c], vzd = VZD[c];      // plane k-1 (d)
+    const T vxu = VXU[c], vyu = VYU[c], vzu = VZU[c];      // plane k+1 (u)
+    const T sxx0 = SXX[c], syy0 = SYY[c], szz0 = SZZ[c], sxy0 = SXY[c], sxz0 = SXZ[c], syz0 = SYZ[c];
+    const T vxe = lane_next(vxc), vye = lane_next(vyc), vze = lane_next(vzc);  // i+1 (e)
+    const T vxw = lane_prev(vxc), vyw = lane_prev(vyc), vzw = lane_prev(vzc);  // i-1 (w)
     const bool iny = in_slab(j, ny, ly);
     const long ox = L.bx + ((long)k * ny + j) * 2 * lx + xs;
-    const long oy = L.by + ((long)k * 2 * ly + slab_pos(j, ny, ly)) * nx + i;
-    const long oz = L.bz + ((long)zs * ny + j) * nx + i;
+    const long oy = L.by + ((long)k * 2 * ly + (iny ? slab_pos(j, ny, ly) : 0)) * nx + ic;
+    const long oz = L.bz + ((long)zs * ny + j) * nx + ic;
     // velocity-derivative memory variables (shared by all four loops: A1)
     T mxx = 0, myx = 0, mzx = 0, mxy = 0, myy = 0, mzy = 0, mxz = 0, myz = 0, mzz = 0;
     if (inx) { mxx = psi[ox]; myx = psi[ox + L.SX]; mzx = psi[ox + 2 * L.SX]; }
-    if (iny) { mxy = psi[oy]; myy = psi[oy + L.SY]; mzy = psi[oy + 2 * L.SY]; }
-    if (inz) { mxz = psi[oz]; myz = psi[oz + L.SZ]; mzz = psi[oz + 2 * L.SZ]; }
-    const T vxc = vx[c], vyc = vy[c], vzc = vz[c];
+    if (iny && incol) { mxy = psi[oy]; myy = psi[oy + L.SY]; mzy = psi[oy + 2 * L.SY]; }
+    if (inz && incol) { mxz = psi[oz]; myz = psi[oz + L.SZ]; mzz = psi[oz + 2 * L.SZ]; }
 
     // loop 1: sigma xx, yy, zz (seismicfdtd.f95:617-674)
     if (p1 && j >= 1) {
-      const long e = c + 1, s = c - nx, d = c - nxy;
-      T dvx_dx = (vx[e] - vxc) * idx, dvy_dx = (vy[e] - vyc) * idx, dvz_dx = (vz[e] - vzc) * idx;
-      T dvy_dy = (vyc - vy[s]) * idy, dvx_dy = (vxc - vx[s]) * idy, dvz_dy = (vzc - vz[s]) * idy;
-      T dvz_dz = (vzc - vz[d]) * idz, dvx_dz = (vxc - vx[d]) * idz, dvy_dz = (vyc - vy[d]) * idz;
+      T dvx_dx = (vxe - vxc) * idx, dvy_dx = (vye - vyc) * idx, dvz_dx = (vze - vzc) * idx;
+      T dvy_dy = (vyc - vym) * idy, dvx_dy = (vxc - vxm) * idy, dvz_dy = (vzc - vzm) * idy;
+      T dvz_dz = (vzc - vzd) * idz, dvx_dz = (vxc - vxd) * idz, dvy_dz = (vyc - vyd) * idz;
       if (inx) {
         mxx = bxh_i * mxx + axh_i * dvx_dx; myx = bxh_i * myx + axh_i * dvy_dx; mzx = bxh_i * mzx + axh_i * dvz_dx;
         dvx_dx = dvx_dx * ikxh_i + mxx; dvy_dx = dvy_dx * ikxh_i + myx; dvz_dx = dvz_dx * ikxh_i + mzx;
@@ -1476,19 +1533,18 @@ k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots
         mzz = bz_k * mzz + az_k * dvz_dz; mxz = bz_k * mxz + az_k * dvx_dz; myz = bz_k * myz + az_k * dvy_dz;
         dvz_dz = dvz_dz * ikz_k + mzz; dvx_dz = dvx_dz * ikz_k + mxz; dvy_dz = dvy_dz * ikz_k + myz;
       }
-      sxx[c] = sxx[c] + (C1[0] * dvx_dx + C1[1] * dvy_dy + C1[2] * dvz_dz + C1[3] * (dvy_dz + dvz_dy) +
-                         C1[4] * (dvx_dz + dvz_dx) + C1[5] * (dvx_dy + dvz_dy)) * dt;  // A3 (c16)
-      syy[c] = syy[c] + (C1[1] * dvx_dx + C1[6] * dvy_dy + C1[7] * dvz_dz + C1[8] * (dvy_dz + dvz_dy) +
-                         C1[9] * (dvx_dz + dvz_dx) + C1[10] * (dvy_dx + dvx_dy)) * dt;
-      szz[c] = szz[c] + (C1[2] * dvx_dx + C1[7] * dvy_dy + C1[11] * dvz_dz + C1[12] * (dvy_dz + dvz_dy) +
-                         C1[13] * (dvx_dz + dvz_dx) + C1[14] * (dvy_dx + dvx_dy)) * dt;
+      SXX[c] = sxx0 + (C1[0] * dvx_dx + C1[1] * dvy_dy + C1[2] * dvz_dz + C1[3] * (dvy_dz + dvz_dy) +
+                       C1[4] * (dvx_dz + dvz_dx) + C1[5] * (dvx_dy + dvz_dy)) * dt;  // A3 (c16)
+      SYY[c] = syy0 + (C1[1] * dvx_dx + C1[6] * dvy_dy + C1[7] * dvz_dz + C1[8] * (dvy_dz + dvz_dy) +
+                       C1[9] * (dvx_dz + dvz_dx) + C1[10] * (dvy_dx + dvx_dy)) * dt;
+      SZZ[c] = szz0 + (C1[2] * dvx_dx + C1[7] * dvy_dy + C1[11] * dvz_dz + C1[12] * (dvy_dz + dvz_dy) +
+                       C1[13] * (dvx_dz + dvz_dx) + C1[14] * (dvy_dx + dvx_dy)) * dt;
     }
     // loop 2: sigma xy (seismicfdtd.f95:677-725); dvz_dx and dvx_dz get no correction
     if (p2 && j <= ny - 2) {
-      const long w = c - 1, n = c + nx, d = c - nxy;
-      T dvx_dx = (vxc - vx[w]) * idx, dvy_dx = (vyc - vy[w]) * idx, dvz_dx = (vzc - vz[w]) * idx;
-      T dvy_dy = (vy[n] - vyc) * idy, dvx_dy = (vx[n] - vxc) * idy, dvz_dy = (vz[n] - vzc) * idy;
-      T dvz_dz = (vzc - vz[d]) * idz, dvx_dz = (vxc - vx[d]) * idz, dvy_dz = (vyc - vy[d]) * idz;
+      T dvx_dx = (vxc - vxw) * idx, dvy_dx = (vyc - vyw) * idx, dvz_dx = (vzc - vzw) * idx;
+      T dvy_dy = (vyn - vyc) * idy, dvx_dy = (vxn - vxc) * idy, dvz_dy = (vzn - vzc) * idy;
+      T dvz_dz = (vzc - vzd) * idz, dvx_dz = (vxc - vxd) * idz, dvy_dz = (vyc - vyd) * idz;
       if (inx) {
         mxx = bx_i * mxx + ax_i * dvx_dx; myx = bx_i * myx + ax_i * dvy_dx; mzx = bx_i * mzx + ax_i * dvz_dx;
         dvx_dx = dvx_dx * ikx_i + mxx; dvy_dx = dvy_dx * ikx_i + myx;
@@ -1502,15 +1558,14 @@ k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots
         mzz = bzh_k * mzz + azh_k * dvz_dz; mxz = bzh_k * mxz + azh_k * dvx_dz; myz = bzh_k * myz + azh_k * dvy_dz;
         dvz_dz = dvz_dz * ikzh_k + mzz; dvy_dz = dvy_dz * ikzh_k + myz;
       }
-      sxy[c] = sxy[c] + (C2[0] * dvx_dx + C2[1] * dvy_dy + C2[2] * dvz_dz + C2[3] * (dvz_dy + dvy_dz) +
-                         C2[4] * (dvz_dx + dvx_dz) + C2[5] * (dvy_dx + dvx_dy)) * dt;
+      SXY[c] = sxy0 + (C2[0] * dvx_dx + C2[1] * dvy_dy + C2[2] * dvz_dz + C2[3] * (dvz_dy + dvy_dz) +
+                       C2[4] * (dvz_dx + dvx_dz) + C2[5] * (dvy_dx + dvx_dy)) * dt;
     }
     // loop 3: sigma xz (seismicfdtd.f95:728-768)
     if (p3 && j >= 1) {
-      const long w = c - 1, s = c - nx, u = c + nxy;
-      T dvx_dx = (vxc - vx[w]) * idx, dvy_dx = (vyc - vy[w]) * idx, dvz_dx = (vzc - vz[w]) * idx;
-      T dvy_dy = (vyc - vy[s]) * idy, dvz_dy = (vzc - vz[s]) * idy, dvx_dy = (vxc - vx[s]) * idy;
-      T dvz_dz = (vz[u] - vzc) * idz, dvx_dz = (vx[u] - vxc) * idz, dvy_dz = (vy[u] - vyc) * idz;
+      T dvx_dx = (vxc - vxw) * idx, dvy_dx = (vyc - vyw) * idx, dvz_dx = (vzc - vzw) * idx;
+      T dvy_dy = (vyc - vym) * idy, dvz_dy = (vzc - vzm) * idy, dvx_dy = (vxc - vxm) * idy;
+      T dvz_dz = (vzu - vzc) * idz, dvx_dz = (vxu - vxc) * idz, dvy_dz = (vyu - vyc) * idz;
       if (inx) {
         mxx = bx_i * mxx + ax_i * dvx_dx; myx = bx_i * myx + ax_i * dvy_dx; mzx = bx_i * mzx + ax_i * dvz_dx;
         dvx_dx = dvx_dx * ikx_i + mxx; dvy_dx = dvy_dx * ikx_i + myx; dvz_dx = dvz_dx * ikx_i + mzx;
@@ -1524,15 +1579,14 @@ k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots
         mzz = bzh_k * mzz + azh_k * dvz_dz; mxz = bzh_k * mxz + azh_k * dvx_dz; myz = bzh_k * myz + azh_k * dvy_dz;
         dvz_dz = dvz_dz * ikzh_k + mzz; dvx_dz = dvx_dz * ikzh_k + mxz; dvy_dz = dvy_dz * ikzh_k + myz;
       }
-      sxz[c] = sxz[c] + (C3[0] * dvx_dx + C3[1] * dvy_dy + C3[2] * dvz_dz + C3[3] * (dvx_dz + dvz_dx) +
-                         C3[4] * (dvx_dz + dvz_dx) + C3[5] * (dvx_dy + dvy_dx)) * dt;  // A3 (c45)
+      SXZ[c] = sxz0 + (C3[0] * dvx_dx + C3[1] * dvy_dy + C3[2] * dvz_dz + C3[3] * (dvx_dz + dvz_dx) +
+                       C3[4] * (dvx_dz + dvz_dx) + C3[5] * (dvx_dy + dvy_dx)) * dt;  // A3 (c45)
     }
     // loop 4: sigma yz (seismicfdtd.f95:771-813)
     if (p4 && j <= ny - 2) {
-      const long e = c + 1, n = c + nx, u = c + nxy;
-      T dvx_dx = (vx[e] - vxc) * idx, dvy_dx = (vy[e] - vyc) * idx, dvz_dx = (vz[e] - vzc) * idx;
-      T dvy_dy = (vy[n] - vyc) * idy, dvx_dy = (vx[n] - vxc) * idy, dvz_dy = (vz[n] - vzc) * idy;
-      T dvz_dz = (vz[u] - vzc) * idz, dvx_dz = (vx[u] - vxc) * idz, dvy_dz = (vy[u] - vyc) * idz;
+      T dvx_dx = (vxe - vxc) * idx, dvy_dx = (vye - vyc) * idx, dvz_dx = (vze - vzc) * idx;
+      T dvy_dy = (vyn - vyc) * idy, dvx_dy = (vxn - vxc) * idy, dvz_dy = (vzn - vzc) * idy;
+      T dvz_dz = (vzu - vzc) * idz, dvx_dz = (vxu - vxc) * idz, dvy_dz = (vyu - vyc) * idz;
       if (inx) {
         mxx = bxh_i * mxx + axh_i * dvx_dx; myx = bxh_i * myx + axh_i * dvy_dx; mzx = bxh_i * mzx + axh_i * dvz_dx;
         dvx_dx = dvx_dx * ikxh_i + mxx; dvy_dx = dvy_dx * ikxh_i + myx; dvz_dx = dvz_dx * ikxh_i + mzx;
@@ -1546,68 +1600,97 @@ k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots
         mzz = bzh_k * mzz + azh_k * dvz_dz; mxz = bzh_k * mxz + azh_k * dvx_dz; myz = bzh_k * myz + azh_k * dvy_dz;
         dvz_dz = dvz_dz * ikzh_k + mzz; dvx_dz = dvx_dz * ikzh_k + mxz; dvy_dz = dvy_dz * ikzh_k + myz;
       }
-      syz[c] = syz[c] + (C4[0] * dvx_dx + C4[1] * dvy_dy + C4[2] * dvz_dz + C4[3] * (dvy_dz + dvz_dy) +
-                         C4[4] * (dvx_dz + dvz_dx) + C4[5] * (dvy_dx + dvx_dy)) * dt;
+      SYZ[c] = syz0 + (C4[0] * dvx_dx + C4[1] * dvy_dy + C4[2] * dvz_dz + C4[3] * (dvy_dz + dvz_dy) +
+                       C4[4] * (dvx_dz + dvz_dx) + C4[5] * (dvy_dx + dvx_dy)) * dt;
     }
-    if (inx) { psi[ox] = mxx; psi[ox + L.SX] = myx; psi[ox + 2 * L.SX] = mzx; }
-    if (iny) { psi[oy] = mxy; psi[oy + L.SY] = myy; psi[oy + 2 * L.SY] = mzy; }
-    if (inz) { psi[oz] = mxz; psi[oz + L.SZ] = myz; psi[oz + 2 * L.SZ] = mzz; }
+    if (own) {
+      if (inx) { psi[ox] = mxx; psi[ox + L.SX] = myx; psi[ox + 2 * L.SX] = mzx; }
+      if (iny) { psi[oy] = mxy; psi[oy + L.SY] = myy; psi[oy + 2 * L.SY] = mzy; }
+      if (inz) { psi[oz] = mxz; psi[oz + L.SZ] = myz; psi[oz + 2 * L.SZ] = mzz; }
+    }
+    vxm = vxc; vym = vyc; vzm = vzc;
+    vxc = vxn; vyc = vyn; vzc = vzn;
+#if SDT_EL25_SYNC
+    __syncthreads();  // keep the block's planes on the same row
+#endif
   }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(64 * KB25, sizeof(T) == 4 ? SDT_EL25V_WAVES : 2)
 k_el25_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
            const int4* __restrict__ work, const int* __restrict__ base, int rel) {
-  const int4 wi = work[blockIdx.x];
-  if (wi.x < 0) return;
-  const DShot& S = shots[wi.x];
+  const int4 wv = work[blockIdx.x];
+  const int s = __builtin_amdgcn_readfirstlane(wv.x);
+  if (s < 0) return;
+  const DShot& S = shots[s];
   const DModel& M = models[S.model];
   const int nx = M.nx, ny = M.ny, nz = M.nz;
-  const int i = wi.y + threadIdx.x, k = wi.z + threadIdx.y;
-  if (i >= nx || k >= nz) return;
+  // a wavefront past the last plane stays (clamped, storing nothing) so that
+  // every wavefront of the block takes the same barriers
+  const int kw = __builtin_amdgcn_readfirstlane(wv.z) + (int)(threadIdx.x >> 6);
+  const bool live = kw < nz;
+  const int k = live ? kw : nz - 1;
+  const int lane = threadIdx.x & 63;
+  const int i = __builtin_amdgcn_readfirstlane(wv.y) - 1 + lane;
+  const bool incol = (i >= 0 && i < nx);
+  const int ic = incol ? i : (i < 0 ? 0 : nx - 1);
+  const bool own = live && incol && lane >= 1 && lane <= EW25;
+  const int j0 = __builtin_amdgcn_readfirstlane(wv.w), j1 = min(j0 + Y25, ny);
   const int it = *base + rel;
   const long nxy = (long)nx * ny;
-  const long p = (long)k * nx + i;
+  const uint32_t p = (uint32_t)k * nx + ic;
   const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz);
   const int lx = M.lx, ly = M.ly, lz = M.lz;
-  const bool inx = in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
+  const bool inx = incol && in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
   const Prof<T> P(&M.prof[0][0]);
-  const T bx_i = P[PB][i], ax_i = P[PA][i], ikx_i = P[PK][i];
-  const T bxh_i = P[PBH][i], axh_i = P[PAH][i], ikxh_i = P[PKH][i];
+  const T bx_i = P[PB][ic], ax_i = P[PA][ic], ikx_i = P[PK][ic];
+  const T bxh_i = P[PBH][ic], axh_i = P[PAH][ic], ikxh_i = P[PKH][ic];
   const T bz_k = P[12 + PB][k], az_k = P[12 + PA][k], ikz_k = P[12 + PK][k];
   const T bzh_k = P[12 + PBH][k], azh_k = P[12 + PAH][k], ikzh_k = P[12 + PKH][k];
   gcptr<T> by = P[6 + PB], ay = P[6 + PA], iky = P[6 + PK];
   gcptr<T> byh = P[6 + PBH], ayh = P[6 + PAH], ikyh = P[6 + PKH];
-  const bool ux = (k >= 1 && i >= 1), uy = (k >= 1 && i <= nx - 2), uz = (k <= nz - 2 && i <= nx - 2);
-  const T dtrx = ux ? cp<T>(M.coef[33])[p] : T(0), dtry = uy ? cp<T>(M.coef[34])[p] : T(0);
-  const T dtrz = uz ? cp<T>(M.coef[35])[p] : T(0);
-  const T omx = cp<T>(M.coef[36])[p], omy = cp<T>(M.coef[37])[p], omz = cp<T>(M.coef[38])[p];
-  const auto sxx = cp<T>(S.f[3]);
-  const auto syy = cp<T>(S.f[4]);
-  const auto szz = cp<T>(S.f[5]);
-  const auto sxy = cp<T>(S.f[6]);
-  const auto sxz = cp<T>(S.f[7]);
-  const auto syz = cp<T>(S.f[8]);
-  auto vx = mp<T>(S.f[0]);
-  auto vy = mp<T>(S.f[1]);
-  auto vz = mp<T>(S.f[2]);
+  // loop ranges of the legacy velocity loops (seismicfdtd.f95:821, :848, :873)
+  const bool ux = own && k >= 1 && i >= 1, uy = own && k >= 1 && i <= nx - 2;
+  const bool uz = own && k <= nz - 2 && i <= nx - 2;
+  const T dtrx = ux ? A32<const T>(M.coef[33])[p] : T(0), dtry = uy ? A32<const T>(M.coef[34])[p] : T(0);
+  const T dtrz = uz ? A32<const T>(M.coef[35])[p] : T(0);
+  const T omx = A32<const T>(M.coef[36])[p], omy = A32<const T>(M.coef[37])[p];
+  const T omz = A32<const T>(M.coef[38])[p];
+  const int kd = k >= 1 ? k - 1 : 0, ku = k <= nz - 2 ? k + 1 : nz - 1;
+  const A32<const T> SXX = cplane<T>(S.f[3], nxy, k), SYY = cplane<T>(S.f[4], nxy, k);
+  const A32<const T> SZZ = cplane<T>(S.f[5], nxy, k), SXY = cplane<T>(S.f[6], nxy, k);
+  const A32<const T> SXZ = cplane<T>(S.f[7], nxy, k), SYZ = cplane<T>(S.f[8], nxy, k);
+  const A32<const T> SXZD = cplane<T>(S.f[7], nxy, kd), SYZD = cplane<T>(S.f[8], nxy, kd);
+  const A32<const T> SZZU = cplane<T>(S.f[5], nxy, ku);
+  const A32<T> VX = plane<T>(S.f[0], nxy, k), VY = plane<T>(S.f[1], nxy, k), VZ = plane<T>(S.f[2], nxy, k);
   auto psi = mp<T>(S.psi);
   const Slab3 L(nx, ny, nz, lx, ly, lz);
-  const int xs = slab_pos(i, nx, lx), zs = slab_pos(k, nz, lz);
+  const int xs = inx ? slab_pos(i, nx, lx) : 0, zs = inz ? slab_pos(k, nz, lz) : 0;
   const bool kedge = (i == 0 || k == 0 || i == nx - 1 || k == nz - 1);
   const bool srccol = (i == S.isrc && k == S.ksrc);
 
-  for (int j = 0; j < ny; ++j) {
-    const long c = (long)k * nxy + (long)j * nx + i;
+  // y window: sxy, syz of row j-1 and syy of row j
+  const uint32_t r0 = (uint32_t)j0 * nx + ic;
+  T sxym = 0, syzm = 0;
+  if (j0 >= 1) { sxym = SXY[r0 - nx]; syzm = SYZ[r0 - nx]; }
+  T syyc = SYY[r0];
+#pragma unroll 1
+  for (int j = j0; j < j1; ++j) {
+    const uint32_t c = (uint32_t)j * nx + ic;
+    const uint32_t cn = j + 1 < ny ? c + nx : c;
+    const T sxx = SXX[c], sxy = SXY[c], sxz = SXZ[c], szz = SZZ[c], syz = SYZ[c];
+    const T syyn = SYY[cn];
+    const T sxzd = SXZD[c], syzd = SYZD[c], szzu = SZZU[c];
+    const T sxxw = lane_prev(sxx), sxye = lane_next(sxy), sxze = lane_next(sxz);
+    T vxn = VX[c], vyn = VY[c], vzn = VZ[c];
     const bool iny = in_slab(j, ny, ly);
     const long ox = L.bx + ((long)k * ny + j) * 2 * lx + xs;
-    const long oy = L.by + ((long)k * 2 * ly + slab_pos(j, ny, ly)) * nx + i;
-    const long oz = L.bz + ((long)zs * ny + j) * nx + i;
-    T vxn = vx[c], vyn = vy[c], vzn = vz[c];
+    const long oy = L.by + ((long)k * 2 * ly + (iny ? slab_pos(j, ny, ly) : 0)) * nx + ic;
+    const long oz = L.bz + ((long)zs * ny + j) * nx + ic;
     // vx, seismicfdtd.f95:821-846
     if (ux && j >= 1) {
-      T a = (sxx[c] - sxx[c - 1]) * idx, b = (sxy[c] - sxy[c - nx]) * idy, d = (sxz[c] - sxz[c - nxy]) * idz;
+      T a = (sxx - sxxw) * idx, b = (sxy - sxym) * idy, d = (sxz - sxzd) * idz;
       if (inx) { T m = bx_i * psi[ox + 3 * L.SX] + ax_i * a; a = a * ikx_i + m; psi[ox + 3 * L.SX] = m; }
       if (iny) { T m = by[j] * psi[oy + 4 * L.SY] + ay[j] * b; b = b * iky[j] + m; psi[oy + 4 * L.SY] = m; }
       if (inz) { T m = bz_k * psi[oz + 4 * L.SZ] + az_k * d; d = d * ikz_k + m; psi[oz + 4 * L.SZ] = m; }
@@ -1615,7 +1698,7 @@ k_el25_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
     }
     // vy, seismicfdtd.f95:848-870
     if (uy && j <= ny - 2) {
-      T a = (sxy[c + 1] - sxy[c]) * idx, b = (syy[c + nx] - syy[c]) * idy, d = (syz[c] - syz[c - nxy]) * idz;
+      T a = (sxye - sxy) * idx, b = (syyn - syyc) * idy, d = (syz - syzd) * idz;
       if (inx) { T m = bxh_i * psi[ox + 4 * L.SX] + axh_i * a; a = a * ikxh_i + m; psi[ox + 4 * L.SX] = m; }
       if (iny) { T m = byh[j] * psi[oy + 3 * L.SY] + ayh[j] * b; b = b * ikyh[j] + m; psi[oy + 3 * L.SY] = m; }
       if (inz) { T m = bz_k * psi[oz + 5 * L.SZ] + az_k * d; d = d * ikz_k + m; psi[oz + 5 * L.SZ] = m; }
@@ -1623,7 +1706,7 @@ k_el25_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
     }
     // vz, seismicfdtd.f95:873-896
     if (uz && j >= 1) {
-      T a = (sxz[c + 1] - sxz[c]) * idx, b = (syz[c] - syz[c - nx]) * idy, d = (szz[c + nxy] - szz[c]) * idz;
+      T a = (sxze - sxz) * idx, b = (syz - syzm) * idy, d = (szzu - szz) * idz;
       if (inx) { T m = bxh_i * psi[ox + 5 * L.SX] + axh_i * a; a = a * ikxh_i + m; psi[ox + 5 * L.SX] = m; }
       if (iny) { T m = by[j] * psi[oy + 5 * L.SY] + ay[j] * b; b = b * iky[j] + m; psi[oy + 5 * L.SY] = m; }
       if (inz) { T m = bzh_k * psi[oz + 3 * L.SZ] + azh_k * d; d = d * ikzh_k + m; psi[oz + 3 * L.SZ] = m; }
@@ -1637,10 +1720,16 @@ k_el25_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
     }
     // Dirichlet faces, seismicfdtd.f95:903-925
     if (kedge || j == 0 || j == ny - 1) { vxn = 0; vyn = 0; vzn = 0; }
-    check_blowup((double)vxn * vxn + (double)vyn * vyn + (double)vzn * vzn, S.blowup, it);
-    vx[c] = vxn;
-    vy[c] = vyn;
-    vz[c] = vzn;
+    if (own) {
+      check_blowup((double)vxn * vxn + (double)vyn * vyn + (double)vzn * vzn, S.blowup, it);
+      VX[c] = vxn;
+      VY[c] = vyn;
+      VZ[c] = vzn;
+    }
+    sxym = sxy; syzm = syz; syyc = syyn;
+#if SDT_EL25_SYNC
+    __syncthreads();
+#endif
   }
 }
 
@@ -2196,6 +2285,25 @@ std::vector<int4> el2_work(const Plan* p) {
   return work;
 }
 
+// Work list of the ELASTIC25 kernels: one item (shot, x0, k0, j0) per block
+// of KB25 planes k0.. x EW25 columns x0.. x Y25 rows j0..  The items of each
+// shot are cut into 8 contiguous runs, one per XCD (item w runs on XCD
+// w % 8), so the blocks an XCD takes in turn are z neighbours that share
+// their boundary planes in its L2.
+std::vector<int4> el25_work(const Plan* p) {
+  std::vector<int4> all;
+  for (int s = 0; s < p->nshots; ++s) {
+    const sdt_model& M = p->hmodels[p->hshots[s].model];
+    for (int j0 = 0; j0 < M.ny; j0 += Y25)
+      for (int x0 = 0; x0 < M.nx; x0 += EW25)
+        for (int k0 = 0; k0 < M.nz; k0 += KB25) all.push_back(make_int4(s, x0, k0, j0));
+  }
+  const size_t run = (all.size() + 7) / 8;
+  std::vector<int4> work(run * 8, make_int4(-1, 0, 0, 0));
+  for (size_t q = 0; q < all.size(); ++q) work[(q % run) * 8 + q / run] = all[q];
+  return work;
+}
+
 // Launch kernel `which` (0: first field update, 1: second field update,
 // 2: receiver gather) of one step.
 template <typename T>
@@ -2226,10 +2334,11 @@ void launch_one(Plan* p, hipStream_t st, int rel, int which) {
       hipLaunchKernelGGL(k_em2_e<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
       break;
     case SDT_ELASTIC25 * 2:
-      hipLaunchKernelGGL(k_el25_stress<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork);
+      hipLaunchKernelGGL(k_el25_stress<T>, grd, dim3(64 * KB25), 0, st, p->dmodels, p->dshots, p->dwork);
       break;
     case SDT_ELASTIC25 * 2 + 1:
-      hipLaunchKernelGGL(k_el25_vel<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
+      hipLaunchKernelGGL(k_el25_vel<T>, grd, dim3(64 * KB25), 0, st, p->dmodels, p->dshots, p->dwork,
+                         p->dbase, rel);
       break;
     case SDT_EM25 * 2:
       hipLaunchKernelGGL(k_em25_h<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork);
@@ -2310,6 +2419,12 @@ int validate(int kind, bool split, int prec, const sdt_model* models, int nmodel
       const long long np = psi_elems(kpsi, M.nx, M.ny, M.nz, M.lx, M.ly, M.lz);
       if ((long long)M.nx * M.nz * es >= (1LL << 32) || np * es >= (1LL << 32))
         return fail(SDT_EINVAL, tag + "grid too large (every array must stay under 4 GiB)");
+    }
+    if (kind == SDT_ELASTIC25) {
+      // the kernels address one z plane of a field with 32-bit byte offsets
+      const long long es = prec == SDT_FP64 ? 8 : 4;
+      if ((long long)M.nx * M.ny * es >= (1LL << 32) || (long long)M.nx * M.nz * es >= (1LL << 32))
+        return fail(SDT_EINVAL, tag + "grid too large (an x-y plane must stay under 4 GiB)");
     }
   }
   for (int s = 0; s < nshots; ++s) {
@@ -2574,12 +2689,13 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
   }
   // work list: (shot, x0, z0); all shots of one tile share blockIdx % 8 (XCD)
   std::vector<int4> work;
+  if (kind == SDT_ELASTIC25) work = el25_work(p);
   if (kind == SDT_ELASTIC2) {
     work = el2_work(p);
     const int nblk = (int)work.size();
     p->el2_grid = std::max(8, std::min(nblk, SDT_EL2_GRID) / 8 * 8);
   }
-  for (int m = 0; m < nmodels && kind != SDT_ELASTIC2; ++m) {
+  for (int m = 0; m < nmodels && kind != SDT_ELASTIC2 && kind != SDT_ELASTIC25; ++m) {
     const sdt_model& M = models[m];
     const int wx = TX, wz = TZ;
     const int tx = (M.nx + wx - 1) / wx, tz = (M.nz + wz - 1) / wz, nt_ = tx * tz;
@@ -2723,7 +2839,7 @@ int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows) {
   if (p->kind != SDT_ELASTIC2) return fail(SDT_EINVAL, "tile shapes apply to ELASTIC2 plans");
   if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
     return fail(SDT_EINVAL, "waves_per_block must be 1, 2 or 4");
-  if (rows < 4 || rows > 256) return fail(SDT_EINVAL, "rows must be in [4, 256]");
+  if (rows < 2 || rows > 256) return fail(SDT_EINVAL, "rows must be in [2, 256]");
   if (waves_per_block == p->el2_wb && rows == p->el2_eh) return SDT_OK;
   // the old work list and the graphs that bake in the launch shape may be in flight
   HIP_TRY(hipDeviceSynchronize());

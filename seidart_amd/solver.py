@@ -191,7 +191,7 @@ class Simulation:
         return tuple(float(v) for v in out)
 
     # ------------------------------------------------------------------
-    TILE_CANDIDATES = tuple((wb, rows) for wb in (4, 2, 1) for rows in (8, 16, 32, 64))
+    TILE_CANDIDATES = tuple((wb, rows) for wb in (4, 2, 1) for rows in (4, 8, 16, 32, 64))
 
     def tiles(self):
         """Current (waves per block, rows) of the ELASTIC2 step kernel."""

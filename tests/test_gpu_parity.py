@@ -90,6 +90,16 @@ def test_batched_uneven_models(oracle_lib, kind):
         off += len(p.shots)
 
 
+@pytest.mark.parametrize("kind", ["elastic25", "em25"])
+def test_3d_tiles_and_y_chunks(oracle_lib, kind):
+    """A 2.5-D grid several tiles wide in x, with more y rows than one work
+    item walks and a z extent that is not a multiple of the block's planes."""
+    p = tiny_problem(kind, seed=58, nx=140, ny=70, nz=23, npml=5, nt=30, nrec=9,
+                     **({"gamma": 1e-3} if kind == "elastic25" else {}))
+    sim = _sim(p, "fp64").run().check()
+    _compare(sim, p, oracle_lib, FP64_TOL)
+
+
 def test_graph_capture_equals_direct_launch():
     p = tiny_problem("elastic2", seed=51, nt=64, nshots=2)
     a = _sim(p, "fp32").run(graph_block=0).check().traces()
@@ -156,14 +166,14 @@ def test_tile_shapes_and_autotune_keep_results(oracle_lib):
     probs = [tiny_problem("elastic2", seed=90 + m, nt=60, gamma=1e-3, nshots=2, **d)
              for m, d in enumerate([dict(nx=300, nz=75, npml=6), dict(nx=57, nz=45, npml=5)])]
     ref = _sim(probs, "fp32").run(graph_block=10).check().traces()
-    for wb, rows in ((1, 16), (2, 64), (4, 8)):
+    for wb, rows in ((1, 16), (2, 64), (4, 8), (1, 2), (4, 4)):
         got = _sim(probs, "fp32").set_tiles(wb, rows).run(graph_block=10).check().traces()
         for a, b in zip(ref, got):
             np.testing.assert_array_equal(a, b)
     s = _sim(probs, "fp32")
     s.run(7)
     res = s.autotune(steps=4)
-    assert len(res) == 12 and s.tiles() in res
+    assert len(res) == 15 and s.tiles() in res
     got = s.run(graph_block=10).check().traces()
     for a, b in zip(ref, got):
         np.testing.assert_array_equal(a, b)
