@@ -99,7 +99,7 @@ def cpu_baseline(problems, budget_s):
 FWI_PARAMS = ("c11", "c13", "c33", "c55", "rho")
 
 
-def fwi_gradient(nt: int = 6000, nshots: int = 16, every: int = 64, fd_cells: int = 0):
+def fwi_gradient(nt: int = 6000, nshots: int = 16, every: int = 64, fd_cells: int = 0, tiles=None):
     """Time one full FWI gradient of the fwi_vti workload (FP32, FP64
     accumulation) against a plain FP32 forward of the same shots; optionally
     check it against central differences of the FP64 forward at fd_cells
@@ -115,7 +115,10 @@ def fwi_gradient(nt: int = 6000, nshots: int = 16, every: int = 64, fd_cells: in
     obs = torch.as_tensor(np.stack(sim.traces()), device="cuda")
     sim.close()
     sim = Simulation(start, precision="fp32")
-    sim.autotune(steps=10)
+    if tiles:
+        sim.set_tiles(*tiles)
+    else:
+        sim.autotune(steps=10)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sim.run(graph_block=every)
@@ -125,7 +128,10 @@ def fwi_gradient(nt: int = 6000, nshots: int = 16, every: int = 64, fd_cells: in
     sim.close()
     del sim
     eng = Elastic2Engine(start, "fp32", every=every)
-    eng.sim.autotune(steps=10)
+    if tiles:
+        eng.sim.set_tiles(*tiles)
+    else:
+        eng.sim.autotune(steps=10)
     prm = {k: torch.tensor(start.coef[k], device="cuda", requires_grad=k in FWI_PARAMS)
            for k in ("c11", "c13", "c15", "c33", "c35", "c55", "rho")}
 
@@ -282,6 +288,29 @@ def measured_traffic(alg_bytes: int):
     return None
 
 
+def copy_rate(gib: float = 2.0, reps: int = 10) -> float:
+    """GB/s of torch's device-to-device copy of `gib` GiB (read + write): what a
+    plain streaming kernel reaches on this device, the practical ceiling a
+    stencil's measured traffic is compared with."""
+    import torch
+    n = int(gib * 2**30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * n * 4 / (ms * 1e-3) / 1e9
+
+
 def copy_calibration(gib: float) -> int:
     """Time torch's device-to-device copy of `gib` GiB (read gib + write gib):
     the achievable-bandwidth reference and the known-size kernel the HBM
@@ -317,6 +346,11 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fwi", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--tiles", default="",
+                    help="fixed tile shape WxR of the 2-D elastic step kernel (e.g. 4x16) instead "
+                         "of the autotuner's choice (profiling runs)")
+    ap.add_argument("--config-steps", type=int, default=400,
+                    help="timed steps of each other configuration (plain_psv runs 2000)")
     ap.add_argument("--fwi-steps", type=int, default=6000)
     ap.add_argument("--fwi-fd", type=int, default=0,
                     help="also check the FWI gradient against central differences at this "
@@ -345,7 +379,12 @@ def main(argv=None):
     cells_per_step = sum(p.nx * p.nz * s for p, s in zip(problems, shots))
 
     sim = Simulation(problems, precision="fp32")
-    tune = sim.autotune(steps=10)   # tile shape of the step kernel (setup, untimed)
+    fixed = tuple(int(v) for v in a.tiles.split("x")) if a.tiles else None
+    if fixed:
+        sim.set_tiles(*fixed)
+        tune = {fixed: float("nan")}
+    else:
+        tune = sim.autotune(steps=10)   # tile shape of the step kernel (setup, untimed)
     gb = a.graph_block if W >= a.graph_block else 0
     sim.run(W, graph_block=gb)
     torch.cuda.synchronize()
@@ -373,6 +412,12 @@ def main(argv=None):
                 "alg_bytes_per_launch": int(dom_bytes),
                 "alg_bytes_per_cell_update": round(dom_bytes / cells_per_step, 2),
                 "gather_ms": round(k_ms[2], 4)}
+    if roofline["traffic"]:
+        # measured bytes per launch over the launch time, against a plain copy
+        cgbs = copy_rate()
+        tgbs = roofline["traffic"] / (k_ms[0] * 1e-3) / 1e9
+        roofline.update({"traffic_GBs": round(tgbs, 1), "copy_GBs": round(cgbs, 1),
+                         "traffic_frac_of_copy": round(tgbs / cgbs, 4)})
     traces32 = sim.traces()
     sim.close()
     del sim
@@ -406,8 +451,8 @@ def main(argv=None):
                  "speedup_fp32_batched": round(value / pv, 3),
                  "fp32_trace_rel_l2_vs_fp64": [float(f"{e:.3e}") for e in errs]}
 
-    fwi = None if a.no_fwi else fwi_gradient(nt=a.fwi_steps, fd_cells=a.fwi_fd)
-    others = None if a.no_configs else config_lines()
+    fwi = None if a.no_fwi else fwi_gradient(nt=a.fwi_steps, fd_cells=a.fwi_fd, tiles=fixed)
+    others = None if a.no_configs else config_lines(a.config_steps)
     cpu = None if a.no_cpu else cpu_baseline(problems, a.cpu_seconds)
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s",
@@ -420,8 +465,9 @@ def main(argv=None):
                                "hipGraph per 50 steps",
                    "cells_per_step": cells_per_step, "shots": sum(shots),
                    "graph_block": a.graph_block,
-                   "tiles": {"chosen": list(min(tune, key=tune.get)),
-                             "ms_per_step": {f"{w}x{r}": round(v, 4) for (w, r), v in tune.items()}}},
+                   "tiles": {"chosen": list(fixed or min(tune, key=tune.get)),
+                             "ms_per_step": {} if fixed else
+                             {f"{w}x{r}": round(v, 4) for (w, r), v in tune.items()}}},
         "gpu_ms_events": round(gpu_ms, 3),
         "roofline": roofline,
         "fp64_plain": plain,

@@ -1377,27 +1377,30 @@ k_em2_e(const DModel* __restrict__ models, const DShot* __restrict__ shots,
 // ELASTIC25: seismic25, legacy/fortran/seismicfdtd.f95:388-949
 // Two launches per step: stresses, then velocities.  A wavefront takes one
 // z plane k, 64 adjacent columns (lanes 1..EW25 are owned; lanes 0 and 63
-// only supply x neighbours, which travel by lane shuffles) and a chunk of Y25
-// rows in y, which it walks with its y neighbours kept in registers.  The 2-D
-// (y-invariant) coefficients of its columns are loaded once per chunk.  A
-// block stacks KB25 wavefronts on consecutive planes, so the z-neighbour rows
-// one of them reads are rows its neighbours read as well.
+// only supply x neighbours, which travel by lane shuffles) and a chunk of at
+// most Y25 rows in y, which it walks with its y neighbours kept in registers.
+// The 2-D (y-invariant) coefficients of its columns are loaded once per
+// chunk; z neighbours are plain loads of the adjacent planes.  A block is
+// XW25 x KB25 wavefronts (columns x planes).  Chunks end at the y-slab edges,
+// and a wavefront clear of every slab runs a body without memory variables.
 // psi slabs (per shot):
 //   x [nz][ny][2lx]: dvx_dx dvy_dx dvz_dx dsxx_dx dsxy_dx dsxz_dx
 //   y [nz][2ly][nx]: dvx_dy dvy_dy dvz_dy dsyy_dy dsxy_dy dsyz_dy
 //   z [2lz][ny][nx]: dvx_dz dvy_dz dvz_dz dszz_dz dsxz_dz dsyz_dz
 // ===========================================================================
 #ifndef SDT_EL25_PLANES
-#define SDT_EL25_PLANES 4
+#define SDT_EL25_PLANES 1
 #endif
 #ifndef SDT_EL25_ROWS
 #define SDT_EL25_ROWS 32
 #endif
-#ifndef SDT_EL25_SYNC
-#define SDT_EL25_SYNC 0
+#ifndef SDT_EL25_XW
+#define SDT_EL25_XW 1
 #endif
 constexpr int EW25 = 62;               // owned columns per wavefront
-constexpr int KB25 = SDT_EL25_PLANES;  // z planes (wavefronts) per block
+constexpr int KB25 = SDT_EL25_PLANES;  // z planes per block
+constexpr int XW25 = SDT_EL25_XW;      // wavefronts side by side in x per block
+constexpr int NT25 = 64 * KB25 * XW25;
 constexpr int Y25 = SDT_EL25_ROWS;     // y rows per work item
 
 struct Slab3 {
@@ -1431,32 +1434,31 @@ __device__ __forceinline__ A32<const T> cplane(const void* f, long nxy, int k) {
 #define SDT_EL25V_WAVES 4
 #endif
 
-template <typename T>
-__global__ void __launch_bounds__(64 * KB25, sizeof(T) == 4 ? SDT_EL25S_WAVES : 2)
-k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-              const int4* __restrict__ work) {
-  const int4 wv = work[blockIdx.x];
+template <typename T, bool PML>
+__device__ __forceinline__ void el25_stress_tile(const DModel* __restrict__ models,
+                                              const DShot* __restrict__ shots, const int4 wv) {
   const int s = __builtin_amdgcn_readfirstlane(wv.x);
-  if (s < 0) return;
   const DShot& S = shots[s];
   const DModel& M = models[S.model];
   const int nx = M.nx, ny = M.ny, nz = M.nz;
   // a wavefront past the last plane stays (clamped, storing nothing) so that
   // every wavefront of the block takes the same barriers
-  const int kw = __builtin_amdgcn_readfirstlane(wv.z) + (int)(threadIdx.x >> 6);
+  const int wx_ = (int)(threadIdx.x >> 6) % XW25;  // x slot in the block
+  const int wv_ = (int)(threadIdx.x >> 6) / XW25;  // plane slot in the block
+  const int kw = __builtin_amdgcn_readfirstlane(wv.z) + wv_;
   const bool live = kw < nz;
   const int k = live ? kw : nz - 1;
   const int lane = threadIdx.x & 63;
-  const int i = __builtin_amdgcn_readfirstlane(wv.y) - 1 + lane;
+  const int i = __builtin_amdgcn_readfirstlane(wv.y) + wx_ * EW25 - 1 + lane;
   const bool incol = (i >= 0 && i < nx);
   const int ic = incol ? i : (i < 0 ? 0 : nx - 1);  // clamped column: loads stay in bounds
   const bool own = live && incol && lane >= 1 && lane <= EW25;
-  const int j0 = __builtin_amdgcn_readfirstlane(wv.w), j1 = min(j0 + Y25, ny);
+  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff, j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
   const long nxy = (long)nx * ny;
   const uint32_t p = (uint32_t)k * nx + ic;
   const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz), dt = T(M.dt);
   const int lx = M.lx, ly = M.ly, lz = M.lz;
-  const bool inx = incol && in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
+  const bool inx = PML && incol && in_slab(i, nx, lx), inz = PML && in_slab(k, nz, lz);
   const Prof<T> P(&M.prof[0][0]);
   gcptr<T> iky = P[6 + PK], ay = P[6 + PA], by = P[6 + PB];
   gcptr<T> ikyh = P[6 + PKH], ayh = P[6 + PAH], byh = P[6 + PBH];
@@ -1505,7 +1507,7 @@ k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots
     const T sxx0 = SXX[c], syy0 = SYY[c], szz0 = SZZ[c], sxy0 = SXY[c], sxz0 = SXZ[c], syz0 = SYZ[c];
     const T vxe = lane_next(vxc), vye = lane_next(vyc), vze = lane_next(vzc);  // i+1 (e)
     const T vxw = lane_prev(vxc), vyw = lane_prev(vyc), vzw = lane_prev(vzc);  // i-1 (w)
-    const bool iny = in_slab(j, ny, ly);
+    const bool iny = PML && in_slab(j, ny, ly);
     const long ox = L.bx + ((long)k * ny + j) * 2 * lx + xs;
     const long oy = L.by + ((long)k * 2 * ly + (iny ? slab_pos(j, ny, ly) : 0)) * nx + ic;
     const long oz = L.bz + ((long)zs * ny + j) * nx + ic;
@@ -1610,39 +1612,56 @@ k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots
     }
     vxm = vxc; vym = vyc; vzm = vzc;
     vxc = vxn; vyc = vyn; vzc = vzn;
-#if SDT_EL25_SYNC
-    __syncthreads();  // keep the block's planes on the same row
-#endif
   }
 }
 
+// A wavefront whose columns, plane and rows all lie outside the C-PML slabs
+// runs the body without memory variables.
 template <typename T>
-__global__ void __launch_bounds__(64 * KB25, sizeof(T) == 4 ? SDT_EL25V_WAVES : 2)
-k_el25_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-           const int4* __restrict__ work, const int* __restrict__ base, int rel) {
+__global__ void __launch_bounds__(NT25, sizeof(T) == 4 ? SDT_EL25S_WAVES : 2)
+k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+          const int4* __restrict__ work) {
   const int4 wv = work[blockIdx.x];
   const int s = __builtin_amdgcn_readfirstlane(wv.x);
   if (s < 0) return;
+  const DModel& M = models[shots[s].model];
+  const int xo = __builtin_amdgcn_readfirstlane(wv.y) + (int)((threadIdx.x >> 6) % XW25) * EW25;
+  const int k = __builtin_amdgcn_readfirstlane(wv.z) + (int)(threadIdx.x >> 6) / XW25;
+  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff, j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
+  const bool interior = xo - 1 >= M.lx && xo + 62 < M.nx - M.lx && k >= M.lz && k < M.nz - M.lz &&
+                        j0 >= M.ly && j1 <= M.ny - M.ly;
+  if (interior)
+    el25_stress_tile<T, false>(models, shots, wv);
+  else
+    el25_stress_tile<T, true>(models, shots, wv);
+}
+
+template <typename T, bool PML>
+__device__ __forceinline__ void el25_vel_tile(const DModel* __restrict__ models,
+                                              const DShot* __restrict__ shots, const int4 wv, const int* __restrict__ base, int rel) {
+  const int s = __builtin_amdgcn_readfirstlane(wv.x);
   const DShot& S = shots[s];
   const DModel& M = models[S.model];
   const int nx = M.nx, ny = M.ny, nz = M.nz;
   // a wavefront past the last plane stays (clamped, storing nothing) so that
   // every wavefront of the block takes the same barriers
-  const int kw = __builtin_amdgcn_readfirstlane(wv.z) + (int)(threadIdx.x >> 6);
+  const int wx_ = (int)(threadIdx.x >> 6) % XW25;  // x slot in the block
+  const int wv_ = (int)(threadIdx.x >> 6) / XW25;  // plane slot in the block
+  const int kw = __builtin_amdgcn_readfirstlane(wv.z) + wv_;
   const bool live = kw < nz;
   const int k = live ? kw : nz - 1;
   const int lane = threadIdx.x & 63;
-  const int i = __builtin_amdgcn_readfirstlane(wv.y) - 1 + lane;
+  const int i = __builtin_amdgcn_readfirstlane(wv.y) + wx_ * EW25 - 1 + lane;
   const bool incol = (i >= 0 && i < nx);
   const int ic = incol ? i : (i < 0 ? 0 : nx - 1);
   const bool own = live && incol && lane >= 1 && lane <= EW25;
-  const int j0 = __builtin_amdgcn_readfirstlane(wv.w), j1 = min(j0 + Y25, ny);
+  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff, j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
   const int it = *base + rel;
   const long nxy = (long)nx * ny;
   const uint32_t p = (uint32_t)k * nx + ic;
   const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz);
   const int lx = M.lx, ly = M.ly, lz = M.lz;
-  const bool inx = incol && in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
+  const bool inx = PML && incol && in_slab(i, nx, lx), inz = PML && in_slab(k, nz, lz);
   const Prof<T> P(&M.prof[0][0]);
   const T bx_i = P[PB][ic], ax_i = P[PA][ic], ikx_i = P[PK][ic];
   const T bxh_i = P[PBH][ic], axh_i = P[PAH][ic], ikxh_i = P[PKH][ic];
@@ -1684,7 +1703,7 @@ k_el25_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
     const T sxzd = SXZD[c], syzd = SYZD[c], szzu = SZZU[c];
     const T sxxw = lane_prev(sxx), sxye = lane_next(sxy), sxze = lane_next(sxz);
     T vxn = VX[c], vyn = VY[c], vzn = VZ[c];
-    const bool iny = in_slab(j, ny, ly);
+    const bool iny = PML && in_slab(j, ny, ly);
     const long ox = L.bx + ((long)k * ny + j) * 2 * lx + xs;
     const long oy = L.by + ((long)k * 2 * ly + (iny ? slab_pos(j, ny, ly) : 0)) * nx + ic;
     const long oz = L.bz + ((long)zs * ny + j) * nx + ic;
@@ -1727,10 +1746,28 @@ k_el25_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
       VZ[c] = vzn;
     }
     sxym = sxy; syzm = syz; syyc = syyn;
-#if SDT_EL25_SYNC
-    __syncthreads();
-#endif
   }
+}
+
+// A wavefront whose columns, plane and rows all lie outside the C-PML slabs
+// runs the body without memory variables.
+template <typename T>
+__global__ void __launch_bounds__(NT25, sizeof(T) == 4 ? SDT_EL25V_WAVES : 2)
+k_el25_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+          const int4* __restrict__ work, const int* __restrict__ base, int rel) {
+  const int4 wv = work[blockIdx.x];
+  const int s = __builtin_amdgcn_readfirstlane(wv.x);
+  if (s < 0) return;
+  const DModel& M = models[shots[s].model];
+  const int xo = __builtin_amdgcn_readfirstlane(wv.y) + (int)((threadIdx.x >> 6) % XW25) * EW25;
+  const int k = __builtin_amdgcn_readfirstlane(wv.z) + (int)(threadIdx.x >> 6) / XW25;
+  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff, j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
+  const bool interior = xo - 1 >= M.lx && xo + 62 < M.nx - M.lx && k >= M.lz && k < M.nz - M.lz &&
+                        j0 >= M.ly && j1 <= M.ny - M.ly;
+  if (interior)
+    el25_vel_tile<T, false>(models, shots, wv, base, rel);
+  else
+    el25_vel_tile<T, true>(models, shots, wv, base, rel);
 }
 
 // ===========================================================================
@@ -2285,18 +2322,27 @@ std::vector<int4> el2_work(const Plan* p) {
   return work;
 }
 
-// Work list of the ELASTIC25 kernels: one item (shot, x0, k0, j0) per block
-// of KB25 planes k0.. x EW25 columns x0.. x Y25 rows j0..  The items of each
-// shot are cut into 8 contiguous runs, one per XCD (item w runs on XCD
-// w % 8), so the blocks an XCD takes in turn are z neighbours that share
-// their boundary planes in its L2.
+// Work list of the ELASTIC25 kernels: one item (shot, x0, k0, j0 | j1 << 16)
+// per block of KB25 planes k0.. x XW25 * EW25 columns x0.. x rows j0..j1-1
+// (at most Y25, cut at the y-slab edges), x fastest.
+// The items of each shot are cut into 8 contiguous runs, one per XCD (item w
+// runs on XCD w % 8), so the blocks an XCD runs at a time cover whole rows
+// of a few planes (long contiguous runs of addresses in each array).
 std::vector<int4> el25_work(const Plan* p) {
   std::vector<int4> all;
   for (int s = 0; s < p->nshots; ++s) {
     const sdt_model& M = p->hmodels[p->hshots[s].model];
-    for (int j0 = 0; j0 < M.ny; j0 += Y25)
-      for (int x0 = 0; x0 < M.nx; x0 += EW25)
-        for (int k0 = 0; k0 < M.nz; k0 += KB25) all.push_back(make_int4(s, x0, k0, j0));
+    // y chunks of at most Y25 rows, cut at the y-slab edges
+    std::vector<int> cuts = {0};
+    const int ya = std::min(M.ly, M.ny), yb = std::max(ya, M.ny - M.ly);
+    if (ya > 0) cuts.push_back(ya);
+    for (int j = ya + Y25; j < yb; j += Y25) cuts.push_back(j);
+    if (yb > cuts.back()) cuts.push_back(yb);
+    if (M.ny > cuts.back()) cuts.push_back(M.ny);
+    for (size_t c = 0; c + 1 < cuts.size(); ++c)
+      for (int k0 = 0; k0 < M.nz; k0 += KB25)
+        for (int x0 = 0; x0 < M.nx; x0 += EW25 * XW25)
+          all.push_back(make_int4(s, x0, k0, cuts[c] | (cuts[c + 1] << 16)));
   }
   const size_t run = (all.size() + 7) / 8;
   std::vector<int4> work(run * 8, make_int4(-1, 0, 0, 0));
@@ -2334,10 +2380,10 @@ void launch_one(Plan* p, hipStream_t st, int rel, int which) {
       hipLaunchKernelGGL(k_em2_e<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
       break;
     case SDT_ELASTIC25 * 2:
-      hipLaunchKernelGGL(k_el25_stress<T>, grd, dim3(64 * KB25), 0, st, p->dmodels, p->dshots, p->dwork);
+      hipLaunchKernelGGL(k_el25_stress<T>, grd, dim3(NT25), 0, st, p->dmodels, p->dshots, p->dwork);
       break;
     case SDT_ELASTIC25 * 2 + 1:
-      hipLaunchKernelGGL(k_el25_vel<T>, grd, dim3(64 * KB25), 0, st, p->dmodels, p->dshots, p->dwork,
+      hipLaunchKernelGGL(k_el25_vel<T>, grd, dim3(NT25), 0, st, p->dmodels, p->dshots, p->dwork,
                          p->dbase, rel);
       break;
     case SDT_EM25 * 2:
@@ -2425,6 +2471,7 @@ int validate(int kind, bool split, int prec, const sdt_model* models, int nmodel
       const long long es = prec == SDT_FP64 ? 8 : 4;
       if ((long long)M.nx * M.ny * es >= (1LL << 32) || (long long)M.nx * M.nz * es >= (1LL << 32))
         return fail(SDT_EINVAL, tag + "grid too large (an x-y plane must stay under 4 GiB)");
+      if (M.ny > 65535) return fail(SDT_EINVAL, tag + "ny above 65535");
     }
   }
   for (int s = 0; s < nshots; ++s) {
