@@ -250,7 +250,7 @@ def config_lines(steps: int = 400):
         traces = {}
         for prec, size in (("fp32", 4), ("fp64", 8)):
             sim = Simulation(p, precision=prec)
-            if p.kind == "elastic2":
+            if p.kind in ("elastic2", "em2"):
                 rec.setdefault("tiles", {})[prec] = list(min((r := sim.autotune(steps=10)), key=r.get))
             sim.run(W, graph_block=50 if W >= 50 else 0)
             torch.cuda.synchronize()

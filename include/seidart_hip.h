@@ -114,8 +114,8 @@ typedef struct sdt_model {
  *          the initial conditions (classes.py:888-898).
  * psi: sdt_psi_elems() elements, zero before the first step, kept between
  *      calls of sdt_plan_run (a run may be split into segments).
- * field_b, psi_b: ELASTIC2 only (NULL elsewhere): a second state buffer of
- *      the same sizes.  Its single-pass step kernel reads one buffer and
+ * field_b, psi_b: ELASTIC2 and EM2 (NULL elsewhere): a second state buffer
+ *      of the same sizes.  Their single-pass step kernels read one buffer and
  *      writes the other; at the end of every sdt_plan_run the state is back
  *      in field / psi, with field_b / psi_b holding scratch or a copy.
  * isrc, jsrc, ksrc: 0-based padded source cell (jsrc ignored in 2-D).
@@ -159,7 +159,8 @@ int sdt_plan_create(int kind, int precision, const sdt_model* models, int nmodel
  * and replays it; 0 or 1 launches kernels directly. */
 int sdt_plan_run(void* plan, int it0, int nsteps, int graph_block, void* stream);
 
-/* ELASTIC2 tile shape of the step kernel (no counterpart in the reference;
+/* ELASTIC2 / EM2 tile shape of the single-pass step kernel (no counterpart
+ * in the reference;
  * for an autotuner, seidart_amd.solver.Simulation.autotune): a block of
  * waves_per_block (1, 2 or 4) wavefronts side by side, each owning 61
  * columns, walks `rows` rows (2..256).  Results do not depend on the shape.

@@ -1270,107 +1270,190 @@ k_el2_adj(const DModel* __restrict__ models, const DShot* __restrict__ shots,
 }
 
 // ===========================================================================
-// EM2: electromag2, legacy/fortran/electromagfdtd.f95:10-326
-// psi slabs: x [nz][2lx]: dEz_dx dHy_dx ; z [2lz][nx]: dEx_dz dHy_dz
+// EM2: electromag2, legacy/fortran/electromagfdtd.f95:10-326, one pass per
+// step.  psi slabs: x [nz][2lx]: dEz_dx dHy_dx ; z [2lz][nx]: dEx_dz dHy_dz.
 // fields: Ex (nx-1 x nz), Ez (nx x nz-1), Hy (nx-1 x nz-1) stored nx*nz.
-// The Hy Dirichlet zeroing that ends each legacy step (:312-315) is applied
-// by the next H update, which reads a zeroed boundary Hy as 0.
+// A wavefront owns columns xo..xo+EW-1 (lanes 1..EW) of rows zo..zo+eh-1 and
+// walks down z: at row r it updates Hy of row r from the old Ex of rows r and
+// r+1 and the old Ez of columns i and i+1 (:249-266), then Ex and Ez of row r
+// from the new Hy of rows r-1 and r and of columns i-1 and i (:273-295), then
+// the source (:300-301) and the Dirichlet edges (:304-311).  The new Hy of
+// row zo-1 is recomputed, so tiles are independent; the state ping-pongs
+// between buffers A and B like the 2-D elastic one.  The Hy zeroing that ends
+// each legacy step (:312-315) is applied by the next update, which reads a
+// boundary Hy as 0, and by k_em2_edges at the end of a run.
+// Every fused multiply-add is written out (contraction off).
 // ===========================================================================
-template <typename T>
-__global__ void __launch_bounds__(NT)
-k_em2_h(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-        const int4* __restrict__ work) {
-  const int4 wi = work[blockIdx.x];
-  if (wi.x < 0) return;
+template <typename T, bool TRAIL, bool PML>
+__device__ __forceinline__ void em2_tile(const DModel* __restrict__ models,
+                                         const DShot* __restrict__ shots, const int4 wi, const int it,
+                                         const int lane, const DAdj* __restrict__ adj, const int slot,
+                                         const int tb, const int eh) {
+#pragma clang fp contract(off)
   const DShot& S = shots[wi.x];
   const DModel& M = models[S.model];
   const int nx = M.nx, nz = M.nz;
-  const int i = wi.y + threadIdx.x, j = wi.z + threadIdx.y;
-  if (i > nx - 2 || j > nz - 2) return;
-  const auto Ex = cp<T>(S.f[0]);
-  const auto Ez = cp<T>(S.f[1]);
-  auto Hy = mp<T>(S.f[2]);
-  const long c = (long)j * nx + i;
+  const int xo = wi.y, zo = wi.z;
+  const int i = xo - 1 + lane;
+  const bool incol = (i >= 0 && i < nx);
+  const int ic = incol ? i : (i < 0 ? 0 : nx - 1);  // clamped column: every load stays in bounds
+  const bool own = incol && lane >= 1 && lane <= EW;
+  const int zend = min(zo + eh, nz);
+  const int par = it & 1;
+  const long nn = (long)nx * nz;
+  const T* tin = nullptr;
+  T* tout = nullptr;
+  if (TRAIL) {
+    const DAdj& A = adj[wi.x];
+    tin = static_cast<const T*>(A.trail[tb]) + (long)slot * A.slot;
+    tout = static_cast<T*>(A.trail[tb]) + (long)(slot + 1) * A.slot;
+  }
+  const A32<const T> exi(TRAIL ? tin : (par ? S.g[0] : S.f[0]));
+  const A32<const T> ezi(TRAIL ? tin + nn : (par ? S.g[1] : S.f[1]));
+  const A32<const T> hyi(TRAIL ? tin + 2 * nn : (par ? S.g[2] : S.f[2]));
+  const A32<T> exo(TRAIL ? tout : (par ? S.f[0] : S.g[0]));
+  const A32<T> ezo(TRAIL ? tout + nn : (par ? S.f[1] : S.g[1]));
+  const A32<T> hyo(TRAIL ? tout + 2 * nn : (par ? S.f[2] : S.g[2]));
+  const A32<const T> pin(TRAIL ? tin + 3 * nn : (par ? S.psi2 : S.psi));
+  const A32<T> pout(TRAIL ? tout + 3 * nn : (par ? S.psi : S.psi2));
+  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dbh = T(M.scal[0]);
   const int lx = M.lx, lz = M.lz;
+  const uint32_t SX = (uint32_t)nz * 2 * lx, SZ = (uint32_t)2 * lz * nx;
+  const bool inx = PML && incol && in_slab(i, nx, lx);
+  const int xs = inx ? slab_pos(i, nx, lx) : 0;
   const Prof<T> P(&M.prof[0][0]);
-  auto psi = mp<T>(S.psi);
-  const long SX = (long)nz * 2 * lx;
-  // Hy, electromagfdtd.f95:249-266
-  T a = (Ex[c + nx] - Ex[c]) * T(1.0 / M.dz);
-  if (in_slab(j, nz, lz)) {
-    const long o = 2 * SX + (long)slab_pos(j, nz, lz) * nx + i;
-    T m = P[12 + PB][j] * psi[o] + P[12 + PA][j] * a;
-    a = a * P[12 + PK][j] + m;
-    psi[o] = m;
+  const T ikx = P[PK][ic], ax = P[PA][ic], bx = P[PB][ic];
+  const T ikxh = P[PKH][ic], axh = P[PAH][ic], bxh = P[PBH][ic];
+  const auto ikz = P[12 + PK], az = P[12 + PA], bz = P[12 + PB];
+  const auto ikzh = P[12 + PKH], azh = P[12 + PAH], bzh = P[12 + PBH];
+  const A32<const T> CAX(M.coef[0]), CBX(M.coef[1]), CAZ(M.coef[2]), CBZ(M.coef[3]);
+  const bool hcol = incol && i <= nx - 2;                  // Hy, Ex columns
+  const bool ezcol = incol && i >= 1 && i <= nx - 2;       // Ez columns
+  const bool is_src_col = (i == S.isrc);
+  const int ksrc = S.ksrc;
+  const auto srcx = cp<T>(S.src[0]), srcz = cp<T>(S.src[2]);
+  const T incx = (srcx != nullptr) ? srcx[it] : T(0);
+  const T incz = (srcz != nullptr) ? srcz[it] : T(0);
+
+  auto row = [&](int r) { return (uint32_t)(r < 0 ? 0 : (r >= nz ? nz - 1 : r)) * nx + ic; };
+  auto valid = [&](int r) { return incol && r >= 0 && r < nz; };
+  T ex_c = valid(zo - 1) ? exi[row(zo - 1)] : T(0);  // old Ex of row r
+  T hy_p = 0;                                         // new Hy of row r-1
+  for (int r = zo - 1; r < zend; ++r) {
+    const uint32_t c = row(r);
+    const bool rv = valid(r);
+    const T ld_exn = exi[row(r + 1)], ld_ez = ezi[c], ld_hy = hyi[c];
+    const T cax = CAX[c], cbx = CBX[c], caz = CAZ[c], cbz = CBZ[c];
+    const int rs = r < 0 ? 0 : (r >= nz ? nz - 1 : r);
+    const bool inz = PML && in_slab(rs, nz, lz), inzc = incol && inz;
+    const uint32_t ox = (uint32_t)rs * 2 * lx + xs;
+    const uint32_t oz = 2 * SX + (inz ? (uint32_t)slab_pos(rs, nz, lz) * nx + ic : 0);
+    T m_ezx = inx ? pin[ox] : T(0);        // dEz_dx
+    T m_hyx = inx ? pin[SX + ox] : T(0);   // dHy_dx
+    T m_exz = inzc ? pin[oz] : T(0);       // dEx_dz
+    T m_hyz = inzc ? pin[SZ + oz] : T(0);  // dHy_dz
+    const T ex_n = valid(r + 1) ? ld_exn : T(0);
+    const T ez_c = rv ? ld_ez : T(0);
+    const T hy_o = rv ? ld_hy : T(0);
+
+    // ---- Hy of row r (:249-266) ----
+    const T ez_e = lane_next(ez_c);  // Ez(i+1, r)
+    T hy = hy_o;
+    if (rv && hcol && r <= nz - 2) {
+      T a = (ex_n - ex_c) * idz;
+      if (inz) {
+        m_exz = fma(bz[rs], m_exz, az[rs] * a);
+        a = fma(a, ikz[rs], m_exz);
+      }
+      T b = (ez_e - ez_c) * idx;
+      if (inx) {
+        m_ezx = fma(bx, m_ezx, ax * b);
+        b = fma(b, ikx, m_ezx);
+      }
+      const bool edge = (i == 0 || i == nx - 2 || r == 0 || r == nz - 2);  // :312-315 of the last step
+      hy = fma(dbh, b + a, edge ? T(0) : hy_o);
+    }
+
+    // ---- Ex, Ez of row r (:273-295), source, Dirichlet ----
+    const T hy_w = lane_prev(hy);  // new Hy(i-1, r)
+    if (own && rv && r >= zo) {
+      T ex = ex_c, ez = ez_c;
+      if (hcol && r >= 1 && r <= nz - 2) {
+        T d = (hy - hy_p) * idz;
+        if (inz) {
+          m_hyz = fma(bzh[rs], m_hyz, azh[rs] * d);
+          d = fma(d, ikzh[rs], m_hyz);
+        }
+        ex = fma(cax, ex, cbx * d);
+      }
+      if (ezcol && r <= nz - 2) {
+        T d = (hy - hy_w) * idx;
+        if (inx) {
+          m_hyx = fma(bxh, m_hyx, axh * d);
+          d = fma(d, ikxh, m_hyx);
+        }
+        ez = fma(caz, ez, cbz * d);
+      }
+      if (is_src_col && r == ksrc) { ex += incx; ez += incz; }
+      if (i == 0 || i == nx - 2 || r == 0 || r == nz - 1) ex = 0;
+      if (i == 0 || i == nx - 1 || r == 0 || r == nz - 2) ez = 0;
+      if (i == nx - 1) ex = 0;  // outside the (nx-1) x nz Ex array
+      if (r == nz - 1) ez = 0;  // outside the nx x (nz-1) Ez array
+      check_blowup((double)ex * ex + (double)ez * ez, S.blowup, it);
+      exo[c] = ex;
+      ezo[c] = ez;
+      hyo[c] = hy;
+      if (inx) { pout[ox] = m_ezx; pout[SX + ox] = m_hyx; }
+      if (inz) { pout[oz] = m_exz; pout[SZ + oz] = m_hyz; }
+    }
+    hy_p = hy;
+    ex_c = ex_n;
   }
-  T b = (Ez[c + 1] - Ez[c]) * T(1.0 / M.dx);
-  if (in_slab(i, nx, lx)) {
-    const long o = (long)j * 2 * lx + slab_pos(i, nx, lx);
-    T m = P[PB][i] * psi[o] + P[PA][i] * b;
-    b = b * P[PK][i] + m;
-    psi[o] = m;
-  }
-  const bool edge = (i == 0 || i == nx - 2 || j == 0 || j == nz - 2);
-  const T hy = edge ? T(0) : Hy[c];
-  Hy[c] = hy + T(M.scal[0]) * (b + a);
 }
 
+// Same work list and launch shape as k_el2_step.
+template <typename T, bool TRAIL, int WB>
+__global__ void __launch_bounds__(64 * WB)
+k_em2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+           const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel,
+           const DAdj* __restrict__ adj, const int* __restrict__ tsel, int eh) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int it = __builtin_amdgcn_readfirstlane(*base + rel);
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const int4 wv = work[w];
+    const int4 wi = make_int4(__builtin_amdgcn_readfirstlane(wv.x),
+                              __builtin_amdgcn_readfirstlane(wv.y) + wave * EW,
+                              __builtin_amdgcn_readfirstlane(wv.z), 0);
+    if (wi.x < 0) continue;
+    const DModel& M = models[shots[wi.x].model];
+    if (wi.y >= M.nx) continue;
+    // columns (halo included) and rows zo-1 .. zo+eh-1 clear of every slab
+    const bool interior = wi.y - 1 >= M.lx && wi.y + 62 < M.nx - M.lx && wi.z - 1 >= M.lz &&
+                          wi.z + eh <= M.nz - M.lz;
+    const int tb = TRAIL ? __builtin_amdgcn_readfirstlane(*tsel) : 0;
+    if (interior)
+      em2_tile<T, TRAIL, false>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh);
+    else
+      em2_tile<T, TRAIL, true>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh);
+  }
+}
+
+// Hy zeroed on its Dirichlet ring (:312-315) in buffer A at the end of a run,
+// so the returned state equals the legacy state.
 template <typename T>
-__global__ void __launch_bounds__(NT)
-k_em2_e(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-        const int4* __restrict__ work, const int* __restrict__ base, int rel) {
-  const int4 wi = work[blockIdx.x];
-  if (wi.x < 0) return;
-  const DShot& S = shots[wi.x];
+__global__ void k_em2_edges(const DModel* __restrict__ models, const DShot* __restrict__ shots) {
+  const DShot& S = shots[blockIdx.y];
   const DModel& M = models[S.model];
   const int nx = M.nx, nz = M.nz;
-  const int i = wi.y + threadIdx.x, j = wi.z + threadIdx.y;
-  if (i >= nx || j >= nz) return;
-  const int it = *base + rel;
-  const auto Hy = cp<T>(S.f[2]);
-  auto Ex = mp<T>(S.f[0]);
-  auto Ez = mp<T>(S.f[1]);
-  const long c = (long)j * nx + i;
-  const int lx = M.lx, lz = M.lz;
-  const Prof<T> P(&M.prof[0][0]);
-  auto psi = mp<T>(S.psi);
-  const long SX = (long)nz * 2 * lx, SZ = (long)2 * lz * nx;
-  T ex = Ex[c], ez = Ez[c];
-  // Ex, electromagfdtd.f95:273-283
-  if (j >= 1 && j <= nz - 2 && i <= nx - 2) {
-    T d = (Hy[c] - Hy[c - nx]) * T(1.0 / M.dz);
-    if (in_slab(j, nz, lz)) {
-      const long o = 2 * SX + SZ + (long)slab_pos(j, nz, lz) * nx + i;
-      T m = P[12 + PBH][j] * psi[o] + P[12 + PAH][j] * d;
-      d = d * P[12 + PKH][j] + m;
-      psi[o] = m;
-    }
-    ex = cp<T>(M.coef[0])[c] * ex + cp<T>(M.coef[1])[c] * d;
+  auto hy = mp<T>(S.f[2]);
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * (nx + nz); q += gridDim.x * blockDim.x) {
+    int i, j;
+    if (q < nx) { i = q; j = 0; }
+    else if (q < 2 * nx) { i = q - nx; j = nz - 2; }
+    else if (q < 2 * nx + nz) { i = 0; j = q - 2 * nx; }
+    else { i = nx - 2; j = q - 2 * nx - nz; }
+    if (i >= 0 && j >= 0) hy[(long)j * nx + i] = T(0);
   }
-  // Ez, electromagfdtd.f95:285-295
-  if (j <= nz - 2 && i >= 1 && i <= nx - 2) {
-    T d = (Hy[c] - Hy[c - 1]) * T(1.0 / M.dx);
-    if (in_slab(i, nx, lx)) {
-      const long o = SX + (long)j * 2 * lx + slab_pos(i, nx, lx);
-      T m = P[PBH][i] * psi[o] + P[PAH][i] * d;
-      d = d * P[PKH][i] + m;
-      psi[o] = m;
-    }
-    ez = cp<T>(M.coef[2])[c] * ez + cp<T>(M.coef[3])[c] * d;
-  }
-  // source, electromagfdtd.f95:300-301
-  if (i == S.isrc && j == S.ksrc) {
-    if (S.src[0]) ex += cp<T>(S.src[0])[it];
-    if (S.src[2]) ez += cp<T>(S.src[2])[it];
-  }
-  // Dirichlet, electromagfdtd.f95:304-311
-  if (i == 0 || i == nx - 2 || j == 0 || j == nz - 1) ex = 0;
-  if (i == 0 || i == nx - 1 || j == 0 || j == nz - 2) ez = 0;
-  if (i == nx - 1) ex = 0;  // outside the (nx-1) x nz Ex array
-  if (j == nz - 1) ez = 0;  // outside the nx x (nz-1) Ez array
-  check_blowup((double)ex * ex + (double)ez * ez, S.blowup, it);
-  Ex[c] = ex;
-  Ez[c] = ez;
 }
 
 // ===========================================================================
@@ -1771,7 +1854,7 @@ k_el25_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
 }
 
 // ===========================================================================
-// EM2 adjoint: the transpose of one k_em2_h + k_em2_e step, for gradients
+// EM2 adjoint: the transpose of one EM2 step (k_em2_step), for gradients
 // of a trace misfit with respect to the E-update coefficients (caEx cbEx
 // caEz cbEz, from permittivity and conductivity).  No counterpart in the
 // reference.  The step as the device runs it: Hy' = m * Hy + dbH (dEz_dx +
@@ -1932,28 +2015,6 @@ k_em2_adj_g(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz);
   Lout[c] = Lout[c] + idz * ((j >= 1 ? W[2 * n + c - nx] : T(0)) - W[2 * n + c]);
   Lout[n + c] = Lout[n + c] + idx * ((i >= 1 ? W[3 * n + c - 1] : T(0)) - W[3 * n + c]);
-}
-
-// Copy a shot's state (fields then psi) between its own buffers and trail
-// slot `slot` (to_trail = 1: state -> trail).  Kinds that update in place
-// recompute through their own buffers and record each state this way.
-template <typename T>
-__global__ void k_state_copy(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-                             const DAdj* __restrict__ adj, int nfield, const int* __restrict__ tsel,
-                             int slot, int to_trail) {
-  const DShot& S = shots[blockIdx.y];
-  const DModel& M = models[S.model];
-  const DAdj& A = adj[blockIdx.y];
-  const long n = (long)M.nx * M.ny * M.nz;
-  T* tr = static_cast<T*>(A.trail[*tsel]) + (long)slot * A.slot;
-  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < A.slot;
-       q += (long)gridDim.x * blockDim.x) {
-    T* f = q < nfield * n ? static_cast<T*>(S.f[q / n]) + q % n : static_cast<T*>(S.psi) + (q - nfield * n);
-    if (to_trail)
-      tr[q] = *f;
-    else
-      *f = tr[q];
-  }
 }
 
 // ===========================================================================
@@ -2260,6 +2321,8 @@ int kind_ncoef(int kind) {
   return 0;
 }
 bool kind_3d(int kind) { return kind == SDT_ELASTIC25 || kind == SDT_EM25; }
+// kinds whose single-pass step reads state A or B and writes the other
+bool pingpong(int kind) { return kind == SDT_ELASTIC2 || kind == SDT_EM2; }
 
 // kind may carry SDT_SPLIT_CPML (ELASTIC2 only: two more memory arrays per slab)
 long long psi_elems(int kind, long long nx, long long ny, long long nz, long long lx,
@@ -2294,6 +2357,26 @@ void launch_el2(Plan* p, hipStream_t st, int rel, const int* tsel) {
     default:
       hipLaunchKernelGGL((k_el2_step<T, TRAIL, SPLIT, 4>), g, dim3(256), 0, st, p->dmodels,
                          p->dshots, p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh);
+      break;
+  }
+}
+
+template <typename T, bool TRAIL>
+void launch_em2(Plan* p, hipStream_t st, int rel, const int* tsel) {
+  const DAdj* adj = TRAIL ? p->dadj : nullptr;
+  const dim3 g(p->el2_grid);
+  switch (p->el2_wb) {
+    case 1:
+      hipLaunchKernelGGL((k_em2_step<T, TRAIL, 1>), g, dim3(64), 0, st, p->dmodels, p->dshots,
+                         p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_em2_step<T, TRAIL, 2>), g, dim3(128), 0, st, p->dmodels, p->dshots,
+                         p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh);
+      break;
+    default:
+      hipLaunchKernelGGL((k_em2_step<T, TRAIL, 4>), g, dim3(256), 0, st, p->dmodels, p->dshots,
+                         p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh);
       break;
   }
 }
@@ -2360,7 +2443,7 @@ void launch_one(Plan* p, hipStream_t st, int rel, int which) {
     if (p->maxrec > 0) {
       const dim3 gg((p->maxrec + 63) / 64, p->nshots);
       hipLaunchKernelGGL(k_gather<T>, gg, dim3(64), 0, st, p->dmodels, p->dshots, p->nshots,
-                         p->ncomp, p->dbase, rel, p->kind == SDT_ELASTIC2 ? 1 : 0);
+                         p->ncomp, p->dbase, rel, pingpong(p->kind) ? 1 : 0);
     }
     return;
   }
@@ -2374,11 +2457,10 @@ void launch_one(Plan* p, hipStream_t st, int rel, int which) {
     case SDT_ELASTIC2 * 2 + 1:
       break;  // the fused step kernel does both phases
     case SDT_EM2 * 2:
-      hipLaunchKernelGGL(k_em2_h<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork);
+      launch_em2<T, false>(p, st, rel, nullptr);
       break;
     case SDT_EM2 * 2 + 1:
-      hipLaunchKernelGGL(k_em2_e<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
-      break;
+      break;  // the fused step kernel does both phases
     case SDT_ELASTIC25 * 2:
       hipLaunchKernelGGL(k_el25_stress<T>, grd, dim3(NT25), 0, st, p->dmodels, p->dshots, p->dwork);
       break;
@@ -2403,9 +2485,11 @@ void launch_copy_back(Plan* p, hipStream_t st) {
 
 template <typename T>
 void launch_finalize(Plan* p, hipStream_t st) {
-  if (p->kind == SDT_EM2 || p->kind == SDT_EM25)
+  if (p->kind == SDT_EM2)
+    hipLaunchKernelGGL(k_em2_edges<T>, dim3(16, p->nshots), dim3(256), 0, st, p->dmodels, p->dshots);
+  if (p->kind == SDT_EM25)
     hipLaunchKernelGGL(k_em_h_edges<T>, dim3(p->nwork), dim3(TX, TZ), 0, st, p->dmodels,
-                       p->dshots, p->dwork, p->kind == SDT_EM25 ? 1 : 0);
+                       p->dshots, p->dwork, 1);
 }
 
 template <typename T>
@@ -2459,7 +2543,7 @@ int validate(int kind, bool split, int prec, const sdt_model* models, int nmodel
     }
     if (kind == SDT_ELASTIC2 && !split && !M.aux)
       return fail(SDT_EINVAL, tag + "aux (b_x_half by row) missing");
-    if (kind == SDT_ELASTIC2) {
+    if (kind == SDT_ELASTIC2 || kind == SDT_EM2) {
       // the step kernel addresses each array with 32-bit byte offsets
       const long long es = prec == SDT_FP64 ? 8 : 4;
       const long long np = psi_elems(kpsi, M.nx, M.ny, M.nz, M.lx, M.ly, M.lz);
@@ -2483,9 +2567,9 @@ int validate(int kind, bool split, int prec, const sdt_model* models, int nmodel
       if (!S.field[f]) return fail(SDT_EINVAL, tag + "field pointer null");
     if (!S.psi && psi_elems(kpsi, M.nx, M.ny, M.nz, M.lx, M.ly, M.lz) > 0)
       return fail(SDT_EINVAL, tag + "psi workspace null");
-    if (kind == SDT_ELASTIC2) {
+    if (pingpong(kind)) {
       for (int f = 0; f < kind_fields(kind); ++f)
-        if (!S.field_b[f]) return fail(SDT_EINVAL, tag + "field_b pointer null (ELASTIC2 ping-pongs)");
+        if (!S.field_b[f]) return fail(SDT_EINVAL, tag + "field_b pointer null (ELASTIC2 and EM2 ping-pong)");
       if (!S.psi_b && psi_elems(kpsi, M.nx, M.ny, M.nz, M.lx, M.ly, M.lz) > 0)
         return fail(SDT_EINVAL, tag + "psi_b workspace null");
     }
@@ -2501,13 +2585,13 @@ int validate(int kind, bool split, int prec, const sdt_model* models, int nmodel
 // State after the last step of a run: EM H edges zeroed; ping-pong kinds
 // copied back into buffer A when the last step wrote buffer B.
 void end_of_run(Plan* p, hipStream_t st, int next_step) {
-  const bool b = (p->kind == SDT_ELASTIC2) && (next_step & 1);
+  const bool b = pingpong(p->kind) && (next_step & 1);
   if (p->prec == SDT_FP64) {
-    launch_finalize<double>(p, st);
     if (b) launch_copy_back<double>(p, st);
+    launch_finalize<double>(p, st);
   } else {
-    launch_finalize<float>(p, st);
     if (b) launch_copy_back<float>(p, st);
+    launch_finalize<float>(p, st);
   }
 }
 
@@ -2552,19 +2636,9 @@ void launch_adjoint_steps(Plan* p, hipStream_t st, int khi, int klo) {
   }
 }
 
-// EM2 updates in place: recompute through the shot's own buffers, loading
-// slot 0 first and recording every new state.
 template <typename T>
 void launch_trail_em2(Plan* p, hipStream_t st, int nsteps) {
-  const dim3 g(256, p->nshots);
-  hipLaunchKernelGGL(k_state_copy<T>, g, dim3(256), 0, st, p->dmodels, p->dshots, p->dadj, 3,
-                     p->dtsel_trail, 0, 0);
-  for (int r = 0; r < nsteps; ++r) {
-    launch_one<T>(p, st, r, 0);
-    launch_one<T>(p, st, r, 1);
-    hipLaunchKernelGGL(k_state_copy<T>, g, dim3(256), 0, st, p->dmodels, p->dshots, p->dadj, 3,
-                       p->dtsel_trail, r + 1, 1);
-  }
+  for (int r = 0; r < nsteps; ++r) launch_em2<T, true>(p, st, r, p->dtsel_trail);
 }
 
 void launch_trail(Plan* p, hipStream_t st, int nsteps) {
@@ -2737,12 +2811,12 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
   // work list: (shot, x0, z0); all shots of one tile share blockIdx % 8 (XCD)
   std::vector<int4> work;
   if (kind == SDT_ELASTIC25) work = el25_work(p);
-  if (kind == SDT_ELASTIC2) {
+  if (kind == SDT_ELASTIC2 || kind == SDT_EM2) {
     work = el2_work(p);
     const int nblk = (int)work.size();
     p->el2_grid = std::max(8, std::min(nblk, SDT_EL2_GRID) / 8 * 8);
   }
-  for (int m = 0; m < nmodels && kind != SDT_ELASTIC2 && kind != SDT_ELASTIC25; ++m) {
+  for (int m = 0; m < nmodels && !pingpong(kind) && kind != SDT_ELASTIC25; ++m) {
     const sdt_model& M = models[m];
     const int wx = TX, wz = TZ;
     const int tx = (M.nx + wx - 1) / wx, tz = (M.nz + wz - 1) / wz, nt_ = tx * tz;
@@ -2883,7 +2957,7 @@ int sdt_plan_check(void* plan, int* first_step, int* per_shot) {
 int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows) {
   Plan* p = static_cast<Plan*>(plan);
   if (!p) return fail(SDT_EINVAL, "plan is null");
-  if (p->kind != SDT_ELASTIC2) return fail(SDT_EINVAL, "tile shapes apply to ELASTIC2 plans");
+  if (!pingpong(p->kind)) return fail(SDT_EINVAL, "tile shapes apply to ELASTIC2 and EM2 plans");
   if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
     return fail(SDT_EINVAL, "waves_per_block must be 1, 2 or 4");
   if (rows < 2 || rows > 256) return fail(SDT_EINVAL, "rows must be in [2, 256]");
@@ -2960,8 +3034,8 @@ int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots, unsigned 
     if (a.trail_len < 2) return fail(SDT_EINVAL, tag + "trail_len must be at least 2");
     const sdt_model& M = p->hmodels[p->hshots[s].model];
     DAdj& D = h[s];
-    if (p->kind == SDT_ELASTIC2) {
-      // the fused adjoint addresses a state and a gradient with 32-bit byte offsets
+    if (p->kind == SDT_ELASTIC2 || p->kind == SDT_EM2) {
+      // the fused kernels address a state and a gradient with 32-bit byte offsets
       const long long es = p->prec == SDT_FP64 ? 8 : 4;
       const long long st = state_elems(p->kind | (p->split ? SDT_SPLIT_CPML : 0), M.nx, M.ny, M.nz,
                                        M.lx, M.ly, M.lz);

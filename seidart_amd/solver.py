@@ -22,6 +22,8 @@ from .problem import FIELDS, KINDS, Problem, aux_b_x_half_by_row, em_dbh, kernel
 __all__ = ["Simulation", "PRECISIONS"]
 
 PRECISIONS = {"fp64": (_hip.SDT_FP64, torch.float64), "fp32": (_hip.SDT_FP32, torch.float32)}
+# kinds whose single-pass step kernel reads state A or B and writes the other
+PINGPONG = ("elastic2", "em2")
 
 
 def _dev_ptr(t: torch.Tensor | None):
@@ -119,10 +121,10 @@ class Simulation:
                 init = (p.init[s] if p.init else None) or {}
                 bufs = {}
                 # one contiguous state (fields then psi), the layout of
-                # sdt_state_elems; the single-pass ELASTIC2 kernel ping-pongs
-                # between state A and state B
+                # sdt_state_elems; the single-pass ELASTIC2 and EM2 kernels
+                # ping-pong between state A and state B
                 nst = _hip.state_elems(self.kid, p.nx, p.ny, p.nz, lx, ly, lz)
-                for tag in (("", "_b") if self.kind == "elastic2" else ("",)):
+                for tag in (("", "_b") if self.kind in PINGPONG else ("",)):
                     st = torch.zeros(nst, dtype=self.dtype, device=self.device)
                     bufs["state" + tag] = st
                     for f, n in enumerate(self.names):
@@ -146,7 +148,7 @@ class Simulation:
                 for f, n in enumerate(self.names):
                     S.field[f] = bufs[n].data_ptr()
                 S.psi = bufs["psi"].data_ptr()
-                if self.kind == "elastic2":
+                if self.kind in PINGPONG:
                     for f, n in enumerate(self.names):
                         S.field_b[f] = bufs[n + "_b"].data_ptr()
                     S.psi_b = bufs["psi_b"].data_ptr()
@@ -191,10 +193,11 @@ class Simulation:
         return tuple(float(v) for v in out)
 
     # ------------------------------------------------------------------
-    TILE_CANDIDATES = tuple((wb, rows) for wb in (4, 2, 1) for rows in (4, 8, 16, 32, 64))
+    TILE_CANDIDATES = tuple((wb, rows) for wb in (4, 2, 1) for rows in (2, 4, 8, 16, 32, 64))
 
     def tiles(self):
-        """Current (waves per block, rows) of the ELASTIC2 step kernel."""
+        """Current (waves per block, rows) of the single-pass step kernel
+        (ELASTIC2, EM2)."""
         wb, rows = C.c_int(), C.c_int()
         _hip.check(_hip.lib().sdt_plan_get_tiles(self._plan, C.byref(wb), C.byref(rows)))
         return wb.value, rows.value
@@ -204,14 +207,15 @@ class Simulation:
         return self
 
     def autotune(self, steps: int = 8, candidates=None):
-        """Pick the ELASTIC2 tile shape that runs this plan's step fastest.
+        """Pick the tile shape of the single-pass step kernel (ELASTIC2, EM2)
+        that runs this plan's step fastest.
 
         Each candidate runs ``steps`` steps (timed by HIP events around the
         step kernel) from the current state; the state, traces and step
         counter are restored afterwards, so the run continues as if nothing
         had happened.  Returns {(wb, rows): ms per step} and keeps the best.
         Other kinds have a single shape: returns {}."""
-        if self.kind != "elastic2":
+        if self.kind not in PINGPONG:
             return {}
         steps = max(1, min(int(steps), self.nt - self.step))
         keep = [(b["state"].clone(), b["state_b"].clone(), b["trace"].clone()) for _, _, b in self.shots]

@@ -100,6 +100,26 @@ def test_3d_tiles_and_y_chunks(oracle_lib, kind):
     _compare(sim, p, oracle_lib, FP64_TOL)
 
 
+def test_em2_tile_shapes_keep_results(oracle_lib):
+    """The single-pass EM 2-D kernel gives bit-identical results for every
+    tile shape, and matches the oracle on several models and shots."""
+    probs = [tiny_problem("em2", seed=95 + m, nt=60, nshots=2, **d)
+             for m, d in enumerate([dict(nx=300, nz=75, npml=6), dict(nx=57, nz=45, npml=5)])]
+    ref = _sim(probs, "fp32").run(graph_block=10).check().traces()
+    for wb, rows in ((1, 16), (2, 64), (4, 8), (1, 2), (4, 4)):
+        got = _sim(probs, "fp32").set_tiles(wb, rows).run(graph_block=10).check().traces()
+        for a, b in zip(ref, got):
+            np.testing.assert_array_equal(a, b)
+    s = _sim(probs, "fp64")
+    s.run(7)
+    s.autotune(steps=4)
+    s.run(graph_block=10).check()
+    off = 0
+    for p in probs:
+        _compare(s, p, oracle_lib, FP64_TOL, shot_offset=off)
+        off += len(p.shots)
+
+
 def test_graph_capture_equals_direct_launch():
     p = tiny_problem("elastic2", seed=51, nt=64, nshots=2)
     a = _sim(p, "fp32").run(graph_block=0).check().traces()
@@ -173,7 +193,7 @@ def test_tile_shapes_and_autotune_keep_results(oracle_lib):
     s = _sim(probs, "fp32")
     s.run(7)
     res = s.autotune(steps=4)
-    assert len(res) == 15 and s.tiles() in res
+    assert len(res) == 18 and s.tiles() in res
     got = s.run(graph_block=10).check().traces()
     for a, b in zip(ref, got):
         np.testing.assert_array_equal(a, b)
