@@ -35,7 +35,8 @@ import numpy as np
 
 from .cpml import (AxisProfile, alpha_max_em, alpha_max_seismic, axis_profile,
                    profiles_from_2d, sigma_max_em, sigma_max_seismic)
-from .fortranio import read_record, read_stream, snapshot_name, write_snapshot
+from .fortranio import em25_block_name, read_record, read_stream, snapshot_name, write_em25_block, \
+    write_snapshot
 from .problem import (CONDUCTIVITY, DENSITY_METHODS, FIELDS, PERMITTIVITY, STIFFNESS,
                       TRACE_COMPONENTS, Problem, Shot)
 
@@ -351,9 +352,16 @@ def write_trace_csv(trace2d: np.ndarray, basefile: str, channel: str, source_xyz
 
 
 def _execute(p: Problem, precision: str, snapshot_every: int, snapshot_single: bool,
-             snapshot_dir: str, graph_block: int, legacy_names: bool):
+             snapshot_dir: str, graph_block: int, legacy_names: bool, snapshot_format: str = "dat"):
+    """Run one problem; write snapshots every ``snapshot_every`` steps, as one
+    record per field and step (``"dat"``), or for 2.5-D EM optionally as one
+    int16 block file of the E fields (``"em25block"``)."""
     from .solver import Simulation
     from ._hip import UnstableError
+    if snapshot_format not in ("dat", "em25block"):
+        raise ValueError("snapshot_format must be 'dat' or 'em25block'")
+    if snapshot_format == "em25block" and p.kind != "em25":
+        raise ValueError("em25block snapshots apply to 2.5-D EM runs")
     sim = Simulation(p, precision=precision)
     nsnap = 0
     if snapshot_every and snapshot_every > 0:
@@ -361,6 +369,7 @@ def _execute(p: Problem, precision: str, snapshot_every: int, snapshot_single: b
         names = FIELDS[p.kind][: len(comps)]
         si = getattr(p, "src_index", None) or p.shots[0].src
         idx = (si[0], si[2]) if not p.is3d else tuple(si)
+        block = {ch: [] for ch in comps}
         done = 0
         while done < p.nt:
             n = min(snapshot_every, p.nt - done)
@@ -368,9 +377,17 @@ def _execute(p: Problem, precision: str, snapshot_every: int, snapshot_single: b
             done += n
             for ch, fname in zip(comps, names):
                 f = sim.field(0, fname)
+                if snapshot_format == "em25block":
+                    block[ch].append(np.asarray(f, dtype=np.float32))
+                    continue
                 write_snapshot(os.path.join(snapshot_dir, snapshot_name(ch, done, idx, legacy_names)),
                                f, single=snapshot_single)
                 nsnap += 1
+        if snapshot_format == "em25block":
+            # steps snapshot_every, 2*snapshot_every, ... stored as block steps 1, 2, ...
+            write_em25_block(os.path.join(snapshot_dir, em25_block_name(1, idx)),
+                             {ch: np.stack(v) for ch, v in block.items()}, first_step=1)
+            nsnap = 1
     else:
         sim.run(graph_block=graph_block)
     unstable = -1
@@ -386,12 +403,12 @@ def _execute(p: Problem, precision: str, snapshot_every: int, snapshot_single: b
 def run_files(jsonfile: str, seismic: bool = True, density_method: str = "legacy",
               precision: str = "fp64", snapshot_every: int = 0, snapshot_single: bool = True,
               receivers=None, graph_block: int = 50, raise_unstable: bool = True,
-              legacy_names: bool = False) -> RunResult:
+              legacy_names: bool = False, snapshot_format: str = "dat") -> RunResult:
     """The reference binary's call (classes.py:910-924) on the same files."""
     p = problem_from_files(jsonfile, seismic, density_method=density_method, receivers=receivers)
     wd = os.path.dirname(os.path.abspath(jsonfile))
     tr, unstable, nsnap = _execute(p, precision, snapshot_every, snapshot_single, wd, graph_block,
-                                   legacy_names)
+                                   legacy_names, snapshot_format)
     csvs = []
     if receivers is not None and len(p.shots[0].rec):
         base = os.path.splitext(jsonfile)[0]
@@ -432,7 +449,8 @@ def run(model, domain=None, material=None, *, num_threads: int = 1, jsonfile: st
 
 def main(argv=None) -> int:
     """``python -m seidart_amd.run project.json seismic=true density_method=geometric
-    [precision=fp32] [snapshot_every=N] [single=true] [receivers=file.xyz]``"""
+    [precision=fp32] [snapshot_every=N] [single=true] [receivers=file.xyz]
+    [snapshot_format=dat|em25block]``"""
     argv = list(sys.argv[1:] if argv is None else argv)
     if not argv:
         print(main.__doc__, file=sys.stderr)
@@ -447,7 +465,8 @@ def main(argv=None) -> int:
     try:
         res = run_files(jsonfile, seismic, dm, kw.get("precision", "fp64"),
                         int(kw.get("snapshot_every", "1")), kw.get("single", "true") == "true",
-                        kw.get("receivers"), int(kw.get("graph_block", "50")))
+                        kw.get("receivers"), int(kw.get("graph_block", "50")),
+                        snapshot_format=kw.get("snapshot_format", "dat"))
     except Exception as e:  # report like the reference solver, but with a status
         print(f"seidart_amd.run: {e}", file=sys.stderr)
         return 1

@@ -120,6 +120,31 @@ def test_em2_tile_shapes_keep_results(oracle_lib):
         off += len(p.shots)
 
 
+def test_em25_block_snapshots_match_record_snapshots(tmp_path):
+    """2.5-D EM snapshots written as one int16 block equal the per-step
+    records of the same run to the block's quantisation step."""
+    import glob
+    from seidart_amd import fortranio, run as dropin
+    p = tiny_problem("em25", seed=61, nx=20, ny=8, nz=16, npml=3, nt=12)
+    tr, unstable, nsnap = dropin._execute(p, "fp64", 4, True, str(tmp_path), 0, False, "em25block")
+    assert unstable < 0 and nsnap == 1
+    blk = glob.glob(str(tmp_path / "EM25.*.blk"))
+    assert len(blk) == 1
+    r = fortranio.read_em25_block(blk[0])
+    assert r["metadata"]["step_count"] == 3 and sorted(r["fields"]) == ["Ex", "Ey", "Ez"]
+    _, _, n2 = dropin._execute(p, "fp64", 4, False, str(tmp_path), 0, False, "dat")
+    assert n2 == 9
+    si = p.shots[0].src
+    for t, it in enumerate((4, 8, 12)):
+        for ch in ("Ex", "Ey", "Ez"):
+            rec = fortranio.read_snapshot(str(tmp_path / fortranio.snapshot_name(ch, it, tuple(si))),
+                                          (p.nz, p.ny, p.nx), single=False)
+            got = np.transpose(r["fields"][ch][t], (2, 1, 0))
+            q = (rec.max() - rec.min()) / 65534
+            tol = 0.5 * q + 4 * np.finfo(np.float32).eps * np.abs(rec).max() + 1e-30
+            assert np.abs(got - rec).max() <= tol, (ch, it)
+
+
 def test_graph_capture_equals_direct_launch():
     p = tiny_problem("elastic2", seed=51, nt=64, nshots=2)
     a = _sim(p, "fp32").run(graph_block=0).check().traces()

@@ -149,6 +149,64 @@ def test_fortran_record_roundtrip_and_errors(tmp_path):
         fortranio.read_record(f)
 
 
+def _em25_bytes(nx, ny, nz, first, recs):
+    """An EM25 block built byte by byte (definitions.py:154-190 layout),
+    without the package's writer: recs = [(step, channel, lo, hi, raw)]."""
+    out = b"SEIDART_EM25B1  " + np.array([1, nx, ny, nz, first, len({r[0] for r in recs}),
+                                           len({r[1] for r in recs})], "<i4").tobytes()
+    for step, ch, lo, hi, raw in recs:
+        out += np.array([step], "<i4").tobytes() + ch.encode() + np.array([lo, hi], "<f4").tobytes()
+        out += np.asarray(raw, "<i2").tobytes()
+    return out
+
+
+def test_em25_block_reader_on_handmade_file(tmp_path):
+    """Reader against a hand-built block: the end points of the int16 range
+    map to min and max, 0 to the midpoint, values land in Fortran (x, y, z)
+    order, max == min gives a constant, gzip is read transparently."""
+    import gzip
+    nx, ny, nz = 3, 2, 2
+    raw = np.array([-32767, 0, 32767, 100, -100, 5, 6, 7, 8, 9, 10, 11], dtype=np.int16)
+    recs = [(4, "Ex", -2.0, 6.0, raw), (4, "Ez", 1.5, 1.5, np.zeros(12, np.int16)),
+            (5, "Ex", 0.0, 1.0, raw[::-1].copy()), (5, "Ez", 0.0, 2.0, raw)]
+    blob = _em25_bytes(nx, ny, nz, 4, recs)
+    for name, data in (("b.blk", blob), ("b.blk.gz", gzip.compress(blob))):
+        pth = tmp_path / name
+        pth.write_bytes(data)
+        r = fortranio.read_em25_block(str(pth))
+        assert r["metadata"] == {"version": 1, "nx": 3, "ny": 2, "nz": 2, "first_step": 4,
+                                 "step_count": 2, "component_count": 2}
+        ex = r["fields"]["Ex"]
+        assert ex.shape == (2, 3, 2, 2)
+        assert ex[0, 0, 0, 0] == np.float32(-2.0)       # raw -32767 -> min
+        assert ex[0, 2, 0, 0] == np.float32(6.0)        # raw +32767 -> max
+        assert abs(ex[0, 1, 0, 0] - 2.0) < 1e-6         # raw 0 -> midpoint
+        # element (x=0, y=1, z=0) is raw[0 + 3*(1 + 2*0)] = raw[3]
+        assert abs(ex[0, 0, 1, 0] - ((100 + 32767) / 65534 * 8.0 - 2.0)) < 1e-5
+        assert np.all(r["fields"]["Ez"][0] == np.float32(1.5))
+        assert r["scales"]["Ex"] == [(4, -2.0, 6.0), (5, 0.0, 1.0)]
+        only = fortranio.read_em25_block(str(pth), channels=["Ez"])
+        assert list(only["fields"]) == ["Ez"]
+
+
+def test_em25_block_writer_roundtrip(tmp_path):
+    rng = np.random.default_rng(7)
+    ex = rng.standard_normal((3, 5, 4, 6)).astype(np.float32) * 1e-3
+    ey = np.full((3, 5, 4, 6), 0.25, np.float32)
+    pth = tmp_path / fortranio.em25_block_name(11, (5, 2, 3))
+    assert pth.name == "EM25.000011.5.2.3.blk"
+    fortranio.write_em25_block(str(pth), {"Ex": ex, "Ey": ey}, first_step=11)
+    r = fortranio.read_em25_block(str(pth))
+    assert r["metadata"]["first_step"] == 11 and r["metadata"]["step_count"] == 3
+    got = np.transpose(r["fields"]["Ex"], (0, 3, 2, 1))   # back to (t, z, y, x)
+    for t in range(3):
+        step = (ex[t].max() - ex[t].min()) / 65534
+        assert np.abs(got[t] - ex[t]).max() <= 0.5 * step * 1.001 + 1e-12
+    assert np.all(r["fields"]["Ey"] == np.float32(0.25))
+    with pytest.raises(ValueError):
+        fortranio.write_em25_block(str(pth), {"Exx": ex}, first_step=0)
+
+
 def test_snapshot_names():
     assert fortranio.snapshot_name("Vx", 7, (128, 30)) == "Vx.000007.128.30.dat"
     assert fortranio.snapshot_name("Vx", 7, (128, 30), legacy_2d_dots=True) == "Vx.000007.128.30..dat"
