@@ -166,6 +166,10 @@ def problem_from_files(jsonfile: str, seismic: bool = True, workdir: str | None 
             coef[n] = read_record(f, shape=shape2)
         else:
             coef[n] = np.zeros(shape2)
+    if seismic and dim == 2.0 and os.path.exists(P("density_gradient.dat")) and "rho" in coef:
+        # (nx, nz) interior factor written by airsurf (definitions.py:553-555)
+        grad = read_record(P("density_gradient.dat")).reshape(nx - 2 * cpml, nz - 2 * cpml)
+        coef["rho"] = _apply_density_gradient(coef["rho"], grad, cpml)
     gamma = {}
     if seismic:
         for a in ("x", "y", "z"):
@@ -225,6 +229,49 @@ def _paint(vals, geometry, cpml):
     return np.pad(a, cpml, mode="edge").T.copy()
 
 
+def air_density_gradient(geometry, air_label, N: int = 2) -> np.ndarray:
+    """Density factor near air (``airsurf``, definitions.py:475-557) on a
+    (nx, nz) label image: a non-air cell next to air (across either axis)
+    gets 0.15, the cell one further in 0.15 + 0.85/2, and so on for N
+    levels, each level halving what is left to 1; every other cell 1.  The
+    nearest level wins.  As in the reference, an air label of None or 0
+    (air listed first) means no gradient.  Returned as (nx, nz)."""
+    g = np.asarray(geometry)
+    if not air_label:
+        return np.ones(g.shape)
+    solid = (g != air_label).astype(np.int64)
+    pct = [0.15]
+    for _ in range(1, N):
+        pct.append(pct[-1] + (1.0 - pct[-1]) / 2.0)
+    out = np.zeros(g.shape)
+    for level in range(N - 1, -1, -1):
+        hit = np.zeros(g.shape)
+        for ax in (0, 1):
+            d = np.diff(solid, axis=ax)          # +1: air -> solid, -1: solid -> air
+            zero = np.zeros_like(np.take(d, [0], axis=ax))
+            up = np.concatenate([zero, d], axis=ax)     # +1 on the solid cell after air
+            dn = np.concatenate([d, zero], axis=ax)     # -1 on the solid cell before air
+            for _ in range(level):                       # walk `level` cells into the solid
+                up = np.concatenate([zero, up], axis=ax)[tuple(
+                    slice(0, -1) if a == ax else slice(None) for a in range(2))]
+                dn = np.concatenate([dn, zero], axis=ax)[tuple(
+                    slice(1, None) if a == ax else slice(None) for a in range(2))]
+            hit = hit + up - dn
+        out[hit > 0] = pct[level]
+    out[out == 0] = 1.0
+    return out
+
+
+def _apply_density_gradient(rho_padded: np.ndarray, grad: np.ndarray, cpml: int) -> np.ndarray:
+    """rho (nz, nx padded) scaled by the (nx, nz) interior gradient, then the
+    C-PML rows and columns refilled from the scaled edge (the legacy order,
+    legacy/v0.0.2/fdtd/seismicFDTD2D.f95:53-80)."""
+    r = rho_padded.copy()
+    nz, nx = r.shape
+    inner = r[cpml:nz - cpml, cpml:nx - cpml] * np.asarray(grad, dtype=np.float64).T
+    return np.pad(inner, cpml, mode="edge")
+
+
 def _sponge(eta, cpml, NPS, eta_max):
     """Additive sponge of definitions.py:1001-1047 on an (nx, nz) array:
     corner products plus full-length edge rows and columns."""
@@ -243,7 +290,8 @@ def _sponge(eta, cpml, NPS, eta_max):
 
 
 def problem_from_objects(model, domain, density_method: str | None = None,
-                         source_convention: str = "legacy", receivers=None) -> Problem:
+                         source_convention: str = "legacy", receivers=None,
+                         material=None) -> Problem:
     """Host setup from a Model / Domain pair after ``Model.build``
     (classes.py:393-547): tables painted on domain.geometry, C-PML profiles
     from the reference formulas, the model's source series and dt."""
@@ -263,6 +311,15 @@ def problem_from_objects(model, domain, density_method: str | None = None,
         tab = model.stiffness_coefficients
         for n in STIFFNESS:
             coef[n] = _paint(tab[n].values, geom, cpml)
+        # density factor near air (classes.py:529-534: 2-D, more than one material)
+        if dim == 2.0 and int(getattr(domain, "nmats", 1)) > 1:
+            # the material list of the Material object (definitions.py:502-505)
+            mlist = getattr(material, "material", None)
+            names = [str(v).strip().lower() for v in
+                     (np.asarray(mlist).ravel() if mlist is not None else [])]
+            air = names.index("air") if "air" in names else None
+            grad = air_density_gradient(geom, air, int(getattr(model, "air_gradient_integer", 2) or 2))
+            coef["rho"] = _apply_density_gradient(coef["rho"], grad, cpml)
         att = getattr(model, "attenuation_coefficients", None)
         if att is not None:
             for a in ("x", "y", "z"):
@@ -431,13 +488,13 @@ def run(model, domain=None, material=None, *, num_threads: int = 1, jsonfile: st
     solver on the GPU.  ``model`` is a Model built by ``Model.build`` (or a
     project JSON path, which takes the file route).  ``num_threads`` is
     accepted for signature compatibility; the host does only setup."""
-    del num_threads, material
+    del num_threads
     if isinstance(model, str):
         return run_files(model, True, density_method or "legacy", precision, snapshot_every,
                          snapshot_single, receivers, graph_block, raise_unstable)
     if domain is None:
         raise ValueError("run(model, domain): the domain is required with a Model object")
-    p = problem_from_objects(model, domain, density_method, receivers=receivers)
+    p = problem_from_objects(model, domain, density_method, receivers=receivers, material=material)
     wd = os.path.dirname(os.path.abspath(jsonfile or getattr(model, "project_file", None) or "."))
     tr, unstable, nsnap = _execute(p, precision, snapshot_every, snapshot_single, wd, graph_block,
                                    False)

@@ -295,6 +295,43 @@ def test_problem_from_files(tmp_path):
     assert p.shots[0].force[1] is None
 
 
+def test_air_density_gradient_closed_forms():
+    """airsurf (definitions.py:475-557): 0.15 on the solid cell next to air,
+    then 0.575, 0.7875 (halving what is left) one and two cells further in,
+    across either axis; diagonal neighbours and air cells stay 1; an air
+    label of 0 or None (air listed first) gives no gradient."""
+    g = np.ones((6, 8), int)
+    g[:, :3] = 2                                   # air in z < 3 of a (nx, nz) image
+    got = dropin.air_density_gradient(g, 2, 3)
+    assert got.shape == (6, 8)
+    np.testing.assert_allclose(got[0], [1, 1, 1, 0.15, 0.575, 0.7875, 1, 1])
+    assert np.all(got == got[0])
+    g = np.ones((7, 7), int)
+    g[3, 3] = 2
+    got = dropin.air_density_gradient(g, 2, 2)
+    for (i, k), v in {(2, 3): 0.15, (4, 3): 0.15, (3, 2): 0.15, (3, 4): 0.15, (1, 3): 0.575,
+                      (3, 5): 0.575, (2, 2): 1.0, (3, 3): 1.0, (0, 0): 1.0}.items():
+        assert got[i, k] == v, (i, k, got[i, k])
+    assert np.all(dropin.air_density_gradient(g, 0) == 1) and np.all(dropin.air_density_gradient(g, None) == 1)
+
+
+def test_density_gradient_file_scales_rho(tmp_path):
+    """density_gradient.dat ((nx, nz) interior) multiplies rho; the C-PML
+    rows and columns repeat the scaled edge."""
+    _write_project(str(tmp_path))
+    base = dropin.problem_from_files(str(tmp_path / "p.json"), seismic=True)
+    nx0, nz0, c = base.nx - 10, base.nz - 10, 5
+    grad = np.ones((nx0, nz0))
+    grad[:, 0] = 0.15
+    grad[3, 4] = 0.5
+    fortranio.write_record(str(tmp_path / "density_gradient.dat"), grad)
+    p = dropin.problem_from_files(str(tmp_path / "p.json"), seismic=True)
+    inner = base.coef["rho"][c:-c, c:-c] * grad.T
+    np.testing.assert_allclose(p.coef["rho"][c:-c, c:-c], inner, rtol=0, atol=0)
+    np.testing.assert_array_equal(p.coef["rho"][:c, c:-c], np.broadcast_to(inner[0], (c, nx0)))
+    np.testing.assert_array_equal(p.coef["c11"], base.coef["c11"])
+
+
 def test_dropin_cli_reports_missing_project(capsys):
     assert dropin.main(["/nonexistent/p.json", "seismic=true"]) == 1
     assert "seidart_amd.run" in capsys.readouterr().err
