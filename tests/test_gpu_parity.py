@@ -90,6 +90,27 @@ def test_batched_uneven_models(oracle_lib, kind):
         off += len(p.shots)
 
 
+def test_el25_fp32_tiles_chunks_segments(oracle_lib):
+    """FP32 2.5-D elastic against the FP64 oracle: traces near the source and
+    every final field, on a grid several tiles wide in x, several chunks long
+    in y and with an odd number of planes, run in odd-length segments."""
+    p = tiny_problem("elastic25", seed=59, nx=140, ny=70, nz=27, npml=5, nt=31, nrec=9, gamma=1e-3)
+    si, sj, sk = p.shots[0].src      # receivers the wave reaches within 31 steps (FP32 range)
+    p.shots[0].rec = np.array([[si + dx, sj + dy, sk + dz] for dx, dy, dz in
+                               ((0, 0, 0), (3, 0, 0), (-3, 1, 0), (0, 3, 1), (2, -2, -2), (1, 1, 3))],
+                              dtype=np.int32)
+    sim = _sim(p, "fp32")
+    for n in (7, 11, 13):
+        sim.run(n)
+    sim.check()
+    o = oracle_lib.OracleRun(p).run()
+    assert rel_l2(sim.traces()[0], o.trace) < FP32_TOL
+    for n in o.names:
+        exp = o.field(n)
+        if np.abs(exp).max() > 0:
+            assert rel_l2(sim.field(0, n), exp) < FP32_TOL, n
+
+
 @pytest.mark.parametrize("kind", ["elastic25", "em25"])
 def test_3d_tiles_and_y_chunks(oracle_lib, kind):
     """A 2.5-D grid several tiles wide in x, with more y rows than one work
