@@ -166,6 +166,23 @@ def test_em25_block_snapshots_match_record_snapshots(tmp_path):
             assert np.abs(got - rec).max() <= tol, (ch, it)
 
 
+def test_common_offset_batched_matches_oracle(tmp_path, oracle_lib):
+    """The batched common-offset survey (all source positions as shots of
+    one plan) against each shot's own oracle run, and its CSV."""
+    from seidart_amd import survey
+    from test_host import _co_files, _write_project
+    _write_project(str(tmp_path))
+    src, rec = _co_files(str(tmp_path))
+    co = survey.CommonOffset(src, "Vz", str(tmp_path / "p.json"), rec, precision="fp64")
+    img = co.co_run(graph_block=0)
+    assert img.shape == (40, 3)
+    q = co.problem()
+    for s in range(3):
+        o = oracle_lib.OracleRun(q, shot=s).run()
+        assert rel_l2(img[:, s], o.trace[:, 1, 0]) < FP64_TOL, s
+    assert np.allclose(np.loadtxt(co.csvfilename, delimiter=","), img)
+
+
 def test_graph_capture_equals_direct_launch():
     p = tiny_problem("elastic2", seed=51, nt=64, nshots=2)
     a = _sim(p, "fp32").run(graph_block=0).check().traces()

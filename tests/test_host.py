@@ -6,7 +6,7 @@ import os
 import numpy as np
 import pytest
 
-from seidart_amd import configs, cpml, fortranio, problem, run as dropin, sources
+from seidart_amd import configs, cpml, fortranio, problem, run as dropin, sources, survey
 
 
 # --------------------------------------------------------------------- C-PML
@@ -330,6 +330,32 @@ def test_density_gradient_file_scales_rho(tmp_path):
     np.testing.assert_allclose(p.coef["rho"][c:-c, c:-c], inner, rtol=0, atol=0)
     np.testing.assert_array_equal(p.coef["rho"][:c, c:-c], np.broadcast_to(inner[0], (c, nx0)))
     np.testing.assert_array_equal(p.coef["c11"], base.coef["c11"])
+
+
+def _co_files(tmp):
+    src, rec = os.path.join(tmp, "src.csv"), os.path.join(tmp, "rec.csv")
+    np.savetxt(src, [[2.0, 0, 1.5], [4.5, 0, 1.5], [7.25, 0, 2.0]], delimiter=",", header="x,y,z",
+               comments="")
+    np.savetxt(rec, [[3.0, 0, 0.5], [5.5, 0, 0.5], [8.0, 0, 1.0]], delimiter=",", header="x,y,z",
+               comments="")
+    return src, rec
+
+
+def test_common_offset_problem(tmp_path):
+    """CommonOffset (common_offset.py:14-283): one shot per source row, its
+    receiver int(xyz / d + cpml), the project's source series and model."""
+    _write_project(str(tmp_path))
+    src, rec = _co_files(str(tmp_path))
+    co = survey.CommonOffset(src, "Vz", str(tmp_path / "p.json"), rec)
+    q = co.problem()
+    assert len(q.shots) == 3 and co.component == 1 and co.time_steps == 40
+    assert q.shots[0].src == (int(2.0 / 0.5) + 5 - 1, 0, int(1.5 / 0.5) + 5 - 1)
+    np.testing.assert_array_equal(q.shots[2].rec, [[int(8.0 / 0.5 + 5), 0, int(1.0 / 0.5 + 5)]])
+    np.testing.assert_array_equal(q.shots[1].force[2], co.base.shots[0].force[2])
+    with pytest.raises(ValueError):
+        survey.CommonOffset(src, "Vq", str(tmp_path / "p.json"), rec)
+    with pytest.raises(ValueError):   # a 2-D run has no y component
+        survey.CommonOffset(src, "Vy", str(tmp_path / "p.json"), rec)
 
 
 def test_dropin_cli_reports_missing_project(capsys):
