@@ -208,9 +208,18 @@ def problem_from_files(jsonfile: str, seismic: bool = True, workdir: str | None 
     if receivers is not None:
         xyz = load_receivers(receivers) if isinstance(receivers, str) else receivers
         rec = receiver_indices(xyz, dx, dy, dz, cpml, dim)
-    p = Problem("elastic25" if (seismic and dim == 2.5) else "elastic2" if seismic else
-                "em25" if dim == 2.5 else "em2", nx, ny, nz, dx, dy, dz, dt, nt, coef, prof,
-                [Shot(src=src, force=tuple(force), rec=rec)], gamma)
+    # initial conditions (Model.run writes them as .T records, classes.py:882-898)
+    kind = ("elastic" if seismic else "em") + ("25" if dim == 2.5 else "2")
+    init = {}
+    for a in ("x", "y", "z"):
+        f = P(f"initialcondition{'V' if seismic else 'E'}{a}.dat")
+        name = ("v" if seismic else "E") + a
+        if os.path.exists(f) and name in FIELDS[kind]:
+            arr = read_record(f, shape=(nz, ny, nx) if dim == 2.5 else shape2)
+            if np.any(arr):
+                init[name] = arr
+    p = Problem(kind, nx, ny, nz, dx, dy, dz, dt, nt, coef, prof,
+                [Shot(src=src, force=tuple(force), rec=rec)], gamma, [init] if init else None)
     p.density_method = density_method
     p.src_index = (xind, yind, zind)
     p.cpml = cpml

@@ -315,6 +315,22 @@ def test_air_density_gradient_closed_forms():
     assert np.all(dropin.air_density_gradient(g, 0) == 1) and np.all(dropin.air_density_gradient(g, None) == 1)
 
 
+def test_initial_condition_files(tmp_path):
+    """initialconditionV{x,z}.dat (Model.run writes them as .T records,
+    classes.py:882-898) become the run's initial fields; all-zero files are
+    ignored."""
+    _write_project(str(tmp_path))
+    base = dropin.problem_from_files(str(tmp_path / "p.json"), seismic=True)
+    assert not base.init
+    vx0 = np.zeros((base.nz, base.nx))
+    vx0[12, 20] = 1e-3
+    fortranio.write_record(str(tmp_path / "initialconditionVx.dat"), vx0)
+    fortranio.write_record(str(tmp_path / "initialconditionVz.dat"), np.zeros((base.nz, base.nx)))
+    p = dropin.problem_from_files(str(tmp_path / "p.json"), seismic=True)
+    assert list(p.init[0]) == ["vx"]
+    np.testing.assert_array_equal(p.init[0]["vx"], vx0)
+
+
 def test_density_gradient_file_scales_rho(tmp_path):
     """density_gradient.dat ((nx, nz) interior) multiplies rho; the C-PML
     rows and columns repeat the scaled edge."""
