@@ -123,6 +123,12 @@ typedef struct sdt_model {
  *      after the update (seismicfdtd.f95:359-361: s(it)*dt/rho(src);
  *      electromagfdtd.f95:300-301: s(it)*dt/eps_r(src)); NULL = none.
  *      Components are (x, y, z); 2-D kinds use x and z.
+ * msrc[c]: ELASTIC2 / ELASTIC25 only (NULL elsewhere): nt per-step stress
+ *      increments (stress rate x dt) added at the source cell after the
+ *      stress update, components (xx, yy, zz, xy, xz, yz); NULL = none.  The
+ *      moment-tensor series of sourcefunction.py:296-372; the reference's
+ *      solver that consumes them is not in its sources, so this injection
+ *      is this library's definition.  2-D kinds use xx, zz and xz.
  * rec: nrec triples (i, j, k), 0-based padded; trace: nt * ncomp * nrec,
  *      trace[it][c][r] = component c at receiver r after step it, with
  *      ncomp = 2 (ELASTIC2: vx vz, EM2: Ex Ez) or 3 (vx vy vz / Ex Ey Ez).
@@ -138,6 +144,7 @@ typedef struct sdt_shot {
   int nrec;
   const int* rec;
   void* trace;
+  const void* msrc[6];
 } sdt_shot;
 
 /* Elements of one shot's psi workspace for a kind and grid (slab storage:

@@ -136,6 +136,14 @@ static void elastic2(or_run_t* r) {
         sxz[c] = sxz[c] + (a15 * dvx_dx + a35 * dvz_dz + a55 * (dvz_dx + dvx_dz)) * DT;
       }
     }
+    /* stress-rate (moment-tensor) source: no counterpart in the legacy code;
+       rate * dt (xx, zz, xz) added at the source cell after the stress update */
+    {
+      const long long c = I2(r->isrc, r->ksrc);
+      if (r->msrc[0]) sxx[c] = sxx[c] + r->msrc[0][it] * DT;
+      if (r->msrc[2]) szz[c] = szz[c] + r->msrc[2][it] * DT;
+      if (r->msrc[4]) sxz[c] = sxz[c] + r->msrc[4][it] * DT;
+    }
     /* vx, seismicfdtd.f95:317-337 (A6: divides by rho(i,j)) */
 #pragma omp parallel for schedule(static)
     for (int j = 1; j < nz; ++j) {
@@ -413,6 +421,14 @@ static void elastic25(or_run_t* r) {
 #undef A4
 #undef A8
         }
+    }
+    /* stress-rate (moment-tensor) source: no counterpart in the legacy code;
+       rate * dt (xx, yy, zz, xy, xz, yz) added at the source cell */
+    {
+      const long long c = I3(r->isrc, r->jsrc, r->ksrc);
+      double* sig[6] = {sxx, syy, szz, sxy, sxz, syz};
+      for (int q = 0; q < 6; ++q)
+        if (r->msrc[q]) sig[q][c] = sig[q][c] + r->msrc[q][it] * dt;
     }
     /* vx, vy: seismicfdtd.f95:821-871 */
 #pragma omp parallel for schedule(static)

@@ -63,6 +63,9 @@ typedef struct {
   int blowup_step;         /* out: first step whose max |v| > 1e25 (or -1)    */
   int threads;             /* OpenMP threads (0 = runtime default)            */
   int split;               /* elastic2 only: 1 = split C-PML variant (below)  */
+  const double* msrc[6];   /* elastic: stress-rate series (xx, yy, zz, xy, xz,
+                              yz), rate * dt added at the source cell after
+                              the stress update; NULL = none                  */
 } or_run_t;
 
 /* Number of doubles of psi workspace a routine needs (kept across segments). */

@@ -28,6 +28,7 @@ class OrRun(C.Structure):
         ("nrec", C.c_int), ("rec", C.POINTER(C.c_int)), ("trace", _P),
         ("f", _P * 9), ("psi", _P),
         ("blowup_step", C.c_int), ("threads", C.c_int), ("split", C.c_int),
+        ("msrc", _P * 6),
     ]
 
 
@@ -103,6 +104,8 @@ class OracleRun:
                                   for v in (pr.kappa, pr.a, pr.b, pr.kappa_h, pr.a_h, pr.b_h)])
         self.src = [None if f is None else np.ascontiguousarray(f, dtype=np.float64)
                     for f in self.shot.force]
+        self.msrc = [None if f is None else np.ascontiguousarray(f, dtype=np.float64)
+                     for f in getattr(self.shot, "moment", (None,) * 6)]
         self.rec = np.ascontiguousarray(np.asarray(self.shot.rec, dtype=np.int32).reshape(-1, 3))
         self.ncomp = ncomp(p.kind)
         self.trace = np.zeros((p.nt, self.ncomp, self.rec.shape[0]))
@@ -135,6 +138,8 @@ class OracleRun:
         r.blowup_step = self.blowup
         r.threads = self.threads
         r.split = 1 if getattr(p, "cpml_split", False) else 0
+        for q in range(6):
+            r.msrc[q] = _ptr(self.msrc[q])
         rc = lib().or_run(self.kind, C.byref(r))
         if rc:
             raise RuntimeError(f"oracle failed with code {rc}")

@@ -42,6 +42,7 @@ class SdtShot(C.Structure):
         ("nrec", C.c_int),
         ("rec", C.c_void_p),
         ("trace", C.c_void_p),
+        ("msrc", C.c_void_p * 6),
     ]
 
 

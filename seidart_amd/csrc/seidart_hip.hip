@@ -85,6 +85,7 @@ struct DShot {
   const int* rec;
   void* trace;
   int* blowup;
+  const void* msrc[6];  // stress increments xx yy zz xy xz yz (elastic)
 };
 
 // Adjoint-side buffers of one shot (elastic 2-D), see sdt_adj_shot.
@@ -316,6 +317,11 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
   const auto srcx = cp<T>(S.src[0]), srcz = cp<T>(S.src[2]);
   const T incx = (srcx != nullptr) ? srcx[it] : T(0);
   const T incz = (srcz != nullptr) ? srcz[it] : T(0);
+  // stress-rate (moment-tensor) source: xx, zz, xz increments of this step
+  const bool msrc = S.msrc[0] || S.msrc[2] || S.msrc[4];
+  const T incsxx = S.msrc[0] ? cp<T>(S.msrc[0])[it] : T(0);
+  const T incszz = S.msrc[2] ? cp<T>(S.msrc[2])[it] : T(0);
+  const T incsxz = S.msrc[4] ? cp<T>(S.msrc[4])[it] : T(0);
 
   auto row = [&](int r) { return (uint32_t)(r < 0 ? 0 : (r >= nz ? nz - 1 : r)) * nx + ic; };
   auto valid = [&](int r) { return incol && r >= 0 && r < nz; };
@@ -460,6 +466,11 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
         dvx_dz = fma(dvx_dz, ikzhr, q_vxz);
       }
       sxz_c = fma(fma(s55, dvz_dx + dvx_dz, fma(s35, dvz_dz, s15 * dvx_dx)), dt, sxz_c);
+    }
+    if (msrc && rv && is_src_col && r == ksrc) {  // every lane that computes the cell adds it
+      sxx_c += incsxx;
+      szz_c += incszz;
+      sxz_c += incsxz;
     }
     if (own && rv && r >= zo && r < zend) {
       sxxo[c] = sxx_c;
@@ -1519,7 +1530,8 @@ __device__ __forceinline__ A32<const T> cplane(const void* f, long nxy, int k) {
 
 template <typename T, bool PML>
 __device__ __forceinline__ void el25_stress_tile(const DModel* __restrict__ models,
-                                              const DShot* __restrict__ shots, const int4 wv) {
+                                              const DShot* __restrict__ shots, const int4 wv,
+                                              const int it) {
   const int s = __builtin_amdgcn_readfirstlane(wv.x);
   const DShot& S = shots[s];
   const DModel& M = models[S.model];
@@ -1692,6 +1704,14 @@ __device__ __forceinline__ void el25_stress_tile(const DModel* __restrict__ mode
       if (inx) { psi[ox] = mxx; psi[ox + L.SX] = myx; psi[ox + 2 * L.SX] = mzx; }
       if (iny) { psi[oy] = mxy; psi[oy + L.SY] = myy; psi[oy + 2 * L.SY] = mzy; }
       if (inz) { psi[oz] = mxz; psi[oz + L.SZ] = myz; psi[oz + 2 * L.SZ] = mzz; }
+      if (i == S.isrc && k == S.ksrc && j == S.jsrc) {  // stress-rate (moment-tensor) source
+        if (S.msrc[0]) SXX[c] = SXX[c] + cp<T>(S.msrc[0])[it];
+        if (S.msrc[1]) SYY[c] = SYY[c] + cp<T>(S.msrc[1])[it];
+        if (S.msrc[2]) SZZ[c] = SZZ[c] + cp<T>(S.msrc[2])[it];
+        if (S.msrc[3]) SXY[c] = SXY[c] + cp<T>(S.msrc[3])[it];
+        if (S.msrc[4]) SXZ[c] = SXZ[c] + cp<T>(S.msrc[4])[it];
+        if (S.msrc[5]) SYZ[c] = SYZ[c] + cp<T>(S.msrc[5])[it];
+      }
     }
     vxm = vxc; vym = vyc; vzm = vzc;
     vxc = vxn; vyc = vyn; vzc = vzn;
@@ -1703,7 +1723,7 @@ __device__ __forceinline__ void el25_stress_tile(const DModel* __restrict__ mode
 template <typename T>
 __global__ void __launch_bounds__(NT25, sizeof(T) == 4 ? SDT_EL25S_WAVES : 2)
 k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-          const int4* __restrict__ work) {
+          const int4* __restrict__ work, const int* __restrict__ base, int rel) {
   const int4 wv = work[blockIdx.x];
   const int s = __builtin_amdgcn_readfirstlane(wv.x);
   if (s < 0) return;
@@ -1713,10 +1733,11 @@ k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots
   const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff, j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
   const bool interior = xo - 1 >= M.lx && xo + 62 < M.nx - M.lx && k >= M.lz && k < M.nz - M.lz &&
                         j0 >= M.ly && j1 <= M.ny - M.ly;
+  const int it = *base + rel;
   if (interior)
-    el25_stress_tile<T, false>(models, shots, wv);
+    el25_stress_tile<T, false>(models, shots, wv, it);
   else
-    el25_stress_tile<T, true>(models, shots, wv);
+    el25_stress_tile<T, true>(models, shots, wv, it);
 }
 
 template <typename T, bool PML>
@@ -2462,7 +2483,8 @@ void launch_one(Plan* p, hipStream_t st, int rel, int which) {
     case SDT_EM2 * 2 + 1:
       break;  // the fused step kernel does both phases
     case SDT_ELASTIC25 * 2:
-      hipLaunchKernelGGL(k_el25_stress<T>, grd, dim3(NT25), 0, st, p->dmodels, p->dshots, p->dwork);
+      hipLaunchKernelGGL(k_el25_stress<T>, grd, dim3(NT25), 0, st, p->dmodels, p->dshots, p->dwork,
+                         p->dbase, rel);
       break;
     case SDT_ELASTIC25 * 2 + 1:
       hipLaunchKernelGGL(k_el25_vel<T>, grd, dim3(NT25), 0, st, p->dmodels, p->dshots, p->dwork,
@@ -2805,6 +2827,7 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
     for (int f = 0; f < 9; ++f) D.g[f] = S.field_b[f];
     D.psi2 = S.psi_b;
     for (int c = 0; c < 3; ++c) D.src[c] = S.src[c];
+    for (int c = 0; c < 6; ++c) D.msrc[c] = (kind == SDT_ELASTIC2 || kind == SDT_ELASTIC25) ? S.msrc[c] : nullptr;
     D.nrec = S.nrec; D.rec = S.rec; D.trace = S.trace;
     D.blowup = p->dflags + s;
   }

@@ -79,13 +79,19 @@ def ncomp(kind: str) -> int:
     return 2 if kind in ("elastic2", "em2") else 3
 
 
+MOMENT_COMPONENTS = ("xx", "yy", "zz", "xy", "xz", "yz")
+
+
 @dataclass
 class Shot:
     """Source cell (i, j, k), the source series per component (x, y, z; None
-    = absent; length nt) and receiver cells (nrec, 3)."""
+    = absent; length nt), receiver cells (nrec, 3) and, for elastic runs,
+    optional stress-rate (moment-tensor) series per component
+    MOMENT_COMPONENTS (sourcefunction.py:296-372)."""
     src: tuple
     force: tuple = (None, None, None)
     rec: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), dtype=np.int32))
+    moment: tuple = (None,) * 6
 
 
 @dataclass
@@ -335,6 +341,23 @@ def source_increments(p: Problem, sh: Shot):
         else:
             div = p.coef[("e11", "e22", "e33")[c]][k, i]
         out.append(f * p.dt / div)
+    return out
+
+
+def stress_source_increments(p: Problem, sh: Shot):
+    """Per-step stress increments at the source cell, per MOMENT_COMPONENTS
+    entry: rate * dt (None = absent).  Only elastic runs carry them; a 2-D run
+    uses xx, zz and xz.  The reference writes these series
+    (seismicsource{xx,...}.dat, sourcefunction.py:358-363) for a solver whose
+    source is not in its sources, so the injection rule is this package's."""
+    if not p.kind.startswith("elastic"):
+        return [None] * 6
+    out = []
+    for q, m in enumerate(getattr(sh, "moment", (None,) * 6)):
+        if m is None or (p.kind == "elastic2" and MOMENT_COMPONENTS[q] not in ("xx", "zz", "xz")):
+            out.append(None)
+            continue
+        out.append(np.asarray(m, dtype=np.float64)[: p.nt] * p.dt)
     return out
 
 

@@ -37,8 +37,8 @@ from .cpml import (AxisProfile, alpha_max_em, alpha_max_seismic, axis_profile,
                    profiles_from_2d, sigma_max_em, sigma_max_seismic)
 from .fortranio import em25_block_name, read_record, read_stream, snapshot_name, write_em25_block, \
     write_snapshot
-from .problem import (CONDUCTIVITY, DENSITY_METHODS, FIELDS, PERMITTIVITY, STIFFNESS,
-                      TRACE_COMPONENTS, Problem, Shot)
+from .problem import (CONDUCTIVITY, DENSITY_METHODS, FIELDS, MOMENT_COMPONENTS, PERMITTIVITY,
+                      STIFFNESS, TRACE_COMPONENTS, Problem, Shot)
 
 __all__ = ["run", "run_files", "problem_from_files", "problem_from_objects", "RunResult",
            "load_receivers", "receiver_indices", "source_index", "christoffel_vmax",
@@ -200,6 +200,15 @@ def problem_from_files(jsonfile: str, seismic: bool = True, workdir: str | None 
         force.append(read_record(f)[:nt] if os.path.exists(f) else None)
     if dim != 2.5:
         force[1] = None
+    # stress-rate (moment-tensor) series, sourcefunction.py:358-363
+    moment = [None] * 6
+    if seismic:
+        for q, comp in enumerate(MOMENT_COMPONENTS):
+            f = P(f"seismicsource{comp}.dat")
+            if os.path.exists(f):
+                m = read_record(f)[:nt]
+                if np.any(m):
+                    moment[q] = m
     xind = int(_jget(S, "xind", default=int(float(S["x"]) / dx)))
     zind = int(_jget(S, "zind", default=int(float(S["z"]) / dz)))
     yind = int(_jget(S, "yind", default=0)) if dim == 2.5 else 0
@@ -219,7 +228,8 @@ def problem_from_files(jsonfile: str, seismic: bool = True, workdir: str | None 
             if np.any(arr):
                 init[name] = arr
     p = Problem(kind, nx, ny, nz, dx, dy, dz, dt, nt, coef, prof,
-                [Shot(src=src, force=tuple(force), rec=rec)], gamma, [init] if init else None)
+                [Shot(src=src, force=tuple(force), rec=rec, moment=tuple(moment))], gamma,
+                [init] if init else None)
     p.density_method = density_method
     p.src_index = (xind, yind, zind)
     p.cpml = cpml
@@ -379,6 +389,12 @@ def problem_from_objects(model, domain, density_method: str | None = None,
                                  float(model.theta or 0), float(getattr(model, "psi", 0) or 0)))
     if dim != 2.5:
         force[1] = None
+    moment = [None] * 6
+    if seismic:  # pointsource sets source_sigma_* (sourcefunction.py:365-370)
+        for q, comp in enumerate(MOMENT_COMPONENTS):
+            m = getattr(model, f"source_sigma_{comp}", None)
+            if m is not None and np.any(m):
+                moment[q] = np.asarray(m, dtype=np.float64)
     xind, zind = int(float(model.x) / dx), int(float(model.z) / dz)
     yind = int(float(model.y) / dy) if dim == 2.5 else 0
     src = source_index(xind, yind, zind, cpml, dim, source_convention)
@@ -396,7 +412,8 @@ def problem_from_objects(model, domain, density_method: str | None = None,
             init[name] = arr.T.copy() if dim != 2.5 else np.transpose(arr, (2, 1, 0)).copy()
     kind = ("elastic" if seismic else "em") + ("25" if dim == 2.5 else "2")
     p = Problem(kind, nx, ny, nz, dx, dy, dz, dt, nt, coef, prof,
-                [Shot(src=src, force=tuple(force), rec=rec)], gamma, [init] if init else None)
+                [Shot(src=src, force=tuple(force), rec=rec, moment=tuple(moment))], gamma,
+                [init] if init else None)
     p.density_method = density_method or getattr(model, "density_method", "legacy") or "legacy"
     p.src_index = (xind, yind, zind)
     p.cpml = cpml

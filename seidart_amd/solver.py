@@ -17,7 +17,7 @@ import torch
 
 from . import _hip
 from .problem import FIELDS, KINDS, Problem, aux_b_x_half_by_row, em_dbh, kernel_coefs, ncomp, \
-    source_increments
+    source_increments, stress_source_increments
 
 __all__ = ["Simulation", "PRECISIONS"]
 
@@ -137,6 +137,8 @@ class Simulation:
                 if all_sources and self.kind in ("elastic2", "em2"):
                     inc = [np.zeros(p.nt) if (v is None and c != 1) else v for c, v in enumerate(inc)]
                 bufs["src"] = [None if v is None else T(np.pad(v, (0, self.nt - v.size))) for v in inc]
+                minc = stress_source_increments(p, sh)
+                bufs["msrc"] = [None if v is None else T(np.pad(v, (0, self.nt - v.size))) for v in minc]
                 rec = np.asarray(sh.rec, dtype=np.int32).reshape(-1, 3)
                 bufs["rec"] = torch.as_tensor(rec, device=self.device) if rec.size else None
                 bufs["trace"] = torch.zeros((self.nt, self.ncomp, max(rec.shape[0], 1)),
@@ -154,6 +156,8 @@ class Simulation:
                     S.psi_b = bufs["psi_b"].data_ptr()
                 for c in range(3):
                     S.src[c] = bufs["src"][c].data_ptr() if bufs["src"][c] is not None else None
+                for c in range(6):
+                    S.msrc[c] = bufs["msrc"][c].data_ptr() if bufs["msrc"][c] is not None else None
                 S.nrec = bufs["nrec"]
                 S.rec = bufs["rec"].data_ptr() if bufs["rec"] is not None else None
                 S.trace = bufs["trace"].data_ptr()

@@ -96,9 +96,10 @@ class CommonOffset:
     def problem(self) -> Problem:
         """All source positions as shots of one problem on the project's model."""
         p = self.base
-        force = p.shots[0].force
+        force, moment = p.shots[0].force, p.shots[0].moment
         rec = self.receiver_cells()
-        shots = [Shot(src=s, force=force, rec=rec[q:q + 1]) for q, s in enumerate(self.source_cells())]
+        shots = [Shot(src=s, force=force, rec=rec[q:q + 1], moment=moment)
+                 for q, s in enumerate(self.source_cells())]
         q = Problem(p.kind, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, p.dt, p.nt, p.coef, p.prof, shots,
                     p.gamma)
         for a in ("density_method", "cpml", "dim", "cpml_split"):

@@ -29,10 +29,33 @@ def test_library_exports_every_declared_symbol():
     assert b"gfx950" in lib.sdt_version()
 
 
-def test_struct_layout_matches_header():
+def test_struct_layout_matches_header(tmp_path):
     # 6 ints, 4 doubles, scal[4], coef[40], prof[3][6], aux
     assert C.sizeof(_hip.SdtModel) == 24 + 32 + 32 + 8 * 40 + 8 * 18 + 8
-    assert C.sizeof(_hip.SdtShot) == 16 + 8 * 9 + 8 + 8 * 9 + 8 + 8 * 3 + 8 + 8 + 8
+    assert C.sizeof(_hip.SdtShot) == 16 + 8 * 9 + 8 + 8 * 9 + 8 + 8 * 3 + 8 + 8 + 8 + 8 * 6
+    # and against the header itself, compiled by the host C compiler
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no host C compiler")
+    inc = os.path.join(os.path.dirname(__file__), "..", "include")
+    src = tmp_path / "probe.c"
+    structs = {"sdt_model": _hip.SdtModel, "sdt_shot": _hip.SdtShot, "sdt_adj_shot": _hip.SdtAdjShot}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "seidart_hip.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                 check=True).stdout.splitlines())
+    for cname, cls in structs.items():
+        assert int(out[cname]) == C.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert int(out[f"{cname}.{f}"]) == getattr(cls, f).offset, (cname, f)
 
 
 @pytest.mark.parametrize("kind,dims,expect", [

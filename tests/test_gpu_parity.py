@@ -183,6 +183,27 @@ def test_common_offset_batched_matches_oracle(tmp_path, oracle_lib):
     assert np.allclose(np.loadtxt(co.csvfilename, delimiter=","), img)
 
 
+@pytest.mark.parametrize("kind,precision", [("elastic2", "fp64"), ("elastic2", "fp32"),
+                                             ("elastic25", "fp64"), ("elastic25", "fp32")])
+def test_moment_tensor_source_matches_oracle(oracle_lib, kind, precision):
+    """Stress-rate (moment-tensor) sources with a force source beside them,
+    on heterogeneous grids, against the oracle; 2-D runs use xx, zz, xz."""
+    kw = dict(nx=70, ny=20, nz=40) if kind == "elastic25" else dict(nx=90, nz=60)
+    p = tiny_problem(kind, seed=66, npml=5, nt=50, nrec=8, gamma=1e-3, **kw)
+    rng = np.random.default_rng(3)
+    f = np.asarray(p.shots[0].force[2], dtype=np.float64)
+    p.shots[0].moment = tuple(f * rng.uniform(-2e6, 2e6) for _ in range(6))
+    si, sj, sk = p.shots[0].src
+    p.shots[0].rec = np.array([[si + d, sj, sk + e] for d, e in ((0, 0), (2, 1), (-3, 2), (1, -2))],
+                              dtype=np.int32)
+    sim = _sim(p, precision).run().check()
+    if precision == "fp64":
+        _compare(sim, p, oracle_lib, FP64_TOL)
+    else:
+        o = oracle_lib.OracleRun(p).run()
+        assert rel_l2(sim.traces()[0], o.trace) < FP32_TOL
+
+
 def test_graph_capture_equals_direct_launch():
     p = tiny_problem("elastic2", seed=51, nt=64, nshots=2)
     a = _sim(p, "fp32").run(graph_block=0).check().traces()

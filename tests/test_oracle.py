@@ -113,3 +113,57 @@ def test_oracle_energy_conserved_without_pml(oracle_lib):
         o.run(it, 40)
         e.append(sum(float(np.sum(o.field(n) ** 2)) for n in ("vx", "vz")))
     assert max(e[2:]) < 50 * min(e[2:])
+
+
+def _homogeneous_iso(kind, n=41, ny=1, nt=24):
+    """Homogeneous isotropic elastic grid without C-PML, a source in the
+    middle and no force: the mirror-symmetry setting of the moment tests."""
+    from seidart_amd.cpml import AxisProfile
+    from seidart_amd.problem import STIFFNESS, Problem, Shot
+    lam, mu, rho = 2.0e9, 1.0e9, 1000.0
+    is3d = kind == "elastic25"
+    shape = (n, n)
+    C = {k: np.zeros(shape) for k in STIFFNESS}
+    for k in ("c11", "c22", "c33"):
+        C[k][:] = lam + 2 * mu
+    for k in ("c12", "c13", "c23"):
+        C[k][:] = lam
+    for k in ("c44", "c55", "c66"):
+        C[k][:] = mu
+    C["rho"][:] = rho
+    dx = 1.0
+    dt = 0.3 * dx / np.sqrt((lam + 2 * mu) / rho)
+    prof = {"x": AxisProfile.trivial(n), "z": AxisProfile.trivial(n)}
+    if is3d:
+        prof["y"] = AxisProfile.trivial(ny)
+    c = n // 2
+    t = np.arange(nt) * dt
+    rate = np.exp(-((t - 8 * dt) / (3 * dt)) ** 2) * 1e6
+    moment = (rate, rate if is3d else None, rate, None, None, None)   # explosive (tnt)
+    sh = Shot(src=(c, ny // 2 if is3d else 0, c), rec=np.zeros((0, 3), np.int32), moment=moment)
+    p = Problem(kind, n, ny if is3d else 1, n, dx, dx, dx, float(dt), nt, C, prof, [sh])
+    p.validate()
+    return p, c
+
+
+def test_oracle_explosive_moment_source_mirror_symmetry(oracle_lib):
+    """An isotropic stress-rate source (xx = zz) at a stress node of a
+    homogeneous isotropic medium: the field is mirror symmetric about the
+    source, vx odd about the node in x (vx[i] at i - 1/2: vx[2c + 1 - i] =
+    -vx[i]) and vz odd in z (vz[j] at j + 1/2: vz[2c - 1 - j] = -vz[j]),
+    sxx, szz even; the wave has not reached the edges."""
+    p, c = _homogeneous_iso("elastic2")
+    o = oracle_lib.OracleRun(p).run()
+    vx, vz, sxx = o.field("vx"), o.field("vz"), o.field("sxx")
+    n = p.nx
+    assert np.abs(vx).max() > 0 and np.abs(vz).max() > 0
+    i = np.arange(2, n - 2)
+    np.testing.assert_allclose(vx[:, 2 * c + 1 - i], -vx[:, i], rtol=0, atol=1e-12 * np.abs(vx).max())
+    np.testing.assert_allclose(vz[2 * c - 1 - i, :], -vz[i, :], rtol=0, atol=1e-12 * np.abs(vz).max())
+    np.testing.assert_allclose(sxx[:, 2 * c - i], sxx[:, i], rtol=0, atol=1e-12 * np.abs(sxx).max())
+    # the increment is rate * dt at the source cell: after one step (zero
+    # velocities before it) sigma_xx there is exactly that
+    o1 = oracle_lib.OracleRun(p).run(0, 1)
+    rate0 = p.shots[0].moment[0][0]
+    assert o1.field("sxx")[c, c] == rate0 * p.dt
+    assert o1.field("szz")[c, c] == rate0 * p.dt and o1.field("sxz")[c, c] == 0.0
