@@ -240,7 +240,8 @@ def config_lines(steps: int = 400):
     out = {}
     cases = [("plain_psv", lambda nt: configs.plain_psv(nt=nt), 2000),
              ("gpr", lambda nt: configs.gpr(nt=nt), steps),
-             ("aniso25", lambda nt: configs.aniso25(nt=nt), min(steps, 200))]
+             ("aniso25", lambda nt: configs.aniso25(nt=nt), min(steps, 200)),
+             ("gpr25", lambda nt: configs.gpr25(nt=nt), min(steps, 200))]
     for name, make, K in cases:
         W = 50
         p = make(W + K)

@@ -301,6 +301,52 @@ def gpr(nt: int = 4000, seed: int = 1002) -> Problem:
     return Problem("em2", nx, 1, nz, d, 1.0, d, dt, nt, coef, prof, [shot])
 
 
+def gpr25(nt: int = 1000, seed: int = 1006, nx0=512, ny0=64, nz0=384, cpml=16) -> Problem:
+    """2.5-D GPR, the y-invariant extension of the 2-D GPR workload at half its
+    x and z extent: 512 x 64 x 384, dx = dy = dz = 0.01 m, C-PML 16; the same
+    snow bands over c-axis-rotated ice (rotation about y per column, so the
+    tensor is y-invariant), a 500 MHz Ez source at (256, 32, 10), 64
+    receivers at z = 10, y = 32 every 8 cells, dt from CFL 0.95 in 3-D.  Not
+    a configuration of the reference benchmark: a measured line for the 2.5-D
+    EM kernels."""
+    rng = np.random.default_rng(seed)
+    d = 0.01
+    nb = nz0 // 10 + 1
+    band_eps = rng.uniform(1.5, 2.2, size=nb)
+    band_sig = 10 ** rng.uniform(-5, -4, size=nb)
+    theta = np.deg2rad(rng.uniform(0, 30, size=nx0))
+    z = np.arange(nz0)
+    band = z // 10
+    C = {n: np.zeros((nz0, nx0)) for n in ("e11", "e12", "e13", "e22", "e23", "e33",
+                                            "s11", "s12", "s13", "s22", "s23", "s33")}
+    snow = z < nz0 // 3
+    for n in ("e11", "e22", "e33"):
+        C[n][snow, :] = band_eps[band[snow]][:, None]
+    c, s = np.cos(theta), np.sin(theta)
+    e1, e3 = 3.15, 3.18
+    ice = ~snow
+    C["e11"][ice, :] = (e1 * c * c + e3 * s * s)[None, :]
+    C["e33"][ice, :] = (e1 * s * s + e3 * c * c)[None, :]
+    C["e13"][ice, :] = ((e3 - e1) * s * c)[None, :]
+    C["e22"][ice, :] = e1
+    for n in ("s11", "s22", "s33"):
+        C[n][:, :] = band_sig[band][:, None]
+    coef = {k: _pad_edge(v, cpml) for k, v in C.items()}
+    nx, ny, nz = nx0 + 2 * cpml, ny0 + 2 * cpml, nz0 + 2 * cpml
+    emin = min(C["e11"].min(), C["e22"].min(), C["e33"].min())
+    vmax = CLIGHT / np.sqrt(emin)
+    dt = 0.95 / (vmax * np.sqrt(3.0 / d ** 2))
+    f0 = 500e6
+    sm = sigma_max_em(cpml, d, 2, 0.001)
+    am = alpha_max_em(1.0, f0)
+    prof = {a: axis_profile(n, cpml, d, sm, am, 5.0, 2, 2, dt, False)
+            for a, n in (("x", nx), ("y", ny), ("z", nz))}
+    fx, fy, fz = point_force(nt, dt, f0, 1.0, "gaus1", xz_rotation=90.0)
+    rec = np.array([[4 + 8 * k + cpml, ny0 // 2 + cpml, 10 + cpml] for k in range(64)], dtype=np.int32)
+    shot = Shot(src=(nx0 // 2 + cpml, ny0 // 2 + cpml, 10 + cpml), force=(fx, fy, fz), rec=rec)
+    return Problem("em25", nx, ny, nz, d, d, d, dt, nt, coef, prof, [shot])
+
+
 def aniso25(nt: int = 3000, seed: int = 1003, nx0=800, ny0=64, nz0=400, cpml=16) -> Problem:
     """2.5-D elastic: 800 x 64 x 400, dx = 1 m, C-PML 16; 8 seeded Voronoi
     regions of ice single-crystal stiffness rotated to a c-axis of dip
@@ -348,6 +394,7 @@ def aniso25(nt: int = 3000, seed: int = 1003, nx0=800, ny0=64, nz0=400, cpml=16)
 
 
 CONFIGS = {
+    "gpr25": gpr25,
     "plain_psv": plain_psv,
     "gpr": gpr,
     "aniso25": aniso25,

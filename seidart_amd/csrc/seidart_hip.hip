@@ -2044,6 +2044,10 @@ k_em2_adj_g(const DModel* __restrict__ models, const DShot* __restrict__ shots,
 //            y [nz][2ly][nx]: dEz_dy dEx_dy dHz_dy dHx_dy
 //            z [2lz][ny][nx]: dEy_dz dEx_dz dHy_dz dHx_dz
 // H boundary zeroing (:1059-1078) is applied by the next H update.
+// Two launches per step (H, then E) in the layout of the 2.5-D elastic
+// kernels: a wavefront takes one plane, 64 columns (62 owned; x neighbours by
+// lane shuffles) and a chunk of rows, walking y with the y neighbours in
+// registers; the z neighbours are loads of the adjacent plane.
 // ===========================================================================
 struct Slab3e {
   long SX, SY, SZ, bx, by, bz;
@@ -2057,159 +2061,221 @@ struct Slab3e {
   }
 };
 
-template <typename T>
-__global__ void __launch_bounds__(NT)
-k_em25_h(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-         const int4* __restrict__ work) {
-  const int4 wi = work[blockIdx.x];
-  if (wi.x < 0) return;
-  const DShot& S = shots[wi.x];
+template <typename T, bool PML>
+__device__ __forceinline__ void em25_h_tile(const DModel* __restrict__ models,
+                                           const DShot* __restrict__ shots, const int4 wv) {
+  const int s = __builtin_amdgcn_readfirstlane(wv.x);
+  const DShot& S = shots[s];
   const DModel& M = models[S.model];
   const int nx = M.nx, ny = M.ny, nz = M.nz;
-  const int i = wi.y + threadIdx.x, k = wi.z + threadIdx.y;
-  if (i > nx - 2 || k > nz - 2) return;
+  const int k = __builtin_amdgcn_readfirstlane(wv.z);
+  const int lane = threadIdx.x & 63;
+  const int i = __builtin_amdgcn_readfirstlane(wv.y) - 1 + lane;
+  const bool incol = (i >= 0 && i < nx);
+  const int ic = incol ? i : (i < 0 ? 0 : nx - 1);
+  // H lives on i <= nx-2, j <= ny-2, k <= nz-2 (the loop ranges of :901-962)
+  const bool own = incol && lane >= 1 && lane <= EW25 && i <= nx - 2 && k <= nz - 2;
+  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff;
+  const int j1 = min(__builtin_amdgcn_readfirstlane(wv.w) >> 16, ny - 1);
   const long nxy = (long)nx * ny;
   const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz), dbH = T(M.scal[0]);
   const int lx = M.lx, ly = M.ly, lz = M.lz;
-  const bool inx = in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
+  const bool inx = PML && incol && in_slab(i, nx, lx), inz = PML && in_slab(k, nz, lz);
   const Prof<T> P(&M.prof[0][0]);
-  const T bx_i = P[PB][i], ax_i = P[PA][i], ikx_i = P[PK][i];
+  const T bx_i = P[PB][ic], ax_i = P[PA][ic], ikx_i = P[PK][ic];
   const T bz_k = P[12 + PB][k], az_k = P[12 + PA][k], ikz_k = P[12 + PK][k];
   const T bzh_k = P[12 + PBH][k], azh_k = P[12 + PAH][k], ikzh_k = P[12 + PKH][k];
   gcptr<T> by = P[6 + PB], ay = P[6 + PA], iky = P[6 + PK];
   gcptr<T> byh = P[6 + PBH], ayh = P[6 + PAH], ikyh = P[6 + PKH];
-  const auto Ex = cp<T>(S.f[0]);
-  const auto Ey = cp<T>(S.f[1]);
-  const auto Ez = cp<T>(S.f[2]);
-  auto Hx = mp<T>(S.f[3]);
-  auto Hy = mp<T>(S.f[4]);
-  auto Hz = mp<T>(S.f[5]);
+  const int ku = k + 1 < nz ? k + 1 : k;
+  const A32<const T> EX = cplane<T>(S.f[0], nxy, k), EY = cplane<T>(S.f[1], nxy, k);
+  const A32<const T> EZ = cplane<T>(S.f[2], nxy, k);
+  const A32<const T> EXU = cplane<T>(S.f[0], nxy, ku), EYU = cplane<T>(S.f[1], nxy, ku);
+  const A32<T> HX = plane<T>(S.f[3], nxy, k), HY = plane<T>(S.f[4], nxy, k), HZ = plane<T>(S.f[5], nxy, k);
   auto psi = mp<T>(S.psi);
   const Slab3e L(nx, ny, nz, lx, ly, lz);
-  const int xs = slab_pos(i, nx, lx), zs = slab_pos(k, nz, lz);
-  for (int j = 0; j <= ny - 2; ++j) {
-    const long c = (long)k * nxy + (long)j * nx + i;
-    const bool iny = in_slab(j, ny, ly);
-    const long ox = L.bx + ((long)k * ny + j) * 2 * lx + xs;
-    const long oy = L.by + ((long)k * 2 * ly + slab_pos(j, ny, ly)) * nx + i;
-    const long oz = L.bz + ((long)zs * ny + j) * nx + i;
-    const bool lo = (i == 0 || j == 0 || k == 0);
-    // Hx, electromagfdtd.f95:901-920
-    {
-      T a = (Ez[c] - Ez[c + nx]) * idy;  // dEz_dy
-      if (iny) { T m = byh[j] * psi[oy] + ayh[j] * a; a = a * ikyh[j] + m; psi[oy] = m; }
-      T b = (Ey[c + nxy] - Ey[c]) * idz;  // dEy_dz
-      if (inz) { T m = bzh_k * psi[oz] + azh_k * b; b = b * ikzh_k + m; psi[oz] = m; }
-      const bool edge = lo || i == nx - 1 || j == ny - 2 || k == nz - 2;
-      Hx[c] = (edge ? T(0) : Hx[c]) + dbH * (b + a);
+  const int xs = inx ? slab_pos(i, nx, lx) : 0, zs = inz ? slab_pos(k, nz, lz) : 0;
+  if (j0 >= j1) return;
+  // y window: Ex, Ez of row j
+  uint32_t c = (uint32_t)j0 * nx + ic;
+  T exc = EX[c], ezc = EZ[c];
+#pragma unroll 1
+  for (int j = j0; j < j1; ++j, c += nx) {
+    const T exn = EX[c + nx], ezn = EZ[c + nx];  // row j+1 (j <= ny-2)
+    const T eyc = EY[c];
+    const T eyu = EYU[c], exu = EXU[c];          // plane k+1
+    const T eze = lane_next(ezc), eye = lane_next(eyc);
+    const T hx0 = HX[c], hy0 = HY[c], hz0 = HZ[c];
+    if (own) {
+      const bool iny = PML && in_slab(j, ny, ly);
+      const long ox = L.bx + ((long)k * ny + j) * 2 * lx + xs;
+      const long oy = L.by + ((long)k * 2 * ly + (iny ? slab_pos(j, ny, ly) : 0)) * nx + ic;
+      const long oz = L.bz + ((long)zs * ny + j) * nx + ic;
+      const bool lo = (i == 0 || j == 0 || k == 0);
+      {  // Hx, :901-920
+        T a = (ezc - ezn) * idy;  // dEz_dy
+        if (iny) { T m = byh[j] * psi[oy] + ayh[j] * a; a = a * ikyh[j] + m; psi[oy] = m; }
+        T b = (eyu - eyc) * idz;  // dEy_dz
+        if (inz) { T m = bzh_k * psi[oz] + azh_k * b; b = b * ikzh_k + m; psi[oz] = m; }
+        const bool edge = lo || i == nx - 1 || j == ny - 2 || k == nz - 2;
+        HX[c] = (edge ? T(0) : hx0) + dbH * (b + a);
+      }
+      {  // Hy, :922-942
+        T a = (exc - exu) * idz;  // dEx_dz
+        if (inz) { T m = bz_k * psi[oz + L.SZ] + az_k * a; a = a * ikz_k + m; psi[oz + L.SZ] = m; }
+        T b = (eze - ezc) * idx;  // dEz_dx
+        if (inx) { T m = bx_i * psi[ox] + ax_i * b; b = b * ikx_i + m; psi[ox] = m; }
+        const bool edge = lo || i == nx - 2 || j == ny - 1 || k == nz - 2;
+        HY[c] = (edge ? T(0) : hy0) + dbH * (b + a);
+      }
+      if (k >= 1) {  // Hz, :944-962
+        T a = (exn - exc) * idy;  // dEx_dy
+        if (iny) { T m = by[j] * psi[oy + L.SY] + ay[j] * a; a = a * iky[j] + m; psi[oy + L.SY] = m; }
+        T b = (eyc - eye) * idx;  // dEy_dx
+        if (inx) { T m = bx_i * psi[ox + L.SX] + ax_i * b; b = b * ikx_i + m; psi[ox + L.SX] = m; }
+        const bool edge = lo || i == nx - 2 || j == ny - 2 || k == nz - 1;
+        HZ[c] = (edge ? T(0) : hz0) + dbH * (b + a);
+      }
     }
-    // Hy, electromagfdtd.f95:922-942
-    {
-      T a = (Ex[c] - Ex[c + nxy]) * idz;  // dEx_dz
-      if (inz) { T m = bz_k * psi[oz + L.SZ] + az_k * a; a = a * ikz_k + m; psi[oz + L.SZ] = m; }
-      T b = (Ez[c + 1] - Ez[c]) * idx;  // dEz_dx
-      if (inx) { T m = bx_i * psi[ox] + ax_i * b; b = b * ikx_i + m; psi[ox] = m; }
-      const bool edge = lo || i == nx - 2 || j == ny - 1 || k == nz - 2;
-      Hy[c] = (edge ? T(0) : Hy[c]) + dbH * (b + a);
-    }
-    // Hz, electromagfdtd.f95:944-962
-    if (k >= 1) {
-      T a = (Ex[c + nx] - Ex[c]) * idy;  // dEx_dy
-      if (iny) { T m = by[j] * psi[oy + L.SY] + ay[j] * a; a = a * iky[j] + m; psi[oy + L.SY] = m; }
-      T b = (Ey[c] - Ey[c + 1]) * idx;  // dEy_dx
-      if (inx) { T m = bx_i * psi[ox + L.SX] + ax_i * b; b = b * ikx_i + m; psi[ox + L.SX] = m; }
-      const bool edge = lo || i == nx - 2 || j == ny - 2 || k == nz - 1;
-      Hz[c] = (edge ? T(0) : Hz[c]) + dbH * (b + a);
-    }
+    exc = exn;
+    ezc = ezn;
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(NT)
-k_em25_e(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-         const int4* __restrict__ work, const int* __restrict__ base, int rel) {
-  const int4 wi = work[blockIdx.x];
-  if (wi.x < 0) return;
-  const DShot& S = shots[wi.x];
+template <typename T, bool PML>
+__device__ __forceinline__ void em25_e_tile(const DModel* __restrict__ models,
+                                           const DShot* __restrict__ shots, const int4 wv,
+                                           const int it) {
+  const int s = __builtin_amdgcn_readfirstlane(wv.x);
+  const DShot& S = shots[s];
   const DModel& M = models[S.model];
   const int nx = M.nx, ny = M.ny, nz = M.nz;
-  const int i = wi.y + threadIdx.x, k = wi.z + threadIdx.y;
-  if (i >= nx || k >= nz) return;
-  const int it = *base + rel;
+  const int k = __builtin_amdgcn_readfirstlane(wv.z);
+  const int lane = threadIdx.x & 63;
+  const int i = __builtin_amdgcn_readfirstlane(wv.y) - 1 + lane;
+  const bool incol = (i >= 0 && i < nx);
+  const int ic = incol ? i : (i < 0 ? 0 : nx - 1);
+  const bool own = incol && lane >= 1 && lane <= EW25;
+  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff;
+  const int j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
   const long nxy = (long)nx * ny;
-  const long p = (long)k * nx + i;
+  const uint32_t p = (uint32_t)k * nx + ic;
   const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz);
   const int lx = M.lx, ly = M.ly, lz = M.lz;
-  const bool inx = in_slab(i, nx, lx), inz = in_slab(k, nz, lz);
+  const bool inx = PML && incol && in_slab(i, nx, lx), inz = PML && in_slab(k, nz, lz);
   const Prof<T> P(&M.prof[0][0]);
-  const T bxh_i = P[PBH][i], axh_i = P[PAH][i], ikxh_i = P[PKH][i];
+  const T bxh_i = P[PBH][ic], axh_i = P[PAH][ic], ikxh_i = P[PKH][ic];
   const T bz_k = P[12 + PB][k], az_k = P[12 + PA][k], ikz_k = P[12 + PK][k];
   const T bzh_k = P[12 + PBH][k], azh_k = P[12 + PAH][k], ikzh_k = P[12 + PKH][k];
   gcptr<T> byh = P[6 + PBH], ayh = P[6 + PAH], ikyh = P[6 + PKH];
-  const bool ux = (k >= 1 && k <= nz - 2 && i <= nx - 2);
-  const bool uy = (k >= 1 && k <= nz - 2 && i >= 1 && i <= nx - 2);
-  const bool uz = (k <= nz - 2 && i >= 1 && i <= nx - 2);
-  const T cax = ux ? cp<T>(M.coef[0])[p] : T(0), cbx = ux ? cp<T>(M.coef[1])[p] : T(0);
-  const T cay = uy ? cp<T>(M.coef[2])[p] : T(0), cby = uy ? cp<T>(M.coef[3])[p] : T(0);
-  const T caz = uz ? cp<T>(M.coef[4])[p] : T(0), cbz = uz ? cp<T>(M.coef[5])[p] : T(0);
-  const auto Hx = cp<T>(S.f[3]);
-  const auto Hy = cp<T>(S.f[4]);
-  const auto Hz = cp<T>(S.f[5]);
-  auto Ex = mp<T>(S.f[0]);
-  auto Ey = mp<T>(S.f[1]);
-  auto Ez = mp<T>(S.f[2]);
+  // loop ranges of :968-1029
+  const bool ux = own && k >= 1 && k <= nz - 2 && i <= nx - 2;
+  const bool uy = own && k >= 1 && k <= nz - 2 && i >= 1 && i <= nx - 2;
+  const bool uz = own && k <= nz - 2 && i >= 1 && i <= nx - 2;
+  const T cax = ux ? A32<const T>(M.coef[0])[p] : T(0), cbx = ux ? A32<const T>(M.coef[1])[p] : T(0);
+  const T cay = uy ? A32<const T>(M.coef[2])[p] : T(0), cby = uy ? A32<const T>(M.coef[3])[p] : T(0);
+  const T caz = uz ? A32<const T>(M.coef[4])[p] : T(0), cbz = uz ? A32<const T>(M.coef[5])[p] : T(0);
+  const int kd = k >= 1 ? k - 1 : 0;
+  const A32<const T> HX = cplane<T>(S.f[3], nxy, k), HY = cplane<T>(S.f[4], nxy, k);
+  const A32<const T> HZ = cplane<T>(S.f[5], nxy, k);
+  const A32<const T> HXD = cplane<T>(S.f[3], nxy, kd), HYD = cplane<T>(S.f[4], nxy, kd);
+  const A32<T> EX = plane<T>(S.f[0], nxy, k), EY = plane<T>(S.f[1], nxy, k), EZ = plane<T>(S.f[2], nxy, k);
   auto psi = mp<T>(S.psi);
   const Slab3e L(nx, ny, nz, lx, ly, lz);
-  const int xs = slab_pos(i, nx, lx), zs = slab_pos(k, nz, lz);
+  const int xs = inx ? slab_pos(i, nx, lx) : 0, zs = inz ? slab_pos(k, nz, lz) : 0;
   const bool srccol = (i == S.isrc && k == S.ksrc);
-  for (int j = 0; j < ny; ++j) {
-    const long c = (long)k * nxy + (long)j * nx + i;
-    const bool iny = in_slab(j, ny, ly);
-    const long ox = L.bx + ((long)k * ny + j) * 2 * lx + xs;
-    const long oy = L.by + ((long)k * 2 * ly + slab_pos(j, ny, ly)) * nx + i;
-    const long oz = L.bz + ((long)zs * ny + j) * nx + i;
-    T ex = Ex[c], ey = Ey[c], ez = Ez[c];
-    // Ex, electromagfdtd.f95:968-985
-    if (ux && j >= 1 && j <= ny - 2) {
-      T a = (Hz[c] - Hz[c - nx]) * idy;  // dHz_dy
-      if (iny) { T m = byh[j] * psi[oy + 2 * L.SY] + ayh[j] * a; a = a * ikyh[j] + m; psi[oy + 2 * L.SY] = m; }
-      T b = (Hy[c - nxy] - Hy[c]) * idz;  // dHy_dz
-      if (inz) { T m = bz_k * psi[oz + 2 * L.SZ] + az_k * b; b = b * ikz_k + m; psi[oz + 2 * L.SZ] = m; }
-      ex = cax * ex + cbx * (a + b);
+  // y window: Hx, Hz of row j-1
+  uint32_t c = (uint32_t)j0 * nx + ic;
+  T hxm = 0, hzm = 0;
+  if (j0 >= 1) { hxm = HX[c - nx]; hzm = HZ[c - nx]; }
+#pragma unroll 1
+  for (int j = j0; j < j1; ++j, c += nx) {
+    const T hxc = HX[c], hyc = HY[c], hzc = HZ[c];
+    const T hyd = HYD[c], hxd = HXD[c];          // plane k-1
+    const T hzw = lane_prev(hzc), hyw = lane_prev(hyc);
+    T ex = EX[c], ey = EY[c], ez = EZ[c];
+    if (own) {
+      const bool iny = PML && in_slab(j, ny, ly);
+      const long ox = L.bx + ((long)k * ny + j) * 2 * lx + xs;
+      const long oy = L.by + ((long)k * 2 * ly + (iny ? slab_pos(j, ny, ly) : 0)) * nx + ic;
+      const long oz = L.bz + ((long)zs * ny + j) * nx + ic;
+      if (ux && j >= 1 && j <= ny - 2) {  // Ex, :968-985
+        T a = (hzc - hzm) * idy;  // dHz_dy
+        if (iny) { T m = byh[j] * psi[oy + 2 * L.SY] + ayh[j] * a; a = a * ikyh[j] + m; psi[oy + 2 * L.SY] = m; }
+        T b = (hyd - hyc) * idz;  // dHy_dz
+        if (inz) { T m = bz_k * psi[oz + 2 * L.SZ] + az_k * b; b = b * ikz_k + m; psi[oz + 2 * L.SZ] = m; }
+        ex = cax * ex + cbx * (a + b);
+      }
+      if (uy && j <= ny - 2) {  // Ey, :988-1006
+        T a = (hzw - hzc) * idx;  // dHz_dx
+        if (inx) { T m = bxh_i * psi[ox + 2 * L.SX] + axh_i * a; a = a * ikxh_i + m; psi[ox + 2 * L.SX] = m; }
+        T b = (hxc - hxd) * idz;  // dHx_dz
+        if (inz) { T m = bzh_k * psi[oz + 3 * L.SZ] + azh_k * b; b = b * ikzh_k + m; psi[oz + 3 * L.SZ] = m; }
+        ey = cay * ey + cby * (a + b);
+      }
+      if (uz && j >= 1 && j <= ny - 2) {  // Ez, :1011-1029
+        T a = (hxm - hxc) * idy;  // dHx_dy
+        if (iny) { T m = byh[j] * psi[oy + 3 * L.SY] + ayh[j] * a; a = a * ikyh[j] + m; psi[oy + 3 * L.SY] = m; }
+        T b = (hyc - hyw) * idx;  // dHy_dx
+        if (inx) { T m = bxh_i * psi[ox + 3 * L.SX] + axh_i * b; b = b * ikxh_i + m; psi[ox + 3 * L.SX] = m; }
+        ez = caz * ez + cbz * (a + b);
+      }
+      if (srccol && j == S.jsrc) {  // source, :1033-1035
+        if (S.src[0]) ex += cp<T>(S.src[0])[it];
+        if (S.src[1]) ey += cp<T>(S.src[1])[it];
+        if (S.src[2]) ez += cp<T>(S.src[2])[it];
+      }
+      // Dirichlet faces, :1038-1057 (and cells outside each staggered field's extent)
+      const bool lo = (i == 0 || j == 0 || k == 0);
+      if (lo || i >= nx - 2 || j == ny - 1 || k == nz - 1) ex = 0;
+      if (lo || i == nx - 1 || j >= ny - 2 || k == nz - 1) ey = 0;
+      if (lo || i == nx - 1 || j == ny - 1 || k >= nz - 2) ez = 0;
+      check_blowup((double)ex * ex + (double)ey * ey + (double)ez * ez, S.blowup, it);
+      EX[c] = ex;
+      EY[c] = ey;
+      EZ[c] = ez;
     }
-    // Ey, electromagfdtd.f95:988-1006
-    if (uy && j <= ny - 2) {
-      T a = (Hz[c - 1] - Hz[c]) * idx;  // dHz_dx
-      if (inx) { T m = bxh_i * psi[ox + 2 * L.SX] + axh_i * a; a = a * ikxh_i + m; psi[ox + 2 * L.SX] = m; }
-      T b = (Hx[c] - Hx[c - nxy]) * idz;  // dHx_dz
-      if (inz) { T m = bzh_k * psi[oz + 3 * L.SZ] + azh_k * b; b = b * ikzh_k + m; psi[oz + 3 * L.SZ] = m; }
-      ey = cay * ey + cby * (a + b);
-    }
-    // Ez, electromagfdtd.f95:1011-1029
-    if (uz && j >= 1 && j <= ny - 2) {
-      T a = (Hx[c - nx] - Hx[c]) * idy;  // dHx_dy
-      if (iny) { T m = byh[j] * psi[oy + 3 * L.SY] + ayh[j] * a; a = a * ikyh[j] + m; psi[oy + 3 * L.SY] = m; }
-      T b = (Hy[c] - Hy[c - 1]) * idx;  // dHy_dx
-      if (inx) { T m = bxh_i * psi[ox + 3 * L.SX] + axh_i * b; b = b * ikxh_i + m; psi[ox + 3 * L.SX] = m; }
-      ez = caz * ez + cbz * (a + b);
-    }
-    // source, electromagfdtd.f95:1033-1035
-    if (srccol && j == S.jsrc) {
-      if (S.src[0]) ex += cp<T>(S.src[0])[it];
-      if (S.src[1]) ey += cp<T>(S.src[1])[it];
-      if (S.src[2]) ez += cp<T>(S.src[2])[it];
-    }
-    // Dirichlet faces, electromagfdtd.f95:1038-1057 (and cells outside each
-    // staggered field's Fortran extent)
-    const bool lo = (i == 0 || j == 0 || k == 0);
-    if (lo || i >= nx - 2 || j == ny - 1 || k == nz - 1) ex = 0;
-    if (lo || i == nx - 1 || j >= ny - 2 || k == nz - 1) ey = 0;
-    if (lo || i == nx - 1 || j == ny - 1 || k >= nz - 2) ez = 0;
-    check_blowup((double)ex * ex + (double)ey * ey + (double)ez * ez, S.blowup, it);
-    Ex[c] = ex;
-    Ey[c] = ey;
-    Ez[c] = ez;
+    hxm = hxc;
+    hzm = hzc;
   }
+}
+
+// A wavefront whose columns, plane and rows (with the neighbours it reads)
+// lie outside every C-PML slab runs the body without memory variables.
+__device__ __forceinline__ bool em25_interior(const DModel& M, int xo, int k, int j0, int j1) {
+  return xo - 1 >= M.lx && xo + 62 < M.nx - M.lx && k >= M.lz && k < M.nz - M.lz &&
+         j0 >= M.ly && j1 <= M.ny - M.ly;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+k_em25_h(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+         const int4* __restrict__ work) {
+  const int4 wv = work[blockIdx.x];
+  const int s = __builtin_amdgcn_readfirstlane(wv.x);
+  if (s < 0) return;
+  const DModel& M = models[shots[s].model];
+  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff, j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
+  if (em25_interior(M, __builtin_amdgcn_readfirstlane(wv.y), __builtin_amdgcn_readfirstlane(wv.z), j0, j1))
+    em25_h_tile<T, false>(models, shots, wv);
+  else
+    em25_h_tile<T, true>(models, shots, wv);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+k_em25_e(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+         const int4* __restrict__ work, const int* __restrict__ base, int rel) {
+  const int4 wv = work[blockIdx.x];
+  const int s = __builtin_amdgcn_readfirstlane(wv.x);
+  if (s < 0) return;
+  const DModel& M = models[shots[s].model];
+  const int it = *base + rel;
+  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff, j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
+  if (em25_interior(M, __builtin_amdgcn_readfirstlane(wv.y), __builtin_amdgcn_readfirstlane(wv.z), j0, j1))
+    em25_e_tile<T, false>(models, shots, wv, it);
+  else
+    em25_e_tile<T, true>(models, shots, wv, it);
 }
 
 // ===========================================================================
@@ -2233,28 +2299,26 @@ __global__ void k_gather(const DModel* __restrict__ models, const DShot* __restr
   for (int f = 0; f < ncomp; ++f) tr[(long)f * S.nrec] = cp<T>(inb ? S.g[f] : S.f[f])[c];
 }
 
-// The H Dirichlet zeroing that closes each legacy EM step (electromagfdtd.f95
-// :312-315, :1059-1078) is folded into the next H update; at the end of a
-// run it is applied here so the returned state equals the legacy state.
+// The H zeroing that closes each legacy 2.5-D EM step (electromagfdtd.f95
+// :1059-1078) is folded into the next H update; at the end of a run it is
+// applied here (one wavefront per work item of the EM25 kernels) so the
+// returned state equals the legacy state.
 template <typename T>
-__global__ void __launch_bounds__(NT)
-k_em_h_edges(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-             const int4* __restrict__ work, int d3) {
-  const int4 wi = work[blockIdx.x];
-  if (wi.x < 0) return;
-  const DShot& S = shots[wi.x];
+__global__ void __launch_bounds__(64)
+k_em25_edges(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+             const int4* __restrict__ work) {
+  const int4 wv = work[blockIdx.x];
+  if (wv.x < 0) return;
+  const DShot& S = shots[wv.x];
   const DModel& M = models[S.model];
   const int nx = M.nx, ny = M.ny, nz = M.nz;
-  const int i = wi.y + threadIdx.x, k = wi.z + threadIdx.y;
-  if (i >= nx || k >= nz) return;
-  if (!d3) {
-    if (i == 0 || i == nx - 2 || k == 0 || k == nz - 2) mp<T>(S.f[2])[(long)k * nx + i] = T(0);
-    return;
-  }
+  const int lane = threadIdx.x & 63;
+  const int i = wv.y - 1 + lane, k = wv.z;
+  if (lane < 1 || lane > EW25 || i >= nx || k >= nz) return;
   auto Hx = mp<T>(S.f[3]);
   auto Hy = mp<T>(S.f[4]);
   auto Hz = mp<T>(S.f[5]);
-  for (int j = 0; j < ny; ++j) {
+  for (int j = wv.w & 0xffff; j < (wv.w >> 16); ++j) {
     const long c = ((long)k * ny + j) * nx + i;
     const bool lo = (i == 0 || j == 0 || k == 0);
     if (lo || i == nx - 1 || j == ny - 2 || k == nz - 2) Hx[c] = T(0);
@@ -2432,7 +2496,7 @@ std::vector<int4> el2_work(const Plan* p) {
 // The items of each shot are cut into 8 contiguous runs, one per XCD (item w
 // runs on XCD w % 8), so the blocks an XCD runs at a time cover whole rows
 // of a few planes (long contiguous runs of addresses in each array).
-std::vector<int4> el25_work(const Plan* p) {
+std::vector<int4> el25_work(const Plan* p, int planes, int xstep) {
   std::vector<int4> all;
   for (int s = 0; s < p->nshots; ++s) {
     const sdt_model& M = p->hmodels[p->hshots[s].model];
@@ -2444,8 +2508,8 @@ std::vector<int4> el25_work(const Plan* p) {
     if (yb > cuts.back()) cuts.push_back(yb);
     if (M.ny > cuts.back()) cuts.push_back(M.ny);
     for (size_t c = 0; c + 1 < cuts.size(); ++c)
-      for (int k0 = 0; k0 < M.nz; k0 += KB25)
-        for (int x0 = 0; x0 < M.nx; x0 += EW25 * XW25)
+      for (int k0 = 0; k0 < M.nz; k0 += planes)
+        for (int x0 = 0; x0 < M.nx; x0 += xstep)
           all.push_back(make_int4(s, x0, k0, cuts[c] | (cuts[c + 1] << 16)));
   }
   const size_t run = (all.size() + 7) / 8;
@@ -2491,10 +2555,11 @@ void launch_one(Plan* p, hipStream_t st, int rel, int which) {
                          p->dbase, rel);
       break;
     case SDT_EM25 * 2:
-      hipLaunchKernelGGL(k_em25_h<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork);
+      hipLaunchKernelGGL(k_em25_h<T>, grd, dim3(64), 0, st, p->dmodels, p->dshots, p->dwork);
       break;
     case SDT_EM25 * 2 + 1:
-      hipLaunchKernelGGL(k_em25_e<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dwork, p->dbase, rel);
+      hipLaunchKernelGGL(k_em25_e<T>, grd, dim3(64), 0, st, p->dmodels, p->dshots, p->dwork, p->dbase,
+                         rel);
       break;
   }
 }
@@ -2510,8 +2575,8 @@ void launch_finalize(Plan* p, hipStream_t st) {
   if (p->kind == SDT_EM2)
     hipLaunchKernelGGL(k_em2_edges<T>, dim3(16, p->nshots), dim3(256), 0, st, p->dmodels, p->dshots);
   if (p->kind == SDT_EM25)
-    hipLaunchKernelGGL(k_em_h_edges<T>, dim3(p->nwork), dim3(TX, TZ), 0, st, p->dmodels,
-                       p->dshots, p->dwork, 1);
+    hipLaunchKernelGGL(k_em25_edges<T>, dim3(p->nwork), dim3(64), 0, st, p->dmodels, p->dshots,
+                       p->dwork);
 }
 
 template <typename T>
@@ -2833,13 +2898,14 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
   }
   // work list: (shot, x0, z0); all shots of one tile share blockIdx % 8 (XCD)
   std::vector<int4> work;
-  if (kind == SDT_ELASTIC25) work = el25_work(p);
+  if (kind == SDT_ELASTIC25) work = el25_work(p, KB25, EW25 * XW25);
+  if (kind == SDT_EM25) work = el25_work(p, 1, EW25);  // one plane, one wavefront per item
   if (kind == SDT_ELASTIC2 || kind == SDT_EM2) {
     work = el2_work(p);
     const int nblk = (int)work.size();
     p->el2_grid = std::max(8, std::min(nblk, SDT_EL2_GRID) / 8 * 8);
   }
-  for (int m = 0; m < nmodels && !pingpong(kind) && kind != SDT_ELASTIC25; ++m) {
+  for (int m = 0; m < nmodels && !pingpong(kind) && !kind_3d(kind); ++m) {
     const sdt_model& M = models[m];
     const int wx = TX, wz = TZ;
     const int tx = (M.nx + wx - 1) / wx, tz = (M.nz + wz - 1) / wz, nt_ = tx * tz;

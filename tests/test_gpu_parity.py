@@ -74,7 +74,7 @@ def test_fp64_matches_legacy_golden(name):
         assert rel_l2(tr[:, c, :], g[f"trace_{ch}"]) < FP64_TOL, ch
 
 
-@pytest.mark.parametrize("kind", ["elastic2", "em2", "elastic25"])
+@pytest.mark.parametrize("kind", ["elastic2", "em2", "elastic25", "em25"])
 def test_batched_uneven_models(oracle_lib, kind):
     """Several models of different size, several shots each, one plan: every
     shot equals its own single-shot oracle run."""
