@@ -404,14 +404,19 @@ def main(argv=None):
     # per-kernel durations (HIP events around each launch, same stream)
     k_ms = sim.profile(PROFILE_STEPS)
     sim.check()
-    dom_bytes = algorithmic_bytes(problems, shots, 4)
+    # one launch of the step kernel advances `spl` steps (two by default: the
+    # state of the first stays in registers), so its algorithmic bytes are
+    # spl times those of one step
+    spl = sim.steps_per_launch
+    dom_bytes = spl * algorithmic_bytes(problems, shots, 4)
     achieved = dom_bytes / (k_ms[0] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(dom_bytes),
-                "kernel": "k_el2_step<float>", "kernel_ms": round(k_ms[0], 4),
+                "kernel": "k_el2_step2<float>" if spl == 2 else "k_el2_step<float>",
+                "kernel_ms": round(k_ms[0], 4), "steps_per_launch": spl,
                 "alg_bytes_per_launch": int(dom_bytes),
-                "alg_bytes_per_cell_update": round(dom_bytes / cells_per_step, 2),
+                "alg_bytes_per_cell_update": round(dom_bytes / spl / cells_per_step, 2),
                 "gather_ms": round(k_ms[2], 4)}
     if roofline["traffic"]:
         # measured bytes per launch over the launch time, against a plain copy

@@ -170,10 +170,23 @@ int sdt_plan_run(void* plan, int it0, int nsteps, int graph_block, void* stream)
  * in the reference;
  * for an autotuner, seidart_amd.solver.Simulation.autotune): a block of
  * waves_per_block (1, 2 or 4) wavefronts side by side, each owning 61
- * columns, walks `rows` rows (2..256).  Results do not depend on the shape.
+ * columns (58 in the two-step ELASTIC2 kernel), walks `rows` rows (2..256).  Results do not depend on the shape.
  * Synchronises the device when the shape changes. */
 int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows);
 int sdt_plan_get_tiles(void* plan, int* waves_per_block, int* rows);
+
+/* ELASTIC2 steps per launch of the step kernel (no counterpart in the
+ * reference): 2 (the default for ELASTIC2 plans) advances two time steps in
+ * one launch, keeping the state of the first in registers, so the state
+ * crosses HBM once per two steps; 1 is the one-step kernel.  Results are
+ * identical bit for bit.  A run of an odd number of steps ends with one
+ * single step; with hipGraph blocks, two-step launches need an even
+ * graph_block (odd blocks fall back to one step per launch).  The adjoint's
+ * trail recomputation always steps one at a time.  In two-step mode
+ * sdt_plan_profile's out_ms[0] is the mean per two-step launch and
+ * out_ms[2] is 0 (the kernel writes the receiver samples itself). */
+int sdt_plan_set_steps_per_launch(void* plan, int steps);
+int sdt_plan_get_steps_per_launch(void* plan, int* steps);
 
 /* Synchronise the plan's last stream and report instability: *first_step
  * gets the first step at which any shot's field exceeded 1e25 in magnitude

@@ -86,6 +86,10 @@ struct DShot {
   void* trace;
   int* blowup;
   const void* msrc[6];  // stress increments xx yy zz xy xz yz (elastic)
+  // receivers by row (ELASTIC2): those on row k are rxid[rrow[k] .. rrow[k+1]),
+  // each a pair (column, receiver index), columns ascending
+  const int* rrow;
+  const int* rxid;
 };
 
 // Adjoint-side buffers of one shot (elastic 2-D), see sdt_adj_shot.
@@ -200,20 +204,38 @@ struct A32 {
   }
 };
 
+// Neighbouring lanes by DPP wavefront shifts (a VALU move, no LDS round
+// trip): lane_next gives lane l the value of lane l+1, lane_prev that of
+// lane l-1; the lane with no neighbour (63, resp. 0) gets 0.  Kernels own
+// only lanes whose neighbours exist.
+__device__ __forceinline__ int dpp_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
 template <typename T>
 __device__ __forceinline__ T lane_next(T v) {  // value of lane l+1
-  return __shfl_down(v, 1, 64);
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, dpp_next(__builtin_bit_cast(int, v)));
+  } else {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)dpp_next((int)(unsigned)b), hi = (unsigned)dpp_next((int)(unsigned)(b >> 32));
+    return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+  }
 }
 template <typename T>
 __device__ __forceinline__ T lane_prev(T v) {  // value of lane l-1
-  return __shfl_up(v, 1, 64);
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, dpp_prev(__builtin_bit_cast(int, v)));
+  } else {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)dpp_prev((int)(unsigned)b), hi = (unsigned)dpp_prev((int)(unsigned)(b >> 32));
+    return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+  }
 }
 
 template <typename T, bool TRAIL, bool SPLIT, bool PML>
 __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
                                          const DShot* __restrict__ shots, const int4 wi, const int it,
                                          const int lane, const DAdj* __restrict__ adj, const int slot,
-                                         const int tb, const int eh);
+                                         const int tb, const int eh, const int par);
 
 // Work: a block of WB wavefronts takes one item (shot, xo, zo): wave w owns
 // columns xo + w*EW .. +EW-1 of rows zo .. zo+eh-1, so the block's row
@@ -224,14 +246,20 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
 // slot 0) and write the new state into the next slot (forward recomputation
 // for the adjoint).
 // SPLIT: the split C-PML variant (see SDT_SPLIT_CPML).
+// The state of step it is read from buffer it & 1 (A = field/psi, B =
+// field_b/psi_b) and written to the other; with prun (the first step of a
+// run of two-step launches, k_el2_step2) it is read from buffer
+// ((it - *prun) / 2) & 1 instead, the buffer those launches leave it in.
 template <typename T, bool TRAIL, bool SPLIT, int WB>
 __global__ void __launch_bounds__(64 * WB, sizeof(T) == 4 ? SDT_EL2_WAVES_PER_SIMD : 3)
 k_el2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
            const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel,
-           const DAdj* __restrict__ adj, const int* __restrict__ tsel, int eh) {
+           const DAdj* __restrict__ adj, const int* __restrict__ tsel, int eh,
+           const int* __restrict__ prun) {
   // wave-uniform values kept in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int it = __builtin_amdgcn_readfirstlane(*base + rel);
+  const int par = prun ? __builtin_amdgcn_readfirstlane(((it - *prun) >> 1) & 1) : (it & 1);
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
     const int4 wv = work[w];
     const int4 wi = make_int4(__builtin_amdgcn_readfirstlane(wv.x),
@@ -246,9 +274,9 @@ k_el2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
                           wi.z + eh + 1 < M.nz - M.lz;
     const int tb = TRAIL ? __builtin_amdgcn_readfirstlane(*tsel) : 0;
     if (interior)
-      el2_tile<T, TRAIL, SPLIT, false>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh);
+      el2_tile<T, TRAIL, SPLIT, false>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh, par);
     else
-      el2_tile<T, TRAIL, SPLIT, true>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh);
+      el2_tile<T, TRAIL, SPLIT, true>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh, par);
   }
 }
 
@@ -256,7 +284,7 @@ template <typename T, bool TRAIL, bool SPLIT, bool PML>
 __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
                                          const DShot* __restrict__ shots, const int4 wi, const int it,
                                          const int lane, const DAdj* __restrict__ adj, const int slot,
-                                         const int tb, const int eh) {
+                                         const int tb, const int eh, const int par) {
   // Every fused multiply-add is written out; nothing else is contracted, so
   // the interior and C-PML bodies (and every tile shape) round identically.
 #pragma clang fp contract(off)
@@ -269,7 +297,6 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
   const int ic = incol ? i : (i < 0 ? 0 : nx - 1);  // clamped column: every load stays in bounds
   const bool own = incol && lane >= 1 && lane <= EW;
   const int zend = min(zo + eh, nz);
-  const int par = it & 1;
   const long nn = (long)nx * nz;
   const T* tin = nullptr;
   T* tout = nullptr;
@@ -544,12 +571,432 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
   }
 }
 
+// ===========================================================================
+// ELASTIC2, two time steps per launch.
+//
+// The walk of k_el2_step with the second step pipelined behind the first.
+// At row r a wavefront computes
+//   A  the stresses of step it at row r      (velocities of it-1, rows r-1..r+1)
+//   B  the velocities of step it at row r-1  (stresses from A, rows r-2..r)
+//   C  the stresses of step it+1 at row r-2  (velocities from B, rows r-3..r-1)
+//   D  the velocities of step it+1 at row r-3 (stresses from C, rows r-4..r-2)
+// so the state of step it lives only in registers and each field, memory
+// variable and coefficient crosses HBM once per two steps.
+// One step makes a cell depend on columns i-1 .. i+2 and rows k-2 .. k+1 of
+// the previous state (the shear stress updates the C-PML memory the normal
+// stresses just updated, quirk A1, so it sees the normal loop's neighbours);
+// two steps on columns i-2 .. i+4 and rows k-4 .. k+2.  Lane l therefore
+// holds column xo - 2 + l and lanes 2 .. 2+EW2-1 are exact after two steps;
+// rows zo-4 .. zo+eh+1 are read to produce rows zo .. zo+eh-1.  Every stage
+// evaluates the expressions of k_el2_step in the same order, so both kernels
+// give identical bits.  The state of step it is never stored, so this kernel
+// also writes the receiver samples of both steps.
+// The state of step it is read from buffer ((it - *it0) / 2) & 1 and the
+// state of it+2 written to the other (it0: first step of the run).
+// ===========================================================================
+constexpr int EW2 = 58;  // owned columns per wavefront
+
+template <typename T, bool SPLIT, bool PML>
+__device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
+                                           const DShot* __restrict__ shots, const int4 wi, const int it,
+                                           const int q, const int lane, const int eh) {
+#pragma clang fp contract(off)
+  // Interior waves (no C-PML) read only columns 1..nx-2 and rows 1..nz-2,
+  // so every edge condition of the scheme holds on every lane there.
+  constexpr bool INT = !PML;
+  const DShot& S = shots[wi.x];
+  const DModel& M = models[S.model];
+  const int nx = M.nx, nz = M.nz;
+  const int xo = wi.y, zo = wi.z;
+  const int i = xo - 2 + lane;
+  const bool incol = INT || (i >= 0 && i < nx);
+  const int ic = INT ? i : (incol ? i : (i < 0 ? 0 : nx - 1));  // clamped column: every load stays in bounds
+  const bool own = incol && lane >= 2 && lane < 2 + EW2;
+  const int ze = min(zo + eh, nz);
+  const A32<const T> vxi(q ? S.g[0] : S.f[0]);
+  const A32<const T> vzi(q ? S.g[1] : S.f[1]);
+  const A32<const T> sxxi(q ? S.g[2] : S.f[2]);
+  const A32<const T> szzi(q ? S.g[3] : S.f[3]);
+  const A32<const T> sxzi(q ? S.g[4] : S.f[4]);
+  const A32<T> vxo(q ? S.f[0] : S.g[0]);
+  const A32<T> vzo(q ? S.f[1] : S.g[1]);
+  const A32<T> sxxo(q ? S.f[2] : S.g[2]);
+  const A32<T> szzo(q ? S.f[3] : S.g[3]);
+  const A32<T> sxzo(q ? S.f[4] : S.g[4]);
+  const A32<const T> pin(q ? S.psi2 : S.psi);
+  const A32<T> pout(q ? S.psi : S.psi2);
+  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dt = T(M.dt);
+  const int lx = M.lx, lz = M.lz;
+  const uint32_t SX = (uint32_t)nz * 2 * lx, SZ = (uint32_t)2 * lz * nx;
+  const uint32_t ZB = (SPLIT ? 6 : 4) * SX;  // first z-slab array
+  const bool inx = PML && incol && in_slab(i, nx, lx);
+  const int xs = inx ? slab_pos(i, nx, lx) : 0;
+  const Prof<T> P(&M.prof[0][0]);
+  const T ikx = P[PK][ic], ax = P[PA][ic], bx = P[PB][ic];
+  const T ikxh = P[PKH][ic], axh = P[PAH][ic], bxh = P[PBH][ic];
+  const auto ikz = P[12 + PK], az = P[12 + PA], bz = P[12 + PB];
+  const auto ikzh = P[12 + PKH], azh = P[12 + PAH], bzh = P[12 + PBH];
+  const auto brow = cp<T>(M.aux);
+  const A32<const T> N11(M.coef[0]), N13(M.coef[1]), N15(M.coef[2]);
+  const A32<const T> N33(M.coef[3]), N35(M.coef[4]);
+  const A32<const T> S15(M.coef[5]), S35(M.coef[6]), S55(M.coef[7]);
+  const A32<const T> DTX(M.coef[8]), DTZ(M.coef[9]);
+  const A32<const T> OMX(M.coef[10]), OMZ(M.coef[11]);
+  const bool mono = M.coef[2] != nullptr;   // c15 / c35 terms present
+  const bool atten = M.coef[10] != nullptr; // gamma present
+  const bool is_src_col = (i == S.isrc);
+  const int ksrc = S.ksrc;
+  const auto srcx = cp<T>(S.src[0]), srcz = cp<T>(S.src[2]);
+  const T incx0 = srcx ? srcx[it] : T(0), incx1 = srcx ? srcx[it + 1] : T(0);
+  const T incz0 = srcz ? srcz[it] : T(0), incz1 = srcz ? srcz[it + 1] : T(0);
+  const bool msrc = S.msrc[0] || S.msrc[2] || S.msrc[4];
+  const T ms0xx = S.msrc[0] ? cp<T>(S.msrc[0])[it] : T(0), ms1xx = S.msrc[0] ? cp<T>(S.msrc[0])[it + 1] : T(0);
+  const T ms0zz = S.msrc[2] ? cp<T>(S.msrc[2])[it] : T(0), ms1zz = S.msrc[2] ? cp<T>(S.msrc[2])[it + 1] : T(0);
+  const T ms0xz = S.msrc[4] ? cp<T>(S.msrc[4])[it] : T(0), ms1xz = S.msrc[4] ? cp<T>(S.msrc[4])[it + 1] : T(0);
+
+  auto crow = [&](int r) { return r < 0 ? 0 : (r >= nz ? nz - 1 : r); };
+  auto cell = [&](int r) { return (uint32_t)crow(r) * nx + ic; };
+  auto valid = [&](int r) { return INT || (incol && r >= 0 && r < nz); };
+
+  struct CN { T n11, n13, n33, s55, n15, n35, s15, s35; };  // stress coefficients of a row
+  struct CV { T dtx, dtz, omx, omz; };                      // velocity coefficients of a row
+  struct MV { T vxx, vzx, vzz, vxz, svxx, svzx, svzz, svxz; };  // memory of velocity derivatives
+  struct MS { T sxx, sxzx, szz, sxzz; };                    // memory of stress derivatives
+
+  auto load_cn = [&](int r) {
+    CN k;
+    const uint32_t c = cell(r);
+    k.n11 = N11[c];
+    k.n13 = N13[c];
+    k.n33 = N33[c];
+    k.s55 = S55[c];
+    k.n15 = mono ? N15[c] : T(0);
+    k.n35 = mono ? N35[c] : T(0);
+    k.s15 = mono ? S15[c] : T(0);
+    k.s35 = mono ? S35[c] : T(0);
+    return k;
+  };
+  auto load_cv = [&](int r) {
+    CV k;
+    const uint32_t c = cell(r);
+    k.dtx = DTX[c];
+    k.dtz = DTZ[c];
+    k.omx = atten ? OMX[c] : T(1);
+    k.omz = atten ? OMZ[c] : T(1);
+    return k;
+  };
+  // slab offsets of row r: x slab (valid where inx), z slab (ZB-based)
+  auto offx = [&](int r) { return inx ? (uint32_t)crow(r) * 2 * lx + xs : 0u; };
+  auto inzs = [&](int r) { return PML && in_slab(crow(r), nz, lz); };
+  auto offz = [&](int r) { return ZB + (inzs(r) ? (uint32_t)slab_pos(crow(r), nz, lz) * nx + ic : 0u); };
+  auto load_mv = [&](int r) {
+    MV m{};
+    if constexpr (PML) {
+      const uint32_t ox = offx(r), oz = offz(r);
+      const bool inzc = incol && inzs(r);
+      m.vxx = inx ? pin[ox] : T(0);
+      m.vzx = inx ? pin[SX + ox] : T(0);
+      m.vzz = inzc ? pin[oz] : T(0);
+      m.vxz = inzc ? pin[SZ + oz] : T(0);
+      if (SPLIT) {
+        m.svxx = inx ? pin[4 * SX + ox] : T(0);
+        m.svzx = inx ? pin[5 * SX + ox] : T(0);
+        m.svzz = inzc ? pin[4 * SZ + oz] : T(0);
+        m.svxz = inzc ? pin[5 * SZ + oz] : T(0);
+      }
+    }
+    return m;
+  };
+  auto load_ms = [&](int r) {
+    MS m{};
+    if constexpr (PML) {
+      const uint32_t ox = offx(r), oz = offz(r);
+      const bool inzc = incol && inzs(r);
+      m.sxx = inx ? pin[2 * SX + ox] : T(0);
+      m.sxzx = inx ? pin[3 * SX + ox] : T(0);
+      m.szz = inzc ? pin[2 * SZ + oz] : T(0);
+      m.sxzz = inzc ? pin[3 * SZ + oz] : T(0);
+    }
+    return m;
+  };
+  auto store_mv = [&](int r, const MV& m) {  // own cells only
+    if constexpr (PML) {
+      const uint32_t ox = offx(r), oz = offz(r);
+      if (inx) {
+        pout[ox] = m.vxx;
+        pout[SX + ox] = m.vzx;
+        if (SPLIT) {
+          pout[4 * SX + ox] = m.svxx;
+          pout[5 * SX + ox] = m.svzx;
+        }
+      }
+      if (inzs(r)) {
+        pout[oz] = m.vzz;
+        pout[SZ + oz] = m.vxz;
+        if (SPLIT) {
+          pout[4 * SZ + oz] = m.svzz;
+          pout[5 * SZ + oz] = m.svxz;
+        }
+      }
+    }
+  };
+  auto store_ms = [&](int r, const MS& m) {  // own cells only
+    if constexpr (PML) {
+      const uint32_t ox = offx(r), oz = offz(r);
+      if (inx) { pout[2 * SX + ox] = m.sxx; pout[3 * SX + ox] = m.sxzx; }
+      if (inzs(r)) { pout[2 * SZ + oz] = m.szz; pout[3 * SZ + oz] = m.sxzz; }
+    }
+  };
+
+  // Stresses of one step at row a (seismicfdtd.f95:247-312): velocities of
+  // rows a-1, a, a+1 in, the row's stresses and memory updated in place.
+  auto stress = [&](int a, T vxm, T vzm, T vxc, T vzc, T vxn, T vzn, T& sxx, T& szz, T& sxz,
+                    const CN& k, MV& m, T isxx, T iszz, T isxz) {
+    const T vx_e = lane_next(vxc), vz_e = lane_next(vzc);
+    const T vx_w = lane_prev(vxc), vz_w = lane_prev(vzc);
+    const bool rv = valid(a);
+    const int rs = crow(a);
+    const bool inx_r = inx && rv, inz_r = incol && rv && inzs(a);
+    T bzr = 0, azr = 0, ikzr = 0, bzhr = 0, azhr = 0, ikzhr = 0, browr = 0;
+    if constexpr (PML) {
+      bzr = bz[rs]; azr = az[rs]; ikzr = ikz[rs];
+      bzhr = bzh[rs]; azhr = azh[rs]; ikzhr = ikzh[rs];
+      browr = SPLIT ? bxh : brow[rs];  // b_x_half(j) (A2), or b_x_half(i)
+    }
+    if (INT || (rv && i <= nx - 2 && a >= 1)) {
+      T dvx_dx = (vx_e - vxc) * idx;
+      T dvz_dz = (vzc - vzm) * idz;
+      T dvz_dx = (vz_e - vzc) * idx;
+      T dvx_dz = (vxc - vxm) * idz;
+      if (PML && inx_r) {
+        m.vxx = fma(browr, m.vxx, axh * dvx_dx);
+        m.vzx = fma(bx, m.vzx, ax * dvz_dx);
+        dvx_dx = fma(dvx_dx, ikxh, m.vxx);
+        dvz_dx = fma(dvz_dx, ikx, m.vzx);
+      }
+      if (PML && inz_r) {
+        m.vzz = fma(bzr, m.vzz, azr * dvz_dz);
+        m.vxz = fma(bzhr, m.vxz, azhr * dvx_dz);
+        dvz_dz = fma(dvz_dz, ikzr, m.vzz);
+        dvx_dz = fma(dvx_dz, ikzhr, m.vxz);
+      }
+      const T sh = dvz_dx + dvx_dz;
+      sxx = fma(fma(k.n15, sh, fma(k.n13, dvz_dz, k.n11 * dvx_dx)), dt, sxx);
+      szz = fma(fma(k.n35, sh, fma(k.n33, dvz_dz, k.n13 * dvx_dx)), dt, szz);
+    }
+    if (INT || (rv && i >= 1 && a <= nz - 2)) {  // same memory arrays (A1) unless split
+      T& q_vxx = SPLIT ? m.svxx : m.vxx;
+      T& q_vzx = SPLIT ? m.svzx : m.vzx;
+      T& q_vzz = SPLIT ? m.svzz : m.vzz;
+      T& q_vxz = SPLIT ? m.svxz : m.vxz;
+      T dvx_dx = (vxc - vx_w) * idx;
+      T dvz_dz = (vzn - vzc) * idz;
+      T dvz_dx = (vzc - vz_w) * idx;
+      T dvx_dz = (vxn - vxc) * idz;
+      if (PML && inx_r) {
+        q_vxx = fma(bxh, q_vxx, axh * dvx_dx);
+        q_vzx = fma(bx, q_vzx, ax * dvz_dx);
+        dvx_dx = fma(dvx_dx, ikxh, q_vxx);
+        dvz_dx = fma(dvz_dx, ikx, q_vzx);
+      }
+      if (PML && inz_r) {
+        q_vzz = fma(bzr, q_vzz, azr * dvz_dz);
+        q_vxz = fma(bzhr, q_vxz, azhr * dvx_dz);
+        dvz_dz = fma(dvz_dz, ikzr, q_vzz);
+        dvx_dz = fma(dvx_dz, ikzhr, q_vxz);
+      }
+      sxz = fma(fma(k.s55, dvz_dx + dvx_dz, fma(k.s35, dvz_dz, k.s15 * dvx_dx)), dt, sxz);
+    }
+    if (msrc && a == ksrc && rv && is_src_col) {
+      sxx += isxx;
+      szz += iszz;
+      sxz += isxz;
+    }
+  };
+  // Velocities of one step at row b (seismicfdtd.f95:317-375): the new
+  // stresses of rows b-1 (sxz), b, b+1 (szz) in.
+  auto vel = [&](int b, T sxx_b, T szz_b, T sxz_b, T sxz_bm, T szz_bn, T vx_old, T vz_old,
+                 const CV& k, MS& m, T ix, T iz, T& vxn, T& vzn) {
+    const T sxx_w = lane_prev(sxx_b);  // sigma_xx(i-1, b)
+    const T sxz_e = lane_next(sxz_b);  // sigma_xz(i+1, b)
+    const bool rv = valid(b);
+    const int rs = crow(b);
+    const bool inzv = PML && inzs(b);
+    T bzv = 0, azv = 0, ikzv = 0, bzhv = 0, azhv = 0, ikzhv = 0;
+    if constexpr (PML) {
+      bzv = bz[rs]; azv = az[rs]; ikzv = ikz[rs];
+      bzhv = bzh[rs]; azhv = azh[rs]; ikzhv = ikzh[rs];
+    }
+    vxn = vx_old;
+    vzn = vz_old;
+    if (INT || (rv && i >= 1 && b >= 1)) {
+      T a = (sxx_b - sxx_w) * idx;
+      T c = (sxz_b - sxz_bm) * idz;
+      if (PML && inx) {
+        m.sxx = fma(bx, m.sxx, ax * a);
+        a = fma(a, ikx, m.sxx);
+      }
+      if (PML && inzv) {
+        m.sxzz = fma(bzv, m.sxzz, azv * c);
+        c = fma(c, ikzv, m.sxzz);
+      }
+      vxn = fma(vxn, k.omx, (a + c) * k.dtx);
+    }
+    if (INT || (rv && i <= nx - 2 && b <= nz - 2)) {
+      T a = (sxz_e - sxz_b) * idx;
+      T c = (szz_bn - szz_b) * idz;
+      if (PML && inx) {
+        m.sxzx = fma(bxh, m.sxzx, axh * a);
+        a = fma(a, ikxh, m.sxzx);
+      }
+      if (PML && inzv) {
+        m.szz = fma(bzhv, m.szz, azhv * c);
+        c = fma(c, ikzhv, m.szz);
+      }
+      vzn = fma(vzn, k.omz, (a + c) * k.dtz);
+    }
+    if (b == ksrc && is_src_col) { vxn += ix; vzn += iz; }   // source, :359-361
+    if (!INT && (!rv || i == 0 || i == nx - 1 || b == 0 || b == nz - 1)) { vxn = 0; vzn = 0; }  // :365-375
+  };
+  // receiver samples of step itn at row b (own cells of an own row)
+  const int nrec = S.nrec;
+  auto gather = [&](int b, int itn, T vx, T vz) {
+    const int q0 = S.rrow[b], q1 = S.rrow[b + 1];
+    auto tr = mp<T>(S.trace) + (long)itn * 2 * nrec;
+    for (int e = q0; e < q1; ++e) {
+      const int rx = S.rxid[2 * e], id = S.rxid[2 * e + 1];
+      if (own && i == rx) {
+        tr[id] = vx;
+        tr[nrec + id] = vz;
+      }
+    }
+  };
+
+  T vx0_m = valid(zo - 4) ? vxi[cell(zo - 4)] : T(0);  // velocities of step it-1, row r-1
+  T vz0_m = valid(zo - 4) ? vzi[cell(zo - 4)] : T(0);
+  T vx0_c = valid(zo - 3) ? vxi[cell(zo - 3)] : T(0);  // row r
+  T vz0_c = valid(zo - 3) ? vzi[cell(zo - 3)] : T(0);
+  T s1xx_p = 0, s1zz_p = 0, s1xz_p = 0, s1xx_pp = 0, s1zz_pp = 0, s1xz_pp = 0;  // step it, rows r-1, r-2
+  T v1x_m = 0, v1z_m = 0, v1x_c = 0, v1z_c = 0;  // step it, rows r-3, r-2
+  T s2xx_p = 0, s2zz_p = 0, s2xz_p = 0, s2xz_pp = 0;  // step it+1, rows r-3 (and r-4)
+  CN kn_p{}, kn_pp{};  // stress coefficients of rows r-1, r-2
+  CV kv_p{}, kv_pp{};  // velocity coefficients of rows r-2, r-3
+  MV mv_p{}, mv_pp{};  // memory after step it, rows r-1, r-2
+  MS ms_p{}, ms_pp{};  // memory after step it, rows r-2, r-3
+  const bool has_rec = nrec > 0;
+  bool bad0 = false, bad1 = false;  // own cell above the blow-up bound at step it / it+1
+
+  for (int r = zo - 3; r <= ze + 2; ++r) {
+    const bool actA = r <= ze + 1, actB = r >= zo - 1 && r <= ze + 1;
+    const bool actC = r >= zo + 1, actD = r >= zo + 3;
+    // loads (addresses clamped in bounds)
+    const T ld_vx = vxi[cell(r + 1)], ld_vz = vzi[cell(r + 1)];
+    const uint32_t cA = cell(r);
+    const T ld_sxx = sxxi[cA], ld_szz = szzi[cA], ld_sxz = sxzi[cA];
+    const CN kn = load_cn(r);
+    MV mv = load_mv(r);
+    const CV kv = load_cv(r - 1);
+    MS ms = load_ms(r - 1);
+    const bool nxt = r + 1 <= ze + 1 && valid(r + 1);
+    const T vx0_n = nxt ? ld_vx : T(0), vz0_n = nxt ? ld_vz : T(0);
+
+    // A: stresses of step it, row r
+    const bool rvA = valid(r);
+    T s1xx_c = rvA ? ld_sxx : T(0), s1zz_c = rvA ? ld_szz : T(0), s1xz_c = rvA ? ld_sxz : T(0);
+    if (actA)
+      stress(r, vx0_m, vz0_m, vx0_c, vz0_c, vx0_n, vz0_n, s1xx_c, s1zz_c, s1xz_c, kn, mv, ms0xx, ms0zz,
+             ms0xz);
+    // B: velocities of step it, row r-1
+    T v1x_n = 0, v1z_n = 0;
+    if (actB) {
+      vel(r - 1, s1xx_p, s1zz_p, s1xz_p, s1xz_pp, s1zz_c, vx0_m, vz0_m, kv, ms, incx0, incz0, v1x_n, v1z_n);
+      if (r - 1 >= zo && r - 1 < ze) {
+        bad0 |= own && !((double)v1x_n * v1x_n + (double)v1z_n * v1z_n <= 1.0e50);
+        if (has_rec) gather(r - 1, it, v1x_n, v1z_n);
+      }
+    }
+    // C: stresses of step it+1, row r-2
+    T s2xx_c = s1xx_pp, s2zz_c = s1zz_pp, s2xz_c = s1xz_pp;
+    if (actC) {
+      MV mc = mv_pp;
+      stress(r - 2, v1x_m, v1z_m, v1x_c, v1z_c, v1x_n, v1z_n, s2xx_c, s2zz_c, s2xz_c, kn_pp, mc, ms1xx,
+             ms1zz, ms1xz);
+      if (own && r - 2 >= zo && r - 2 < ze) {
+        const uint32_t c = cell(r - 2);
+        sxxo[c] = s2xx_c;
+        szzo[c] = s2zz_c;
+        sxzo[c] = s2xz_c;
+        store_mv(r - 2, mc);
+      }
+    }
+    // D: velocities of step it+1, row r-3
+    if (actD) {
+      MS md = ms_pp;
+      T v2x, v2z;
+      vel(r - 3, s2xx_p, s2zz_p, s2xz_p, s2xz_pp, s2zz_c, v1x_m, v1z_m, kv_pp, md, incx1, incz1, v2x, v2z);
+      if (r - 3 < ze) {
+        bad1 |= own && !((double)v2x * v2x + (double)v2z * v2z <= 1.0e50);
+        if (own) {
+          const uint32_t c = cell(r - 3);
+          vxo[c] = v2x;
+          vzo[c] = v2z;
+          store_ms(r - 3, md);
+        }
+        if (has_rec) gather(r - 3, it + 1, v2x, v2z);
+      }
+    }
+    vx0_m = vx0_c; vz0_m = vz0_c; vx0_c = vx0_n; vz0_c = vz0_n;
+    s1xx_pp = s1xx_p; s1zz_pp = s1zz_p; s1xz_pp = s1xz_p;
+    s1xx_p = s1xx_c; s1zz_p = s1zz_c; s1xz_p = s1xz_c;
+    v1x_m = v1x_c; v1z_m = v1z_c; v1x_c = v1x_n; v1z_c = v1z_n;
+    s2xz_pp = s2xz_p; s2xx_p = s2xx_c; s2zz_p = s2zz_c; s2xz_p = s2xz_c;
+    kn_pp = kn_p; kn_p = kn; kv_pp = kv_p; kv_p = kv;
+    mv_pp = mv_p; mv_p = mv; ms_pp = ms_p; ms_p = ms;
+  }
+  if (bad0)
+    check_blowup(HUGE_VAL, S.blowup, it);
+  else if (bad1)
+    check_blowup(HUGE_VAL, S.blowup, it + 1);
+}
+
+// Work: as k_el2_step, with EW2 owned columns per wavefront.
+template <typename T, bool SPLIT, int WB>
+__global__ void __launch_bounds__(64 * WB, sizeof(T) == 4 ? SDT_EL2_WAVES_PER_SIMD : 2)
+k_el2_step2(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+            const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel,
+            const int* __restrict__ it0, int eh) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int it = __builtin_amdgcn_readfirstlane(*base + rel);
+  const int q = __builtin_amdgcn_readfirstlane(((it - *it0) >> 1) & 1);
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const int4 wv = work[w];
+    const int4 wi = make_int4(__builtin_amdgcn_readfirstlane(wv.x),
+                              __builtin_amdgcn_readfirstlane(wv.y) + wave * EW2,
+                              __builtin_amdgcn_readfirstlane(wv.z), 0);
+    if (wi.x < 0) continue;
+    const DModel& M = models[shots[wi.x].model];
+    if (wi.y >= M.nx) continue;
+    // every cell the wave reads (lanes, rows zo-4 .. ze+1) outside the C-PML slabs
+    const int ze = min(wi.z + eh, M.nz);
+    // and off the outer ring (columns 0, nx-1 and rows 0, nz-1 and their
+    // neighbours)
+    const bool interior = wi.y - 2 >= M.lx + 1 && wi.y + 61 <= M.nx - M.lx - 2 &&
+                          wi.z - 4 >= M.lz + 1 && ze + 1 <= M.nz - M.lz - 2;
+    if (interior)
+      el2x2_tile<T, SPLIT, false>(models, shots, wi, it, q, threadIdx.x & 63, eh);
+    else
+      el2x2_tile<T, SPLIT, true>(models, shots, wi, it, q, threadIdx.x & 63, eh);
+  }
+}
+
 // After a run whose last step wrote buffer B, copy B's fields and psi into A
 // so the caller always finds the state in field[] / psi (A); B keeps an
-// identical copy, which the next (odd) step reads.
+// identical copy, which the next (odd) step reads.  With to_b the copy runs
+// the other way (a run of two-step launches that ends in A before an odd
+// step).
 template <typename T>
 __global__ void k_copy_back(const DShot* __restrict__ shots, int nfield, const long* __restrict__ nel,
-                            const long* __restrict__ npsi) {
+                            const long* __restrict__ npsi, int to_b) {
   const int s = blockIdx.y;
   const DShot& S = shots[s];
   const long n = nel[s], np = npsi[s];
@@ -558,10 +1005,16 @@ __global__ void k_copy_back(const DShot* __restrict__ shots, int nfield, const l
     if (q < nfield * n) {
       const int f = (int)(q / n);
       const long e = q - (long)f * n;
-      mp<T>(S.f[f])[e] = cp<T>(S.g[f])[e];
+      if (to_b)
+        mp<T>(S.g[f])[e] = cp<T>(S.f[f])[e];
+      else
+        mp<T>(S.f[f])[e] = cp<T>(S.g[f])[e];
     } else {
       const long e = q - nfield * n;
-      mp<T>(S.psi)[e] = cp<T>(S.psi2)[e];
+      if (to_b)
+        mp<T>(S.psi2)[e] = cp<T>(S.psi)[e];
+      else
+        mp<T>(S.psi)[e] = cp<T>(S.psi2)[e];
     }
   }
 }
@@ -2284,7 +2737,8 @@ k_em25_e(const DModel* __restrict__ models, const DShot* __restrict__ shots,
 // ===========================================================================
 template <typename T>
 __global__ void k_gather(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-                         int nshots, int ncomp, const int* __restrict__ base, int rel, int pingpong) {
+                         int nshots, int ncomp, const int* __restrict__ base, int rel, int pingpong,
+                         const int* __restrict__ prun) {
   const int s = blockIdx.y;
   if (s >= nshots) return;
   const DShot& S = shots[s];
@@ -2295,7 +2749,9 @@ __global__ void k_gather(const DModel* __restrict__ models, const DShot* __restr
   const auto q = cp<int>(S.rec) + 3 * r;
   const long c = ((long)q[2] * M.ny + q[1]) * M.nx + q[0];
   auto tr = mp<T>(S.trace) + ((long)it * ncomp) * S.nrec + r;
-  const bool inb = pingpong && ((it + 1) & 1);  // state after step it
+  // state after step it (after the single step that ends a run of
+  // two-step launches, with prun)
+  const bool inb = pingpong && (prun ? ((((it - *prun) >> 1) & 1) ^ 1) : ((it + 1) & 1));
   for (int f = 0; f < ncomp; ++f) tr[(long)f * S.nrec] = cp<T>(inb ? S.g[f] : S.f[f])[c];
 }
 
@@ -2330,6 +2786,12 @@ k_em25_edges(const DModel* __restrict__ models, const DShot* __restrict__ shots,
 __global__ void k_set_int(int* p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
+__global__ void k_set_int2(int* p, int v, int* p2, int v2) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *p = v;
+    *p2 = v2;
+  }
+}
 __global__ void k_add_int(int* p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = *p + v;
 }
@@ -2354,6 +2816,13 @@ struct Plan {
   int el2_grid = 0;  // persistent grid of the ELASTIC2 step kernel
   int el2_wb = SDT_EL2_WAVES_PER_BLOCK;  // tile: wavefronts per block
   int el2_eh = SDT_EL2_ROWS;             // tile: rows
+  int spl = 1;                 // steps per launch of the ELASTIC2 step kernel (1 or 2)
+  int4* dwork2 = nullptr;      // work list of the two-step kernel
+  int nwork2 = 0;
+  int el2x2_grid = 0;
+  int* dit0 = nullptr;         // first step of the current run of two-step launches
+  int* drec = nullptr;         // receivers by row of every shot (DShot::rrow, rxid)
+  bool graph_two = false;      // the cached graph holds two-step launches
   int* dflags = nullptr;  // per shot first unstable step (INT_MAX = none)
   int* dbase = nullptr;   // step index of rel = 0
   long* dnel = nullptr;   // per shot: field elements (ping-pong copy-back)
@@ -2427,21 +2896,41 @@ long long psi_elems(int kind, long long nx, long long ny, long long nz, long lon
 
 // One launch of the single-pass ELASTIC2 step with the plan's tile shape.
 template <typename T, bool TRAIL, bool SPLIT>
-void launch_el2(Plan* p, hipStream_t st, int rel, const int* tsel) {
+void launch_el2(Plan* p, hipStream_t st, int rel, const int* tsel, const int* prun = nullptr) {
   const DAdj* adj = TRAIL ? p->dadj : nullptr;
   const dim3 g(p->el2_grid);
   switch (p->el2_wb) {
     case 1:
       hipLaunchKernelGGL((k_el2_step<T, TRAIL, SPLIT, 1>), g, dim3(64), 0, st, p->dmodels,
-                         p->dshots, p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh);
+                         p->dshots, p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh, prun);
       break;
     case 2:
       hipLaunchKernelGGL((k_el2_step<T, TRAIL, SPLIT, 2>), g, dim3(128), 0, st, p->dmodels,
-                         p->dshots, p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh);
+                         p->dshots, p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh, prun);
       break;
     default:
       hipLaunchKernelGGL((k_el2_step<T, TRAIL, SPLIT, 4>), g, dim3(256), 0, st, p->dmodels,
-                         p->dshots, p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh);
+                         p->dshots, p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh, prun);
+      break;
+  }
+}
+
+// One launch of the two-step ELASTIC2 kernel (steps base+rel, base+rel+1).
+template <typename T, bool SPLIT>
+void launch_el2x2(Plan* p, hipStream_t st, int rel) {
+  const dim3 g(p->el2x2_grid);
+  switch (p->el2_wb) {
+    case 1:
+      hipLaunchKernelGGL((k_el2_step2<T, SPLIT, 1>), g, dim3(64), 0, st, p->dmodels, p->dshots,
+                         p->dwork2, p->nwork2, p->dbase, rel, p->dit0, p->el2_eh);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_el2_step2<T, SPLIT, 2>), g, dim3(128), 0, st, p->dmodels, p->dshots,
+                         p->dwork2, p->nwork2, p->dbase, rel, p->dit0, p->el2_eh);
+      break;
+    default:
+      hipLaunchKernelGGL((k_el2_step2<T, SPLIT, 4>), g, dim3(256), 0, st, p->dmodels, p->dshots,
+                         p->dwork2, p->nwork2, p->dbase, rel, p->dit0, p->el2_eh);
       break;
   }
 }
@@ -2469,9 +2958,9 @@ void launch_em2(Plan* p, hipStream_t st, int rel, const int* tsel) {
 // Work list of the ELASTIC2 step kernel for the plan's tile shape: one item
 // (shot, x0, z0) per block of el2_wb adjacent x tiles; the items of one
 // block-tile over all shots of its model share blockIdx % 8 (one XCD's L2).
-std::vector<int4> el2_work(const Plan* p) {
+std::vector<int4> el2_work(const Plan* p, int ew = EW) {
   std::vector<int4> work;
-  const int bw = EW * p->el2_wb, eh = p->el2_eh;
+  const int bw = ew * p->el2_wb, eh = p->el2_eh;
   for (int m = 0; m < p->nmodels; ++m) {
     const sdt_model& M = p->hmodels[m];
     const int tx = (M.nx + bw - 1) / bw, tz = (M.nz + eh - 1) / eh, ntile = tx * tz;
@@ -2528,7 +3017,7 @@ void launch_one(Plan* p, hipStream_t st, int rel, int which) {
     if (p->maxrec > 0) {
       const dim3 gg((p->maxrec + 63) / 64, p->nshots);
       hipLaunchKernelGGL(k_gather<T>, gg, dim3(64), 0, st, p->dmodels, p->dshots, p->nshots,
-                         p->ncomp, p->dbase, rel, pingpong(p->kind) ? 1 : 0);
+                         p->ncomp, p->dbase, rel, pingpong(p->kind) ? 1 : 0, (const int*)nullptr);
     }
     return;
   }
@@ -2565,9 +3054,9 @@ void launch_one(Plan* p, hipStream_t st, int rel, int which) {
 }
 
 template <typename T>
-void launch_copy_back(Plan* p, hipStream_t st) {
+void launch_copy_back(Plan* p, hipStream_t st, int to_b = 0) {
   hipLaunchKernelGGL(k_copy_back<T>, dim3(1024, p->nshots), dim3(256), 0, st, p->dshots,
-                     kind_fields(p->kind), p->dnel, p->dnpsi);
+                     kind_fields(p->kind), p->dnel, p->dnpsi, to_b);
 }
 
 template <typename T>
@@ -2591,6 +3080,44 @@ void launch_steps(Plan* p, hipStream_t st, int n) {
     else
       launch_step<float>(p, st, r);
   }
+}
+
+// Launch `which` of a run of two-step launches from the base step: pair
+// r/2 (steps r, r+1) for even r, or the single step r that ends an odd
+// run (one-step kernel and receiver gather, reading the buffer the pairs
+// left the state in).
+template <typename T>
+void launch_two(Plan* p, hipStream_t st, int r, bool single) {
+  if (!single) {
+    if (p->split)
+      launch_el2x2<T, true>(p, st, r);
+    else
+      launch_el2x2<T, false>(p, st, r);
+    return;
+  }
+  if (p->split)
+    launch_el2<T, false, true>(p, st, r, nullptr, p->dit0);
+  else
+    launch_el2<T, false, false>(p, st, r, nullptr, p->dit0);
+  if (p->maxrec > 0)
+    hipLaunchKernelGGL(k_gather<T>, dim3((p->maxrec + 63) / 64, p->nshots), dim3(64), 0, st, p->dmodels,
+                       p->dshots, p->nshots, p->ncomp, p->dbase, r, 1, (const int*)p->dit0);
+}
+
+void launch_pairs(Plan* p, hipStream_t st, int n) {
+  for (int r = 0; r < n; r += 2) {
+    const bool single = r + 1 == n;
+    if (p->prec == SDT_FP64)
+      launch_two<double>(p, st, r, single);
+    else
+      launch_two<float>(p, st, r, single);
+  }
+}
+
+// Whether a run of n steps in graph blocks of graph_block uses two-step
+// launches (the blocks must hold whole pairs).
+bool use_two(const Plan* p, int n, int graph_block) {
+  return p->spl == 2 && n >= 2 && (graph_block <= 1 || n < graph_block || graph_block % 2 == 0);
 }
 
 int validate(int kind, bool split, int prec, const sdt_model* models, int nmodels,
@@ -2670,14 +3197,19 @@ int validate(int kind, bool split, int prec, const sdt_model* models, int nmodel
 }
 
 // State after the last step of a run: EM H edges zeroed; ping-pong kinds
-// copied back into buffer A when the last step wrote buffer B.
-void end_of_run(Plan* p, hipStream_t st, int next_step) {
-  const bool b = pingpong(p->kind) && (next_step & 1);
+// copied back into buffer A when the last step wrote buffer B.  After a run
+// of two-step launches (final_b: whether it ended in B, else -1) A always
+// holds the state, and so does B before an odd step, as after one-step runs.
+void end_of_run(Plan* p, hipStream_t st, int next_step, int final_b = -1) {
+  const bool b = pingpong(p->kind) && (final_b < 0 ? (next_step & 1) != 0 : final_b != 0);
+  const bool to_b = final_b == 0 && (next_step & 1);
   if (p->prec == SDT_FP64) {
     if (b) launch_copy_back<double>(p, st);
+    if (to_b) launch_copy_back<double>(p, st, 1);
     launch_finalize<double>(p, st);
   } else {
     if (b) launch_copy_back<float>(p, st);
+    if (to_b) launch_copy_back<float>(p, st, 1);
     launch_finalize<float>(p, st);
   }
 }
@@ -2785,6 +3317,23 @@ void drop_adjoint_graphs(Plan* p) {
 }
 
 // Replay (capturing on first use) the segment graph (kind, key) on st.
+// (Re)build the work list of the two-step ELASTIC2 kernel for the plan's
+// tile shape.
+int upload_work2(Plan* p) {
+  const std::vector<int4> work = el2_work(p, EW2);
+  int4* dw = nullptr;
+  HIP_TRY(hipMalloc(&dw, sizeof(int4) * work.size()));
+  if (hipMemcpy(dw, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(dw);
+    return fail(SDT_EDEVICE, "hipMemcpy failed (work list)");
+  }
+  if (p->dwork2) (void)hipFree(p->dwork2);
+  p->dwork2 = dw;
+  p->nwork2 = (int)work.size();
+  p->el2x2_grid = std::max(8, std::min(p->nwork2, SDT_EL2_GRID) / 8 * 8);
+  return SDT_OK;
+}
+
 template <typename F>
 int replay_segment(Plan* p, int kind, long long key, hipStream_t st, F&& enqueue) {
   for (auto& x : p->seg_graphs)
@@ -2865,6 +3414,40 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
     }
     p->maxrec = std::max(p->maxrec, S.nrec);
   }
+  // receivers by row (two-step ELASTIC2 kernel): per shot rrow[nz + 1] then
+  // (column, index) pairs sorted by (row, column)
+  std::vector<int> rows_cols;
+  std::vector<long> rec_off(nshots, -1);
+  if (kind == SDT_ELASTIC2) {
+    for (int s = 0; s < nshots; ++s) {
+      const sdt_shot& S = shots[s];
+      if (S.nrec <= 0) continue;
+      std::vector<int> rec(3 * (size_t)S.nrec);
+      if (hipMemcpy(rec.data(), S.rec, rec.size() * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return bail(fail(SDT_EDEVICE, "cannot read receiver indices"));
+      const int nz = models[S.model].nz;
+      std::vector<int> order(S.nrec);
+      for (int r = 0; r < S.nrec; ++r) order[r] = r;
+      std::sort(order.begin(), order.end(), [&](int a, int b) {
+        return rec[3 * a + 2] != rec[3 * b + 2] ? rec[3 * a + 2] < rec[3 * b + 2] : rec[3 * a] < rec[3 * b];
+      });
+      rec_off[s] = (long)rows_cols.size();
+      std::vector<int> rrow(nz + 1, 0);
+      for (int r = 0; r < S.nrec; ++r) rrow[rec[3 * r + 2] + 1]++;
+      for (int k = 0; k < nz; ++k) rrow[k + 1] += rrow[k];
+      rows_cols.insert(rows_cols.end(), rrow.begin(), rrow.end());
+      for (int r : order) {
+        rows_cols.push_back(rec[3 * r]);
+        rows_cols.push_back(r);
+      }
+    }
+    if (!rows_cols.empty()) {
+      if (hipMalloc(&p->drec, sizeof(int) * rows_cols.size()) != hipSuccess ||
+          hipMemcpy(p->drec, rows_cols.data(), sizeof(int) * rows_cols.size(), hipMemcpyHostToDevice) !=
+              hipSuccess)
+        return bail(fail(SDT_EDEVICE, "hipMalloc/hipMemcpy failed (receivers by row)"));
+    }
+  }
   // device descriptors
   std::vector<DModel> dm(nmodels);
   for (int m = 0; m < nmodels; ++m) {
@@ -2880,7 +3463,7 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
     D.aux = M.aux;
   }
   if (hipMalloc(&p->dflags, sizeof(int) * nshots) != hipSuccess ||
-      hipMalloc(&p->dbase, sizeof(int)) != hipSuccess)
+      hipMalloc(&p->dbase, sizeof(int)) != hipSuccess || hipMalloc(&p->dit0, sizeof(int)) != hipSuccess)
     return bail(fail(SDT_EDEVICE, "hipMalloc failed (flags)"));
   std::vector<DShot> ds(nshots);
   for (int s = 0; s < nshots; ++s) {
@@ -2895,6 +3478,8 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
     for (int c = 0; c < 6; ++c) D.msrc[c] = (kind == SDT_ELASTIC2 || kind == SDT_ELASTIC25) ? S.msrc[c] : nullptr;
     D.nrec = S.nrec; D.rec = S.rec; D.trace = S.trace;
     D.blowup = p->dflags + s;
+    D.rrow = rec_off[s] < 0 ? nullptr : p->drec + rec_off[s];
+    D.rxid = rec_off[s] < 0 ? nullptr : p->drec + rec_off[s] + models[S.model].nz + 1;
   }
   // work list: (shot, x0, z0); all shots of one tile share blockIdx % 8 (XCD)
   std::vector<int4> work;
@@ -2950,8 +3535,13 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
     return bail(fail(SDT_EDEVICE, "hipMemcpy failed (descriptors)"));
   hipLaunchKernelGGL(k_fill_int, dim3((nshots + 255) / 256), dim3(256), 0, nullptr, p->dflags,
                      nshots, INT_MAX);
-  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, nullptr, p->dbase, 0);
+  hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(1), 0, nullptr, p->dbase, 0, p->dit0, 0);
   if (hipDeviceSynchronize() != hipSuccess) return bail(fail(SDT_EDEVICE, "init kernels failed"));
+  if (kind == SDT_ELASTIC2) {
+    rc = upload_work2(p);
+    if (rc) return bail(rc);
+    p->spl = 2;
+  }
   if (hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(SDT_EDEVICE, "hipStreamCreate failed"));
   *out_plan = p;
@@ -2966,23 +3556,33 @@ int sdt_plan_run(void* plan, int it0, int nsteps, int graph_block, void* stream)
   hipStream_t st = static_cast<hipStream_t>(stream);
   p->last = st;
   if (nsteps == 0) return SDT_OK;
-  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, p->dbase, it0);
+  const bool two = use_two(p, nsteps, graph_block);
+  hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(1), 0, st, p->dbase, it0, p->dit0, it0);
   int done = 0;
   if (graph_block > 1 && nsteps >= graph_block) {
-    if (p->graph_block != graph_block || !p->gexec) {
+    if (p->graph_block != graph_block || !p->gexec || p->graph_two != two) {
       if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
       if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
       HIP_TRY(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
-      launch_steps(p, p->cap, graph_block);
+      if (two)
+        launch_pairs(p, p->cap, graph_block);
+      else
+        launch_steps(p, p->cap, graph_block);
       hipLaunchKernelGGL(k_add_int, dim3(1), dim3(1), 0, p->cap, p->dbase, graph_block);
       HIP_TRY(hipStreamEndCapture(p->cap, &p->graph));
       HIP_TRY(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
       p->graph_block = graph_block;
+      p->graph_two = two;
     }
     for (; done + graph_block <= nsteps; done += graph_block) HIP_TRY(hipGraphLaunch(p->gexec, st));
   }
-  if (done < nsteps) launch_steps(p, st, nsteps - done);
-  end_of_run(p, st, it0 + nsteps);
+  if (done < nsteps) {
+    if (two)
+      launch_pairs(p, st, nsteps - done);
+    else
+      launch_steps(p, st, nsteps - done);
+  }
+  end_of_run(p, st, it0 + nsteps, two ? ((nsteps / 2 + (nsteps & 1)) & 1) : -1);
   HIP_TRY(hipGetLastError());
   return SDT_OK;
 }
@@ -2997,8 +3597,39 @@ int sdt_plan_profile(void* plan, int it0, int nsteps, void* stream, double* out_
   hipEvent_t ev[4];
   for (int q = 0; q < 4; ++q) HIP_TRY(hipEventCreate(&ev[q]));
   double acc[3] = {0, 0, 0};
-  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, p->dbase, it0);
+  const bool two = use_two(p, nsteps, 0);
+  hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(1), 0, st, p->dbase, it0, p->dit0, it0);
   int rc = SDT_OK;
+  if (two) {
+    // two-step launches: slot 0 is the mean per launch (two steps); the
+    // single step that ends an odd run is not timed
+    int npair = 0;
+    for (int s = 0; s < nsteps && rc == SDT_OK; s += 2) {
+      const bool single = s + 1 == nsteps;
+      if (hipEventRecord(ev[0], st) != hipSuccess) { rc = fail(SDT_EDEVICE, "hipEventRecord"); break; }
+      if (p->prec == SDT_FP64)
+        launch_two<double>(p, st, s, single);
+      else
+        launch_two<float>(p, st, s, single);
+      if (hipEventRecord(ev[1], st) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess) {
+        rc = fail(SDT_EDEVICE, "event sync failed");
+        break;
+      }
+      if (single) break;
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) { rc = fail(SDT_EDEVICE, "elapsed"); break; }
+      acc[0] += ms;
+      ++npair;
+    }
+    if (rc == SDT_OK) end_of_run(p, st, it0 + nsteps, (nsteps / 2 + (nsteps & 1)) & 1);
+    for (int q = 0; q < 4; ++q) (void)hipEventDestroy(ev[q]);
+    if (rc) return rc;
+    out_ms[0] = npair ? acc[0] / npair : 0.0;
+    out_ms[1] = 0.0;
+    out_ms[2] = 0.0;
+    HIP_TRY(hipGetLastError());
+    return SDT_OK;
+  }
   for (int s = 0; s < nsteps && rc == SDT_OK; ++s) {
     for (int w = 0; w < 3; ++w) {
       if (hipEventRecord(ev[w], st) != hipSuccess) { rc = fail(SDT_EDEVICE, "hipEventRecord"); break; }
@@ -3066,9 +3697,30 @@ int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows) {
   p->dwork = dw;
   p->nwork = (int)work.size();
   p->el2_grid = std::max(8, std::min(p->nwork, SDT_EL2_GRID) / 8 * 8);
+  if (p->dwork2) {
+    const int rc = upload_work2(p);
+    if (rc) return rc;
+  }
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
   if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
   drop_adjoint_graphs(p);
+  return SDT_OK;
+}
+
+int sdt_plan_set_steps_per_launch(void* plan, int steps) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p) return fail(SDT_EINVAL, "plan is null");
+  if (steps != 1 && steps != 2) return fail(SDT_EINVAL, "steps per launch must be 1 or 2");
+  if (steps == 2 && p->kind != SDT_ELASTIC2)
+    return fail(SDT_EINVAL, "two steps per launch apply to ELASTIC2 plans");
+  p->spl = steps;
+  return SDT_OK;
+}
+
+int sdt_plan_get_steps_per_launch(void* plan, int* steps) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p || !steps) return fail(SDT_EINVAL, "null argument");
+  *steps = p->spl;
   return SDT_OK;
 }
 
@@ -3269,6 +3921,9 @@ void sdt_plan_destroy(void* plan) {
   if (p->dmodels) (void)hipFree(p->dmodels);
   if (p->dshots) (void)hipFree(p->dshots);
   if (p->dwork) (void)hipFree(p->dwork);
+  if (p->dwork2) (void)hipFree(p->dwork2);
+  if (p->dit0) (void)hipFree(p->dit0);
+  if (p->drec) (void)hipFree(p->drec);
   if (p->dflags) (void)hipFree(p->dflags);
   if (p->dbase) (void)hipFree(p->dbase);
   if (p->dnel) (void)hipFree(p->dnel);

@@ -187,7 +187,9 @@ class Simulation:
     def profile(self, nsteps: int, stream=None):
         """Run the next ``nsteps`` steps with every kernel bracketed by HIP
         events; return mean ms per launch of (first field kernel, second
-        field kernel, receiver gather)."""
+        field kernel, receiver gather).  With two steps per launch
+        (``steps_per_launch``) the first is the two-step kernel, per launch,
+        and the gather is 0 (the kernel writes the receiver samples)."""
         if stream is None:
             stream = torch.cuda.current_stream(self.device)
         out = (C.c_double * 3)()
@@ -195,6 +197,18 @@ class Simulation:
                                                C.c_void_p(stream.cuda_stream), out))
         self.step += nsteps
         return tuple(float(v) for v in out)
+
+    @property
+    def steps_per_launch(self) -> int:
+        """Time steps one launch of the ELASTIC2 step kernel advances (2 by
+        default for ELASTIC2, else 1); results do not depend on it."""
+        v = C.c_int()
+        _hip.check(_hip.lib().sdt_plan_get_steps_per_launch(self._plan, C.byref(v)))
+        return v.value
+
+    @steps_per_launch.setter
+    def steps_per_launch(self, steps: int):
+        _hip.check(_hip.lib().sdt_plan_set_steps_per_launch(self._plan, int(steps)))
 
     # ------------------------------------------------------------------
     TILE_CANDIDATES = tuple((wb, rows) for wb in (4, 2, 1) for rows in (2, 4, 8, 16, 32, 64))
@@ -217,22 +231,28 @@ class Simulation:
         Each candidate runs ``steps`` steps (timed by HIP events around the
         step kernel) from the current state; the state, traces and step
         counter are restored afterwards, so the run continues as if nothing
-        had happened.  Returns {(wb, rows): ms per step} and keeps the best.
+        had happened.  Returns {(wb, rows): ms per step} and keeps the best
+        (``steps`` is rounded up to whole two-step launches where those run).
         Other kinds have a single shape: returns {}."""
         if self.kind not in PINGPONG:
             return {}
         steps = max(1, min(int(steps), self.nt - self.step))
+        spl = self.steps_per_launch
+        if spl == 2 and steps % 2 and steps + 1 <= self.nt - self.step:
+            steps += 1
+        per = spl if steps >= 2 else 1      # steps per timed launch
+        warm = min(spl, self.nt - self.step)
         keep = [(b["state"].clone(), b["state_b"].clone(), b["trace"].clone()) for _, _, b in self.shots]
         step0 = self.step
         res = {}
         for wb, rows in (candidates or self.TILE_CANDIDATES):
             self.set_tiles(wb, rows)
-            self.profile(1)                       # first launch of this shape
+            self.profile(warm)                    # first launch of this shape
             self.step = step0
             for (_, _, b), (a0, b0, t0) in zip(self.shots, keep):
                 b["state"].copy_(a0)
                 b["state_b"].copy_(b0)
-            res[(wb, rows)] = self.profile(steps)[0]
+            res[(wb, rows)] = self.profile(steps)[0] / per
             self.step = step0
             for (_, _, b), (a0, b0, t0) in zip(self.shots, keep):
                 b["state"].copy_(a0)
