@@ -132,6 +132,14 @@ __device__ __forceinline__ gptr<T> mp(void* p) {
   return (gptr<T>)p;
 }
 
+// Read-only array through the constant address space: a wave-uniform index
+// becomes a scalar load into SGPRs (only for data no kernel of the launch
+// writes).
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* kp(const void* p) {
+  return (const __attribute__((address_space(4))) T*)p;
+}
+
 // C-PML profile table of a model: P[axis * 6 + q] (q as in PK..PBH).
 template <typename T>
 struct Prof {
@@ -595,22 +603,34 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
 // state of it+2 written to the other (it0: first step of the run).
 // ===========================================================================
 constexpr int EW2 = 58;  // owned columns per wavefront
+#ifndef SDT_EL2X2_WAVES
+#define SDT_EL2X2_WAVES 4   // FP32 wavefronts per SIMD the register budget allows
+#endif
+#ifndef SDT_EL2X2_PREFETCH
+#define SDT_EL2X2_PREFETCH 0
+#endif
+#ifndef SDT_EL2X2_WAVES_INT
+#define SDT_EL2X2_WAVES_INT 5  // the same for the interior-only kernel
+#endif
 
-template <typename T, bool SPLIT, bool PML>
+template <typename T, bool SPLIT, int BODY>
 __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
                                            const DShot* __restrict__ shots, const int4 wi, const int it,
                                            const int q, const int lane, const int eh) {
 #pragma clang fp contract(off)
-  // Interior waves (no C-PML) read only columns 1..nx-2 and rows 1..nz-2,
-  // so every edge condition of the scheme holds on every lane there.
-  constexpr bool INT = !PML;
+  // BODY 0: an interior wave (no C-PML) reads only columns 1..nx-2 and rows
+  // 1..nz-2, so every edge condition of the scheme holds on every lane;
+  // BODY 1: its columns are interior (every condition on the column holds,
+  // no x slab) but its rows may reach a z slab or the outer rows;
+  // BODY 2: anything else.
+  constexpr bool INT = BODY == 0, XI = BODY < 2, PML = BODY > 0;
   const DShot& S = shots[wi.x];
   const DModel& M = models[S.model];
   const int nx = M.nx, nz = M.nz;
   const int xo = wi.y, zo = wi.z;
   const int i = xo - 2 + lane;
-  const bool incol = INT || (i >= 0 && i < nx);
-  const int ic = INT ? i : (incol ? i : (i < 0 ? 0 : nx - 1));  // clamped column: every load stays in bounds
+  const bool incol = XI || (i >= 0 && i < nx);
+  const int ic = XI ? i : (incol ? i : (i < 0 ? 0 : nx - 1));  // clamped column: every load stays in bounds
   const bool own = incol && lane >= 2 && lane < 2 + EW2;
   const int ze = min(zo + eh, nz);
   const A32<const T> vxi(q ? S.g[0] : S.f[0]);
@@ -629,14 +649,16 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
   const int lx = M.lx, lz = M.lz;
   const uint32_t SX = (uint32_t)nz * 2 * lx, SZ = (uint32_t)2 * lz * nx;
   const uint32_t ZB = (SPLIT ? 6 : 4) * SX;  // first z-slab array
-  const bool inx = PML && incol && in_slab(i, nx, lx);
+  const bool inx = !XI && incol && in_slab(i, nx, lx);
   const int xs = inx ? slab_pos(i, nx, lx) : 0;
   const Prof<T> P(&M.prof[0][0]);
   const T ikx = P[PK][ic], ax = P[PA][ic], bx = P[PB][ic];
   const T ikxh = P[PKH][ic], axh = P[PAH][ic], bxh = P[PBH][ic];
-  const auto ikz = P[12 + PK], az = P[12 + PA], bz = P[12 + PB];
-  const auto ikzh = P[12 + PKH], azh = P[12 + PAH], bzh = P[12 + PBH];
-  const auto brow = cp<T>(M.aux);
+  // z profiles and b_x_half by row: one value per row, read through the
+  // scalar (constant) path into SGPRs; nothing writes them during a launch
+  const auto ikz = kp<T>(M.prof[2][PK]), az = kp<T>(M.prof[2][PA]), bz = kp<T>(M.prof[2][PB]);
+  const auto ikzh = kp<T>(M.prof[2][PKH]), azh = kp<T>(M.prof[2][PAH]), bzh = kp<T>(M.prof[2][PBH]);
+  const auto brow = kp<T>(M.aux);
   const A32<const T> N11(M.coef[0]), N13(M.coef[1]), N15(M.coef[2]);
   const A32<const T> N33(M.coef[3]), N35(M.coef[4]);
   const A32<const T> S15(M.coef[5]), S35(M.coef[6]), S55(M.coef[7]);
@@ -763,7 +785,7 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
       bzhr = bzh[rs]; azhr = azh[rs]; ikzhr = ikzh[rs];
       browr = SPLIT ? bxh : brow[rs];  // b_x_half(j) (A2), or b_x_half(i)
     }
-    if (INT || (rv && i <= nx - 2 && a >= 1)) {
+    if (INT || (rv && (XI || i <= nx - 2) && a >= 1)) {
       T dvx_dx = (vx_e - vxc) * idx;
       T dvz_dz = (vzc - vzm) * idz;
       T dvz_dx = (vz_e - vzc) * idx;
@@ -784,7 +806,7 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
       sxx = fma(fma(k.n15, sh, fma(k.n13, dvz_dz, k.n11 * dvx_dx)), dt, sxx);
       szz = fma(fma(k.n35, sh, fma(k.n33, dvz_dz, k.n13 * dvx_dx)), dt, szz);
     }
-    if (INT || (rv && i >= 1 && a <= nz - 2)) {  // same memory arrays (A1) unless split
+    if (INT || (rv && (XI || i >= 1) && a <= nz - 2)) {  // same memory arrays (A1) unless split
       T& q_vxx = SPLIT ? m.svxx : m.vxx;
       T& q_vzx = SPLIT ? m.svzx : m.vzx;
       T& q_vzz = SPLIT ? m.svzz : m.vzz;
@@ -829,7 +851,7 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
     }
     vxn = vx_old;
     vzn = vz_old;
-    if (INT || (rv && i >= 1 && b >= 1)) {
+    if (INT || (rv && (XI || i >= 1) && b >= 1)) {
       T a = (sxx_b - sxx_w) * idx;
       T c = (sxz_b - sxz_bm) * idz;
       if (PML && inx) {
@@ -842,7 +864,7 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
       }
       vxn = fma(vxn, k.omx, (a + c) * k.dtx);
     }
-    if (INT || (rv && i <= nx - 2 && b <= nz - 2)) {
+    if (INT || (rv && (XI || i <= nx - 2) && b <= nz - 2)) {
       T a = (sxz_e - sxz_b) * idx;
       T c = (szz_bn - szz_b) * idz;
       if (PML && inx) {
@@ -856,7 +878,10 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
       vzn = fma(vzn, k.omz, (a + c) * k.dtz);
     }
     if (b == ksrc && is_src_col) { vxn += ix; vzn += iz; }   // source, :359-361
-    if (!INT && (!rv || i == 0 || i == nx - 1 || b == 0 || b == nz - 1)) { vxn = 0; vzn = 0; }  // :365-375
+    if (!INT && (!rv || (!XI && (i == 0 || i == nx - 1)) || b == 0 || b == nz - 1)) {  // :365-375
+      vxn = 0;
+      vzn = 0;
+    }
   };
   // receiver samples of step itn at row b (own cells of an own row)
   const int nrec = S.nrec;
@@ -886,17 +911,49 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
   const bool has_rec = nrec > 0;
   bool bad0 = false, bad1 = false;  // own cell above the blow-up bound at step it / it+1
 
+  // Every load of iteration r (addresses clamped in bounds).  With
+  // SDT_EL2X2_PREFETCH the loads of iteration r+1 are issued before
+  // iteration r computes, so each wavefront keeps two rows in flight.
+  struct Ld {
+    T vx, vz, sxx, szz, sxz;
+    CN kn;
+    MV mv;
+    CV kv;
+    MS ms;
+  };
+  auto load_it = [&](int r) {
+    Ld L;
+    L.vx = vxi[cell(r + 1)];
+    L.vz = vzi[cell(r + 1)];
+    const uint32_t cA = cell(r);
+    L.sxx = sxxi[cA];
+    L.szz = szzi[cA];
+    L.sxz = sxzi[cA];
+    L.kn = load_cn(r);
+    L.mv = load_mv(r);
+    L.kv = load_cv(r - 1);
+    L.ms = load_ms(r - 1);
+    return L;
+  };
+  constexpr bool PF = SDT_EL2X2_PREFETCH && INT;
+  Ld ahead{};
+  if constexpr (PF) ahead = load_it(zo - 3);
+
   for (int r = zo - 3; r <= ze + 2; ++r) {
     const bool actA = r <= ze + 1, actB = r >= zo - 1 && r <= ze + 1;
     const bool actC = r >= zo + 1, actD = r >= zo + 3;
-    // loads (addresses clamped in bounds)
-    const T ld_vx = vxi[cell(r + 1)], ld_vz = vzi[cell(r + 1)];
-    const uint32_t cA = cell(r);
-    const T ld_sxx = sxxi[cA], ld_szz = szzi[cA], ld_sxz = sxzi[cA];
-    const CN kn = load_cn(r);
-    MV mv = load_mv(r);
-    const CV kv = load_cv(r - 1);
-    MS ms = load_ms(r - 1);
+    Ld L;
+    if constexpr (PF) {
+      L = ahead;
+      ahead = load_it(r + 1);
+    } else {
+      L = load_it(r);
+    }
+    const T ld_vx = L.vx, ld_vz = L.vz, ld_sxx = L.sxx, ld_szz = L.szz, ld_sxz = L.sxz;
+    const CN kn = L.kn;
+    MV mv = L.mv;
+    const CV kv = L.kv;
+    MS ms = L.ms;
     const bool nxt = r + 1 <= ze + 1 && valid(r + 1);
     const T vx0_n = nxt ? ld_vx : T(0), vz0_n = nxt ? ld_vz : T(0);
 
@@ -959,9 +1016,14 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
     check_blowup(HUGE_VAL, S.blowup, it + 1);
 }
 
-// Work: as k_el2_step, with EW2 owned columns per wavefront.
-template <typename T, bool SPLIT, int WB>
-__global__ void __launch_bounds__(64 * WB, sizeof(T) == 4 ? SDT_EL2_WAVES_PER_SIMD : 2)
+// Work: as k_el2_step, with EW2 owned columns per wavefront.  INT: every
+// item of the work list is an interior block (el2x2_interior), so the kernel
+// holds only the interior body and its smaller register budget allows more
+// wavefronts per SIMD; otherwise each wavefront picks its body.
+template <typename T, bool SPLIT, int WB, bool INT>
+// (the split variant's C-PML body needs a larger budget: fewer wavefronts)
+__global__ void __launch_bounds__(64 * WB, sizeof(T) == 4 ? (INT ? SDT_EL2X2_WAVES_INT : SPLIT ? 3 : SDT_EL2X2_WAVES)
+                                                          : (INT ? 3 : SPLIT ? 1 : 2))
 k_el2_step2(const DModel* __restrict__ models, const DShot* __restrict__ shots,
             const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel,
             const int* __restrict__ it0, int eh) {
@@ -976,16 +1038,18 @@ k_el2_step2(const DModel* __restrict__ models, const DShot* __restrict__ shots,
     if (wi.x < 0) continue;
     const DModel& M = models[shots[wi.x].model];
     if (wi.y >= M.nx) continue;
-    // every cell the wave reads (lanes, rows zo-4 .. ze+1) outside the C-PML slabs
+    // the columns (lanes), resp. rows (zo-4 .. ze+1), the wave reads outside
+    // the C-PML slabs and off the outer ring (columns 0, nx-1 and rows 0,
+    // nz-1 and their neighbours)
     const int ze = min(wi.z + eh, M.nz);
-    // and off the outer ring (columns 0, nx-1 and rows 0, nz-1 and their
-    // neighbours)
-    const bool interior = wi.y - 2 >= M.lx + 1 && wi.y + 61 <= M.nx - M.lx - 2 &&
-                          wi.z - 4 >= M.lz + 1 && ze + 1 <= M.nz - M.lz - 2;
-    if (interior)
-      el2x2_tile<T, SPLIT, false>(models, shots, wi, it, q, threadIdx.x & 63, eh);
+    const bool xint = wi.y - 2 >= M.lx + 1 && wi.y + 61 <= M.nx - M.lx - 2;
+    const bool zint = wi.z - 4 >= M.lz + 1 && ze + 1 <= M.nz - M.lz - 2;
+    if (INT || (xint && zint))
+      el2x2_tile<T, SPLIT, 0>(models, shots, wi, it, q, threadIdx.x & 63, eh);
+    else if (xint)
+      el2x2_tile<T, SPLIT, 1>(models, shots, wi, it, q, threadIdx.x & 63, eh);
     else
-      el2x2_tile<T, SPLIT, true>(models, shots, wi, it, q, threadIdx.x & 63, eh);
+      el2x2_tile<T, SPLIT, 2>(models, shots, wi, it, q, threadIdx.x & 63, eh);
   }
 }
 
@@ -2817,9 +2881,9 @@ struct Plan {
   int el2_wb = SDT_EL2_WAVES_PER_BLOCK;  // tile: wavefronts per block
   int el2_eh = SDT_EL2_ROWS;             // tile: rows
   int spl = 1;                 // steps per launch of the ELASTIC2 step kernel (1 or 2)
-  int4* dwork2 = nullptr;      // work list of the two-step kernel
-  int nwork2 = 0;
-  int el2x2_grid = 0;
+  int4* dwork2 = nullptr;      // work list of the two-step kernel: interior blocks first
+  int nwork2 = 0, nwork2i = 0;
+  int el2x2_grid = 0, el2x2_grid_i = 0;
   int* dit0 = nullptr;         // first step of the current run of two-step launches
   int* drec = nullptr;         // receivers by row of every shot (DShot::rrow, rxid)
   bool graph_two = false;      // the cached graph holds two-step launches
@@ -2915,24 +2979,32 @@ void launch_el2(Plan* p, hipStream_t st, int rel, const int* tsel, const int* pr
   }
 }
 
-// One launch of the two-step ELASTIC2 kernel (steps base+rel, base+rel+1).
-template <typename T, bool SPLIT>
-void launch_el2x2(Plan* p, hipStream_t st, int rel) {
-  const dim3 g(p->el2x2_grid);
+// One two-step ELASTIC2 step pair (steps base+rel, base+rel+1): the
+// interior blocks, then the blocks that touch a C-PML slab or the outer ring.
+template <typename T, bool SPLIT, bool INT>
+void launch_el2x2_list(Plan* p, hipStream_t st, int rel, const int4* work, int n, int grid) {
+  if (n <= 0) return;
+  const dim3 g(grid);
   switch (p->el2_wb) {
     case 1:
-      hipLaunchKernelGGL((k_el2_step2<T, SPLIT, 1>), g, dim3(64), 0, st, p->dmodels, p->dshots,
-                         p->dwork2, p->nwork2, p->dbase, rel, p->dit0, p->el2_eh);
+      hipLaunchKernelGGL((k_el2_step2<T, SPLIT, 1, INT>), g, dim3(64), 0, st, p->dmodels, p->dshots,
+                         work, n, p->dbase, rel, p->dit0, p->el2_eh);
       break;
     case 2:
-      hipLaunchKernelGGL((k_el2_step2<T, SPLIT, 2>), g, dim3(128), 0, st, p->dmodels, p->dshots,
-                         p->dwork2, p->nwork2, p->dbase, rel, p->dit0, p->el2_eh);
+      hipLaunchKernelGGL((k_el2_step2<T, SPLIT, 2, INT>), g, dim3(128), 0, st, p->dmodels, p->dshots,
+                         work, n, p->dbase, rel, p->dit0, p->el2_eh);
       break;
     default:
-      hipLaunchKernelGGL((k_el2_step2<T, SPLIT, 4>), g, dim3(256), 0, st, p->dmodels, p->dshots,
-                         p->dwork2, p->nwork2, p->dbase, rel, p->dit0, p->el2_eh);
+      hipLaunchKernelGGL((k_el2_step2<T, SPLIT, 4, INT>), g, dim3(256), 0, st, p->dmodels, p->dshots,
+                         work, n, p->dbase, rel, p->dit0, p->el2_eh);
       break;
   }
+}
+template <typename T, bool SPLIT>
+void launch_el2x2(Plan* p, hipStream_t st, int rel) {
+  launch_el2x2_list<T, SPLIT, true>(p, st, rel, p->dwork2, p->nwork2i, p->el2x2_grid_i);
+  launch_el2x2_list<T, SPLIT, false>(p, st, rel, p->dwork2 + p->nwork2i, p->nwork2 - p->nwork2i,
+                                     p->el2x2_grid);
 }
 
 template <typename T, bool TRAIL>
@@ -2958,7 +3030,22 @@ void launch_em2(Plan* p, hipStream_t st, int rel, const int* tsel) {
 // Work list of the ELASTIC2 step kernel for the plan's tile shape: one item
 // (shot, x0, z0) per block of el2_wb adjacent x tiles; the items of one
 // block-tile over all shots of its model share blockIdx % 8 (one XCD's L2).
-std::vector<int4> el2_work(const Plan* p, int ew = EW) {
+// Whether every wavefront of the two-step kernel's block at (x0, z0) takes
+// the interior body (the test of k_el2_step2 for each of its wavefronts).
+bool el2x2_interior(const sdt_model& M, int x0, int z0, int wb, int eh) {
+  const int ze = std::min(z0 + eh, M.nz);
+  if (!(z0 - 4 >= M.lz + 1 && ze + 1 <= M.nz - M.lz - 2)) return false;
+  for (int w = 0; w < wb; ++w) {
+    const int xw = x0 + w * EW2;
+    if (xw >= M.nx) break;
+    if (!(xw - 2 >= M.lx + 1 && xw + 61 <= M.nx - M.lx - 2)) return false;
+  }
+  return true;
+}
+
+// which: 0 every block, 1 the interior blocks of the two-step kernel, 2 the
+// others.
+std::vector<int4> el2_work(const Plan* p, int ew = EW, int which = 0) {
   std::vector<int4> work;
   const int bw = ew * p->el2_wb, eh = p->el2_eh;
   for (int m = 0; m < p->nmodels; ++m) {
@@ -2968,12 +3055,17 @@ std::vector<int4> el2_work(const Plan* p, int ew = EW) {
     for (int s = 0; s < p->nshots; ++s)
       if (p->hshots[s].model == m) ms.push_back(s);
     if (ms.empty()) continue;
-    for (int tg = 0; tg < (ntile + 7) / 8; ++tg)
+    std::vector<int> tl;
+    for (int t = 0; t < ntile; ++t)
+      if (which == 0 || el2x2_interior(M, (t % tx) * bw, (t / tx) * eh, p->el2_wb, eh) == (which == 1))
+        tl.push_back(t);
+    const int nt = (int)tl.size();
+    for (int tg = 0; tg < (nt + 7) / 8; ++tg)
       for (int s : ms)
         for (int tt = 0; tt < 8; ++tt) {
-          const int t = tg * 8 + tt;
-          work.push_back(t < ntile ? make_int4(s, (t % tx) * bw, (t / tx) * eh, 0)
-                                   : make_int4(-1, 0, 0, 0));
+          const int k = tg * 8 + tt;
+          work.push_back(k < nt ? make_int4(s, (tl[k] % tx) * bw, (tl[k] / tx) * eh, 0)
+                                : make_int4(-1, 0, 0, 0));
         }
   }
   return work;
@@ -3320,7 +3412,11 @@ void drop_adjoint_graphs(Plan* p) {
 // (Re)build the work list of the two-step ELASTIC2 kernel for the plan's
 // tile shape.
 int upload_work2(Plan* p) {
-  const std::vector<int4> work = el2_work(p, EW2);
+  std::vector<int4> work = el2_work(p, EW2, 1);
+  const int ni = (int)work.size();
+  const std::vector<int4> edge = el2_work(p, EW2, 2);
+  work.insert(work.end(), edge.begin(), edge.end());
+  if (work.empty()) work.push_back(make_int4(-1, 0, 0, 0));
   int4* dw = nullptr;
   HIP_TRY(hipMalloc(&dw, sizeof(int4) * work.size()));
   if (hipMemcpy(dw, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice) != hipSuccess) {
@@ -3330,7 +3426,9 @@ int upload_work2(Plan* p) {
   if (p->dwork2) (void)hipFree(p->dwork2);
   p->dwork2 = dw;
   p->nwork2 = (int)work.size();
-  p->el2x2_grid = std::max(8, std::min(p->nwork2, SDT_EL2_GRID) / 8 * 8);
+  p->nwork2i = ni;
+  p->el2x2_grid_i = std::max(8, std::min(ni, SDT_EL2_GRID) / 8 * 8);
+  p->el2x2_grid = std::max(8, std::min(p->nwork2 - ni, SDT_EL2_GRID) / 8 * 8);
   return SDT_OK;
 }
 
