@@ -383,9 +383,10 @@ def main(argv=None):
     fixed = tuple(int(v) for v in a.tiles.split("x")) if a.tiles else None
     if fixed:
         sim.set_tiles(*fixed)
-        tune = {fixed: float("nan")}
+        tune = {(sim.steps_per_launch, *fixed): float("nan")}
     else:
         tune = sim.autotune(steps=10)   # tile shape of the step kernel (setup, untimed)
+    sim_spl, sim_tiles = sim.steps_per_launch, sim.tiles()
     gb = a.graph_block if W >= a.graph_block else 0
     sim.run(W, graph_block=gb)
     torch.cuda.synchronize()
@@ -471,9 +472,10 @@ def main(argv=None):
                                "hipGraph per 50 steps",
                    "cells_per_step": cells_per_step, "shots": sum(shots),
                    "graph_block": a.graph_block,
-                   "tiles": {"chosen": list(fixed or min(tune, key=tune.get)),
+                   "tiles": {"chosen": {"steps_per_launch": sim_spl, "waves": sim_tiles[0],
+                                        "rows": sim_tiles[1]},
                              "ms_per_step": {} if fixed else
-                             {f"{w}x{r}": round(v, 4) for (w, r), v in tune.items()}}},
+                             {f"{n}-step {w}x{r}": round(v, 4) for (n, w, r), v in tune.items()}}},
         "gpu_ms_events": round(gpu_ms, 3),
         "roofline": roofline,
         "fp64_plain": plain,

@@ -212,29 +212,40 @@ struct A32 {
   }
 };
 
-// Neighbouring lanes by DPP wavefront shifts (a VALU move, no LDS round
-// trip): lane_next gives lane l the value of lane l+1, lane_prev that of
-// lane l-1; the lane with no neighbour (63, resp. 0) gets 0.  Kernels own
-// only lanes whose neighbours exist.
-__device__ __forceinline__ int dpp_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
-__device__ __forceinline__ int dpp_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
 template <typename T>
 __device__ __forceinline__ T lane_next(T v) {  // value of lane l+1
+  return __shfl_down(v, 1, 64);
+}
+template <typename T>
+__device__ __forceinline__ T lane_prev(T v) {  // value of lane l-1
+  return __shfl_up(v, 1, 64);
+}
+
+// The same by DPP wavefront shifts (a VALU move the compiler folds into the
+// instruction that uses it, no LDS round trip): dpp_next gives lane l the
+// value of lane l+1, dpp_prev that of lane l-1; the lane with no neighbour
+// (63, resp. 0) gets 0.  Faster in the two-step kernel; in the one-step
+// kernels the LDS form measured faster (they are latency-bound at lower
+// occupancy, and a VALU result feeding a DPP read needs wait states).
+__device__ __forceinline__ int dpp_next_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_prev_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+template <typename T>
+__device__ __forceinline__ T dpp_next(T v) {
   if constexpr (sizeof(T) == 4) {
-    return __builtin_bit_cast(T, dpp_next(__builtin_bit_cast(int, v)));
+    return __builtin_bit_cast(T, dpp_next_i(__builtin_bit_cast(int, v)));
   } else {
     const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = (unsigned)dpp_next((int)(unsigned)b), hi = (unsigned)dpp_next((int)(unsigned)(b >> 32));
+    const unsigned lo = (unsigned)dpp_next_i((int)(unsigned)b), hi = (unsigned)dpp_next_i((int)(unsigned)(b >> 32));
     return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
   }
 }
 template <typename T>
-__device__ __forceinline__ T lane_prev(T v) {  // value of lane l-1
+__device__ __forceinline__ T dpp_prev(T v) {
   if constexpr (sizeof(T) == 4) {
-    return __builtin_bit_cast(T, dpp_prev(__builtin_bit_cast(int, v)));
+    return __builtin_bit_cast(T, dpp_prev_i(__builtin_bit_cast(int, v)));
   } else {
     const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = (unsigned)dpp_prev((int)(unsigned)b), hi = (unsigned)dpp_prev((int)(unsigned)(b >> 32));
+    const unsigned lo = (unsigned)dpp_prev_i((int)(unsigned)b), hi = (unsigned)dpp_prev_i((int)(unsigned)(b >> 32));
     return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
   }
 }
@@ -606,6 +617,9 @@ constexpr int EW2 = 58;  // owned columns per wavefront
 #ifndef SDT_EL2X2_WAVES
 #define SDT_EL2X2_WAVES 4   // FP32 wavefronts per SIMD the register budget allows
 #endif
+#ifndef SDT_EL2X2_MIN_CELLS
+#define SDT_EL2X2_MIN_CELLS 4000000LL  // cells per step from which plans default to two steps per launch
+#endif
 #ifndef SDT_EL2X2_PREFETCH
 #define SDT_EL2X2_PREFETCH 0
 #endif
@@ -774,8 +788,8 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
   // rows a-1, a, a+1 in, the row's stresses and memory updated in place.
   auto stress = [&](int a, T vxm, T vzm, T vxc, T vzc, T vxn, T vzn, T& sxx, T& szz, T& sxz,
                     const CN& k, MV& m, T isxx, T iszz, T isxz) {
-    const T vx_e = lane_next(vxc), vz_e = lane_next(vzc);
-    const T vx_w = lane_prev(vxc), vz_w = lane_prev(vzc);
+    const T vx_e = dpp_next(vxc), vz_e = dpp_next(vzc);
+    const T vx_w = dpp_prev(vxc), vz_w = dpp_prev(vzc);
     const bool rv = valid(a);
     const int rs = crow(a);
     const bool inx_r = inx && rv, inz_r = incol && rv && inzs(a);
@@ -839,8 +853,8 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
   // stresses of rows b-1 (sxz), b, b+1 (szz) in.
   auto vel = [&](int b, T sxx_b, T szz_b, T sxz_b, T sxz_bm, T szz_bn, T vx_old, T vz_old,
                  const CV& k, MS& m, T ix, T iz, T& vxn, T& vzn) {
-    const T sxx_w = lane_prev(sxx_b);  // sigma_xx(i-1, b)
-    const T sxz_e = lane_next(sxz_b);  // sigma_xz(i+1, b)
+    const T sxx_w = dpp_prev(sxx_b);  // sigma_xx(i-1, b)
+    const T sxz_e = dpp_next(sxz_b);  // sigma_xz(i+1, b)
     const bool rv = valid(b);
     const int rs = crow(b);
     const bool inzv = PML && inzs(b);
@@ -3638,7 +3652,12 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
   if (kind == SDT_ELASTIC2) {
     rc = upload_work2(p);
     if (rc) return bail(rc);
-    p->spl = 2;
+    // Two steps per launch pay off once the plan has enough tiles to fill the
+    // device; a small plan is bound by the latency of one tile's walk, which
+    // the two-step kernel lengthens (measured: 4M cells per step and above).
+    long long cells = 0;
+    for (int s = 0; s < nshots; ++s) cells += (long long)models[shots[s].model].nx * models[shots[s].model].nz;
+    p->spl = cells >= SDT_EL2X2_MIN_CELLS ? 2 : 1;
   }
   if (hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(SDT_EDEVICE, "hipStreamCreate failed"));

@@ -201,7 +201,8 @@ class Simulation:
     @property
     def steps_per_launch(self) -> int:
         """Time steps one launch of the ELASTIC2 step kernel advances (2 by
-        default for ELASTIC2, else 1); results do not depend on it."""
+        default for ELASTIC2 plans of at least 4M cells per step, else 1;
+        ``autotune`` times both); results do not depend on it."""
         v = C.c_int()
         _hip.check(_hip.lib().sdt_plan_get_steps_per_launch(self._plan, C.byref(v)))
         return v.value
@@ -226,40 +227,53 @@ class Simulation:
 
     def autotune(self, steps: int = 8, candidates=None):
         """Pick the tile shape of the single-pass step kernel (ELASTIC2, EM2)
-        that runs this plan's step fastest.
+        and, for ELASTIC2, the steps per launch (2 or 1) that run this plan's
+        steps fastest.
 
-        Each candidate runs ``steps`` steps (timed by HIP events around the
-        step kernel) from the current state; the state, traces and step
+        Each candidate runs ``steps`` steps as one hipGraph block (captured
+        once, then replayed and timed with HIP events on the stream, receiver
+        gather included) from the current state; the state, traces and step
         counter are restored afterwards, so the run continues as if nothing
-        had happened.  Returns {(wb, rows): ms per step} and keeps the best
-        (``steps`` is rounded up to whole two-step launches where those run).
-        Other kinds have a single shape: returns {}."""
+        had happened.  Returns {(steps_per_launch, wb, rows): ms per step}
+        and keeps the best (``steps`` is rounded up to a multiple of 4, so
+        both modes end a block with the state in the same buffer).  Other
+        kinds have a single shape: returns {}."""
         if self.kind not in PINGPONG:
             return {}
-        steps = max(1, min(int(steps), self.nt - self.step))
-        spl = self.steps_per_launch
-        if spl == 2 and steps % 2 and steps + 1 <= self.nt - self.step:
-            steps += 1
-        per = spl if steps >= 2 else 1      # steps per timed launch
-        warm = min(spl, self.nt - self.step)
+        left = self.nt - self.step
+        steps = min(-(-max(int(steps), 4) // 4) * 4, left)
+        if steps < 1:
+            return {}
+        spls = (2, 1) if self.kind == "elastic2" else (1,)
         keep = [(b["state"].clone(), b["state_b"].clone(), b["trace"].clone()) for _, _, b in self.shots]
         step0 = self.step
+        stream = torch.cuda.current_stream(self.device)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+        def restore(traces=False):
+            self.step = step0
+            for (_, _, b), (a0, b0, t0) in zip(self.shots, keep):
+                b["state"].copy_(a0)
+                b["state_b"].copy_(b0)
+                if traces:
+                    b["trace"].copy_(t0)
+
         res = {}
-        for wb, rows in (candidates or self.TILE_CANDIDATES):
-            self.set_tiles(wb, rows)
-            self.profile(warm)                    # first launch of this shape
-            self.step = step0
-            for (_, _, b), (a0, b0, t0) in zip(self.shots, keep):
-                b["state"].copy_(a0)
-                b["state_b"].copy_(b0)
-            res[(wb, rows)] = self.profile(steps)[0] / per
-            self.step = step0
-            for (_, _, b), (a0, b0, t0) in zip(self.shots, keep):
-                b["state"].copy_(a0)
-                b["state_b"].copy_(b0)
-                b["trace"].copy_(t0)
+        for spl in spls:
+            self.steps_per_launch = spl
+            for wb, rows in (candidates or self.TILE_CANDIDATES):
+                self.set_tiles(wb, rows)
+                self.run(steps, graph_block=steps)    # capture this shape
+                restore()
+                ev0.record(stream)
+                self.run(steps, graph_block=steps)
+                ev1.record(stream)
+                ev1.synchronize()
+                res[(spl, wb, rows)] = ev0.elapsed_time(ev1) / steps
+                restore(traces=True)
         best = min(res, key=res.get)
-        self.set_tiles(*best)
+        self.steps_per_launch = best[0]
+        self.set_tiles(*best[1:])
         torch.cuda.synchronize(self.device)
         return res
 

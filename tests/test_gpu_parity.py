@@ -277,7 +277,7 @@ def test_tile_shapes_and_autotune_keep_results(oracle_lib):
     s = _sim(probs, "fp32")
     s.run(7)
     res = s.autotune(steps=4)
-    assert len(res) == 18 and s.tiles() in res
+    assert len(res) == 36 and (s.steps_per_launch, *s.tiles()) in res
     got = s.run(graph_block=10).check().traces()
     for a, b in zip(ref, got):
         np.testing.assert_array_equal(a, b)
