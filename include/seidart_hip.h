@@ -175,9 +175,9 @@ int sdt_plan_run(void* plan, int it0, int nsteps, int graph_block, void* stream)
 int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows);
 int sdt_plan_get_tiles(void* plan, int* waves_per_block, int* rows);
 
-/* ELASTIC2 steps per launch of the step kernel (no counterpart in the
- * reference): 2 (the default for ELASTIC2 plans of at least 4M cells per
- * step, summed over shots; smaller plans default to 1) advances two time steps in
+/* ELASTIC2 / EM2 steps per launch of the step kernel (no counterpart in the
+ * reference): 2 (the default for plans of at least 4M cells per step,
+ * summed over shots; smaller plans default to 1) advances two time steps in
  * one launch, keeping the state of the first in registers, so the state
  * crosses HBM once per two steps; 1 is the one-step kernel.  Results are
  * identical bit for bit.  A run of an odd number of steps ends with one

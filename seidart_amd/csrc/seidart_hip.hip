@@ -86,7 +86,7 @@ struct DShot {
   void* trace;
   int* blowup;
   const void* msrc[6];  // stress increments xx yy zz xy xz yz (elastic)
-  // receivers by row (ELASTIC2): those on row k are rxid[rrow[k] .. rrow[k+1]),
+  // receivers by row (ELASTIC2, EM2): those on row k are rxid[rrow[k] .. rrow[k+1]),
   // each a pair (column, receiver index), columns ascending
   const int* rrow;
   const int* rxid;
@@ -1830,7 +1830,7 @@ template <typename T, bool TRAIL, bool PML>
 __device__ __forceinline__ void em2_tile(const DModel* __restrict__ models,
                                          const DShot* __restrict__ shots, const int4 wi, const int it,
                                          const int lane, const DAdj* __restrict__ adj, const int slot,
-                                         const int tb, const int eh) {
+                                         const int tb, const int eh, const int par) {
 #pragma clang fp contract(off)
   const DShot& S = shots[wi.x];
   const DModel& M = models[S.model];
@@ -1841,7 +1841,6 @@ __device__ __forceinline__ void em2_tile(const DModel* __restrict__ models,
   const int ic = incol ? i : (i < 0 ? 0 : nx - 1);  // clamped column: every load stays in bounds
   const bool own = incol && lane >= 1 && lane <= EW;
   const int zend = min(zo + eh, nz);
-  const int par = it & 1;
   const long nn = (long)nx * nz;
   const T* tin = nullptr;
   T* tout = nullptr;
@@ -1953,14 +1952,17 @@ __device__ __forceinline__ void em2_tile(const DModel* __restrict__ models,
   }
 }
 
-// Same work list and launch shape as k_el2_step.
+// Same work list and launch shape as k_el2_step, and the same buffer
+// choice (prun: the single step that ends a run of two-step launches).
 template <typename T, bool TRAIL, int WB>
 __global__ void __launch_bounds__(64 * WB)
 k_em2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
            const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel,
-           const DAdj* __restrict__ adj, const int* __restrict__ tsel, int eh) {
+           const DAdj* __restrict__ adj, const int* __restrict__ tsel, int eh,
+           const int* __restrict__ prun) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int it = __builtin_amdgcn_readfirstlane(*base + rel);
+  const int par = prun ? __builtin_amdgcn_readfirstlane(((it - *prun) >> 1) & 1) : (it & 1);
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
     const int4 wv = work[w];
     const int4 wi = make_int4(__builtin_amdgcn_readfirstlane(wv.x),
@@ -1974,9 +1976,266 @@ k_em2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
                           wi.z + eh <= M.nz - M.lz;
     const int tb = TRAIL ? __builtin_amdgcn_readfirstlane(*tsel) : 0;
     if (interior)
-      em2_tile<T, TRAIL, false>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh);
+      em2_tile<T, TRAIL, false>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh, par);
     else
-      em2_tile<T, TRAIL, true>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh);
+      em2_tile<T, TRAIL, true>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh, par);
+  }
+}
+
+// ===========================================================================
+// EM2, two time steps per launch: the walk of k_em2_step with the second
+// step one row behind the first.  At row r a wavefront computes
+//   A  Hy of step it at row r        (E of it-1, rows r, r+1)
+//   B  Ex, Ez of step it at row r     (Hy from A, rows r-1, r)
+//   C  Hy of step it+1 at row r-1     (E from B, rows r-1, r)
+//   D  Ex, Ez of step it+1 at row r-1 (Hy from C, rows r-2, r-1)
+// so the state of step it lives only in registers.  One step makes a cell
+// depend on columns i-1 .. i+1 and rows k-1 .. k+1 of the previous state,
+// so lane l holds column xo - 2 + l, lanes 2 .. 2+EW2E-1 are exact after two
+// steps, and rows zo-2 .. zo+eh read produce rows zo .. zo+eh-1.  The
+// expressions are those of k_em2_step in the same order (identical bits),
+// the Hy zeroing that ends step it is applied where step it+1 reads it, and
+// the receiver samples of both steps are written here.  Buffers as in
+// k_el2_step2.
+// ===========================================================================
+constexpr int EW2E = 60;
+
+template <typename T, bool PML>
+__device__ __forceinline__ void em2x2_tile(const DModel* __restrict__ models,
+                                           const DShot* __restrict__ shots, const int4 wi, const int it,
+                                           const int q, const int lane, const int eh) {
+#pragma clang fp contract(off)
+  // interior waves read only columns 1..nx-3 and rows 1..nz-3 outside the
+  // C-PML slabs: every edge condition holds on every lane
+  constexpr bool INT = !PML;
+  const DShot& S = shots[wi.x];
+  const DModel& M = models[S.model];
+  const int nx = M.nx, nz = M.nz;
+  const int xo = wi.y, zo = wi.z;
+  const int i = xo - 2 + lane;
+  const bool incol = INT || (i >= 0 && i < nx);
+  const int ic = INT ? i : (incol ? i : (i < 0 ? 0 : nx - 1));  // clamped column
+  const bool own = incol && lane >= 2 && lane < 2 + EW2E;
+  const int ze = min(zo + eh, nz);
+  const A32<const T> exi(q ? S.g[0] : S.f[0]);
+  const A32<const T> ezi(q ? S.g[1] : S.f[1]);
+  const A32<const T> hyi(q ? S.g[2] : S.f[2]);
+  const A32<T> exo(q ? S.f[0] : S.g[0]);
+  const A32<T> ezo(q ? S.f[1] : S.g[1]);
+  const A32<T> hyo(q ? S.f[2] : S.g[2]);
+  const A32<const T> pin(q ? S.psi2 : S.psi);
+  const A32<T> pout(q ? S.psi : S.psi2);
+  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dbh = T(M.scal[0]);
+  const int lx = M.lx, lz = M.lz;
+  const uint32_t SX = (uint32_t)nz * 2 * lx, SZ = (uint32_t)2 * lz * nx;
+  const bool inx = PML && incol && in_slab(i, nx, lx);
+  const int xs = inx ? slab_pos(i, nx, lx) : 0;
+  const Prof<T> P(&M.prof[0][0]);
+  const T ikx = P[PK][ic], ax = P[PA][ic], bx = P[PB][ic];
+  const T ikxh = P[PKH][ic], axh = P[PAH][ic], bxh = P[PBH][ic];
+  const auto ikz = kp<T>(M.prof[2][PK]), az = kp<T>(M.prof[2][PA]), bz = kp<T>(M.prof[2][PB]);
+  const auto ikzh = kp<T>(M.prof[2][PKH]), azh = kp<T>(M.prof[2][PAH]), bzh = kp<T>(M.prof[2][PBH]);
+  const A32<const T> CAX(M.coef[0]), CBX(M.coef[1]), CAZ(M.coef[2]), CBZ(M.coef[3]);
+  const bool hcol = INT || (incol && i <= nx - 2);              // Hy, Ex columns
+  const bool ezcol = INT || (incol && i >= 1 && i <= nx - 2);   // Ez columns
+  const bool is_src_col = (i == S.isrc);
+  const int ksrc = S.ksrc;
+  const auto srcx = cp<T>(S.src[0]), srcz = cp<T>(S.src[2]);
+  const T incx0 = srcx ? srcx[it] : T(0), incx1 = srcx ? srcx[it + 1] : T(0);
+  const T incz0 = srcz ? srcz[it] : T(0), incz1 = srcz ? srcz[it + 1] : T(0);
+
+  auto crow = [&](int r) { return r < 0 ? 0 : (r >= nz ? nz - 1 : r); };
+  auto cell = [&](int r) { return (uint32_t)crow(r) * nx + ic; };
+  auto valid = [&](int r) { return INT || (incol && r >= 0 && r < nz); };
+  auto inzs = [&](int r) { return PML && in_slab(crow(r), nz, lz); };
+
+  struct MX { T ezx, hyx, exz, hyz; };  // memory: dEz/dx, dHy/dx (x slab); dEx/dz, dHy/dz (z slab)
+  struct CE { T cax, cbx, caz, cbz; };
+  auto offx = [&](int r) { return inx ? (uint32_t)crow(r) * 2 * lx + xs : 0u; };
+  auto offz = [&](int r) { return 2 * SX + (inzs(r) ? (uint32_t)slab_pos(crow(r), nz, lz) * nx + ic : 0u); };
+  auto load_m = [&](int r) {
+    MX m{};
+    if constexpr (PML) {
+      const uint32_t ox = offx(r), oz = offz(r);
+      const bool inzc = incol && inzs(r);
+      m.ezx = inx ? pin[ox] : T(0);
+      m.hyx = inx ? pin[SX + ox] : T(0);
+      m.exz = inzc ? pin[oz] : T(0);
+      m.hyz = inzc ? pin[SZ + oz] : T(0);
+    }
+    return m;
+  };
+  auto store_m = [&](int r, const MX& m) {  // own cells only
+    if constexpr (PML) {
+      const uint32_t ox = offx(r), oz = offz(r);
+      if (inx) { pout[ox] = m.ezx; pout[SX + ox] = m.hyx; }
+      if (inzs(r)) { pout[oz] = m.exz; pout[SZ + oz] = m.hyz; }
+    }
+  };
+  // Hy of one step at row r (electromagfdtd.f95:249-266): the previous step's
+  // Ex of rows r, r+1, Ez of row r (columns i, i+1) and Hy of row r
+  auto hfield = [&](int r, T ex_c, T ex_n, T ez_c, T hy_o, MX& m) {
+    const T ez_e = dpp_next(ez_c);  // Ez(i+1, r)
+    const bool rv = valid(r);
+    const int rs = crow(r);
+    const bool inz = PML && in_slab(rs, nz, lz);
+    T hy = hy_o;
+    if (INT || (rv && hcol && r <= nz - 2)) {
+      T a = (ex_n - ex_c) * idz;
+      if (PML && inz) {
+        m.exz = fma(bz[rs], m.exz, az[rs] * a);
+        a = fma(a, ikz[rs], m.exz);
+      }
+      T b = (ez_e - ez_c) * idx;
+      if (PML && inx) {
+        m.ezx = fma(bx, m.ezx, ax * b);
+        b = fma(b, ikx, m.ezx);
+      }
+      const bool edge = !INT && (i == 0 || i == nx - 2 || r == 0 || r == nz - 2);  // :312-315
+      hy = fma(dbh, b + a, edge ? T(0) : hy_o);
+    }
+    return hy;
+  };
+  // Ex, Ez of one step at row r (:273-295), source, Dirichlet: the new Hy of
+  // rows r-1, r (column i-1 for Ez)
+  auto efield = [&](int r, T hy, T hy_p, T ex_c, T ez_c, const CE& k, MX& m, T ix, T iz, T& ex, T& ez) {
+    const T hy_w = dpp_prev(hy);  // new Hy(i-1, r)
+    const bool rv = valid(r);
+    const int rs = crow(r);
+    const bool inz = PML && in_slab(rs, nz, lz);
+    ex = ex_c;
+    ez = ez_c;
+    if (INT || (rv && hcol && r >= 1 && r <= nz - 2)) {
+      T d = (hy - hy_p) * idz;
+      if (PML && inz) {
+        m.hyz = fma(bzh[rs], m.hyz, azh[rs] * d);
+        d = fma(d, ikzh[rs], m.hyz);
+      }
+      ex = fma(k.cax, ex, k.cbx * d);
+    }
+    if (INT || (rv && ezcol && r <= nz - 2)) {
+      T d = (hy - hy_w) * idx;
+      if (PML && inx) {
+        m.hyx = fma(bxh, m.hyx, axh * d);
+        d = fma(d, ikxh, m.hyx);
+      }
+      ez = fma(k.caz, ez, k.cbz * d);
+    }
+    if (r == ksrc && is_src_col) { ex += ix; ez += iz; }
+    if (!INT) {
+      if (i == 0 || i == nx - 2 || r == 0 || r == nz - 1) ex = 0;
+      if (i == 0 || i == nx - 1 || r == 0 || r == nz - 2) ez = 0;
+      if (i == nx - 1) ex = 0;  // outside the (nx-1) x nz Ex array
+      if (r == nz - 1) ez = 0;  // outside the nx x (nz-1) Ez array
+      if (!rv) { ex = 0; ez = 0; }
+    }
+  };
+  const int nrec = S.nrec;
+  auto gather = [&](int b, int itn, T ex, T ez) {
+    const int q0 = S.rrow[b], q1 = S.rrow[b + 1];
+    auto tr = mp<T>(S.trace) + (long)itn * 2 * nrec;
+    for (int e = q0; e < q1; ++e) {
+      const int rx = S.rxid[2 * e], id = S.rxid[2 * e + 1];
+      if (own && i == rx) {
+        tr[id] = ex;
+        tr[nrec + id] = ez;
+      }
+    }
+  };
+
+  T ex0_c = valid(zo - 2) ? exi[cell(zo - 2)] : T(0);  // Ex of step it-1, row r
+  T hy1_p = 0;                                          // Hy of step it, row r-1
+  T ex1_p = 0, ez1_p = 0;                               // E of step it, row r-1
+  T hy2_p = 0;                                          // Hy of step it+1, row r-2
+  CE k_p{};                                             // coefficients of row r-1
+  MX m_p{};                                             // memory of row r-1 after step it
+  const bool has_rec = nrec > 0;
+  bool bad0 = false, bad1 = false;
+
+  for (int r = zo - 2; r <= ze; ++r) {
+    const bool actB = r >= zo - 1, actC = r >= zo, actD = r >= zo + 1;
+    const uint32_t c = cell(r);
+    const T ld_exn = exi[cell(r + 1)], ld_ez = ezi[c], ld_hy = hyi[c];
+    CE k;
+    k.cax = CAX[c];
+    k.cbx = CBX[c];
+    k.caz = CAZ[c];
+    k.cbz = CBZ[c];
+    MX m = load_m(r);
+    const bool rv = valid(r);
+    const T ex0_n = valid(r + 1) ? ld_exn : T(0);
+    const T ez0_c = rv ? ld_ez : T(0);
+    const T hy0 = rv ? ld_hy : T(0);
+
+    // A, B: step it at row r
+    const T hy1_c = hfield(r, ex0_c, ex0_n, ez0_c, hy0, m);
+    T ex1_c = 0, ez1_c = 0;
+    if (actB) {
+      efield(r, hy1_c, hy1_p, ex0_c, ez0_c, k, m, incx0, incz0, ex1_c, ez1_c);
+      if (r >= zo && r < ze) {
+        bad0 |= own && !((double)ex1_c * ex1_c + (double)ez1_c * ez1_c <= 1.0e50);
+        if (has_rec) gather(r, it, ex1_c, ez1_c);
+      }
+    }
+    // C, D: step it+1 at row r-1
+    T hy2_c = 0;
+    if (actC) {
+      MX mc = m_p;
+      hy2_c = hfield(r - 1, ex1_p, ex1_c, ez1_p, hy1_p, mc);
+      if (actD && r - 1 < ze) {
+        T ex2, ez2;
+        efield(r - 1, hy2_c, hy2_p, ex1_p, ez1_p, k_p, mc, incx1, incz1, ex2, ez2);
+        bad1 |= own && !((double)ex2 * ex2 + (double)ez2 * ez2 <= 1.0e50);
+        if (own) {
+          const uint32_t co = cell(r - 1);
+          exo[co] = ex2;
+          ezo[co] = ez2;
+          hyo[co] = hy2_c;
+          store_m(r - 1, mc);
+        }
+        if (has_rec) gather(r - 1, it + 1, ex2, ez2);
+      }
+    }
+    ex0_c = ex0_n;
+    hy1_p = hy1_c;
+    ex1_p = ex1_c;
+    ez1_p = ez1_c;
+    hy2_p = hy2_c;
+    k_p = k;
+    m_p = m;
+  }
+  if (bad0)
+    check_blowup(HUGE_VAL, S.blowup, it);
+  else if (bad1)
+    check_blowup(HUGE_VAL, S.blowup, it + 1);
+}
+
+// Work: as k_em2_step, with EW2E owned columns per wavefront.
+template <typename T, int WB>
+__global__ void __launch_bounds__(64 * WB)
+k_em2_step2(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+            const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel,
+            const int* __restrict__ it0, int eh) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int it = __builtin_amdgcn_readfirstlane(*base + rel);
+  const int q = __builtin_amdgcn_readfirstlane(((it - *it0) >> 1) & 1);
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const int4 wv = work[w];
+    const int4 wi = make_int4(__builtin_amdgcn_readfirstlane(wv.x),
+                              __builtin_amdgcn_readfirstlane(wv.y) + wave * EW2E,
+                              __builtin_amdgcn_readfirstlane(wv.z), 0);
+    if (wi.x < 0) continue;
+    const DModel& M = models[shots[wi.x].model];
+    if (wi.y >= M.nx) continue;
+    // every column (lanes) and row (zo-2 .. ze+1) the wave reads outside the
+    // C-PML slabs and clear of the columns and rows the edge rules name
+    const int ze = min(wi.z + eh, M.nz);
+    const bool interior = wi.y - 2 >= M.lx + 1 && wi.y + 61 <= M.nx - M.lx - 3 &&
+                          wi.z - 2 >= M.lz + 1 && ze + 1 <= M.nz - M.lz - 3;
+    if (interior)
+      em2x2_tile<T, false>(models, shots, wi, it, q, threadIdx.x & 63, eh);
+    else
+      em2x2_tile<T, true>(models, shots, wi, it, q, threadIdx.x & 63, eh);
   }
 }
 
@@ -3022,21 +3281,43 @@ void launch_el2x2(Plan* p, hipStream_t st, int rel) {
 }
 
 template <typename T, bool TRAIL>
-void launch_em2(Plan* p, hipStream_t st, int rel, const int* tsel) {
+void launch_em2(Plan* p, hipStream_t st, int rel, const int* tsel, const int* prun = nullptr) {
   const DAdj* adj = TRAIL ? p->dadj : nullptr;
   const dim3 g(p->el2_grid);
   switch (p->el2_wb) {
     case 1:
       hipLaunchKernelGGL((k_em2_step<T, TRAIL, 1>), g, dim3(64), 0, st, p->dmodels, p->dshots,
-                         p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh);
+                         p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh, prun);
       break;
     case 2:
       hipLaunchKernelGGL((k_em2_step<T, TRAIL, 2>), g, dim3(128), 0, st, p->dmodels, p->dshots,
-                         p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh);
+                         p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh, prun);
       break;
     default:
       hipLaunchKernelGGL((k_em2_step<T, TRAIL, 4>), g, dim3(256), 0, st, p->dmodels, p->dshots,
-                         p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh);
+                         p->dwork, p->nwork, p->dbase, rel, adj, tsel, p->el2_eh, prun);
+      break;
+  }
+}
+
+// One two-step EM2 launch (steps base+rel, base+rel+1).
+template <typename T>
+void launch_em2x2(Plan* p, hipStream_t st, int rel) {
+  const dim3 g(p->el2x2_grid);
+  const int4* w = p->dwork2;
+  const int n = p->nwork2;
+  switch (p->el2_wb) {
+    case 1:
+      hipLaunchKernelGGL((k_em2_step2<T, 1>), g, dim3(64), 0, st, p->dmodels, p->dshots, w, n, p->dbase,
+                         rel, p->dit0, p->el2_eh);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_em2_step2<T, 2>), g, dim3(128), 0, st, p->dmodels, p->dshots, w, n, p->dbase,
+                         rel, p->dit0, p->el2_eh);
+      break;
+    default:
+      hipLaunchKernelGGL((k_em2_step2<T, 4>), g, dim3(256), 0, st, p->dmodels, p->dshots, w, n, p->dbase,
+                         rel, p->dit0, p->el2_eh);
       break;
   }
 }
@@ -3194,14 +3475,19 @@ void launch_steps(Plan* p, hipStream_t st, int n) {
 // left the state in).
 template <typename T>
 void launch_two(Plan* p, hipStream_t st, int r, bool single) {
+  const bool em = p->kind == SDT_EM2;
   if (!single) {
-    if (p->split)
+    if (em)
+      launch_em2x2<T>(p, st, r);
+    else if (p->split)
       launch_el2x2<T, true>(p, st, r);
     else
       launch_el2x2<T, false>(p, st, r);
     return;
   }
-  if (p->split)
+  if (em)
+    launch_em2<T, false>(p, st, r, nullptr, p->dit0);
+  else if (p->split)
     launch_el2<T, false, true>(p, st, r, nullptr, p->dit0);
   else
     launch_el2<T, false, false>(p, st, r, nullptr, p->dit0);
@@ -3426,9 +3712,11 @@ void drop_adjoint_graphs(Plan* p) {
 // (Re)build the work list of the two-step ELASTIC2 kernel for the plan's
 // tile shape.
 int upload_work2(Plan* p) {
-  std::vector<int4> work = el2_work(p, EW2, 1);
+  // ELASTIC2: interior blocks first, then the others; EM2: one list
+  const bool em = p->kind == SDT_EM2;
+  std::vector<int4> work = em ? std::vector<int4>() : el2_work(p, EW2, 1);
   const int ni = (int)work.size();
-  const std::vector<int4> edge = el2_work(p, EW2, 2);
+  const std::vector<int4> edge = em ? el2_work(p, EW2E, 0) : el2_work(p, EW2, 2);
   work.insert(work.end(), edge.begin(), edge.end());
   if (work.empty()) work.push_back(make_int4(-1, 0, 0, 0));
   int4* dw = nullptr;
@@ -3526,11 +3814,11 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
     }
     p->maxrec = std::max(p->maxrec, S.nrec);
   }
-  // receivers by row (two-step ELASTIC2 kernel): per shot rrow[nz + 1] then
+  // receivers by row (two-step kernels): per shot rrow[nz + 1] then
   // (column, index) pairs sorted by (row, column)
   std::vector<int> rows_cols;
   std::vector<long> rec_off(nshots, -1);
-  if (kind == SDT_ELASTIC2) {
+  if (kind == SDT_ELASTIC2 || kind == SDT_EM2) {
     for (int s = 0; s < nshots; ++s) {
       const sdt_shot& S = shots[s];
       if (S.nrec <= 0) continue;
@@ -3649,7 +3937,7 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
                      nshots, INT_MAX);
   hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(1), 0, nullptr, p->dbase, 0, p->dit0, 0);
   if (hipDeviceSynchronize() != hipSuccess) return bail(fail(SDT_EDEVICE, "init kernels failed"));
-  if (kind == SDT_ELASTIC2) {
+  if (kind == SDT_ELASTIC2 || kind == SDT_EM2) {
     rc = upload_work2(p);
     if (rc) return bail(rc);
     // Two steps per launch pay off once the plan has enough tiles to fill the
@@ -3828,8 +4116,8 @@ int sdt_plan_set_steps_per_launch(void* plan, int steps) {
   Plan* p = static_cast<Plan*>(plan);
   if (!p) return fail(SDT_EINVAL, "plan is null");
   if (steps != 1 && steps != 2) return fail(SDT_EINVAL, "steps per launch must be 1 or 2");
-  if (steps == 2 && p->kind != SDT_ELASTIC2)
-    return fail(SDT_EINVAL, "two steps per launch apply to ELASTIC2 plans");
+  if (steps == 2 && p->kind != SDT_ELASTIC2 && p->kind != SDT_EM2)
+    return fail(SDT_EINVAL, "two steps per launch apply to ELASTIC2 and EM2 plans");
   p->spl = steps;
   return SDT_OK;
 }

@@ -200,8 +200,8 @@ class Simulation:
 
     @property
     def steps_per_launch(self) -> int:
-        """Time steps one launch of the ELASTIC2 step kernel advances (2 by
-        default for ELASTIC2 plans of at least 4M cells per step, else 1;
+        """Time steps one launch of the ELASTIC2 / EM2 step kernel advances
+        (2 by default for plans of at least 4M cells per step, else 1;
         ``autotune`` times both); results do not depend on it."""
         v = C.c_int()
         _hip.check(_hip.lib().sdt_plan_get_steps_per_launch(self._plan, C.byref(v)))
@@ -227,8 +227,7 @@ class Simulation:
 
     def autotune(self, steps: int = 8, candidates=None):
         """Pick the tile shape of the single-pass step kernel (ELASTIC2, EM2)
-        and, for ELASTIC2, the steps per launch (2 or 1) that run this plan's
-        steps fastest.
+        and the steps per launch (2 or 1) that run this plan's steps fastest.
 
         Each candidate runs ``steps`` steps as one hipGraph block (captured
         once, then replayed and timed with HIP events on the stream, receiver
@@ -244,7 +243,7 @@ class Simulation:
         steps = min(-(-max(int(steps), 4) // 4) * 4, left)
         if steps < 1:
             return {}
-        spls = (2, 1) if self.kind == "elastic2" else (1,)
+        spls = (2, 1)
         keep = [(b["state"].clone(), b["state_b"].clone(), b["trace"].clone()) for _, _, b in self.shots]
         step0 = self.step
         stream = torch.cuda.current_stream(self.device)

@@ -1,4 +1,4 @@
-"""Two time steps per launch (ELASTIC2, the default): the state of every shot
+"""Two time steps per launch (ELASTIC2 and EM2): the state of every shot
 (fields and C-PML memory) and every receiver sample are bit-identical to one
 step per launch, for both precisions, both C-PML variants, every tile shape,
 graph blocks, odd run lengths and runs that start at odd steps; and the FP64
@@ -125,3 +125,47 @@ def test_two_step_reports_the_same_unstable_step():
             s.check()
         steps.append(list(s.unstable))
     assert steps[0] == steps[1] and min(steps[0]) >= 0
+
+
+def _em_problems(nt, seed=400):
+    d = [dict(nx=150, nz=75, npml=6), dict(nx=61, nz=37, npml=5), dict(nx=40, nz=130, npml=7)]
+    probs = [tiny_problem("em2", seed=seed + m, nt=nt, nshots=2, nrec=9, **kw)
+             for m, kw in enumerate(d)]
+    for p in probs:
+        p.shots[0].rec = np.concatenate([p.shots[0].rec, np.array(
+            [[0, 0, 0], [p.nx - 1, 0, p.nz - 1], [p.nx - 2, 0, p.nz - 2], [1, 0, 1]], dtype=np.int32)])
+    return probs
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_em2_two_step_equals_one_step(precision):
+    probs = _em_problems(61)
+    probs[1].shots[1].src = (probs[1].nx - 3, 0, 2)   # source in the C-PML corner
+    ref = _snap(_sim(probs, precision, 1).run(graph_block=10).check())
+    for gb, tiles in ((10, None), (0, (1, 2)), (6, (2, 16)), (4, (4, 64))):
+        s = _sim(probs, precision, 2)
+        if tiles:
+            s.set_tiles(*tiles)
+        _assert_same(ref, _snap(s.run(graph_block=gb).check()))
+
+
+def test_em2_two_step_segments_and_oracle(oracle_lib):
+    probs = _em_problems(47, seed=410)
+    ref = _snap(_sim(probs, "fp64", 1).run().check())
+    s = _sim(probs, "fp64", 2)
+    for n, gb in ((1, 0), (3, 0), (6, 4), (5, 5), (9, 2)):
+        s.run(n, graph_block=gb)
+    s.steps_per_launch = 1
+    s.run(7)
+    s.steps_per_launch = 2
+    s.run(graph_block=4).check()
+    _assert_same(ref, _snap(s))
+    p = probs[0]
+    tr = s.traces()
+    for k in range(len(p.shots)):
+        o = oracle_lib.OracleRun(p, shot=k).run()
+        assert rel_l2(tr[k], o.trace) < FP64_TOL
+        for n in o.names:
+            exp = o.field(n)
+            if np.abs(exp).max() > 0:
+                assert rel_l2(s.field(k, n), exp) < FP64_TOL, n
