@@ -241,13 +241,14 @@ def config_lines(steps: int = 400):
     cases = [("plain_psv", lambda nt: configs.plain_psv(nt=nt), 2000),
              ("gpr", lambda nt: configs.gpr(nt=nt), steps),
              ("aniso25", lambda nt: configs.aniso25(nt=nt), min(steps, 200)),
-             ("gpr25", lambda nt: configs.gpr25(nt=nt), min(steps, 200))]
+             ("gpr25", lambda nt: configs.gpr25(nt=nt), min(steps, 200)),
+             ("gpr_survey", lambda nt: configs.gpr_survey(nt=nt), min(steps, 200))]
     for name, make, K in cases:
         W = 50
         p = make(W + K)
         cells = p.nx * p.ny * p.nz * len(p.shots)
         rec = {"grid": f"{p.nx}x{p.ny}x{p.nz}" if p.is3d else f"{p.nx}x{p.nz}", "kind": p.kind,
-               "steps": K}
+               "shots": len(p.shots), "steps": K}
         traces = {}
         for prec, size in (("fp32", 4), ("fp64", 8)):
             sim = Simulation(p, precision=prec)

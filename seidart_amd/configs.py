@@ -393,7 +393,21 @@ def aniso25(nt: int = 3000, seed: int = 1003, nx0=800, ny0=64, nz0=400, cpml=16)
                    gamma={"x": gpad, "y": gpad, "z": gpad})
 
 
+
+def gpr_survey(nt: int = 1000, nshots: int = 32, seed: int = 1002) -> Problem:
+    """The 2-D GPR model as a common-offset survey: `nshots` source positions
+    along z = 10 (every 30 cells from x = 20), the same 128 surface
+    receivers for each.  Not a configuration of the reference benchmark: the
+    multi-shot EM line (what `CommonOffset.co_run` runs as one plan)."""
+    p = gpr(nt=nt, seed=seed)
+    sh = p.shots[0]
+    p.shots = [Shot(src=(20 + 30 * k + 20, 0, sh.src[2]), force=sh.force, rec=sh.rec)
+               for k in range(nshots)]
+    return p
+
+
 CONFIGS = {
+    "gpr_survey": gpr_survey,
     "gpr25": gpr25,
     "plain_psv": plain_psv,
     "gpr": gpr,
