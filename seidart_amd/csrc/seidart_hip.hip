@@ -148,6 +148,32 @@ struct Prof {
   __device__ gcptr<T> operator[](int q) const { return cp<T>(p[q]); }
 };
 
+// Receiver samples of step itn at row b of one shot (2-D kinds), called by
+// every lane of a wavefront that owns lanes lo..hi of row b (lane l holds
+// column col0 + l; a, c: the two components on each lane).  The receivers
+// of the row (DShot::rrow / rxid) are taken 64 at a time, one per lane,
+// which fetches its value from the lane that holds its column and writes it
+// if this wavefront owns that column; trace layout [it][comp][rec].
+template <typename T>
+__device__ __forceinline__ void gather_row(const DShot& S, int b, int itn, int lane, int col0, int lo,
+                                           int hi, T a, T c) {
+  const int q0 = S.rrow[b], q1 = S.rrow[b + 1];
+  const int nrec = S.nrec;
+  auto tr = mp<T>(S.trace) + (long)itn * 2 * nrec;
+  for (int e0 = q0; e0 < q1; e0 += 64) {
+    const int e = e0 + lane;
+    const bool has = e < q1;
+    const int rx = has ? S.rxid[2 * e] : col0 - 1, id = has ? S.rxid[2 * e + 1] : 0;
+    const int src = rx - col0;  // lane holding column rx
+    const bool mine = has && src >= lo && src <= hi;
+    const T va = __shfl(a, src & 63, 64), vc = __shfl(c, src & 63, 64);
+    if (mine) {
+      tr[id] = va;
+      tr[nrec + id] = vc;
+    }
+  }
+}
+
 // Flag instability: |v|^2 > 1e50 or non-finite (seismicfdtd.f95:378-379).
 __device__ __forceinline__ void check_blowup(double v2, int* flag, int it) {
   if (!(v2 <= 1.0e50)) atomicMin(flag, it);
@@ -543,6 +569,7 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
     // ---- velocities of row rr = r - 1 (seismicfdtd.f95:317-375) ----
     const T sxx_w = lane_prev(sxx_p);  // sigma_xx(i-1, rr)
     const T sxz_e = lane_next(sxz_p);  // sigma_xz(i+1, rr)
+    T gvx = 0, gvz = 0;  // new velocities of the owned cells (receiver samples)
     if (own && rr >= zo && rr < zend) {
       T vxn = vx_m, vzn = vz_m;
       if (ux && rr >= 1) {
@@ -578,7 +605,10 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
       check_blowup((double)vxn * vxn + (double)vzn * vzn, S.blowup, it);
       vxo[cv] = vxn;
       vzo[cv] = vzn;
+      gvx = vxn;
+      gvz = vzn;
     }
+    if (!TRAIL && S.nrec > 0 && rr >= zo && rr < zend) gather_row<T>(S, rr, it, lane, xo - 1, 1, EW, gvx, gvz);
     sxz_pp = sxz_p;
     sxx_p = sxx_c;
     szz_p = szz_c;
@@ -897,19 +927,9 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
       vzn = 0;
     }
   };
-  // receiver samples of step itn at row b (own cells of an own row)
+  // receiver samples of step itn at row b (an own row; every lane calls)
   const int nrec = S.nrec;
-  auto gather = [&](int b, int itn, T vx, T vz) {
-    const int q0 = S.rrow[b], q1 = S.rrow[b + 1];
-    auto tr = mp<T>(S.trace) + (long)itn * 2 * nrec;
-    for (int e = q0; e < q1; ++e) {
-      const int rx = S.rxid[2 * e], id = S.rxid[2 * e + 1];
-      if (own && i == rx) {
-        tr[id] = vx;
-        tr[nrec + id] = vz;
-      }
-    }
-  };
+  auto gather = [&](int b, int itn, T vx, T vz) { gather_row<T>(S, b, itn, lane, xo - 2, 2, 1 + EW2, vx, vz); };
 
   T vx0_m = valid(zo - 4) ? vxi[cell(zo - 4)] : T(0);  // velocities of step it-1, row r-1
   T vz0_m = valid(zo - 4) ? vzi[cell(zo - 4)] : T(0);
@@ -1917,6 +1937,7 @@ __device__ __forceinline__ void em2_tile(const DModel* __restrict__ models,
 
     // ---- Ex, Ez of row r (:273-295), source, Dirichlet ----
     const T hy_w = lane_prev(hy);  // new Hy(i-1, r)
+    T gex = 0, gez = 0;  // new E of the owned cells (receiver samples)
     if (own && rv && r >= zo) {
       T ex = ex_c, ez = ez_c;
       if (hcol && r >= 1 && r <= nz - 2) {
@@ -1944,9 +1965,12 @@ __device__ __forceinline__ void em2_tile(const DModel* __restrict__ models,
       exo[c] = ex;
       ezo[c] = ez;
       hyo[c] = hy;
+      gex = ex;
+      gez = ez;
       if (inx) { pout[ox] = m_ezx; pout[SX + ox] = m_hyx; }
       if (inz) { pout[oz] = m_exz; pout[SZ + oz] = m_hyz; }
     }
+    if (!TRAIL && S.nrec > 0 && r >= zo) gather_row<T>(S, r, it, lane, xo - 1, 1, EW, gex, gez);
     hy_p = hy;
     ex_c = ex_n;
   }
@@ -2131,17 +2155,7 @@ __device__ __forceinline__ void em2x2_tile(const DModel* __restrict__ models,
     }
   };
   const int nrec = S.nrec;
-  auto gather = [&](int b, int itn, T ex, T ez) {
-    const int q0 = S.rrow[b], q1 = S.rrow[b + 1];
-    auto tr = mp<T>(S.trace) + (long)itn * 2 * nrec;
-    for (int e = q0; e < q1; ++e) {
-      const int rx = S.rxid[2 * e], id = S.rxid[2 * e + 1];
-      if (own && i == rx) {
-        tr[id] = ex;
-        tr[nrec + id] = ez;
-      }
-    }
-  };
+  auto gather = [&](int b, int itn, T ex, T ez) { gather_row<T>(S, b, itn, lane, xo - 2, 2, 1 + EW2E, ex, ez); };
 
   T ex0_c = valid(zo - 2) ? exi[cell(zo - 2)] : T(0);  // Ex of step it-1, row r
   T hy1_p = 0;                                          // Hy of step it, row r-1
@@ -3074,8 +3088,7 @@ k_em25_e(const DModel* __restrict__ models, const DShot* __restrict__ shots,
 // ===========================================================================
 template <typename T>
 __global__ void k_gather(const DModel* __restrict__ models, const DShot* __restrict__ shots,
-                         int nshots, int ncomp, const int* __restrict__ base, int rel, int pingpong,
-                         const int* __restrict__ prun) {
+                         int nshots, int ncomp, const int* __restrict__ base, int rel) {
   const int s = blockIdx.y;
   if (s >= nshots) return;
   const DShot& S = shots[s];
@@ -3086,10 +3099,7 @@ __global__ void k_gather(const DModel* __restrict__ models, const DShot* __restr
   const auto q = cp<int>(S.rec) + 3 * r;
   const long c = ((long)q[2] * M.ny + q[1]) * M.nx + q[0];
   auto tr = mp<T>(S.trace) + ((long)it * ncomp) * S.nrec + r;
-  // state after step it (after the single step that ends a run of
-  // two-step launches, with prun)
-  const bool inb = pingpong && (prun ? ((((it - *prun) >> 1) & 1) ^ 1) : ((it + 1) & 1));
-  for (int f = 0; f < ncomp; ++f) tr[(long)f * S.nrec] = cp<T>(inb ? S.g[f] : S.f[f])[c];
+  for (int f = 0; f < ncomp; ++f) tr[(long)f * S.nrec] = cp<T>(S.f[f])[c];
 }
 
 // The H zeroing that closes each legacy 2.5-D EM step (electromagfdtd.f95
@@ -3401,10 +3411,11 @@ void launch_one(Plan* p, hipStream_t st, int rel, int which) {
   const dim3 blk(TX, TZ);
   const dim3 grd(p->nwork);
   if (which == 2) {
-    if (p->maxrec > 0) {
+    // the 2-D step kernels write their receiver samples themselves
+    if (p->maxrec > 0 && kind_3d(p->kind)) {
       const dim3 gg((p->maxrec + 63) / 64, p->nshots);
       hipLaunchKernelGGL(k_gather<T>, gg, dim3(64), 0, st, p->dmodels, p->dshots, p->nshots,
-                         p->ncomp, p->dbase, rel, pingpong(p->kind) ? 1 : 0, (const int*)nullptr);
+                         p->ncomp, p->dbase, rel);
     }
     return;
   }
@@ -3471,8 +3482,7 @@ void launch_steps(Plan* p, hipStream_t st, int n) {
 
 // Launch `which` of a run of two-step launches from the base step: pair
 // r/2 (steps r, r+1) for even r, or the single step r that ends an odd
-// run (one-step kernel and receiver gather, reading the buffer the pairs
-// left the state in).
+// run (one-step kernel, reading the buffer the pairs left the state in).
 template <typename T>
 void launch_two(Plan* p, hipStream_t st, int r, bool single) {
   const bool em = p->kind == SDT_EM2;
@@ -3491,9 +3501,6 @@ void launch_two(Plan* p, hipStream_t st, int r, bool single) {
     launch_el2<T, false, true>(p, st, r, nullptr, p->dit0);
   else
     launch_el2<T, false, false>(p, st, r, nullptr, p->dit0);
-  if (p->maxrec > 0)
-    hipLaunchKernelGGL(k_gather<T>, dim3((p->maxrec + 63) / 64, p->nshots), dim3(64), 0, st, p->dmodels,
-                       p->dshots, p->nshots, p->ncomp, p->dbase, r, 1, (const int*)p->dit0);
 }
 
 void launch_pairs(Plan* p, hipStream_t st, int n) {
