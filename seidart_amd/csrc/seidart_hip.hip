@@ -1060,7 +1060,7 @@ __global__ void __launch_bounds__(64 * WB, sizeof(T) == 4 ? (INT ? SDT_EL2X2_WAV
                                                           : (INT ? 3 : SPLIT ? 1 : 2))
 k_el2_step2(const DModel* __restrict__ models, const DShot* __restrict__ shots,
             const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel,
-            const int* __restrict__ it0, int eh) {
+            const int* __restrict__ it0, int eh0) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int it = __builtin_amdgcn_readfirstlane(*base + rel);
   const int q = __builtin_amdgcn_readfirstlane(((it - *it0) >> 1) & 1);
@@ -1072,6 +1072,7 @@ k_el2_step2(const DModel* __restrict__ models, const DShot* __restrict__ shots,
     if (wi.x < 0) continue;
     const DModel& M = models[shots[wi.x].model];
     if (wi.y >= M.nx) continue;
+    const int eh = wv.w > 0 ? __builtin_amdgcn_readfirstlane(wv.w) : eh0;  // rows of this tile
     // the columns (lanes), resp. rows (zo-4 .. ze+1), the wave reads outside
     // the C-PML slabs and off the outer ring (columns 0, nx-1 and rows 0,
     // nz-1 and their neighbours)
@@ -2229,7 +2230,7 @@ template <typename T, int WB>
 __global__ void __launch_bounds__(64 * WB)
 k_em2_step2(const DModel* __restrict__ models, const DShot* __restrict__ shots,
             const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel,
-            const int* __restrict__ it0, int eh) {
+            const int* __restrict__ it0, int eh0) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int it = __builtin_amdgcn_readfirstlane(*base + rel);
   const int q = __builtin_amdgcn_readfirstlane(((it - *it0) >> 1) & 1);
@@ -2241,6 +2242,7 @@ k_em2_step2(const DModel* __restrict__ models, const DShot* __restrict__ shots,
     if (wi.x < 0) continue;
     const DModel& M = models[shots[wi.x].model];
     if (wi.y >= M.nx) continue;
+    const int eh = wv.w > 0 ? __builtin_amdgcn_readfirstlane(wv.w) : eh0;  // rows of this tile
     // every column (lanes) and row (zo-2 .. ze+1) the wave reads outside the
     // C-PML slabs and clear of the columns and rows the edge rules name
     const int ze = min(wi.z + eh, M.nz);
@@ -3348,29 +3350,60 @@ bool el2x2_interior(const sdt_model& M, int x0, int z0, int wb, int eh) {
   return true;
 }
 
+// Row cuts of one model's tiles: rows of at most eh, and with zt < zb also
+// cuts at zt and zb, so the tiles between (those the kernel can run with
+// its interior body) start exactly where that first becomes possible.
+std::vector<int> row_cuts(int nz, int eh, int zt, int zb) {
+  std::vector<int> c = {0};
+  auto run = [&](int a, int b) {
+    for (int z = a + eh; z < b; z += eh) c.push_back(z);
+    if (b > c.back()) c.push_back(b);
+  };
+  if (zt > 0 && zt < zb && zb < nz) {
+    run(0, zt);
+    run(zt, zb);
+    run(zb, nz);
+  } else {
+    run(0, nz);
+  }
+  return c;
+}
+
 // which: 0 every block, 1 the interior blocks of the two-step kernel, 2 the
-// others.
+// others.  ew: owned columns per wavefront; two-step lists (ew = EW2, EW2E)
+// cut their rows at the interior limits of their kernel and carry each
+// tile's height in .w (0: the plan's rows).
 std::vector<int4> el2_work(const Plan* p, int ew = EW, int which = 0) {
   std::vector<int4> work;
   const int bw = ew * p->el2_wb, eh = p->el2_eh;
   for (int m = 0; m < p->nmodels; ++m) {
     const sdt_model& M = p->hmodels[m];
-    const int tx = (M.nx + bw - 1) / bw, tz = (M.nz + eh - 1) / eh, ntile = tx * tz;
+    // first and last-plus-one interior rows of k_el2_step2 / k_em2_step2
+    const int zt = ew == EW2 ? M.lz + 5 : ew == EW2E ? M.lz + 3 : 0;
+    const int zb = ew == EW2 ? M.nz - M.lz - 3 : ew == EW2E ? M.nz - M.lz - 4 : 0;
+    const std::vector<int> cuts = row_cuts(M.nz, eh, zt, zb);
+    const int tx = (M.nx + bw - 1) / bw, tz = (int)cuts.size() - 1, ntile = tx * tz;
     std::vector<int> ms;
     for (int s = 0; s < p->nshots; ++s)
       if (p->hshots[s].model == m) ms.push_back(s);
     if (ms.empty()) continue;
     std::vector<int> tl;
-    for (int t = 0; t < ntile; ++t)
-      if (which == 0 || el2x2_interior(M, (t % tx) * bw, (t / tx) * eh, p->el2_wb, eh) == (which == 1))
+    for (int t = 0; t < ntile; ++t) {
+      const int z0 = cuts[t / tx], h = cuts[t / tx + 1] - z0;
+      if (which == 0 || el2x2_interior(M, (t % tx) * bw, z0, p->el2_wb, h) == (which == 1))
         tl.push_back(t);
+    }
     const int nt = (int)tl.size();
     for (int tg = 0; tg < (nt + 7) / 8; ++tg)
       for (int s : ms)
         for (int tt = 0; tt < 8; ++tt) {
           const int k = tg * 8 + tt;
-          work.push_back(k < nt ? make_int4(s, (tl[k] % tx) * bw, (tl[k] / tx) * eh, 0)
-                                : make_int4(-1, 0, 0, 0));
+          if (k >= nt) {
+            work.push_back(make_int4(-1, 0, 0, 0));
+            continue;
+          }
+          const int z0 = cuts[tl[k] / tx], h = cuts[tl[k] / tx + 1] - z0;
+          work.push_back(make_int4(s, (tl[k] % tx) * bw, z0, ew == EW ? 0 : h));
         }
   }
   return work;
