@@ -3350,6 +3350,22 @@ bool el2x2_interior(const sdt_model& M, int x0, int z0, int wb, int eh) {
   return true;
 }
 
+// Default rows per tile of the 2-D step kernels: the tallest of 32, 16, 8, 4
+// rows that still gives the plan 1024 blocks or more (4 per CU), else 2 --
+// a single shot on a small grid needs short tiles to fill the device.
+int default_rows(const Plan* p) {
+  const int bw = EW * p->el2_wb;
+  for (int rows : {32, 16, 8, 4}) {
+    long long blocks = 0;
+    for (const sdt_shot& S : p->hshots) {
+      const sdt_model& M = p->hmodels[S.model];
+      blocks += (long long)((M.nx + bw - 1) / bw) * ((M.nz + rows - 1) / rows);
+    }
+    if (blocks >= 1024) return rows;
+  }
+  return 2;
+}
+
 // Row cuts of one model's tiles: rows of at most eh, and with zt < zb also
 // cuts at zt and zb, so the tiles between (those the kernel can run with
 // its interior body) start exactly where that first becomes possible.
@@ -3926,6 +3942,7 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
   if (kind == SDT_ELASTIC25) work = el25_work(p, KB25, EW25 * XW25);
   if (kind == SDT_EM25) work = el25_work(p, 1, EW25);  // one plane, one wavefront per item
   if (kind == SDT_ELASTIC2 || kind == SDT_EM2) {
+    p->el2_eh = default_rows(p);
     work = el2_work(p);
     const int nblk = (int)work.size();
     p->el2_grid = std::max(8, std::min(nblk, SDT_EL2_GRID) / 8 * 8);
