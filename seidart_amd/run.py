@@ -104,13 +104,21 @@ def _voigt6(row):
 
 
 def christoffel_vmax(row, direction) -> float:
-    """Largest phase velocity along a unit direction from the Christoffel
-    matrix Gamma_ik = C_ijkl n_j n_l / rho (materials.py:1014-1172)."""
+    """Largest phase velocity along a direction, as the reference's C-PML
+    setup computes it (``get_christoffel_matrix``, materials.py:1014-1172):
+    the Voigt matrix is expanded to a 3x3x3x3 tensor with a factor 1/sqrt(2)
+    on each shear index pair (``voigt_to_full_tensor``, materials.py:903-966,
+    so that C_1212 = c66 / 2 and C_1112 = c16 / sqrt(2)), then
+    Gamma_ik = C_ijkl n_j n_l / rho and v = sqrt(max(eig(Gamma), 0)).  The
+    factor is kept so that sigma_max, and with it every C-PML coefficient,
+    equals the reference's; the P speed of a medium without shear couplings
+    is unaffected."""
     from .configs import voigt_to_tensor
     n = np.asarray(direction, dtype=np.float64)
     n = n / np.linalg.norm(n)
     rho = float(row["rho"])
-    G = np.einsum("ijkl,j,l->ik", voigt_to_tensor(_voigt6(row)), n, n) / rho
+    f = np.array([1.0, 1.0, 1.0, 2 ** -0.5, 2 ** -0.5, 2 ** -0.5])
+    G = np.einsum("ijkl,j,l->ik", voigt_to_tensor(_voigt6(row) * np.outer(f, f)), n, n) / rho
     return float(np.sqrt(max(np.max(np.linalg.eigvalsh(G)), 0.0)))
 
 

@@ -124,14 +124,132 @@ def test_rotate_stiffness_invariants():
     b = configs.rotate_stiffness(C, R2 @ R1)
     np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-2)
     np.testing.assert_allclose(configs.rotate_stiffness(C, np.eye(3)), C, rtol=0, atol=0)
-    # phase velocity along R n in the rotated medium equals that along n
+    # phase velocities along R n in the rotated medium equal those along n
     n = np.array([0.3, -0.5, 0.8])
-    row = lambda C6: dict({f"c{i + 1}{j + 1}": C6[i, j] for i in range(6) for j in range(i, 6)}, rho=917.0)
-    v0 = dropin.christoffel_vmax(row(C), n)
-    v1 = dropin.christoffel_vmax(row(configs.rotate_stiffness(C, R1)), R1 @ n)
-    assert v1 == pytest.approx(v0, rel=1e-12)
-    # ice along the c-axis: P speed sqrt(c33/rho)
-    assert dropin.christoffel_vmax(row(C), (0, 0, 1)) == pytest.approx(np.sqrt(15.01e9 / 917), rel=1e-12)
+
+    def speeds(C6, d):
+        G = np.einsum("ijkl,j,l->ik", configs.voigt_to_tensor(C6), d, d) / 917.0
+        return np.linalg.eigvalsh(G)
+    np.testing.assert_allclose(speeds(configs.rotate_stiffness(C, R1), R1 @ n), speeds(C, n), rtol=1e-12)
+
+
+def _row(C6, rho=917.0):
+    return dict({f"c{i + 1}{j + 1}": C6[i, j] for i in range(6) for j in range(i, 6)}, rho=rho)
+
+
+def test_christoffel_vmax_follows_the_reference_convention():
+    """sigma_max of the object route uses the reference's Christoffel
+    velocities (materials.py:903-966, 1014-1172: shear index pairs scaled
+    by 1/sqrt(2)); closed forms of its matrix along x and z."""
+    C = configs._ice_crystal_voigt()
+    # ice along the c-axis: Gamma = diag(c55/2, c44/2, c33) / rho
+    assert dropin.christoffel_vmax(_row(C), (0, 0, 1)) == pytest.approx(np.sqrt(15.01e9 / 917), rel=1e-12)
+    # a shear-dominated medium along x: Gamma = diag(c11, c66/2, c55/2) / rho
+    S = np.diag([1e9, 1e9, 1e9, 3e9, 5e9, 8e9])
+    assert dropin.christoffel_vmax(_row(S, 1000.0), (1, 0, 0)) == pytest.approx(np.sqrt(4e9 / 1000), rel=1e-12)
+    assert dropin.christoffel_vmax(_row(S, 1000.0), (0, 0, 1)) == pytest.approx(np.sqrt(2.5e9 / 1000), rel=1e-12)
+    # a c16 coupling along x: the block [[c11, c16/sqrt2], [c16/sqrt2, c66/2]]
+    K = np.diag([2e9, 1e9, 1e9, 1e9, 1e9, 6e9])
+    K[0, 5] = K[5, 0] = 1.5e9
+    a, b, c = 2e9, 3e9, 1.5e9 / np.sqrt(2)
+    lam = 0.5 * (a + b + np.sqrt((a - b) ** 2 + 4 * c * c))
+    assert dropin.christoffel_vmax(_row(K, 1000.0), (1, 0, 0)) == pytest.approx(np.sqrt(lam / 1000), rel=1e-12)
+    # an isotropic medium: the P speed along the axes; along (1, 0, 1) the
+    # scaled shear terms give the xz block [[c11 + c55/2, c13 + c55/2],
+    # [c13 + c55/2, c33 + c55/2]] / 2, largest eigenvalue (c11 + c13 + c55) / 2
+    iso = np.zeros((6, 6))
+    iso[:3, :3] = 3e9
+    iso[np.arange(3), np.arange(3)] = 7e9
+    iso[np.arange(3, 6), np.arange(3, 6)] = 2e9
+    for d in ((1, 0, 0), (0, 0, 1)):
+        assert dropin.christoffel_vmax(_row(iso, 2000.0), d) == pytest.approx(np.sqrt(7e9 / 2000), rel=1e-12)
+    assert dropin.christoffel_vmax(_row(iso, 2000.0), (1, 0, 1)) == pytest.approx(
+        np.sqrt((7e9 + 3e9 + 2e9) / 2 / 2000), rel=1e-12)
+
+
+def _objects(seismic=True):
+    """Duck-typed Model / Domain after Model.build: three materials in the
+    table, two painted (labels 0 and 2); material 1, absent from the
+    geometry, is the fastest and must not set sigma_max."""
+    import pandas as pd
+    from types import SimpleNamespace as NS
+    nx0, nz0, cpml_ = 9, 7, 3
+    geom = np.zeros((nx0, nz0), dtype=int)
+    geom[:, 4:] = 2
+    geom[3, 1] = 2
+    if seismic:
+        rows = []
+        for c11, c66, c16, rho in ((4e9, 1e9, 0.0, 2000.0), (9e10, 1e9, 0.0, 2000.0),
+                                   (2e9, 7e9, 1.5e9, 1000.0)):
+            r = {k: 0.0 for k in problem.STIFFNESS}
+            r.update(c11=c11, c22=c11, c33=c11, c12=0.3 * c11, c13=0.3 * c11, c23=0.3 * c11,
+                     c44=c66, c55=c66, c66=c66, c16=c16, rho=rho)
+            rows.append(r)
+        tab = dict(stiffness_coefficients=pd.DataFrame(rows), attenuation_coefficients=None)
+    else:
+        e = pd.DataFrame([{k: v for k, v in zip(problem.PERMITTIVITY, (3.2, 0, 0, 3.2, 0, 3.1 + m))}
+                          for m in range(3)])
+        c = pd.DataFrame([{k: v for k, v in zip(problem.CONDUCTIVITY, (1e-4 * (m + 1), 0, 0, 1e-4, 0, 2e-4))}
+                          for m in range(3)])
+        tab = dict(permittivity_coefficients=e, conductivity_coefficients=c)
+    nt = 12
+    model = NS(is_seismic=seismic, time_steps=nt, f0=100.0, dt=1e-4 if seismic else 1e-11,
+               velocity_scaling_factor=1.0, x=2.0, y=0.0, z=1.5,
+               sourcefunction_x=np.linspace(0, 1, nt), sourcefunction_y=np.zeros(nt),
+               sourcefunction_z=np.linspace(1, 0, nt), density_method="harmonic", **tab)
+    domain = NS(dim=2.0, cpml=cpml_, nx=nx0, nz=nz0, ny=1, dx=0.5, dz=0.25, dy=1.0, geometry=geom,
+                nmats=1, NP=2.0, NPA=1.0, Rcoef=0.001, kappa_max=5.0, alpha_max_scalar=1.0)
+    return model, domain
+
+
+def test_problem_from_objects_seismic():
+    """The object route (Model.run replacement): tables painted on the
+    geometry and edge-extended into the C-PML (tensor2dat), sigma_max from
+    the largest axis speed of the painted materials in the reference's
+    Christoffel convention (cpmlcompute, definitions.py:612-652), the
+    model's dt, source series and source cell."""
+    model, domain = _objects(True)
+    p = dropin.problem_from_objects(model, domain)
+    cp_, geom = domain.cpml, domain.geometry
+    tab = model.stiffness_coefficients
+    assert (p.nx, p.nz, p.dt, p.nt) == (9 + 6, 7 + 6, model.dt, 12)
+    for k in ("c11", "c16", "c55", "rho"):
+        exp = tab[k].values[geom[np.clip(np.arange(p.nx) - cp_, 0, 8)][:, np.clip(np.arange(p.nz) - cp_, 0, 6)]]
+        np.testing.assert_array_equal(p.coef[k], exp.T, err_msg=k)
+    # x: material 2 along x, Gamma block [[c11, c16/sqrt2], [c16/sqrt2, c66/2]] / rho;
+    # z: max(c33 / rho) over materials 0 and 2 (c55 / 2 < c33 for both)
+    a, b, c = 2e9, 3.5e9, 1.5e9 / np.sqrt(2)
+    vx = np.sqrt(0.5 * (a + b + np.sqrt((a - b) ** 2 + 4 * c * c)) / 1000.0)
+    vx = max(vx, np.sqrt(4e9 / 2000.0))
+    vz = max(np.sqrt(4e9 / 2000.0), np.sqrt(3.5e9 / 1000.0), np.sqrt(2e9 / 1000.0))
+    for ax, n, d, v in (("x", p.nx, domain.dx, vx), ("z", p.nz, domain.dz, vz)):
+        exp = cpml.axis_profile(n, cp_, d, cpml.sigma_max_seismic(v, cp_, 2.0, 0.001),
+                                cpml.alpha_max_seismic(1.0, 100.0), 5.0, 2.0, 1.0, model.dt, True)
+        for f in ("kappa", "a", "b", "kappa_h", "a_h", "b_h"):
+            np.testing.assert_allclose(getattr(p.prof[ax], f), getattr(exp, f), rtol=1e-14, atol=0,
+                                       err_msg=(ax, f))
+    sh = p.shots[0]
+    np.testing.assert_array_equal(sh.force[0], model.sourcefunction_x)
+    np.testing.assert_array_equal(sh.force[2], model.sourcefunction_z)
+    assert sh.force[1] is None
+    assert sh.src == dropin.source_index(int(2.0 / 0.5), 0, int(1.5 / 0.25), cp_, 2.0, "legacy")
+    assert p.density_method == "harmonic"
+
+
+def test_problem_from_objects_em():
+    """EM: permittivity and conductivity painted; sigma_max from the EM
+    formula with deltamin = min(dx, dz) and c (definitions.py:653-689)."""
+    model, domain = _objects(False)
+    p = dropin.problem_from_objects(model, domain)
+    np.testing.assert_array_equal(p.coef["e33"][domain.cpml, domain.cpml:-domain.cpml],
+                                  3.1 + domain.geometry[:, 0])
+    sm = cpml.sigma_max_em(domain.cpml, 0.25, 2.0, 0.001)
+    for ax, n, d in (("x", p.nx, 0.5), ("z", p.nz, 0.25)):
+        exp = cpml.axis_profile(n, domain.cpml, d, sm, cpml.alpha_max_em(1.0, 100.0), 5.0, 2.0, 1.0,
+                                model.dt, False)
+        for f in ("kappa", "a", "b", "kappa_h", "a_h", "b_h"):
+            np.testing.assert_allclose(getattr(p.prof[ax], f), getattr(exp, f), rtol=1e-14, atol=0,
+                                       err_msg=(ax, f))
 
 
 # -------------------------------------------------------------- file formats
