@@ -4146,23 +4146,32 @@ int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows) {
   if (waves_per_block == p->el2_wb && rows == p->el2_eh) return SDT_OK;
   // the old work list and the graphs that bake in the launch shape may be in flight
   HIP_TRY(hipDeviceSynchronize());
+  // all or nothing: on a failure the plan keeps its tiles and work lists
+  const int owb = p->el2_wb, oeh = p->el2_eh;
   p->el2_wb = waves_per_block;
   p->el2_eh = rows;
   const std::vector<int4> work = el2_work(p);
   int4* dw = nullptr;
-  HIP_TRY(hipMalloc(&dw, sizeof(int4) * work.size()));
-  if (hipMemcpy(dw, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice) != hipSuccess) {
-    (void)hipFree(dw);
-    return fail(SDT_EDEVICE, "hipMemcpy failed (work list)");
+  int rc = SDT_OK;
+  if (hipMalloc(&dw, sizeof(int4) * work.size()) != hipSuccess) {
+    dw = nullptr;
+    rc = fail(SDT_EDEVICE, "hipMalloc failed (work list)");
+  } else if (hipMemcpy(dw, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice) !=
+             hipSuccess) {
+    rc = fail(SDT_EDEVICE, "hipMemcpy failed (work list)");
+  } else if (p->dwork2) {
+    rc = upload_work2(p);  // replaces the two-step list only when it succeeds
+  }
+  if (rc != SDT_OK) {
+    if (dw) (void)hipFree(dw);
+    p->el2_wb = owb;
+    p->el2_eh = oeh;
+    return rc;
   }
   (void)hipFree(p->dwork);
   p->dwork = dw;
   p->nwork = (int)work.size();
   p->el2_grid = std::max(8, std::min(p->nwork, SDT_EL2_GRID) / 8 * 8);
-  if (p->dwork2) {
-    const int rc = upload_work2(p);
-    if (rc) return rc;
-  }
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
   if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
   drop_adjoint_graphs(p);
