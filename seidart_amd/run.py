@@ -436,9 +436,12 @@ def problem_from_objects(model, domain, density_method: str | None = None,
 def write_trace_csv(trace2d: np.ndarray, basefile: str, channel: str, source_xyz) -> str:
     """nt x nrec CSV without header, named like Array.save
     ('{base}-{channel}-{sx}-{sy}-{sz}.csv', arraybuild.py:1031-1056)."""
+    import pandas as pd
     src = "-".join(str(v) for v in np.asarray(source_xyz, dtype=np.float64))
     fn = f"{basefile}-{channel}-{src}.csv"
-    np.savetxt(fn, np.asarray(trace2d), delimiter=",")
+    # the reference's writer (DataFrame.to_csv, shortest round-trip floats),
+    # so the files are byte-identical to Array.save's
+    pd.DataFrame(np.asarray(trace2d)).to_csv(fn, header=False, index=False)
     return fn
 
 

@@ -13,6 +13,7 @@ from __future__ import annotations
 import os
 
 import numpy as np
+import pandas as pd
 
 from .problem import TRACE_COMPONENTS, Problem, Shot
 from .run import problem_from_files, source_index, write_trace_csv
@@ -128,7 +129,7 @@ class CommonOffset:
         (arraybuild.py:1031-1056: project base, channel, last source x-y-z)."""
         if output_basefile:
             fn = output_basefile + ".csv"
-            np.savetxt(fn, self.timeseries, delimiter=",")
+            pd.DataFrame(self.timeseries).to_csv(fn, header=False, index=False)  # Array.save's writer
         else:
             base = os.path.splitext(self.project_file)[0]
             fn = write_trace_csv(self.timeseries, base, self.channel, self.source_xyz)

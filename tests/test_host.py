@@ -331,6 +331,17 @@ def test_snapshot_names():
     assert fortranio.snapshot_name("Ez", 123456, (1, 2, 3)) == "Ez.123456.1.2.3.dat"
 
 
+def test_trace_csv_name_and_format(tmp_path):
+    """Array.save's CSV (arraybuild.py:1031-1056): '{base}-{channel}-{x}-{y}-{z}.csv',
+    nt x nrec, no header, no index, floats in their shortest round-trip form."""
+    tr = np.array([[0.1, -2.5e-7], [1.0 / 3.0, 0.0], [1e300, -0.0]])
+    fn = dropin.write_trace_csv(tr, str(tmp_path / "proj"), "Vz", np.array([12.5, 0.0, 3.0]))
+    assert fn.endswith("proj-Vz-12.5-0.0-3.0.csv")
+    text = open(fn).read().splitlines()
+    assert text == ["0.1,-2.5e-07", "0.3333333333333333,0.0", "1e+300,-0.0"]
+    np.testing.assert_array_equal(np.loadtxt(fn, delimiter=","), tr)
+
+
 # ------------------------------------------------------------ kernel coefs
 def test_kernel_coef_averages_match_loops():
     """The vectorised staggered-node averages equal the legacy loop bodies
