@@ -342,6 +342,7 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
   const int ic = incol ? i : (i < 0 ? 0 : nx - 1);  // clamped column: every load stays in bounds
   const bool own = incol && lane >= 1 && lane <= EW;
   const int zend = min(zo + eh, nz);
+  const bool has_rec = !TRAIL && S.nrec > 0 && S.rrow[zend] > S.rrow[zo];  // receivers on own rows
   const long nn = (long)nx * nz;
   const T* tin = nullptr;
   T* tout = nullptr;
@@ -608,7 +609,7 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
       gvx = vxn;
       gvz = vzn;
     }
-    if (!TRAIL && S.nrec > 0 && rr >= zo && rr < zend) gather_row<T>(S, rr, it, lane, xo - 1, 1, EW, gvx, gvz);
+    if (has_rec && rr >= zo && rr < zend) gather_row<T>(S, rr, it, lane, xo - 1, 1, EW, gvx, gvz);
     sxz_pp = sxz_p;
     sxx_p = sxx_c;
     szz_p = szz_c;
@@ -942,7 +943,8 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
   CV kv_p{}, kv_pp{};  // velocity coefficients of rows r-2, r-3
   MV mv_p{}, mv_pp{};  // memory after step it, rows r-1, r-2
   MS ms_p{}, ms_pp{};  // memory after step it, rows r-2, r-3
-  const bool has_rec = nrec > 0;
+  // receivers on the tile's own rows at all (one test per tile, not per row)
+  const bool has_rec = nrec > 0 && S.rrow[ze] > S.rrow[zo];
   bool bad0 = false, bad1 = false;  // own cell above the blow-up bound at step it / it+1
 
   // Every load of iteration r (addresses clamped in bounds).  With
@@ -1862,6 +1864,7 @@ __device__ __forceinline__ void em2_tile(const DModel* __restrict__ models,
   const int ic = incol ? i : (i < 0 ? 0 : nx - 1);  // clamped column: every load stays in bounds
   const bool own = incol && lane >= 1 && lane <= EW;
   const int zend = min(zo + eh, nz);
+  const bool has_rec = !TRAIL && S.nrec > 0 && S.rrow[zend] > S.rrow[zo];  // receivers on own rows
   const long nn = (long)nx * nz;
   const T* tin = nullptr;
   T* tout = nullptr;
@@ -1971,7 +1974,7 @@ __device__ __forceinline__ void em2_tile(const DModel* __restrict__ models,
       if (inx) { pout[ox] = m_ezx; pout[SX + ox] = m_hyx; }
       if (inz) { pout[oz] = m_exz; pout[SZ + oz] = m_hyz; }
     }
-    if (!TRAIL && S.nrec > 0 && r >= zo) gather_row<T>(S, r, it, lane, xo - 1, 1, EW, gex, gez);
+    if (has_rec && r >= zo) gather_row<T>(S, r, it, lane, xo - 1, 1, EW, gex, gez);
     hy_p = hy;
     ex_c = ex_n;
   }
@@ -2164,7 +2167,7 @@ __device__ __forceinline__ void em2x2_tile(const DModel* __restrict__ models,
   T hy2_p = 0;                                          // Hy of step it+1, row r-2
   CE k_p{};                                             // coefficients of row r-1
   MX m_p{};                                             // memory of row r-1 after step it
-  const bool has_rec = nrec > 0;
+  const bool has_rec = nrec > 0 && S.rrow[ze] > S.rrow[zo];  // receivers on own rows (per tile)
   bool bad0 = false, bad1 = false;
 
   for (int r = zo - 2; r <= ze; ++r) {
