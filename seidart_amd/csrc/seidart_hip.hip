@@ -667,8 +667,11 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
   // 1..nz-2, so every edge condition of the scheme holds on every lane;
   // BODY 1: its columns are interior (every condition on the column holds,
   // no x slab) but its rows may reach a z slab or the outer rows;
-  // BODY 2: anything else.
-  constexpr bool INT = BODY == 0, XI = BODY < 2, PML = BODY > 0;
+  // BODY 2: anything else; BODY 3 (below): x-slab columns of a tile whose
+  // rows are all off the z slabs and the outer rows.
+  // BODY 3: as 2, but every row the wave reads is off the z slabs and the
+  // outer rows (x-slab columns of a z-interior tile).
+  constexpr bool INT = BODY == 0, XI = BODY < 2, PML = BODY > 0, ZI = BODY == 3;
   const DShot& S = shots[wi.x];
   const DModel& M = models[S.model];
   const int nx = M.nx, nz = M.nz;
@@ -754,7 +757,7 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
   };
   // slab offsets of row r: x slab (valid where inx), z slab (ZB-based)
   auto offx = [&](int r) { return inx ? (uint32_t)crow(r) * 2 * lx + xs : 0u; };
-  auto inzs = [&](int r) { return PML && in_slab(crow(r), nz, lz); };
+  auto inzs = [&](int r) { return PML && !ZI && in_slab(crow(r), nz, lz); };
   auto offz = [&](int r) { return ZB + (inzs(r) ? (uint32_t)slab_pos(crow(r), nz, lz) * nx + ic : 0u); };
   auto load_mv = [&](int r) {
     MV m{};
@@ -1085,6 +1088,8 @@ k_el2_step2(const DModel* __restrict__ models, const DShot* __restrict__ shots,
       el2x2_tile<T, SPLIT, 0>(models, shots, wi, it, q, threadIdx.x & 63, eh);
     else if (xint)
       el2x2_tile<T, SPLIT, 1>(models, shots, wi, it, q, threadIdx.x & 63, eh);
+    else if (zint)
+      el2x2_tile<T, SPLIT, 3>(models, shots, wi, it, q, threadIdx.x & 63, eh);
     else
       el2x2_tile<T, SPLIT, 2>(models, shots, wi, it, q, threadIdx.x & 63, eh);
   }
