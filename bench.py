@@ -4,10 +4,13 @@ Workload (one "step" = one time step of every shot of the batch):
 batched 2-D elastic, 96 shots over three layered snow / firn / ice models
 (512x256, 1024x512, 2048x768 cells + C-PML 20, dx = 0.5 m), 150 Hz, FP32,
 all shots advanced together in the same kernel launches, hipGraph blocks of
-50 steps.  Beside it, in the same run:
-  * the plain FP64 GPU path (one shot per plan, no batching) on one shot of
-    each model, for the FP32-batched / FP64-plain speed ratio and the FP32
-    trace error against FP64;
+up to 50 steps captured before the timed region.  Beside it, in the same run:
+  * the full 3000-step FP32 batched run and the plain FP64 GPU path (every
+    one of the 96 shots alone in its own plan, no batching, 3000 steps), for
+    the FP32-batched / FP64-plain speed ratio and the FP32 trace error of
+    every shot against its FP64 run;
+  * the other reference configurations at their full lengths (plain_psv,
+    gpr, aniso25), FP32 and FP64, with the FP32 trace error;
   * the oracle (FP64 C restatement of the legacy loops, OpenMP) on a bounded
     sample of the same workload on the host cores (cpu_baseline);
   * the FWI gradient workload (fwi_vti: 2-D VTI elastic 1024x512 + C-PML 20,
@@ -58,12 +61,25 @@ def algorithmic_bytes(problems, nshots_by_model, itemsize):
     return tot * itemsize
 
 
-def cpu_baseline(problems, budget_s):
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(budget_s):
     """Oracle (FP64 C restatement of the legacy loops, OpenMP) on one shot of
-    the largest model, on the host cores, for about budget_s seconds of
-    stepping; plus a short single-thread sample for the per-core rate."""
+    the largest model of the workload, on the host cores, for about budget_s
+    seconds of stepping from the start of the run; plus a shorter
+    single-thread sample for the per-core rate."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
+    from seidart_amd import configs
+    from seidart_amd.problem import Problem
     pyoracle.build()
     try:
         cores = len(os.sched_getaffinity(0))
@@ -71,28 +87,30 @@ def cpu_baseline(problems, budget_s):
         cores = os.cpu_count() or 1
     if os.environ.get("OMP_NUM_THREADS"):
         cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
-    p = problems[-1]
-    shot = len(p.shots) // 2
+    q = configs.batched_layered()[-1]
+    p = Problem(q.kind, q.nx, q.ny, q.nz, q.dx, q.dy, q.dz, q.dt, q.nt, q.coef, q.prof,
+                [q.shots[len(q.shots) // 2]], q.gamma)
 
-    def timed(threads, budget, max_steps):
-        o = pyoracle.OracleRun(p, shot=shot, threads=threads)
+    def timed(threads, budget):
+        o = pyoracle.OracleRun(p, threads=threads)
         o.run(0, 2)  # first touch of every array
         t0 = time.perf_counter()
         steps = 2
-        while time.perf_counter() - t0 < budget and steps < max_steps:
-            n = min(25, max_steps - steps)
+        while time.perf_counter() - t0 < budget and steps < p.nt:
+            n = min(25, p.nt - steps)
             o.run(steps, n)
             steps += n
         return steps - 2, time.perf_counter() - t0
 
-    n, el = timed(cores, budget_s, p.nt)
-    n1, el1 = timed(1, min(3.0, budget_s / 4), p.nt)
+    n, el = timed(cores, budget_s)
+    n1, el1 = timed(1, min(4.0, budget_s / 4))
     cells = p.nx * p.nz
     return {"value": round(cells * n / el / 1e9, 4), "unit": "Gcell-updates/s", "cores": cores,
-            "kind": "port",
-            "sample": f"oracle (FP64 C/OpenMP restatement of seismic2) on 1 shot of the "
-                      f"{p.nx}x{p.nz} padded model, {n} steps in {el:.1f} s on {cores} threads; "
-                      f"1 thread: {cells * n1 / el1 / 1e9:.4f} Gcell-updates/s ({n1} steps)",
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"oracle (FP64 C/OpenMP restatement of seismic2) on the middle shot of the "
+                      f"{p.nx}x{p.nz} padded model (2048x768 + C-PML 20), steps 2..{n + 1} of its "
+                      f"{p.nt} in {el:.1f} s on {cores} threads; 1 thread: "
+                      f"{cells * n1 / el1 / 1e9:.4f} Gcell-updates/s ({n1} steps in {el1:.1f} s)",
             "value_1core": round(cells * n1 / el1 / 1e9, 4)}
 
 
@@ -119,6 +137,7 @@ def fwi_gradient(nt: int = 6000, nshots: int = 16, every: int = 64, fd_cells: in
         sim.set_tiles(*tiles)
     else:
         sim.autotune(steps=10)
+    sim.capture_graph(every)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sim.run(graph_block=every)
@@ -229,23 +248,27 @@ def alg_bytes_per_step(problems, itemsize):
     return tot * itemsize
 
 
-def config_lines(steps: int = 400):
-    """Throughput of the other configurations at one GPU: bounded runs of
-    `steps` steps (after 50 warmup steps; plain_psv runs its full 2000), FP32
-    and FP64, each with its algorithmic bytes per cell-update, the fraction of
-    the HBM peak and the FP32 trace error against the FP64 run."""
+def config_lines(survey_steps: int = 200):
+    """Throughput and FP32 accuracy of the other configurations at one GPU.
+    The reference configurations run their full length (plain_psv 2000
+    steps, gpr 4000, aniso25 3000) in FP32 and FP64, after the hipGraph of
+    their block is captured; the two measured lines that are not reference
+    configurations (gpr25, gpr_survey) run `survey_steps` steps.  Each line
+    gives Gcell-updates/s, the algorithmic bytes per cell-update, the
+    fraction of the HBM peak and the FP32 trace error against the FP64 run
+    over the same steps."""
     import torch
     from seidart_amd import configs
     from seidart_amd.solver import Simulation
     out = {}
-    cases = [("plain_psv", lambda nt: configs.plain_psv(nt=nt), 2000),
-             ("gpr", lambda nt: configs.gpr(nt=nt), steps),
-             ("aniso25", lambda nt: configs.aniso25(nt=nt), min(steps, 200)),
-             ("gpr25", lambda nt: configs.gpr25(nt=nt), min(steps, 200)),
-             ("gpr_survey", lambda nt: configs.gpr_survey(nt=nt), min(steps, 200))]
+    cases = [("plain_psv", configs.plain_psv, None),
+             ("gpr", configs.gpr, None),
+             ("aniso25", configs.aniso25, None),
+             ("gpr25", configs.gpr25, survey_steps),
+             ("gpr_survey", configs.gpr_survey, survey_steps)]
     for name, make, K in cases:
-        W = 50
-        p = make(W + K)
+        p = make() if K is None else make(nt=K)
+        K = p.nt
         cells = p.nx * p.ny * p.nz * len(p.shots)
         rec = {"grid": f"{p.nx}x{p.ny}x{p.nz}" if p.is3d else f"{p.nx}x{p.nz}", "kind": p.kind,
                "shots": len(p.shots), "steps": K}
@@ -254,14 +277,14 @@ def config_lines(steps: int = 400):
             sim = Simulation(p, precision=prec)
             if p.kind in ("elastic2", "em2"):
                 rec.setdefault("tiles", {})[prec] = list(min((r := sim.autotune(steps=10)), key=r.get))
-            sim.run(W, graph_block=50 if W >= 50 else 0)
+            sim.capture_graph(50)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             sim.run(K, graph_block=50)
             torch.cuda.synchronize()
             el = time.perf_counter() - t0
             sim.check()
-            traces[prec] = sim.traces()[0][W:]
+            traces[prec] = sim.traces()[0]
             sim.close()
             gcs = cells * K / el / 1e9
             bpc = alg_bytes_per_step([p], size) / cells
@@ -270,6 +293,7 @@ def config_lines(steps: int = 400):
         t32, t64 = traces["fp32"], traces["fp64"]
         rec["fp32_trace_rel_l2_vs_fp64"] = float(f"{np.linalg.norm(t32 - t64) / np.linalg.norm(t64):.3e}")
         out[name] = rec
+        print(f"bench: {name} done", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     return out
 
@@ -351,8 +375,9 @@ def main(argv=None):
     ap.add_argument("--tiles", default="",
                     help="fixed tile shape WxR of the 2-D elastic step kernel (e.g. 4x16) instead "
                          "of the autotuner's choice (profiling runs)")
-    ap.add_argument("--config-steps", type=int, default=400,
-                    help="timed steps of each other configuration (plain_psv runs 2000)")
+    ap.add_argument("--config-steps", type=int, default=200,
+                    help="steps of the measured lines that are not reference configurations "
+                         "(gpr25, gpr_survey); the reference configurations run their full length")
     ap.add_argument("--fwi-steps", type=int, default=6000)
     ap.add_argument("--fwi-fd", type=int, default=0,
                     help="also check the FWI gradient against central differences at this "
@@ -373,7 +398,12 @@ def main(argv=None):
     from seidart_amd.solver import Simulation
 
     K, W = a.steps, a.warmup
-    nt = W + K + PROFILE_STEPS
+    FULL = 3000                       # the workload's own length
+    nt = max(W + K + PROFILE_STEPS, FULL)
+    # hipGraph block of the timed run: at most --graph-block and at most K
+    # steps, even (two steps per launch); captured before the timed region
+    gb = min(a.graph_block, K) // 2 * 2
+    gb = gb if gb >= 2 else 0
     t0 = time.perf_counter()
     problems = configs.batched_layered(nt=nt)
     setup_s = time.perf_counter() - t0
@@ -388,15 +418,15 @@ def main(argv=None):
     else:
         tune = sim.autotune(steps=10)   # tile shape of the step kernel (setup, untimed)
     sim_spl, sim_tiles = sim.steps_per_launch, sim.tiles()
-    gb = a.graph_block if W >= a.graph_block else 0
-    sim.run(W, graph_block=gb)
+    captured = sim.capture_graph(gb) if gb else False
+    sim.run(W, graph_block=gb if W >= gb else 0)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev0.record(stream)
-    sim.run(K, graph_block=a.graph_block)
+    sim.run(K, graph_block=gb)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -426,43 +456,76 @@ def main(argv=None):
         tgbs = roofline["traffic"] / (k_ms[0] * 1e-3) / 1e9
         roofline.update({"traffic_GBs": round(tgbs, 1), "copy_GBs": round(cgbs, 1),
                          "traffic_frac_of_copy": round(tgbs / cgbs, 4)})
-    traces32 = sim.traces()
     sim.close()
     del sim
     torch.cuda.empty_cache()
+    print("bench: headline timed", file=sys.stderr, flush=True)
 
     plain = None
     if not a.no_plain:
-        # plain FP64 GPU path: one shot per plan, sample shot = middle shot of each model
+        # Accuracy and the plain FP64 path over the workload's full length,
+        # independent of --steps: one FP32 batched run of all 96 shots, then
+        # every shot alone in its own FP64 plan (tile shape autotuned on the
+        # first shot of each model), timed.
+        full = configs.batched_layered(nt=FULL)
+        sb = Simulation(full, precision="fp32")
+        if fixed:
+            sb.set_tiles(*fixed)
+        else:
+            sb.autotune(steps=10)
+        sb.capture_graph(50)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sb.run(graph_block=50)
+        torch.cuda.synchronize()
+        full_s = time.perf_counter() - t1
+        sb.check()
+        traces32 = sb.traces()
+        sb.close()
+        del sb
+        torch.cuda.empty_cache()
         tot_cells, tot_s, errs = 0, 0.0, []
-        off = 0
-        for m, p in enumerate(problems):
-            s = len(p.shots) // 2
-            single = Problem(p.kind, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, p.dt, p.nt, p.coef,
-                             p.prof, [p.shots[s]], p.gamma)
-            sp = Simulation(single, precision="fp64")
-            sp.autotune(steps=12)   # tile shape and steps per launch (setup, untimed)
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            sp.run(nt, graph_block=a.graph_block)
-            torch.cuda.synchronize()
-            tot_s += time.perf_counter() - t1
-            sp.check()
-            tot_cells += p.nx * p.nz * nt
-            t64 = sp.traces()[0]
-            t32 = traces32[off + s]
-            errs.append(float(np.linalg.norm(t32 - t64) / np.linalg.norm(t64)))
-            sp.close()
-            off += len(p.shots)
+        k = 0
+        for p in full:
+            best = None
+            for sh in p.shots:
+                single = Problem(p.kind, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, p.dt, p.nt, p.coef,
+                                 p.prof, [sh], p.gamma)
+                sp = Simulation(single, precision="fp64")
+                if best is None:
+                    r = sp.autotune(steps=12)   # tile shape and steps per launch (setup, untimed)
+                    best = min(r, key=r.get) if r else None
+                elif best:
+                    sp.steps_per_launch = best[0]
+                    sp.set_tiles(*best[1:])
+                sp.capture_graph(50)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                sp.run(graph_block=50)
+                torch.cuda.synchronize()
+                tot_s += time.perf_counter() - t1
+                sp.check()
+                tot_cells += p.nx * p.nz * p.nt
+                t64 = sp.traces()[0]
+                errs.append(float(np.linalg.norm(traces32[k] - t64) / np.linalg.norm(t64)))
+                sp.close()
+                k += 1
         pv = tot_cells / tot_s / 1e9
         plain = {"value": round(pv, 3), "unit": "Gcell-updates/s", "dtype": "f64",
-                 "sample": "middle shot of each model, one plan per shot (autotuned), full run",
+                 "sample": f"all {k} shots, each alone in its own FP64 plan (tiles autotuned per "
+                           f"model), {FULL} steps each",
                  "speedup_fp32_batched": round(value / pv, 3),
-                 "fp32_trace_rel_l2_vs_fp64": [float(f"{e:.3e}") for e in errs]}
+                 "fp32_batched_full_run_Gcell_updates_s": round(cells_per_step * FULL / full_s / 1e9, 3),
+                 "fp32_trace_rel_l2_vs_fp64": {
+                     "shots": len(errs), "steps": FULL, "max": float(f"{max(errs):.3e}"),
+                     "median": float(f"{np.median(errs):.3e}"),
+                     "per_model_max": [float(f"{max(errs[o:o + n]):.3e}") for o, n in
+                                       zip(np.cumsum([0] + shots[:-1]), shots)]}}
+        print("bench: fp64 plain and accuracy done", file=sys.stderr, flush=True)
 
     fwi = None if a.no_fwi else fwi_gradient(nt=a.fwi_steps, fd_cells=a.fwi_fd, tiles=fixed)
     others = None if a.no_configs else config_lines(a.config_steps)
-    cpu = None if a.no_cpu else cpu_baseline(problems, a.cpu_seconds)
+    cpu = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s",
         "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(wall / K * 1e3, 4),
@@ -470,10 +533,12 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": "batched_layered: 2-D elastic, 96 shots (32 per model) over 3 "
                                "layered snow/firn/ice models 512x256, 1024x512, 2048x768 "
-                               "(+C-PML 20), dx 0.5 m, 150 Hz, one launch set per step, "
-                               "hipGraph per 50 steps",
+                               f"(+C-PML 20), dx 0.5 m, 150 Hz, FP32, all shots in the same "
+                               f"launches, {K} timed steps after {W} warmup steps" +
+                               (f", hipGraph blocks of {gb} steps (captured before timing)"
+                                if gb else ", direct launches"),
                    "cells_per_step": cells_per_step, "shots": sum(shots),
-                   "graph_block": a.graph_block,
+                   "graph_block": gb if captured else 0,
                    "tiles": {"chosen": {"steps_per_launch": sim_spl, "waves": sim_tiles[0],
                                         "rows": sim_tiles[1]},
                              "ms_per_step": {} if fixed else

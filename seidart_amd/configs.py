@@ -179,7 +179,7 @@ FWI_PARAMS = ("c11", "c13", "c33", "c55", "rho")
 
 
 def fwi_vti(nt: int = 6000, seed: int = 1005, nshots: int = 16, nx0: int = 1024, nz0: int = 512,
-            cpml: int = 20, nrec: int = 256):
+            cpml: int = 20, nrec: int = 256, window=None):
     """FWI gradient workload, 2-D VTI elastic: 1024 x 512 cells, dx = dz = 1 m,
     C-PML 20, per-cell (C11, C13, C33, C55, rho).
 
@@ -191,7 +191,12 @@ def fwi_vti(nt: int = 6000, seed: int = 1005, nshots: int = 16, nx0: int = 1024,
     sigma 30 cells.  16 vertical-force Ricker shots of 80 Hz evenly spaced
     along x at z = 4, 256 receivers every 4 cells at z = 2, dt with CFL
     1/sqrt(2) (classes.py:469-485), split C-PML variant.  Returns (true,
-    start) Problems that share sources, receivers, dt and C-PML."""
+    start) Problems that share sources, receivers, dt and C-PML.
+
+    ``window`` = (x0, x1, z0, z1): the same models cut to cells x0..x1-1,
+    z0..z1-1 (with their own C-PML), the dt of the full model, ``nshots``
+    sources spread along x at z = 4 and receivers every 4 cells at z = 2
+    inside the window."""
     from scipy.ndimage import gaussian_filter
     rng = np.random.default_rng(seed)
     dx = 1.0
@@ -213,6 +218,12 @@ def fwi_vti(nt: int = 6000, seed: int = 1005, nshots: int = 16, nx0: int = 1024,
     start = {k: gaussian_filter(v, 30.0, mode="nearest") for k, v in true.items()}
     vmax = max(float(np.sqrt(np.max(np.maximum(M["c11"], M["c33"]) / M["rho"]))) for M in (true, start))
     dt = (1 / np.sqrt(2.0)) / (vmax * np.sqrt(2.0 / dx ** 2))
+    if window is not None:
+        x0, x1, z0, z1 = window
+        true = {k: v[z0:z1, x0:x1].copy() for k, v in true.items()}
+        start = {k: v[z0:z1, x0:x1].copy() for k, v in start.items()}
+        nx0 = x1 - x0
+        nrec = nx0 // 4
     srcs = [(int((k + 0.5) * nx0 / nshots), 4) for k in range(nshots)]
     step = nx0 // nrec
     recs = [(step // 2 + step * k, 2) for k in range(nrec)]
@@ -347,13 +358,19 @@ def gpr25(nt: int = 1000, seed: int = 1006, nx0=512, ny0=64, nz0=384, cpml=16) -
     return Problem("em25", nx, ny, nz, d, d, d, dt, nt, coef, prof, [shot])
 
 
-def aniso25(nt: int = 3000, seed: int = 1003, nx0=800, ny0=64, nz0=400, cpml=16) -> Problem:
+def aniso25(nt: int = 3000, seed: int = 1003, nx0=800, ny0=64, nz0=400, cpml=16,
+            window=None) -> Problem:
     """2.5-D elastic: 800 x 64 x 400, dx = 1 m, C-PML 16; 8 seeded Voronoi
     regions of ice single-crystal stiffness rotated to a c-axis of dip
     U[0, 60] deg and azimuth U[0, 360] deg (21 Cij), rho 917, attenuation
     gamma ~ U[0, 5e-4] per region on every velocity component; 100 Hz
     vertical force at (400, 32, 10); 200 receivers at z = 2, y = 32, x = 4k;
-    dt from the reference formula with CFL 1/sqrt(3)."""
+    dt from the reference formula with CFL 1/sqrt(3).
+
+    ``window`` = (x0, x1, z0, z1): the same model cut to cells x0..x1-1,
+    z0..z1-1 (with its own C-PML), the dt of the full model, the source and
+    the receivers that fall inside the window (source coordinates shift with
+    the window)."""
     rng = np.random.default_rng(seed)
     d = 1.0
     nreg = 8
@@ -377,6 +394,17 @@ def aniso25(nt: int = 3000, seed: int = 1003, nx0=800, ny0=64, nz0=400, cpml=16)
         for n in np.eye(3):
             Gam = np.einsum("ijkl,j,l->ik", voigt_to_tensor(C6), n, n) / 917.0
             vmax = max(vmax, float(np.sqrt(np.max(np.linalg.eigvalsh(Gam)))))
+    sx, sz = 400, 10
+    rx = 4 * np.arange(200)
+    if window is not None:
+        x0, x1, z0, z1 = window
+        C = {k: v[z0:z1, x0:x1] for k, v in C.items()}
+        gam = gam[z0:z1, x0:x1]
+        nx0, nz0 = x1 - x0, z1 - z0
+        sx, sz = sx - x0, sz - z0
+        rx = rx[(rx >= x0) & (rx < x1)] - x0
+        if not (0 <= sx < nx0 and 0 <= sz < nz0 and 0 <= 2 - z0 < nz0):
+            raise ValueError("the window must hold the source and the receiver line")
     coef = {k: _pad_edge(v, cpml) for k, v in C.items()}
     gpad = _pad_edge(gam, cpml)
     nx, ny, nz = nx0 + 2 * cpml, ny0 + 2 * cpml, nz0 + 2 * cpml
@@ -387,8 +415,9 @@ def aniso25(nt: int = 3000, seed: int = 1003, nx0=800, ny0=64, nz0=400, cpml=16)
     prof = {a: axis_profile(n, cpml, d, sm, am, 5.0, 2, 2, dt, True)
             for a, n in (("x", nx), ("y", ny), ("z", nz))}
     fx, fy, fz = point_force(nt, dt, f0, 1.0, "gaus1", xz_rotation=90.0)
-    rec = np.array([[4 * k + cpml, 32 + cpml, 2 + cpml] for k in range(200)], dtype=np.int32)
-    shot = Shot(src=(400 + cpml, 32 + cpml, 10 + cpml), force=(fx, fy, fz), rec=rec)
+    rz = 2 - (window[2] if window is not None else 0)
+    rec = np.array([[x + cpml, 32 + cpml, rz + cpml] for x in rx], dtype=np.int32)
+    shot = Shot(src=(sx + cpml, 32 + cpml, sz + cpml), force=(fx, fy, fz), rec=rec)
     return Problem("elastic25", nx, ny, nz, d, d, d, dt, nt, coef, prof, [shot],
                    gamma={"x": gpad, "y": gpad, "z": gpad})
 

@@ -2311,6 +2311,15 @@ constexpr int XW25 = SDT_EL25_XW;      // wavefronts side by side in x per block
 constexpr int NT25 = 64 * KB25 * XW25;
 constexpr int Y25 = SDT_EL25_ROWS;     // y rows per work item
 
+// The y chunk [j0, j1) of a 2.5-D work item travels in one int as two
+// unsigned 16-bit halves (ny <= 65535 is checked at plan creation); the
+// decode shifts as unsigned so chunks ending at or above 32768 stay positive.
+__host__ __device__ inline int ychunk_pack(int j0, int j1) {
+  return (int)((unsigned)j0 | ((unsigned)j1 << 16));
+}
+__host__ __device__ inline int ychunk_j0(int w) { return (int)((unsigned)w & 0xffffu); }
+__host__ __device__ inline int ychunk_j1(int w) { return (int)((unsigned)w >> 16); }
+
 struct Slab3 {
   long SX, SY, SZ;       // elements per memory-variable array on each axis
   long bx, by, bz;       // base offsets of the x, y, z groups
@@ -2362,7 +2371,7 @@ __device__ __forceinline__ void el25_stress_tile(const DModel* __restrict__ mode
   const bool incol = (i >= 0 && i < nx);
   const int ic = incol ? i : (i < 0 ? 0 : nx - 1);  // clamped column: loads stay in bounds
   const bool own = live && incol && lane >= 1 && lane <= EW25;
-  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff, j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
+  const int j0 = ychunk_j0(__builtin_amdgcn_readfirstlane(wv.w)), j1 = ychunk_j1(__builtin_amdgcn_readfirstlane(wv.w));
   const long nxy = (long)nx * ny;
   const uint32_t p = (uint32_t)k * nx + ic;
   const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz), dt = T(M.dt);
@@ -2544,7 +2553,7 @@ k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots
   const DModel& M = models[shots[s].model];
   const int xo = __builtin_amdgcn_readfirstlane(wv.y) + (int)((threadIdx.x >> 6) % XW25) * EW25;
   const int k = __builtin_amdgcn_readfirstlane(wv.z) + (int)(threadIdx.x >> 6) / XW25;
-  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff, j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
+  const int j0 = ychunk_j0(__builtin_amdgcn_readfirstlane(wv.w)), j1 = ychunk_j1(__builtin_amdgcn_readfirstlane(wv.w));
   const bool interior = xo - 1 >= M.lx && xo + 62 < M.nx - M.lx && k >= M.lz && k < M.nz - M.lz &&
                         j0 >= M.ly && j1 <= M.ny - M.ly;
   const int it = *base + rel;
@@ -2573,7 +2582,7 @@ __device__ __forceinline__ void el25_vel_tile(const DModel* __restrict__ models,
   const bool incol = (i >= 0 && i < nx);
   const int ic = incol ? i : (i < 0 ? 0 : nx - 1);
   const bool own = live && incol && lane >= 1 && lane <= EW25;
-  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff, j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
+  const int j0 = ychunk_j0(__builtin_amdgcn_readfirstlane(wv.w)), j1 = ychunk_j1(__builtin_amdgcn_readfirstlane(wv.w));
   const int it = *base + rel;
   const long nxy = (long)nx * ny;
   const uint32_t p = (uint32_t)k * nx + ic;
@@ -2679,7 +2688,7 @@ k_el25_vel(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   const DModel& M = models[shots[s].model];
   const int xo = __builtin_amdgcn_readfirstlane(wv.y) + (int)((threadIdx.x >> 6) % XW25) * EW25;
   const int k = __builtin_amdgcn_readfirstlane(wv.z) + (int)(threadIdx.x >> 6) / XW25;
-  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff, j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
+  const int j0 = ychunk_j0(__builtin_amdgcn_readfirstlane(wv.w)), j1 = ychunk_j1(__builtin_amdgcn_readfirstlane(wv.w));
   const bool interior = xo - 1 >= M.lx && xo + 62 < M.nx - M.lx && k >= M.lz && k < M.nz - M.lz &&
                         j0 >= M.ly && j1 <= M.ny - M.ly;
   if (interior)
@@ -2889,8 +2898,8 @@ __device__ __forceinline__ void em25_h_tile(const DModel* __restrict__ models,
   const int ic = incol ? i : (i < 0 ? 0 : nx - 1);
   // H lives on i <= nx-2, j <= ny-2, k <= nz-2 (the loop ranges of :901-962)
   const bool own = incol && lane >= 1 && lane <= EW25 && i <= nx - 2 && k <= nz - 2;
-  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff;
-  const int j1 = min(__builtin_amdgcn_readfirstlane(wv.w) >> 16, ny - 1);
+  const int j0 = ychunk_j0(__builtin_amdgcn_readfirstlane(wv.w));
+  const int j1 = min(ychunk_j1(__builtin_amdgcn_readfirstlane(wv.w)), ny - 1);
   const long nxy = (long)nx * ny;
   const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz), dbH = T(M.scal[0]);
   const int lx = M.lx, ly = M.ly, lz = M.lz;
@@ -2970,8 +2979,8 @@ __device__ __forceinline__ void em25_e_tile(const DModel* __restrict__ models,
   const bool incol = (i >= 0 && i < nx);
   const int ic = incol ? i : (i < 0 ? 0 : nx - 1);
   const bool own = incol && lane >= 1 && lane <= EW25;
-  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff;
-  const int j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
+  const int j0 = ychunk_j0(__builtin_amdgcn_readfirstlane(wv.w));
+  const int j1 = ychunk_j1(__builtin_amdgcn_readfirstlane(wv.w));
   const long nxy = (long)nx * ny;
   const uint32_t p = (uint32_t)k * nx + ic;
   const T idx = T(1.0 / M.dx), idy = T(1.0 / M.dy), idz = T(1.0 / M.dz);
@@ -3069,7 +3078,7 @@ k_em25_h(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   const int s = __builtin_amdgcn_readfirstlane(wv.x);
   if (s < 0) return;
   const DModel& M = models[shots[s].model];
-  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff, j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
+  const int j0 = ychunk_j0(__builtin_amdgcn_readfirstlane(wv.w)), j1 = ychunk_j1(__builtin_amdgcn_readfirstlane(wv.w));
   if (em25_interior(M, __builtin_amdgcn_readfirstlane(wv.y), __builtin_amdgcn_readfirstlane(wv.z), j0, j1))
     em25_h_tile<T, false>(models, shots, wv);
   else
@@ -3085,7 +3094,7 @@ k_em25_e(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   if (s < 0) return;
   const DModel& M = models[shots[s].model];
   const int it = *base + rel;
-  const int j0 = __builtin_amdgcn_readfirstlane(wv.w) & 0xffff, j1 = __builtin_amdgcn_readfirstlane(wv.w) >> 16;
+  const int j0 = ychunk_j0(__builtin_amdgcn_readfirstlane(wv.w)), j1 = ychunk_j1(__builtin_amdgcn_readfirstlane(wv.w));
   if (em25_interior(M, __builtin_amdgcn_readfirstlane(wv.y), __builtin_amdgcn_readfirstlane(wv.z), j0, j1))
     em25_e_tile<T, false>(models, shots, wv, it);
   else
@@ -3131,7 +3140,7 @@ k_em25_edges(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   auto Hx = mp<T>(S.f[3]);
   auto Hy = mp<T>(S.f[4]);
   auto Hz = mp<T>(S.f[5]);
-  for (int j = wv.w & 0xffff; j < (wv.w >> 16); ++j) {
+  for (int j = ychunk_j0(wv.w); j < ychunk_j1(wv.w); ++j) {
     const long c = ((long)k * ny + j) * nx + i;
     const bool lo = (i == 0 || j == 0 || k == 0);
     if (lo || i == nx - 1 || j == ny - 2 || k == nz - 2) Hx[c] = T(0);
@@ -3453,7 +3462,7 @@ std::vector<int4> el25_work(const Plan* p, int planes, int xstep) {
     for (size_t c = 0; c + 1 < cuts.size(); ++c)
       for (int k0 = 0; k0 < M.nz; k0 += planes)
         for (int x0 = 0; x0 < M.nx; x0 += xstep)
-          all.push_back(make_int4(s, x0, k0, cuts[c] | (cuts[c + 1] << 16)));
+          all.push_back(make_int4(s, x0, k0, ychunk_pack(cuts[c], cuts[c + 1])));
   }
   const size_t run = (all.size() + 7) / 8;
   std::vector<int4> work(run * 8, make_int4(-1, 0, 0, 0));

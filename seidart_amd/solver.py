@@ -276,6 +276,26 @@ class Simulation:
         torch.cuda.synchronize(self.device)
         return res
 
+    def capture_graph(self, graph_block: int):
+        """Capture (and replay once) the hipGraph of ``graph_block`` steps that
+        later ``run(..., graph_block=graph_block)`` calls replay, then put the
+        state, traces and step counter back as they were: a run after this
+        equals a run without it, minus the capture cost.  Returns False
+        (doing nothing) when fewer than ``graph_block`` steps remain."""
+        if graph_block < 2 or self.nt - self.step < graph_block:
+            return False
+        keep = [{k: b[k].clone() for k in ("state", "state_b", "trace") if k in b}
+                for _, _, b in self.shots]
+        step0 = self.step
+        self.run(graph_block, graph_block=graph_block)
+        torch.cuda.synchronize(self.device)
+        self.step = step0
+        for (_, _, b), k in zip(self.shots, keep):
+            for n, t in k.items():
+                b[n].copy_(t)
+        torch.cuda.synchronize(self.device)
+        return True
+
     def check(self):
         """Synchronise; raise UnstableError if any shot blew up."""
         first = C.c_int(-1)

@@ -1,0 +1,252 @@
+"""The benchmark configurations themselves, at their real grids and lengths,
+through the C ABI.
+
+Bars: FP64 device traces (and, where stated, final fields) within 1e-10
+relative L2 of the FP64 oracle; FP32 device traces within 1e-5 relative L2
+of the FP64 device run of the same configuration; FWI gradients within 1e-3
+of central finite differences of the FP64 forward and within 1e-9 of torch
+FP64 autograd of the pure-PyTorch restatement.
+
+* plain_psv        2-D P-SV 256 x 256 + C-PML 20, 2000 steps (FP64 vs oracle,
+                   FP32 vs FP64).
+* gpr              2-D GPR 1024 x 768 + C-PML 20, 4000 steps (FP64 vs oracle,
+                   FP32 vs FP64).
+* aniso25          2.5-D, 21 Cij, 800 x 64 x 400 + C-PML 16: FP32 vs FP64 over
+                   all 3000 steps; FP64 vs oracle on a 128 x 64 x 64 window of
+                   the same model (the oracle's 2.5-D loops take ~1 s per step
+                   of the full grid on 16 cores).
+* batched_layered  96 FP32 shots in one plan vs each shot's own FP64
+                   single-shot run, 3000 steps; one shot of the 512 x 256
+                   model FP64 vs the oracle.
+* fwi_vti          1024 x 512 + C-PML 20, one shot, 6000 steps: FP64 and FP32
+                   adjoint gradients of C11 C13 C33 C55 rho vs central
+                   differences at 20 seeded cells per parameter; torch FP64
+                   autograd on a 128 x 64 window of the model.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import rel_l2
+from seidart_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FP32_TOL = 1e-5
+
+
+def _sim(p, precision, **kw):
+    from seidart_amd.solver import Simulation
+    return Simulation(p, precision=precision, **kw)
+
+
+def _run(p, precision, graph_block=50):
+    sim = _sim(p, precision)
+    sim.run(graph_block=graph_block).check()
+    return sim
+
+
+def _fields_match(sim, o, shot=0, tol=FP64_TOL):
+    for n in o.names:
+        exp = o.field(n)
+        if np.abs(exp).max() == 0:
+            continue
+        err = rel_l2(sim.field(shot, n), exp)
+        assert err < tol, (n, err)
+
+
+@pytest.mark.parametrize("name", ["plain_psv", "gpr"])
+def test_2d_config_fp64_matches_oracle_and_fp32_within_bar(oracle_lib, name):
+    p = getattr(configs, name)()
+    o = oracle_lib.OracleRun(p).run()
+    assert o.blowup < 0
+    s64 = _run(p, "fp64")
+    t64 = s64.traces()[0]
+    err64 = rel_l2(t64, o.trace)
+    print(f"{name}: FP64 traces vs oracle {err64:.3e} over {p.nt} steps")
+    assert err64 < FP64_TOL
+    _fields_match(s64, o)
+    s64.close()
+    s32 = _run(p, "fp32")
+    err32 = rel_l2(s32.traces()[0], t64)
+    print(f"{name}: FP32 traces vs FP64 {err32:.3e}")
+    assert err32 < FP32_TOL
+    s32.close()
+
+
+def test_aniso25_fp32_within_bar_full_run():
+    p = configs.aniso25()
+    t = {}
+    for prec in ("fp64", "fp32"):
+        s = _run(p, prec)
+        t[prec] = s.traces()[0]
+        s.close()
+        torch.cuda.empty_cache()
+    assert np.abs(t["fp64"]).max() > 0
+    err = rel_l2(t["fp32"], t["fp64"])
+    print(f"aniso25: FP32 traces vs FP64 {err:.3e} over {p.nt} steps, {p.nx}x{p.ny}x{p.nz}")
+    assert err < FP32_TOL
+
+
+def test_aniso25_window_fp64_matches_oracle(oracle_lib):
+    """The aniso25 model (8 rotated-ice regions, 21 Cij, attenuation) on a
+    128 x 64 x 64 window around the source, 240 steps (past the Ricker
+    peak at step ~122): traces and every final field."""
+    p = configs.aniso25(nt=240, window=(336, 464, 0, 64))
+    o = oracle_lib.OracleRun(p).run()
+    s = _run(p, "fp64", graph_block=40)
+    err = rel_l2(s.traces()[0], o.trace)
+    print(f"aniso25 window {p.nx}x{p.ny}x{p.nz}: FP64 traces vs oracle {err:.3e}")
+    assert np.abs(o.trace).max() > 0 and err < FP64_TOL
+    _fields_match(s, o)
+    s32 = _run(p, "fp32", graph_block=40)
+    assert rel_l2(s32.traces()[0], o.trace) < FP32_TOL
+
+
+def test_batched_layered_every_fp32_shot_against_its_fp64_single_shot_run():
+    from seidart_amd.problem import Problem
+    probs = configs.batched_layered()
+    sim = _sim(probs, "fp32")
+    sim.autotune(steps=8)
+    sim.run(graph_block=50).check()
+    t32 = sim.traces()
+    sim.close()
+    del sim
+    torch.cuda.empty_cache()
+    errs = []
+    k = 0
+    for p in probs:
+        for sh in p.shots:
+            single = Problem(p.kind, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, p.dt, p.nt, p.coef,
+                             p.prof, [sh], p.gamma)
+            s64 = _run(single, "fp64")
+            t64 = s64.traces()[0]
+            s64.close()
+            assert np.abs(t64).max() > 0
+            errs.append(rel_l2(t32[k], t64))
+            k += 1
+    print(f"batched_layered: {len(errs)} shots, FP32 batched vs FP64 single-shot rel L2 "
+          f"max {max(errs):.3e}, median {np.median(errs):.3e}")
+    assert k == 96
+    assert max(errs) < FP32_TOL
+
+
+def test_batched_layered_small_model_shot_fp64_matches_oracle(oracle_lib):
+    """Shot 16 of the 512 x 256 model over all 3000 steps: traces and final
+    fields against the oracle."""
+    from seidart_amd.problem import Problem
+    p = configs.batched_layered()[0]
+    single = Problem(p.kind, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, p.dt, p.nt, p.coef, p.prof,
+                     [p.shots[16]], p.gamma)
+    o = oracle_lib.OracleRun(single).run()
+    s = _run(single, "fp64")
+    err = rel_l2(s.traces()[0], o.trace)
+    print(f"batched_layered 512x256 shot 16: FP64 vs oracle {err:.3e}")
+    assert err < FP64_TOL
+    _fields_match(s, o)
+
+
+FWI = ("c11", "c13", "c33", "c55", "rho")
+PHYS = ("c11", "c13", "c15", "c33", "c35", "c55", "rho")
+
+
+def _fwi_problem(nt, shot=None, window=None, nshots=16):
+    from seidart_amd.problem import Problem
+    true, start = configs.fwi_vti(nt=nt, nshots=nshots, window=window)
+    if shot is not None:
+        true, start = (Problem(q.kind, q.nx, q.ny, q.nz, q.dx, q.dy, q.dz, q.dt, q.nt, q.coef,
+                               q.prof, [q.shots[shot]], q.gamma) for q in (true, start))
+        for q in (true, start):
+            q.cpml_split = True
+    return true, start
+
+
+def _grads(engine, p, obs):
+    from seidart_amd.adjoint import elastic2_traces
+    dev = engine.device
+    prm = {k: torch.tensor(p.coef[k], dtype=torch.float64, device=dev, requires_grad=k in FWI)
+           for k in PHYS}
+    tr = elastic2_traces(engine, [prm])
+    J = 0.5 * ((tr - obs.to(dev)) ** 2).sum()
+    J.backward()
+    return float(J.detach()), {k: prm[k].grad.double().cpu().numpy() for k in FWI}
+
+
+def test_fwi_vti_gradient_against_finite_differences():
+    """One shot of the FWI workload over all 6000 steps, checkpoints every 64
+    steps: the FP64 and FP32 adjoint gradients against central differences
+    of the FP64 forward (relative step 1e-4) at 20 seeded cells per
+    parameter, drawn among the cells whose gradient is at least 1% of that
+    parameter's largest (smaller ones are round-off bound)."""
+    from seidart_amd.adjoint import Elastic2Engine, elastic2_traces
+    true, start = _fwi_problem(6000, shot=8)
+    s = _run(true, "fp64", graph_block=64)
+    obs = torch.as_tensor(s.traces()[0], device="cuda")[None]
+    s.close()
+    eng = Elastic2Engine(start, "fp64", every=64)
+    J64, g64 = _grads(eng, start, obs)
+    e32 = Elastic2Engine(start, "fp32", every=64)
+    J32, g32 = _grads(e32, start, obs)
+    e32.close()
+    assert J64 > 0 and abs(J32 - J64) < 1e-3 * J64
+    base = {k: torch.tensor(start.coef[k], dtype=torch.float64, device="cuda") for k in PHYS}
+
+    def J(prm):
+        with torch.no_grad():
+            return float(0.5 * ((elastic2_traces(eng, [prm]) - obs) ** 2).sum())
+
+    rng = np.random.default_rng(2025)
+    worst = {}
+    for k in FWI:
+        g = g64[k]
+        cand = np.argwhere(np.abs(g) >= 0.01 * np.abs(g).max())
+        cells = cand[rng.choice(len(cand), size=min(20, len(cand)), replace=False)]
+        e64, e32_ = [], []
+        for j, i in cells:
+            h = 1e-4 * abs(float(start.coef[k][j, i]))
+            up = dict(base)
+            dn = dict(base)
+            up[k] = base[k].clone()
+            dn[k] = base[k].clone()
+            up[k][j, i] += h
+            dn[k][j, i] -= h
+            fd = (J(up) - J(dn)) / (2 * h)
+            e64.append(abs(g[j, i] - fd) / abs(fd))
+            e32_.append(abs(g32[k][j, i] - fd) / abs(fd))
+        worst[k] = (len(cells), max(e64), max(e32_))
+        print(f"fwi_vti FD {k}: {worst[k]}", flush=True)
+    print("fwi_vti FD check (cells, max rel err FP64 adjoint, FP32 adjoint):", worst)
+    assert all(n == 20 for n, _, _ in worst.values())
+    assert all(a < 1e-3 and b < 1e-3 for _, a, b in worst.values()), worst
+
+
+def test_fwi_vti_window_gradient_matches_torch_autograd():
+    """Two shots on a 128 x 64 window of the FWI model (its own C-PML 20,
+    split variant), 800 steps: GPU FP64 adjoint gradients and misfit against
+    torch FP64 autograd of the pure-PyTorch restatement, FP32 within 1e-3."""
+    from seidart_amd.adjoint import Elastic2Engine
+    from test_adjoint import _torch_ref_args
+    from torch_elastic2 import run_elastic2
+    win = (448, 576, 0, 64)
+    true, start = _fwi_problem(800, window=win, nshots=2)
+    s = _run(true, "fp64", graph_block=40)
+    obs = torch.as_tensor(np.stack(s.traces()), device="cuda")
+    s.close()
+    Jg, gg = _grads(Elastic2Engine(start, "fp64", every=64), start, obs)
+    prm = {k: torch.tensor(start.coef[k], dtype=torch.float64, requires_grad=k in FWI) for k in PHYS}
+    Jr = 0
+    for sh in range(len(start.shots)):
+        tr = run_elastic2(**_torch_ref_args(start, sh, params=prm))
+        Jr = Jr + 0.5 * ((tr - obs[sh].cpu()) ** 2).sum()
+    Jr.backward()
+    assert abs(Jg - float(Jr)) <= 1e-12 * abs(float(Jr))
+    for k in FWI:
+        ref = prm[k].grad.numpy()
+        assert np.abs(ref).max() > 0, k
+        err = rel_l2(gg[k], ref)
+        print(f"fwi_vti window: {k} GPU FP64 adjoint vs torch autograd {err:.3e}")
+        assert err < 1e-9, (k, err)
+    _, g32 = _grads(Elastic2Engine(start, "fp32", every=64), start, obs)
+    for k in FWI:
+        assert rel_l2(g32[k], gg[k]) < 1e-3, (k, rel_l2(g32[k], gg[k]))
