@@ -42,17 +42,20 @@ PROFILE_STEPS = 20
 def algorithmic_bytes(problems, nshots_by_model, itemsize):
     """Compulsory HBM bytes of one launch of the single-pass 2-D elastic step
     kernel over the whole batch: per shot, read and write vx vz sxx szz sxz
-    once and read and write each of the 8 C-PML memory arrays on its slab
-    cells once; per model (shared by its shots), read its coefficient arrays
-    once (up to 12: 8 staggered-node Cij averages, dt/rho x2, 1-gamma x2;
-    the c15/c35 averages and 1-gamma are absent for isotropic, lossless
-    models) plus the 1-D C-PML profiles."""
+    once and read and write each C-PML memory array on its slab cells once
+    (4 + 4 arrays with the legacy sharing, 6 + 6 with the split C-PML); per
+    model (shared by its shots), read its coefficient arrays once (up to 12:
+    8 staggered-node Cij averages, dt/rho x2, 1-gamma x2; the c15/c35
+    averages and 1-gamma are absent for isotropic, lossless models) plus the
+    1-D C-PML profiles."""
+    from seidart_amd import _hip
     from seidart_amd.problem import kernel_coefs
     tot = 0
     for p, S in zip(problems, nshots_by_model):
         n = p.nx * p.nz
         lx, _, lz = p.slabs()
-        psi = 4 * (p.nz * 2 * lx) + 4 * (2 * lz * p.nx)
+        kid = _hip.SDT_ELASTIC2 | (_hip.SDT_SPLIT_CPML if getattr(p, "cpml_split", False) else 0)
+        psi = _hip.psi_elems(kid, p.nx, p.ny, p.nz, lx, 0, lz)
         prof = 6 * p.nx + 7 * p.nz
         K = kernel_coefs(p)
         ncoef = 12 - (4 if all(not np.any(K[q]) for q in (2, 4, 5, 6)) else 0) \
@@ -79,7 +82,6 @@ def cpu_baseline(budget_s):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from seidart_amd import configs
-    from seidart_amd.problem import Problem
     pyoracle.build()
     try:
         cores = len(os.sched_getaffinity(0))
@@ -88,8 +90,7 @@ def cpu_baseline(budget_s):
     if os.environ.get("OMP_NUM_THREADS"):
         cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
     q = configs.batched_layered()[-1]
-    p = Problem(q.kind, q.nx, q.ny, q.nz, q.dx, q.dy, q.dz, q.dt, q.nt, q.coef, q.prof,
-                [q.shots[len(q.shots) // 2]], q.gamma)
+    p = q.with_shots([q.shots[len(q.shots) // 2]])
 
     def timed(threads, budget):
         o = pyoracle.OracleRun(p, threads=threads)
@@ -107,7 +108,7 @@ def cpu_baseline(budget_s):
     cells = p.nx * p.nz
     return {"value": round(cells * n / el / 1e9, 4), "unit": "Gcell-updates/s", "cores": cores,
             "kind": "port", "cpu_model": cpu_model(),
-            "sample": f"oracle (FP64 C/OpenMP restatement of seismic2) on the middle shot of the "
+            "sample": f"oracle (FP64 C/OpenMP restatement of seismic2, split C-PML) on the middle shot of the "
                       f"{p.nx}x{p.nz} padded model (2048x768 + C-PML 20), steps 2..{n + 1} of its "
                       f"{p.nt} in {el:.1f} s on {cores} threads; 1 thread: "
                       f"{cells * n1 / el1 / 1e9:.4f} Gcell-updates/s ({n1} steps in {el1:.1f} s)",
@@ -370,6 +371,9 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-legacy", action="store_true",
+                    help="skip the legacy-C-PML leg (throughput and FP32 error of the workload "
+                         "with the reference's shared C-PML memory arrays)")
     ap.add_argument("--no-fwi", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--tiles", default="",
@@ -394,7 +398,6 @@ def main(argv=None):
 
     import torch
     from seidart_amd import configs
-    from seidart_amd.problem import Problem
     from seidart_amd.solver import Simulation
 
     K, W = a.steps, a.warmup
@@ -489,8 +492,7 @@ def main(argv=None):
         for p in full:
             best = None
             for sh in p.shots:
-                single = Problem(p.kind, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, p.dt, p.nt, p.coef,
-                                 p.prof, [sh], p.gamma)
+                single = p.with_shots([sh])
                 sp = Simulation(single, precision="fp64")
                 if best is None:
                     r = sp.autotune(steps=12)   # tile shape and steps per launch (setup, untimed)
@@ -523,6 +525,51 @@ def main(argv=None):
                                        zip(np.cumsum([0] + shots[:-1]), shots)]}}
         print("bench: fp64 plain and accuracy done", file=sys.stderr, flush=True)
 
+    legacy = None
+    if not a.no_legacy:
+        # The same workload with the legacy sharing of the C-PML memory
+        # arrays (the reference's arithmetic): throughput over the same K
+        # timed steps, and the FP32 error of the middle shot of each model
+        # against its FP64 single-shot run over the full length, where the
+        # legacy C-PML's growing mode amplifies rounding differences.
+        lp = configs.batched_layered(nt=nt, cpml_split=False)
+        sl = Simulation(lp, precision="fp32")
+        if fixed:
+            sl.set_tiles(*fixed)
+        else:
+            sl.autotune(steps=10)
+        if gb:
+            sl.capture_graph(gb)
+        sl.run(W, graph_block=gb if W >= gb else 0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sl.run(K, graph_block=gb)
+        torch.cuda.synchronize()
+        lv = cells_per_step * K / (time.perf_counter() - t1) / 1e9
+        sl.check()
+        sl.close()
+        del sl
+        lfull = configs.batched_layered(nt=FULL, cpml_split=False)
+        sb = Simulation(lfull, precision="fp32")
+        sb.run(graph_block=50).check()
+        l32 = sb.traces()
+        sb.close()
+        del sb
+        lerr, off = [], 0
+        for p in lfull:
+            s = len(p.shots) // 2
+            single = p.with_shots([p.shots[s]])
+            sp = Simulation(single, precision="fp64")
+            sp.run(graph_block=50).check()
+            t64 = sp.traces()[0]
+            lerr.append(float(f"{np.linalg.norm(l32[off + s] - t64) / np.linalg.norm(t64):.3e}"))
+            sp.close()
+            off += len(p.shots)
+        legacy = {"value": round(lv, 3), "unit": "Gcell-updates/s", "dtype": "f32",
+                  "fp32_trace_rel_l2_vs_fp64_middle_shots": lerr, "steps": FULL}
+        torch.cuda.empty_cache()
+        print("bench: legacy C-PML leg done", file=sys.stderr, flush=True)
+
     fwi = None if a.no_fwi else fwi_gradient(nt=a.fwi_steps, fd_cells=a.fwi_fd, tiles=fixed)
     others = None if a.no_configs else config_lines(a.config_steps)
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
@@ -533,7 +580,7 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": "batched_layered: 2-D elastic, 96 shots (32 per model) over 3 "
                                "layered snow/firn/ice models 512x256, 1024x512, 2048x768 "
-                               f"(+C-PML 20), dx 0.5 m, 150 Hz, FP32, all shots in the same "
+                               f"(+C-PML 20, split memory arrays), dx 0.5 m, 150 Hz, FP32, all shots in the same "
                                f"launches, {K} timed steps after {W} warmup steps" +
                                (f", hipGraph blocks of {gb} steps (captured before timing)"
                                 if gb else ", direct launches"),
@@ -546,6 +593,7 @@ def main(argv=None):
         "gpu_ms_events": round(gpu_ms, 3),
         "roofline": roofline,
         "fp64_plain": plain,
+        "legacy_cpml": legacy,
         "fwi_grad": fwi,
         "other_configs": others,
         "cpu_baseline": cpu,

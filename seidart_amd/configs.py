@@ -236,23 +236,29 @@ def fwi_vti(nt: int = 6000, seed: int = 1005, nshots: int = 16, nx0: int = 1024,
     return out
 
 
-def plain_psv(nt: int = 2000, seed: int = 1001) -> Problem:
+def plain_psv(nt: int = 2000, seed: int = 1001, cpml_split: bool = True) -> Problem:
     """2-D P-SV, 256 x 256, dx = dz = 0.5 m, C-PML 20, 200 Hz vertical force at
     (128, 30), dt = 0.9 dx / (sqrt(2) Vmax), 64 receivers at z = 30 every
-    4 cells."""
+    4 cells.  ``cpml_split``: the split C-PML (stable; the default) or the
+    legacy sharing of the memory arrays, whose C-PML feeds a growing mode
+    (its energy rises by ~1e13 over the 2000 steps)."""
     rng = np.random.default_rng(seed)
     rho, vp, vs, _ = snow_firn_ice(256, 256, 0.5, rng, 40.0, 100.0)
     dt = 0.9 * 0.5 / (np.sqrt(2.0) * ICE[1])
     recs = [(2 + 4 * k, 30) for k in range(64)]
-    return _elastic2_problem(rho, vp, vs, 0.5, 0.5, 20, dt, nt, 200.0, [(128, 30)], recs)
+    p = _elastic2_problem(rho, vp, vs, 0.5, 0.5, 20, dt, nt, 200.0, [(128, 30)], recs)
+    p.cpml_split = bool(cpml_split)
+    return p
 
 
 def batched_layered(nt: int = 3000, seed: int = 1004, shots_per_model: int = 32,
-                    sizes=((512, 256), (1024, 512), (2048, 768))):
+                    sizes=((512, 256), (1024, 512), (2048, 768)), cpml_split: bool = True):
     """Three layered models of uneven size, dx = dz = 0.5 m, C-PML 20, 32 shots
     each at z = 10, x_k = (k + 1/2) nx / 32, 150 Hz vertical force, dt from the
     reference formula with CFL 1/sqrt(2) (classes.py:469-485), receivers every
-    4 cells at z = 10."""
+    4 cells at z = 10.  ``cpml_split``: the split C-PML (stable; the default)
+    or the legacy sharing of the memory arrays, which grows without bound
+    from step ~1200 on these models."""
     rng = np.random.default_rng(seed)
     dx = 0.5
     vmax = ICE[1]
@@ -263,6 +269,7 @@ def batched_layered(nt: int = 3000, seed: int = 1004, shots_per_model: int = 32,
         srcs = [(int((k + 0.5) * nx0 / shots_per_model), 10) for k in range(shots_per_model)]
         recs = [(x, 10) for x in range(2, nx0 - 1, 4)]
         out.append(_elastic2_problem(rho, vp, vs, dx, dx, 20, dt, nt, 150.0, srcs, recs))
+        out[-1].cpml_split = bool(cpml_split)
     return out
 
 

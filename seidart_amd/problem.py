@@ -128,6 +128,17 @@ class Problem:
     def cells(self) -> int:
         return self.nx * self.ny * self.nz
 
+    def with_shots(self, shots) -> "Problem":
+        """The same model (coefficients, profiles, attenuation, C-PML variant,
+        density method and other set-up attributes) with other shots; the
+        initial conditions are dropped unless there is one per new shot."""
+        import copy
+        q = copy.copy(self)
+        q.shots = list(shots)
+        if self.init is not None and len(self.init) != len(q.shots):
+            q.init = None
+        return q
+
     def slabs(self):
         lx = slab_width(self.prof["x"])
         lz = slab_width(self.prof["z"])

@@ -445,11 +445,135 @@ def write_trace_csv(trace2d: np.ndarray, basefile: str, channel: str, source_xyz
     return fn
 
 
+# Extents (z, [y,] x slices) of each snapshot channel as the legacy solver
+# writes it (write_image2 / write_image3 calls, electromagfdtd.f95:323-324,
+# :1086-1088; the elastic velocities are written on the full padded grid).
+def _legacy_extent(kind: str, ch: str):
+    if kind == "em2":
+        return {"Ex": (slice(None), slice(0, -1)), "Ez": (slice(0, -1), slice(None))}[ch]
+    if kind == "em25":
+        return {"Ex": (slice(None), slice(None), slice(0, -1)),
+                "Ey": (slice(None), slice(0, -1), slice(None)),
+                "Ez": (slice(0, -1), slice(None), slice(None))}[ch]
+    return (slice(None),) * (3 if kind.endswith("25") else 2)
+
+
+class SnapshotWriter:
+    """Snapshots of a running Simulation without stalling it.
+
+    At a snapshot step the compute stream records an event; a copy stream
+    waits for it, converts each channel's field into a device staging buffer
+    of the file's precision (FP32 ``sngl`` records by default,
+    readwrite_routines.f95:489-494) and copies that into a pinned host
+    buffer, while the compute stream waits only for the device-side staging
+    copy before its next steps.  A writer thread writes each record once its
+    copy has landed.  Two pinned buffers per channel alternate, so the host
+    write of one snapshot overlaps the device work of the next.
+
+    ``layout``: 'padded' writes every channel on the full padded grid (what
+    the reference's reader ``read_dat`` reshapes, definitions.py:77-136);
+    'legacy' writes each channel at the extent the legacy solver writes
+    (EM: Ex (nx-1) x nz, Ez nx x (nz-1), ...).  ``block``: collect the frames
+    for one EM25 block file instead of per-step records."""
+
+    def __init__(self, sim, p: Problem, outdir: str, single: bool, legacy_names: bool,
+                 layout: str = "padded", block: bool = False):
+        import queue
+        import threading
+        import torch
+        if layout not in ("padded", "legacy"):
+            raise ValueError("snapshot layout must be 'padded' or 'legacy'")
+        self.sim, self.p, self.outdir = sim, p, outdir
+        self.single, self.legacy_names, self.block = single, legacy_names, block
+        self.comps = TRACE_COMPONENTS[p.kind]
+        self.names = FIELDS[p.kind][: len(self.comps)]
+        si = getattr(p, "src_index", None) or p.shots[0].src
+        self.idx = (si[0], si[2]) if not p.is3d else tuple(si)
+        self.ext = {ch: (_legacy_extent(p.kind, ch) if layout == "legacy"
+                         else (slice(None),) * len(p.field_shape)) for ch in self.comps}
+        dt = torch.float32 if (single or block) else torch.float64
+        dev = sim.device
+        views = {ch: sim.shots[0][2][n][self.ext[ch]] for ch, n in zip(self.comps, self.names)}
+        self.views = views
+        self.staging = {ch: torch.empty(v.shape, dtype=dt, device=dev) for ch, v in views.items()}
+        self.host = [{ch: torch.empty(v.shape, dtype=dt, pin_memory=True) for ch, v in views.items()}
+                     for _ in range(2)]
+        self.copy_stream = torch.cuda.Stream(device=dev)
+        self.free = [threading.Semaphore(1), threading.Semaphore(1)]
+        self.q = queue.Queue()
+        self.frames = {ch: [] for ch in self.comps}
+        self.count = 0
+        self.error = None
+        self.slot = 0
+        self.thread = threading.Thread(target=self._loop, daemon=True)
+        self.thread.start()
+
+    def capture(self, step: int):
+        """Snapshot of the state after ``step`` steps (1-based file index)."""
+        import torch
+        if self.error is not None:
+            raise self.error
+        slot = self.slot
+        self.slot ^= 1
+        self.free[slot].acquire()               # the writer is done with this buffer
+        compute = torch.cuda.current_stream(self.sim.device)
+        done = torch.cuda.Event()
+        done.record(compute)
+        staged = torch.cuda.Event()
+        landed = torch.cuda.Event()
+        with torch.cuda.stream(self.copy_stream):
+            self.copy_stream.wait_event(done)
+            for ch in self.comps:
+                self.staging[ch].copy_(self.views[ch])
+            staged.record(self.copy_stream)
+            for ch in self.comps:
+                self.host[slot][ch].copy_(self.staging[ch], non_blocking=True)
+            landed.record(self.copy_stream)
+        compute.wait_event(staged)              # later steps overwrite the fields
+        self.q.put((slot, step, landed))
+
+    def _loop(self):
+        while True:
+            item = self.q.get()
+            if item is None:
+                return
+            slot, step, landed = item
+            try:
+                landed.synchronize()
+                for ch in self.comps:
+                    a = self.host[slot][ch].numpy()
+                    if self.block:
+                        self.frames[ch].append(a.copy())
+                        continue
+                    fn = os.path.join(self.outdir, snapshot_name(ch, step, self.idx, self.legacy_names))
+                    write_snapshot(fn, a, single=self.single)
+                    self.count += 1
+            except Exception as e:  # surfaced by capture / close
+                self.error = e
+            finally:
+                self.free[slot].release()
+
+    def close(self):
+        """Wait for every pending record; write the EM25 block file if any."""
+        self.q.put(None)
+        self.thread.join()
+        if self.error is not None:
+            raise self.error
+        if self.block and self.frames[self.comps[0]]:
+            # steps snapshot_every, 2*snapshot_every, ... stored as block steps 1, 2, ...
+            write_em25_block(os.path.join(self.outdir, em25_block_name(1, self.idx)),
+                             {ch: np.stack(v) for ch, v in self.frames.items()}, first_step=1)
+            self.count = 1
+        return self.count
+
+
 def _execute(p: Problem, precision: str, snapshot_every: int, snapshot_single: bool,
-             snapshot_dir: str, graph_block: int, legacy_names: bool, snapshot_format: str = "dat"):
+             snapshot_dir: str, graph_block: int, legacy_names: bool, snapshot_format: str = "dat",
+             snapshot_layout: str = "padded"):
     """Run one problem; write snapshots every ``snapshot_every`` steps, as one
     record per field and step (``"dat"``), or for 2.5-D EM optionally as one
-    int16 block file of the E fields (``"em25block"``)."""
+    int16 block file of the E fields (``"em25block"``).  Snapshot copies and
+    file writes overlap the following steps (SnapshotWriter)."""
     from .solver import Simulation
     from ._hip import UnstableError
     if snapshot_format not in ("dat", "em25block"):
@@ -459,29 +583,17 @@ def _execute(p: Problem, precision: str, snapshot_every: int, snapshot_single: b
     sim = Simulation(p, precision=precision)
     nsnap = 0
     if snapshot_every and snapshot_every > 0:
-        comps = TRACE_COMPONENTS[p.kind]
-        names = FIELDS[p.kind][: len(comps)]
-        si = getattr(p, "src_index", None) or p.shots[0].src
-        idx = (si[0], si[2]) if not p.is3d else tuple(si)
-        block = {ch: [] for ch in comps}
-        done = 0
-        while done < p.nt:
-            n = min(snapshot_every, p.nt - done)
-            sim.run(n, graph_block=graph_block if n >= graph_block else 0)
-            done += n
-            for ch, fname in zip(comps, names):
-                f = sim.field(0, fname)
-                if snapshot_format == "em25block":
-                    block[ch].append(np.asarray(f, dtype=np.float32))
-                    continue
-                write_snapshot(os.path.join(snapshot_dir, snapshot_name(ch, done, idx, legacy_names)),
-                               f, single=snapshot_single)
-                nsnap += 1
-        if snapshot_format == "em25block":
-            # steps snapshot_every, 2*snapshot_every, ... stored as block steps 1, 2, ...
-            write_em25_block(os.path.join(snapshot_dir, em25_block_name(1, idx)),
-                             {ch: np.stack(v) for ch, v in block.items()}, first_step=1)
-            nsnap = 1
+        w = SnapshotWriter(sim, p, snapshot_dir, snapshot_single, legacy_names, snapshot_layout,
+                           block=snapshot_format == "em25block")
+        try:
+            done = 0
+            while done < p.nt:
+                n = min(snapshot_every, p.nt - done)
+                sim.run(n, graph_block=graph_block if n >= graph_block else 0)
+                done += n
+                w.capture(done)
+        finally:
+            nsnap = w.close()
     else:
         sim.run(graph_block=graph_block)
     unstable = -1
@@ -497,12 +609,13 @@ def _execute(p: Problem, precision: str, snapshot_every: int, snapshot_single: b
 def run_files(jsonfile: str, seismic: bool = True, density_method: str = "legacy",
               precision: str = "fp64", snapshot_every: int = 0, snapshot_single: bool = True,
               receivers=None, graph_block: int = 50, raise_unstable: bool = True,
-              legacy_names: bool = False, snapshot_format: str = "dat") -> RunResult:
+              legacy_names: bool = False, snapshot_format: str = "dat",
+              snapshot_layout: str = "padded") -> RunResult:
     """The reference binary's call (classes.py:910-924) on the same files."""
     p = problem_from_files(jsonfile, seismic, density_method=density_method, receivers=receivers)
     wd = os.path.dirname(os.path.abspath(jsonfile))
     tr, unstable, nsnap = _execute(p, precision, snapshot_every, snapshot_single, wd, graph_block,
-                                   legacy_names, snapshot_format)
+                                   legacy_names, snapshot_format, snapshot_layout)
     csvs = []
     if receivers is not None and len(p.shots[0].rec):
         base = os.path.splitext(jsonfile)[0]
@@ -544,7 +657,7 @@ def run(model, domain=None, material=None, *, num_threads: int = 1, jsonfile: st
 def main(argv=None) -> int:
     """``python -m seidart_amd.run project.json seismic=true density_method=geometric
     [precision=fp32] [snapshot_every=N] [single=true] [receivers=file.xyz]
-    [snapshot_format=dat|em25block]``"""
+    [snapshot_format=dat|em25block] [snapshot_layout=padded|legacy] [legacy_names=false]``"""
     argv = list(sys.argv[1:] if argv is None else argv)
     if not argv:
         print(main.__doc__, file=sys.stderr)
@@ -560,7 +673,9 @@ def main(argv=None) -> int:
         res = run_files(jsonfile, seismic, dm, kw.get("precision", "fp64"),
                         int(kw.get("snapshot_every", "1")), kw.get("single", "true") == "true",
                         kw.get("receivers"), int(kw.get("graph_block", "50")),
-                        snapshot_format=kw.get("snapshot_format", "dat"))
+                        legacy_names=kw.get("legacy_names", "false").lower() == "true",
+                        snapshot_format=kw.get("snapshot_format", "dat"),
+                        snapshot_layout=kw.get("snapshot_layout", "padded"))
     except Exception as e:  # report like the reference solver, but with a status
         print(f"seidart_amd.run: {e}", file=sys.stderr)
         return 1

@@ -41,7 +41,43 @@ def case_problem(name):
     return p2
 
 
+# snapshot records of the legacy writer (write_image2, FP64 records of each
+# channel at its legacy extent) kept as raw bytes for these cases and steps
+SNAPSHOT_CASES = ("elastic2", "em2")
+SNAPSHOT_STEPS = (20, 40, 60, 80)
+
+
+def snapshot_fixtures():
+    """legacy_snapshots_{case}.npz: for each kept record, its channel, step
+    and the raw bytes of the file the legacy solver wrote (record markers
+    included)."""
+    import glob
+    import tempfile
+    for name in SNAPSHOT_CASES:
+        p = case_problem(name)
+        tmp = tempfile.mkdtemp(prefix="legacy_snap_")
+        legacy_ref.run_legacy(p, keep_dir=tmp)
+        out = {}
+        q = 0
+        for ch in ("Vx", "Vz") if name.startswith("elastic") else ("Ex", "Ez"):
+            for it in SNAPSHOT_STEPS:
+                files = glob.glob(os.path.join(tmp, f"{ch}.{it:06d}.*"))
+                assert len(files) == 1, files
+                with open(files[0], "rb") as fh:
+                    raw = fh.read()
+                out[f"channel_{q}"] = np.array(ch)
+                out[f"step_{q}"] = np.int64(it)
+                out[f"bytes_{q}"] = np.frombuffer(raw, dtype=np.uint8)
+                q += 1
+        for f in glob.glob(os.path.join(tmp, "*")):
+            os.remove(f)
+        os.rmdir(tmp)
+        np.savez_compressed(os.path.join(HERE, f"legacy_snapshots_{name}.npz"), **out)
+        print("snapshots", name, q)
+
+
 def main():
+    snapshot_fixtures()
     for name in CASES:
         p = case_problem(name)
         leg = legacy_ref.run_legacy(p)

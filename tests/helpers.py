@@ -130,3 +130,49 @@ def rel_l2(a, b):
     b = np.asarray(b, dtype=np.float64)
     den = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+
+
+def write_reference_project(p, tmp, cpml):
+    """A 2-D Problem as the reference's solver inputs (the files Model.build
+    leaves, classes.py:393-547): the project JSON, coefficient records,
+    legacy 1-D C-PML stream files, source records, and a receiver file
+    (x, y, z in metres).  The source sits where the file route places it
+    (0-based padded index xind + cpml - 1).  Returns (jsonfile, receiver
+    file)."""
+    import json
+    import os
+    from seidart_amd.fortranio import write_record, write_stream
+    from seidart_amd.problem import CONDUCTIVITY, PERMITTIVITY, STIFFNESS
+    seismic = p.kind.startswith("elastic")
+    i, _, k = (int(v) for v in p.shots[0].src)
+    xind, zind = i - cpml + 1, k - cpml + 1
+    src = {"dt": p.dt, "time_steps": p.nt, "x": xind * p.dx, "y": 0.0, "z": zind * p.dz,
+           "xind": xind, "yind": 0, "zind": zind, "source_frequency": 100.0, "x-z_rotation": 0,
+           "x-y_rotation": 0, "y-z_rotation": 0, "amplitude": 1.0, "source_wavelet": "gaus1",
+           "source_type": "ac"}
+    prj = {"Domain": {"dim": 2, "nx": p.nx - 2 * cpml, "ny": 1, "nz": p.nz - 2 * cpml, "dx": p.dx,
+                      "dy": 1, "dz": p.dz, "cpml": cpml, "nmats": 1, "alpha_max_scalar": 1.0,
+                      "kappa_max": 5, "NP": 2, "NPA": 2, "Rcoef": 0.001, "image_file": None},
+           "Seismic" if seismic else "Electromagnetic": {"Source": src}}
+    jf = os.path.join(tmp, "golden.json")
+    with open(jf, "w") as fh:
+        json.dump(prj, fh)
+    for n in (STIFFNESS if seismic else PERMITTIVITY + CONDUCTIVITY):
+        if n in p.coef:
+            write_record(os.path.join(tmp, f"{n}.dat"), p.coef[n])
+    for a, g in p.gamma.items():
+        write_record(os.path.join(tmp, f"gamma_{a}.dat"), g)
+    for a in ("x", "z"):
+        pr = p.prof[a]
+        for q, h, v in (("kappa", "", pr.kappa), ("acoef", "", pr.a), ("bcoef", "", pr.b),
+                        ("kappa", "_half", pr.kappa_h), ("acoef", "_half", pr.a_h),
+                        ("bcoef", "_half", pr.b_h)):
+            write_stream(os.path.join(tmp, f"{q}{a}{h}_cpml.dat"), v)
+    stem = "seismicsource" if seismic else "electromagneticsource"
+    for a, f in zip("xyz", p.shots[0].force):
+        write_record(os.path.join(tmp, f"{stem}{a}.dat"), np.zeros(p.nt) if f is None else f)
+    rec = np.asarray(p.shots[0].rec)
+    rf = os.path.join(tmp, "receivers.xyz")
+    np.savetxt(rf, np.stack([(rec[:, 0] - cpml) * p.dx, np.zeros(len(rec)), (rec[:, 2] - cpml) * p.dz], 1),
+               delimiter=",", header="x,y,z", comments="")
+    return jf, rf

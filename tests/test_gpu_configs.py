@@ -8,16 +8,17 @@ of central finite differences of the FP64 forward and within 1e-9 of torch
 FP64 autograd of the pure-PyTorch restatement.
 
 * plain_psv        2-D P-SV 256 x 256 + C-PML 20, 2000 steps (FP64 vs oracle,
-                   FP32 vs FP64).
+                   FP32 vs FP64), split and legacy C-PML.
 * gpr              2-D GPR 1024 x 768 + C-PML 20, 4000 steps (FP64 vs oracle,
                    FP32 vs FP64).
 * aniso25          2.5-D, 21 Cij, 800 x 64 x 400 + C-PML 16: FP32 vs FP64 over
-                   all 3000 steps; FP64 vs oracle on a 128 x 64 x 64 window of
-                   the same model (the oracle's 2.5-D loops take ~1 s per step
-                   of the full grid on 16 cores).
+                   the first 1000 steps (the legacy 2.5-D scheme grows without
+                   bound from step ~1200 on this model); FP64 vs oracle on a
+                   128 x 64 x 64 window of the same model (the oracle's 2.5-D
+                   loops take ~1 s per step of the full grid on 16 cores).
 * batched_layered  96 FP32 shots in one plan vs each shot's own FP64
-                   single-shot run, 3000 steps; one shot of the 512 x 256
-                   model FP64 vs the oracle.
+                   single-shot run, 3000 steps (split C-PML); one shot of the
+                   512 x 256 model FP64 vs the oracle (split and legacy).
 * fwi_vti          1024 x 512 + C-PML 20, one shot, 6000 steps: FP64 and FP32
                    adjoint gradients of C11 C13 C33 C55 rho vs central
                    differences at 20 seeded cells per parameter; torch FP64
@@ -56,15 +57,18 @@ def _fields_match(sim, o, shot=0, tol=FP64_TOL):
         assert err < tol, (n, err)
 
 
-@pytest.mark.parametrize("name", ["plain_psv", "gpr"])
-def test_2d_config_fp64_matches_oracle_and_fp32_within_bar(oracle_lib, name):
-    p = getattr(configs, name)()
+@pytest.mark.parametrize("name,split", [("plain_psv", True), ("plain_psv", False), ("gpr", None)])
+def test_2d_config_fp64_matches_oracle_and_fp32_within_bar(oracle_lib, name, split):
+    """Full runs: the split C-PML (the configurations' default) and the
+    legacy sharing of the memory arrays (the reference's arithmetic) for the
+    elastic grid, the EM grid as is."""
+    p = configs.gpr() if split is None else configs.plain_psv(cpml_split=split)
     o = oracle_lib.OracleRun(p).run()
     assert o.blowup < 0
     s64 = _run(p, "fp64")
     t64 = s64.traces()[0]
     err64 = rel_l2(t64, o.trace)
-    print(f"{name}: FP64 traces vs oracle {err64:.3e} over {p.nt} steps")
+    print(f"{name} (split C-PML {split}): FP64 traces vs oracle {err64:.3e} over {p.nt} steps")
     assert err64 < FP64_TOL
     _fields_match(s64, o)
     s64.close()
@@ -75,18 +79,36 @@ def test_2d_config_fp64_matches_oracle_and_fp32_within_bar(oracle_lib, name):
     s32.close()
 
 
-def test_aniso25_fp32_within_bar_full_run():
+def test_aniso25_fp32_within_bar_and_legacy_growth():
+    """The full 832 x 96 x 432 grid.  The 2.5-D scheme is the legacy one
+    (seismic25, kept for parity): its c16 / c45 terms as written
+    (seismicfdtd.f95:657, :766) make the operator non-symmetric and its
+    C-PML shares memory arrays between the four stress loops; on this model
+    both feed modes that grow from step ~1200 (the oracle shows each alone:
+    the growth stays with C-PML off, and with c16 = c45 = 0).  FP32 tracks
+    FP64 within the bar over the first 1000 steps; by step 3000 the FP64 run
+    itself is dominated by the growing mode (kinetic energy up > 1e12),
+    which amplifies rounding differences without bound."""
     p = configs.aniso25()
-    t = {}
-    for prec in ("fp64", "fp32"):
-        s = _run(p, prec)
-        t[prec] = s.traces()[0]
+    sims = {prec: _sim(p, prec) for prec in ("fp64", "fp32")}
+    energy = {}
+    for step in (1000, 3000):
+        for prec, s in sims.items():
+            s.run(step - s.step, graph_block=50)
+            energy[(prec, step)] = sum(float((s.shots[0][2][n].double() ** 2).sum())
+                                       for n in ("vx", "vy", "vz"))
+        if step == 1000:
+            t64 = sims["fp64"].traces()[0][:1000]
+            t32 = sims["fp32"].traces()[0][:1000]
+            assert np.abs(t64).max() > 0
+            err = rel_l2(t32, t64)
+            print(f"aniso25: FP32 traces vs FP64 {err:.3e} over 1000 steps, {p.nx}x{p.ny}x{p.nz}")
+            assert err < FP32_TOL
+    g = energy[("fp64", 3000)] / energy[("fp64", 1000)]
+    print(f"aniso25: FP64 kinetic energy step 3000 / step 1000 = {g:.3e}")
+    assert g > 1e6
+    for s in sims.values():
         s.close()
-        torch.cuda.empty_cache()
-    assert np.abs(t["fp64"]).max() > 0
-    err = rel_l2(t["fp32"], t["fp64"])
-    print(f"aniso25: FP32 traces vs FP64 {err:.3e} over {p.nt} steps, {p.nx}x{p.ny}x{p.nz}")
-    assert err < FP32_TOL
 
 
 def test_aniso25_window_fp64_matches_oracle(oracle_lib):
@@ -105,7 +127,8 @@ def test_aniso25_window_fp64_matches_oracle(oracle_lib):
 
 
 def test_batched_layered_every_fp32_shot_against_its_fp64_single_shot_run():
-    from seidart_amd.problem import Problem
+    """All 96 shots (split C-PML, the configuration's default) in one FP32
+    plan, each against its own FP64 single-shot plan, 3000 steps."""
     probs = configs.batched_layered()
     sim = _sim(probs, "fp32")
     sim.autotune(steps=8)
@@ -118,8 +141,7 @@ def test_batched_layered_every_fp32_shot_against_its_fp64_single_shot_run():
     k = 0
     for p in probs:
         for sh in p.shots:
-            single = Problem(p.kind, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, p.dt, p.nt, p.coef,
-                             p.prof, [sh], p.gamma)
+            single = p.with_shots([sh])
             s64 = _run(single, "fp64")
             t64 = s64.traces()[0]
             s64.close()
@@ -132,17 +154,16 @@ def test_batched_layered_every_fp32_shot_against_its_fp64_single_shot_run():
     assert max(errs) < FP32_TOL
 
 
-def test_batched_layered_small_model_shot_fp64_matches_oracle(oracle_lib):
+@pytest.mark.parametrize("split", [True, False])
+def test_batched_layered_small_model_shot_fp64_matches_oracle(oracle_lib, split):
     """Shot 16 of the 512 x 256 model over all 3000 steps: traces and final
-    fields against the oracle."""
-    from seidart_amd.problem import Problem
-    p = configs.batched_layered()[0]
-    single = Problem(p.kind, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, p.dt, p.nt, p.coef, p.prof,
-                     [p.shots[16]], p.gamma)
+    fields against the oracle, with the split C-PML and the legacy one."""
+    p = configs.batched_layered(cpml_split=split)[0]
+    single = p.with_shots([p.shots[16]])
     o = oracle_lib.OracleRun(single).run()
     s = _run(single, "fp64")
     err = rel_l2(s.traces()[0], o.trace)
-    print(f"batched_layered 512x256 shot 16: FP64 vs oracle {err:.3e}")
+    print(f"batched_layered 512x256 shot 16 (split C-PML {split}): FP64 vs oracle {err:.3e}")
     assert err < FP64_TOL
     _fields_match(s, o)
 
@@ -152,13 +173,9 @@ PHYS = ("c11", "c13", "c15", "c33", "c35", "c55", "rho")
 
 
 def _fwi_problem(nt, shot=None, window=None, nshots=16):
-    from seidart_amd.problem import Problem
     true, start = configs.fwi_vti(nt=nt, nshots=nshots, window=window)
     if shot is not None:
-        true, start = (Problem(q.kind, q.nx, q.ny, q.nz, q.dx, q.dy, q.dz, q.dt, q.nt, q.coef,
-                               q.prof, [q.shots[shot]], q.gamma) for q in (true, start))
-        for q in (true, start):
-            q.cpml_split = True
+        true, start = (q.with_shots([q.shots[shot]]) for q in (true, start))
     return true, start
 
 

@@ -99,11 +99,9 @@ def test_split_cpml_energy_bounded_on_fwi_model():
     (the device's 1e25 check trips)."""
     from seidart_amd._hip import UnstableError
     from seidart_amd import configs
-    from seidart_amd.problem import Problem
     _, q = configs.fwi_vti()
-    p = Problem(q.kind, q.nx, q.ny, q.nz, q.dx, q.dy, q.dz, q.dt, q.nt, q.coef, q.prof,
-                [q.shots[8]], q.gamma)
-    p.cpml_split = True
+    p = q.with_shots([q.shots[8]])
+    assert p.cpml_split
     sim = _sim(p, "fp32")
     e = []
     for _ in range(12):

@@ -353,3 +353,54 @@ def test_oracle_split_cpml_energy_decays(oracle_lib):
     assert max(s[3:]) < 1e-7 * max(s[:2])
     assert all(b <= a * 1.0001 for a, b in zip(s[3:], s[4:])), s
     assert e[False][-1] > 1e6 * max(e[False][:2])
+
+
+@pytest.mark.parametrize("name", ["elastic2", "em2"])
+def test_legacy_snapshot_fixtures_match_oracle(oracle_lib, name):
+    """The legacy writer's snapshot records (tests/golden/legacy_snapshots_*):
+    one Fortran record of FP64 values at the channel's legacy extent (EM:
+    Ex (nx-1) x nz, Ez nx x (nz-1); readwrite_routines.f95:473-497), equal to
+    the oracle's fields after those steps."""
+    import struct
+    g = np.load(os.path.join(GOLDEN, f"legacy_snapshots_{name}.npz"))
+    p = case_problem(name)
+    n = len([k for k in g.files if k.startswith("bytes_")])
+    steps = sorted({int(g[f"step_{q}"]) for q in range(n)})
+    o = oracle_lib.OracleRun(p)
+    done = 0
+    fields = {}
+    for it in steps:
+        o.run(done, it - done)
+        done = it
+        for ch in ("Vx", "Vz", "Ex", "Ez"):
+            if CHAN[ch] in o.names:
+                fields[(ch, it)] = o.field(CHAN[ch]).copy()
+    ext = {"Vx": (slice(None), slice(None)), "Vz": (slice(None), slice(None)),
+           "Ex": (slice(None), slice(0, -1)), "Ez": (slice(0, -1), slice(None))}
+    for q in range(n):
+        ch, it, raw = str(g[f"channel_{q}"]), int(g[f"step_{q}"]), g[f"bytes_{q}"].tobytes()
+        exp = fields[(ch, it)][ext[ch]]
+        (m,) = struct.unpack("<i", raw[:4])
+        assert m == exp.size * 8 == len(raw) - 8 and struct.unpack("<i", raw[-4:])[0] == m
+        got = np.frombuffer(raw[4:-4], dtype=np.float64).reshape(exp.shape)
+        assert np.abs(exp).max() > 0
+        assert rel_l2(got, exp) < 1e-13, (ch, it)
+
+
+@pytest.mark.parametrize("name", ["elastic2", "em2"])
+def test_reference_project_files_give_the_golden_problem(tmp_path, name):
+    """The golden case written in the reference's file contract reads back
+    (seidart_amd.run.problem_from_files) as the same Problem."""
+    from helpers import write_reference_project
+    from seidart_amd import run as dropin
+    p = case_problem(name)
+    jf, rf = write_reference_project(p, str(tmp_path), cpml=CASES[name]["npml"])
+    q = dropin.problem_from_files(jf, seismic=p.kind.startswith("elastic"), receivers=rf)
+    assert (q.nx, q.nz, q.nt, q.dt) == (p.nx, p.nz, p.nt, p.dt)
+    assert tuple(q.shots[0].src) == tuple(p.shots[0].src)
+    np.testing.assert_array_equal(q.shots[0].rec, p.shots[0].rec)
+    for k, v in p.coef.items():
+        np.testing.assert_array_equal(q.coef[k], v)
+    for a in ("x", "z"):
+        for f in ("kappa", "a", "b", "kappa_h", "a_h", "b_h"):
+            np.testing.assert_array_equal(getattr(q.prof[a], f), getattr(p.prof[a], f))
