@@ -198,6 +198,27 @@ int sdt_plan_get_steps_per_launch(void* plan, int* steps);
  * gets the step per shot (-1 = stable). */
 int sdt_plan_check(void* plan, int* first_step, int* per_shot);
 
+/* Allocation-free run.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  One call does what
+ * sdt_plan_create + sdt_plan_run(it0, nsteps, graph_block 0) +
+ * sdt_plan_check + sdt_plan_destroy do, with the same kernels and results,
+ * on the caller's buffers (fields, psi, field_b / psi_b, sources, receivers,
+ * traces), but calls no hipMalloc / hipFree: the run's descriptors, work
+ * lists, receiver index, per-shot flags and step counters live in a static
+ * device array of sdt_noalloc_scratch_bytes() bytes per GPU (8 MiB), held
+ * until the process ends.  Inputs whose descriptors and work lists need more
+ * are rejected with SDT_EINVAL before any device work.  Every call on one
+ * GPU shares that array and resets what it uses; calls that overlap on one
+ * GPU (other threads, other streams) race.  Kernels run on `stream`; the
+ * call synchronises it before returning, and reports instability like
+ * sdt_plan_check (first_step, per_shot optional).  No hipGraph is used;
+ * two steps per launch as in sdt_plan_create's default. */
+int sdt_run_noalloc(int kind, int precision, const sdt_model* models, int nmodels,
+                    const sdt_shot* shots, int nshots, int nt, int it0, int nsteps, void* stream,
+                    int* first_step, int* per_shot);
+long long sdt_noalloc_scratch_bytes(void);
+
 /* Measurement hook (no counterpart in the reference): run steps it0 ..
  * it0+nsteps-1 like sdt_plan_run (no graph), bracketing every kernel with
  * HIP events on `stream`, and return the mean duration in ms of

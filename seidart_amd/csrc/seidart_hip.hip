@@ -31,6 +31,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -3784,14 +3785,29 @@ void drop_adjoint_graphs(Plan* p) {
 // Replay (capturing on first use) the segment graph (kind, key) on st.
 // (Re)build the work list of the two-step ELASTIC2 kernel for the plan's
 // tile shape.
-int upload_work2(Plan* p) {
-  // ELASTIC2: interior blocks first, then the others; EM2: one list
+// Work list of the two-step kernels for the plan's tile shape.  ELASTIC2:
+// interior blocks first (their count in *ni), then the others; EM2: one list.
+std::vector<int4> build_work2(const Plan* p, int* ni) {
   const bool em = p->kind == SDT_EM2;
   std::vector<int4> work = em ? std::vector<int4>() : el2_work(p, EW2, 1);
-  const int ni = (int)work.size();
+  *ni = (int)work.size();
   const std::vector<int4> edge = em ? el2_work(p, EW2E, 0) : el2_work(p, EW2, 2);
   work.insert(work.end(), edge.begin(), edge.end());
   if (work.empty()) work.push_back(make_int4(-1, 0, 0, 0));
+  return work;
+}
+
+// Grid sizes of the two-step launches for a list of nw items, ni interior.
+void set_work2_grids(Plan* p, int nw, int ni) {
+  p->nwork2 = nw;
+  p->nwork2i = ni;
+  p->el2x2_grid_i = std::max(8, std::min(ni, SDT_EL2_GRID) / 8 * 8);
+  p->el2x2_grid = std::max(8, std::min(p->nwork2 - ni, SDT_EL2_GRID) / 8 * 8);
+}
+
+int upload_work2(Plan* p) {
+  int ni = 0;
+  const std::vector<int4> work = build_work2(p, &ni);
   int4* dw = nullptr;
   HIP_TRY(hipMalloc(&dw, sizeof(int4) * work.size()));
   if (hipMemcpy(dw, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice) != hipSuccess) {
@@ -3800,10 +3816,7 @@ int upload_work2(Plan* p) {
   }
   if (p->dwork2) (void)hipFree(p->dwork2);
   p->dwork2 = dw;
-  p->nwork2 = (int)work.size();
-  p->nwork2i = ni;
-  p->el2x2_grid_i = std::max(8, std::min(ni, SDT_EL2_GRID) / 8 * 8);
-  p->el2x2_grid = std::max(8, std::min(p->nwork2 - ni, SDT_EL2_GRID) / 8 * 8);
+  set_work2_grids(p, (int)work.size(), ni);
   return SDT_OK;
 }
 
@@ -3826,6 +3839,211 @@ int replay_segment(Plan* p, int kind, long long key, hipStream_t st, F&& enqueue
   p->seg_graphs.push_back(x);
   HIP_TRY(hipGraphLaunch(x.e, st));
   return SDT_OK;
+}
+
+// What a plan's device arrays hold, built on the host: model and shot
+// descriptors, work lists, the receivers-by-row index and the per-shot
+// sizes.  The descriptors' pointers into the plan's own arrays (blowup,
+// rrow, rxid) are set by stage_bind once those arrays have addresses.
+struct Stage {
+  std::vector<DModel> dm;
+  std::vector<DShot> ds;
+  std::vector<int4> work;       // one-step ELASTIC2 / EM2 list, or the 2.5-D list
+  std::vector<int4> work2;      // two-step list (ELASTIC2, EM2)
+  std::vector<int> rows_cols;   // receivers by row (ELASTIC2, EM2)
+  std::vector<long> rec_off;    // per shot offset into rows_cols, -1 = none
+  long long rows_cols_size = 0; // known before the receivers are read
+  std::vector<long> nel, npsi;
+};
+
+// Host half of plan creation: validate the inputs and fill the plan's
+// settings and everything of `S` except the receivers-by-row index.  Reads
+// no device memory.
+int stage_host(Plan* p, int kind, int prec, const sdt_model* models, int nmodels,
+               const sdt_shot* shots, int nshots, int nt, Stage& S) {
+  const bool split = (kind & SDT_SPLIT_CPML) != 0;
+  kind &= ~SDT_SPLIT_CPML;
+  if (split && kind != SDT_ELASTIC2) return fail(SDT_EINVAL, "SDT_SPLIT_CPML applies to ELASTIC2 only");
+  int rc = validate(kind, split, prec, models, nmodels, shots, nshots, nt);
+  if (rc) return rc;
+  p->kind = kind;
+  p->split = split;
+  p->prec = prec;
+  p->nmodels = nmodels;
+  p->nshots = nshots;
+  p->nt = nt;
+  p->ncomp = kind_3d(kind) ? 3 : 2;
+  p->hmodels.assign(models, models + nmodels);
+  p->hshots.assign(shots, shots + nshots);
+  S.rec_off.assign(nshots, -1);
+  for (int s = 0; s < nshots; ++s) {
+    p->maxrec = std::max(p->maxrec, shots[s].nrec);
+    if (pingpong(kind) && shots[s].nrec > 0) {
+      S.rec_off[s] = (long)S.rows_cols_size;
+      S.rows_cols_size += models[shots[s].model].nz + 1 + 2LL * shots[s].nrec;
+    }
+  }
+  S.dm.assign(nmodels, DModel());
+  for (int m = 0; m < nmodels; ++m) {
+    const sdt_model& M = models[m];
+    DModel& D = S.dm[m];
+    D.nx = M.nx; D.ny = M.ny; D.nz = M.nz;
+    D.lx = M.lx; D.ly = kind_3d(kind) ? M.ly : 0; D.lz = M.lz;
+    D.dx = M.dx; D.dy = M.dy; D.dz = M.dz; D.dt = M.dt;
+    for (int q = 0; q < 4; ++q) D.scal[q] = M.scal[q];
+    for (int q = 0; q < 40; ++q) D.coef[q] = M.coef[q];
+    for (int a = 0; a < 3; ++a)
+      for (int q = 0; q < 6; ++q) D.prof[a][q] = M.prof[a][q];
+    D.aux = M.aux;
+    p->maxnx = std::max(p->maxnx, M.nx);
+    p->maxnz = std::max(p->maxnz, M.nz);
+  }
+  S.ds.assign(nshots, DShot());
+  for (int s = 0; s < nshots; ++s) {
+    const sdt_shot& H = shots[s];
+    DShot& D = S.ds[s];
+    D.model = H.model; D.isrc = H.isrc; D.jsrc = kind_3d(kind) ? H.jsrc : 0; D.ksrc = H.ksrc;
+    for (int f = 0; f < 9; ++f) D.f[f] = H.field[f];
+    D.psi = H.psi;
+    for (int f = 0; f < 9; ++f) D.g[f] = H.field_b[f];
+    D.psi2 = H.psi_b;
+    for (int c = 0; c < 3; ++c) D.src[c] = H.src[c];
+    for (int c = 0; c < 6; ++c) D.msrc[c] = (kind == SDT_ELASTIC2 || kind == SDT_ELASTIC25) ? H.msrc[c] : nullptr;
+    D.nrec = H.nrec; D.rec = H.rec; D.trace = H.trace;
+    D.blowup = nullptr;
+    D.rrow = nullptr;
+    D.rxid = nullptr;
+  }
+  // work lists: (shot, x0, z0); all shots of one tile share blockIdx % 8 (XCD)
+  if (kind == SDT_ELASTIC25) S.work = el25_work(p, KB25, EW25 * XW25);
+  if (kind == SDT_EM25) S.work = el25_work(p, 1, EW25);  // one plane, one wavefront per item
+  if (pingpong(kind)) {
+    p->el2_eh = default_rows(p);
+    S.work = el2_work(p);
+    const int nblk = (int)S.work.size();
+    p->el2_grid = std::max(8, std::min(nblk, SDT_EL2_GRID) / 8 * 8);
+    int ni = 0;
+    S.work2 = build_work2(p, &ni);
+    set_work2_grids(p, (int)S.work2.size(), ni);
+    // Two steps per launch pay off once the plan has enough tiles to fill the
+    // device; a small plan is bound by the latency of one tile's walk, which
+    // the two-step kernel lengthens (measured: 4M cells per step and above).
+    long long cells = 0;
+    for (int s = 0; s < nshots; ++s) cells += (long long)models[shots[s].model].nx * models[shots[s].model].nz;
+    p->spl = cells >= SDT_EL2X2_MIN_CELLS ? 2 : 1;
+  }
+  if (S.work.empty()) S.work.push_back(make_int4(-1, 0, 0, 0));
+  p->nwork = (int)S.work.size();
+  S.nel.resize(nshots);
+  S.npsi.resize(nshots);
+  for (int s = 0; s < nshots; ++s) {
+    const sdt_model& M = models[shots[s].model];
+    S.nel[s] = (long)M.nx * M.ny * M.nz;
+    S.npsi[s] = (long)psi_elems(kind | (split ? SDT_SPLIT_CPML : 0), M.nx, M.ny, M.nz, M.lx,
+                                kind_3d(kind) ? M.ly : 0, M.lz);
+  }
+  return SDT_OK;
+}
+
+// Read every shot's receivers back from the device (synchronous copies),
+// check them against the grid, and build the receivers-by-row index of the
+// 2-D kinds: per shot rrow[nz + 1] then (column, index) pairs sorted by
+// (row, column).
+int stage_receivers(const Plan* p, Stage& S) {
+  S.rows_cols.clear();
+  S.rows_cols.reserve((size_t)S.rows_cols_size);
+  for (int s = 0; s < p->nshots; ++s) {
+    const sdt_shot& H = p->hshots[s];
+    if (H.nrec <= 0) continue;
+    std::vector<int> rec(3 * (size_t)H.nrec);
+    if (hipMemcpy(rec.data(), H.rec, rec.size() * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(SDT_EDEVICE, "cannot read receiver indices (not device memory?)");
+    const sdt_model& M = p->hmodels[H.model];
+    for (int r = 0; r < H.nrec; ++r) {
+      const int i = rec[3 * r], j = rec[3 * r + 1], k = rec[3 * r + 2];
+      if (i < 0 || i >= M.nx || k < 0 || k >= M.nz || j < 0 || j >= M.ny)
+        return fail(SDT_EINVAL, "shot " + std::to_string(s) + ": receiver " + std::to_string(r) +
+                                    " outside the padded grid");
+    }
+    if (S.rec_off[s] < 0) continue;
+    std::vector<int> order(H.nrec);
+    for (int r = 0; r < H.nrec; ++r) order[r] = r;
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+      return rec[3 * a + 2] != rec[3 * b + 2] ? rec[3 * a + 2] < rec[3 * b + 2] : rec[3 * a] < rec[3 * b];
+    });
+    std::vector<int> rrow(M.nz + 1, 0);
+    for (int r = 0; r < H.nrec; ++r) rrow[rec[3 * r + 2] + 1]++;
+    for (int k = 0; k < M.nz; ++k) rrow[k + 1] += rrow[k];
+    S.rows_cols.insert(S.rows_cols.end(), rrow.begin(), rrow.end());
+    for (int r : order) {
+      S.rows_cols.push_back(rec[3 * r]);
+      S.rows_cols.push_back(r);
+    }
+  }
+  return SDT_OK;
+}
+
+// Point each shot descriptor at its flag and its part of the receiver index.
+void stage_bind(const Plan* p, Stage& S) {
+  for (int s = 0; s < p->nshots; ++s) {
+    DShot& D = S.ds[s];
+    D.blowup = p->dflags + s;
+    D.rrow = S.rec_off[s] < 0 ? nullptr : p->drec + S.rec_off[s];
+    D.rxid = S.rec_off[s] < 0 ? nullptr : p->drec + S.rec_off[s] + p->hmodels[p->hshots[s].model].nz + 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fixed scratch of the allocation-free run (sdt_run_noalloc): descriptors,
+// work lists, receiver index, flags and step counters of one call, carved
+// from one static device array per GPU.  Host code never passes `scratch`
+// itself to the device: it uses the address hipGetSymbolAddress returns for
+// the current device (scratch_ptr).
+// ---------------------------------------------------------------------------
+#ifndef SDT_NOALLOC_SCRATCH_BYTES
+#define SDT_NOALLOC_SCRATCH_BYTES (8u << 20)
+#endif
+__device__ __align__(256) unsigned char scratch[SDT_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch (looked up once per device).
+int scratch_ptr(unsigned char** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SDT_EDEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(SDT_EDEVICE, "hipGetSymbolAddress returned null for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  return SDT_OK;
+}
+
+// Offsets of one call's arrays inside the scratch (256-byte aligned).
+struct ScratchLayout {
+  size_t models, shots, work, work2, rec, flags, ints, nel, npsi, total;
+};
+ScratchLayout scratch_layout(const Plan* p, const Stage& S) {
+  auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+  ScratchLayout L;
+  size_t o = 0;
+  L.models = o; o = up(o + sizeof(DModel) * p->nmodels);
+  L.shots = o;  o = up(o + sizeof(DShot) * p->nshots);
+  L.work = o;   o = up(o + sizeof(int4) * S.work.size());
+  L.work2 = o;  o = up(o + sizeof(int4) * S.work2.size());
+  L.rec = o;    o = up(o + sizeof(int) * (size_t)S.rows_cols_size);
+  L.flags = o;  o = up(o + sizeof(int) * p->nshots);
+  L.ints = o;   o = up(o + 2 * sizeof(int));
+  L.nel = o;    o = up(o + sizeof(long) * p->nshots);
+  L.npsi = o;   o = up(o + sizeof(long) * p->nshots);
+  L.total = o;
+  return L;
 }
 
 }  // namespace
@@ -3851,176 +4069,49 @@ int sdt_plan_create(int kind, int prec, const sdt_model* models, int nmodels,
                     const sdt_shot* shots, int nshots, int nt, void** out_plan) {
   if (!out_plan) return fail(SDT_EINVAL, "out_plan is null");
   *out_plan = nullptr;
-  const bool split = (kind & SDT_SPLIT_CPML) != 0;
-  kind &= ~SDT_SPLIT_CPML;
-  if (split && kind != SDT_ELASTIC2) return fail(SDT_EINVAL, "SDT_SPLIT_CPML applies to ELASTIC2 only");
-  int rc = validate(kind, split, prec, models, nmodels, shots, nshots, nt);
-  if (rc) return rc;
-  // Receiver indices must be checked on the host before any kernel reads them.
-  // They live in device memory, so copy each shot's list back once.
   Plan* p = new Plan();
-  p->kind = kind;
-  p->split = split;
-  p->prec = prec;
-  p->nmodels = nmodels;
-  p->nshots = nshots;
-  p->nt = nt;
-  p->ncomp = kind_3d(kind) ? 3 : 2;
-  p->hmodels.assign(models, models + nmodels);
-  p->hshots.assign(shots, shots + nshots);
   auto bail = [&](int code) {
     sdt_plan_destroy(p);
     return code;
   };
-  for (int s = 0; s < nshots; ++s) {
-    const sdt_shot& S = shots[s];
-    if (S.nrec <= 0) continue;
-    std::vector<int> rec(3 * (size_t)S.nrec);
-    if (hipMemcpy(rec.data(), S.rec, rec.size() * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
-      return bail(fail(SDT_EDEVICE, "cannot read receiver indices (not device memory?)"));
-    const sdt_model& M = models[S.model];
-    for (int r = 0; r < S.nrec; ++r) {
-      const int i = rec[3 * r], j = rec[3 * r + 1], k = rec[3 * r + 2];
-      if (i < 0 || i >= M.nx || k < 0 || k >= M.nz || j < 0 || j >= M.ny)
-        return bail(fail(SDT_EINVAL, "shot " + std::to_string(s) + ": receiver " +
-                                         std::to_string(r) + " outside the padded grid"));
-    }
-    p->maxrec = std::max(p->maxrec, S.nrec);
-  }
-  // receivers by row (two-step kernels): per shot rrow[nz + 1] then
-  // (column, index) pairs sorted by (row, column)
-  std::vector<int> rows_cols;
-  std::vector<long> rec_off(nshots, -1);
-  if (kind == SDT_ELASTIC2 || kind == SDT_EM2) {
-    for (int s = 0; s < nshots; ++s) {
-      const sdt_shot& S = shots[s];
-      if (S.nrec <= 0) continue;
-      std::vector<int> rec(3 * (size_t)S.nrec);
-      if (hipMemcpy(rec.data(), S.rec, rec.size() * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
-        return bail(fail(SDT_EDEVICE, "cannot read receiver indices"));
-      const int nz = models[S.model].nz;
-      std::vector<int> order(S.nrec);
-      for (int r = 0; r < S.nrec; ++r) order[r] = r;
-      std::sort(order.begin(), order.end(), [&](int a, int b) {
-        return rec[3 * a + 2] != rec[3 * b + 2] ? rec[3 * a + 2] < rec[3 * b + 2] : rec[3 * a] < rec[3 * b];
-      });
-      rec_off[s] = (long)rows_cols.size();
-      std::vector<int> rrow(nz + 1, 0);
-      for (int r = 0; r < S.nrec; ++r) rrow[rec[3 * r + 2] + 1]++;
-      for (int k = 0; k < nz; ++k) rrow[k + 1] += rrow[k];
-      rows_cols.insert(rows_cols.end(), rrow.begin(), rrow.end());
-      for (int r : order) {
-        rows_cols.push_back(rec[3 * r]);
-        rows_cols.push_back(r);
-      }
-    }
-    if (!rows_cols.empty()) {
-      if (hipMalloc(&p->drec, sizeof(int) * rows_cols.size()) != hipSuccess ||
-          hipMemcpy(p->drec, rows_cols.data(), sizeof(int) * rows_cols.size(), hipMemcpyHostToDevice) !=
-              hipSuccess)
-        return bail(fail(SDT_EDEVICE, "hipMalloc/hipMemcpy failed (receivers by row)"));
-    }
-  }
-  // device descriptors
-  std::vector<DModel> dm(nmodels);
-  for (int m = 0; m < nmodels; ++m) {
-    const sdt_model& M = models[m];
-    DModel& D = dm[m];
-    D.nx = M.nx; D.ny = M.ny; D.nz = M.nz;
-    D.lx = M.lx; D.ly = kind_3d(kind) ? M.ly : 0; D.lz = M.lz;
-    D.dx = M.dx; D.dy = M.dy; D.dz = M.dz; D.dt = M.dt;
-    for (int q = 0; q < 4; ++q) D.scal[q] = M.scal[q];
-    for (int q = 0; q < 40; ++q) D.coef[q] = M.coef[q];
-    for (int a = 0; a < 3; ++a)
-      for (int q = 0; q < 6; ++q) D.prof[a][q] = M.prof[a][q];
-    D.aux = M.aux;
+  Stage S;
+  int rc = stage_host(p, kind, prec, models, nmodels, shots, nshots, nt, S);
+  if (rc) return bail(rc);
+  rc = stage_receivers(p, S);
+  if (rc) return bail(rc);
+  if (!S.rows_cols.empty()) {
+    if (hipMalloc(&p->drec, sizeof(int) * S.rows_cols.size()) != hipSuccess ||
+        hipMemcpy(p->drec, S.rows_cols.data(), sizeof(int) * S.rows_cols.size(), hipMemcpyHostToDevice) !=
+            hipSuccess)
+      return bail(fail(SDT_EDEVICE, "hipMalloc/hipMemcpy failed (receivers by row)"));
   }
   if (hipMalloc(&p->dflags, sizeof(int) * nshots) != hipSuccess ||
       hipMalloc(&p->dbase, sizeof(int)) != hipSuccess || hipMalloc(&p->dit0, sizeof(int)) != hipSuccess)
     return bail(fail(SDT_EDEVICE, "hipMalloc failed (flags)"));
-  std::vector<DShot> ds(nshots);
-  for (int s = 0; s < nshots; ++s) {
-    const sdt_shot& S = shots[s];
-    DShot& D = ds[s];
-    D.model = S.model; D.isrc = S.isrc; D.jsrc = kind_3d(kind) ? S.jsrc : 0; D.ksrc = S.ksrc;
-    for (int f = 0; f < 9; ++f) D.f[f] = S.field[f];
-    D.psi = S.psi;
-    for (int f = 0; f < 9; ++f) D.g[f] = S.field_b[f];
-    D.psi2 = S.psi_b;
-    for (int c = 0; c < 3; ++c) D.src[c] = S.src[c];
-    for (int c = 0; c < 6; ++c) D.msrc[c] = (kind == SDT_ELASTIC2 || kind == SDT_ELASTIC25) ? S.msrc[c] : nullptr;
-    D.nrec = S.nrec; D.rec = S.rec; D.trace = S.trace;
-    D.blowup = p->dflags + s;
-    D.rrow = rec_off[s] < 0 ? nullptr : p->drec + rec_off[s];
-    D.rxid = rec_off[s] < 0 ? nullptr : p->drec + rec_off[s] + models[S.model].nz + 1;
-  }
-  // work list: (shot, x0, z0); all shots of one tile share blockIdx % 8 (XCD)
-  std::vector<int4> work;
-  if (kind == SDT_ELASTIC25) work = el25_work(p, KB25, EW25 * XW25);
-  if (kind == SDT_EM25) work = el25_work(p, 1, EW25);  // one plane, one wavefront per item
-  if (kind == SDT_ELASTIC2 || kind == SDT_EM2) {
-    p->el2_eh = default_rows(p);
-    work = el2_work(p);
-    const int nblk = (int)work.size();
-    p->el2_grid = std::max(8, std::min(nblk, SDT_EL2_GRID) / 8 * 8);
-  }
-  for (int m = 0; m < nmodels && !pingpong(kind) && !kind_3d(kind); ++m) {
-    const sdt_model& M = models[m];
-    const int wx = TX, wz = TZ;
-    const int tx = (M.nx + wx - 1) / wx, tz = (M.nz + wz - 1) / wz, nt_ = tx * tz;
-    std::vector<int> ms;
-    for (int s = 0; s < nshots; ++s)
-      if (shots[s].model == m) ms.push_back(s);
-    if (ms.empty()) continue;
-    for (int tg = 0; tg < (nt_ + 7) / 8; ++tg)
-      for (int s : ms)
-        for (int tt = 0; tt < 8; ++tt) {
-          const int t = tg * 8 + tt;
-          if (t < nt_) work.push_back(make_int4(s, (t % tx) * wx, (t / tx) * wz, 0));
-          else work.push_back(make_int4(-1, 0, 0, 0));
-        }
-  }
-  p->nwork = (int)work.size();
-  for (int m = 0; m < nmodels; ++m) {
-    p->maxnx = std::max(p->maxnx, models[m].nx);
-    p->maxnz = std::max(p->maxnz, models[m].nz);
-  }
-  {
-    std::vector<long> nel(nshots), npsi(nshots);
-    for (int s = 0; s < nshots; ++s) {
-      const sdt_model& M = models[shots[s].model];
-      nel[s] = (long)M.nx * M.ny * M.nz;
-      npsi[s] = (long)psi_elems(kind | (split ? SDT_SPLIT_CPML : 0), M.nx, M.ny, M.nz, M.lx,
-                                kind_3d(kind) ? M.ly : 0, M.lz);
-    }
-    if (hipMalloc(&p->dnel, sizeof(long) * nshots) != hipSuccess ||
-        hipMalloc(&p->dnpsi, sizeof(long) * nshots) != hipSuccess ||
-        hipMemcpy(p->dnel, nel.data(), sizeof(long) * nshots, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->dnpsi, npsi.data(), sizeof(long) * nshots, hipMemcpyHostToDevice) != hipSuccess)
-      return bail(fail(SDT_EDEVICE, "hipMalloc/hipMemcpy failed (sizes)"));
-  }
+  stage_bind(p, S);
+  if (hipMalloc(&p->dnel, sizeof(long) * nshots) != hipSuccess ||
+      hipMalloc(&p->dnpsi, sizeof(long) * nshots) != hipSuccess ||
+      hipMemcpy(p->dnel, S.nel.data(), sizeof(long) * nshots, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->dnpsi, S.npsi.data(), sizeof(long) * nshots, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(SDT_EDEVICE, "hipMalloc/hipMemcpy failed (sizes)"));
   if (hipMalloc(&p->dmodels, sizeof(DModel) * nmodels) != hipSuccess ||
       hipMalloc(&p->dshots, sizeof(DShot) * nshots) != hipSuccess ||
-      hipMalloc(&p->dwork, sizeof(int4) * work.size()) != hipSuccess)
+      hipMalloc(&p->dwork, sizeof(int4) * S.work.size()) != hipSuccess)
     return bail(fail(SDT_EDEVICE, "hipMalloc failed (descriptors)"));
-  if (hipMemcpy(p->dmodels, dm.data(), sizeof(DModel) * nmodels, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(p->dshots, ds.data(), sizeof(DShot) * nshots, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(p->dwork, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpy(p->dmodels, S.dm.data(), sizeof(DModel) * nmodels, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->dshots, S.ds.data(), sizeof(DShot) * nshots, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->dwork, S.work.data(), sizeof(int4) * S.work.size(), hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(SDT_EDEVICE, "hipMemcpy failed (descriptors)"));
+  if (!S.work2.empty()) {
+    if (hipMalloc(&p->dwork2, sizeof(int4) * S.work2.size()) != hipSuccess ||
+        hipMemcpy(p->dwork2, S.work2.data(), sizeof(int4) * S.work2.size(), hipMemcpyHostToDevice) !=
+            hipSuccess)
+      return bail(fail(SDT_EDEVICE, "hipMalloc/hipMemcpy failed (work list)"));
+  }
   hipLaunchKernelGGL(k_fill_int, dim3((nshots + 255) / 256), dim3(256), 0, nullptr, p->dflags,
                      nshots, INT_MAX);
   hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(1), 0, nullptr, p->dbase, 0, p->dit0, 0);
   if (hipDeviceSynchronize() != hipSuccess) return bail(fail(SDT_EDEVICE, "init kernels failed"));
-  if (kind == SDT_ELASTIC2 || kind == SDT_EM2) {
-    rc = upload_work2(p);
-    if (rc) return bail(rc);
-    // Two steps per launch pay off once the plan has enough tiles to fill the
-    // device; a small plan is bound by the latency of one tile's walk, which
-    // the two-step kernel lengthens (measured: 4M cells per step and above).
-    long long cells = 0;
-    for (int s = 0; s < nshots; ++s) cells += (long long)models[shots[s].model].nx * models[shots[s].model].nz;
-    p->spl = cells >= SDT_EL2X2_MIN_CELLS ? 2 : 1;
-  }
   if (hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(SDT_EDEVICE, "hipStreamCreate failed"));
   *out_plan = p;
@@ -4152,6 +4243,84 @@ int sdt_plan_check(void* plan, int* first_step, int* per_shot) {
     return fail(SDT_EUNSTABLE, "code became unstable and blew up at step " + std::to_string(first));
   return SDT_OK;
 }
+
+int sdt_run_noalloc(int kind, int prec, const sdt_model* models, int nmodels,
+                    const sdt_shot* shots, int nshots, int nt, int it0, int nsteps, void* stream,
+                    int* first_step, int* per_shot) {
+  if (first_step) *first_step = -1;
+  if (it0 < 0 || nsteps < 0 || (long long)it0 + nsteps > nt)
+    return fail(SDT_EINVAL, "step range outside [0, nt)");
+  // The plan lives on this stack frame; its device arrays point into the
+  // scratch, so nothing of it is freed (sdt_plan_destroy is never called).
+  Plan P;
+  Plan* p = &P;
+  Stage S;
+  int rc = stage_host(p, kind, prec, models, nmodels, shots, nshots, nt, S);
+  if (rc) return rc;
+  const ScratchLayout L = scratch_layout(p, S);
+  if (L.total > (size_t)SDT_NOALLOC_SCRATCH_BYTES)
+    return fail(SDT_EINVAL, "sdt_run_noalloc: this run needs " + std::to_string(L.total) +
+                                " bytes of scratch (descriptors, work lists, receiver index) but the "
+                                "library holds " + std::to_string((size_t)SDT_NOALLOC_SCRATCH_BYTES) +
+                                "; use sdt_plan_create / sdt_plan_run");
+  unsigned char* base = nullptr;
+  rc = scratch_ptr(&base);
+  if (rc) return rc;
+  rc = stage_receivers(p, S);
+  if (rc) return rc;
+  p->dmodels = reinterpret_cast<DModel*>(base + L.models);
+  p->dshots = reinterpret_cast<DShot*>(base + L.shots);
+  p->dwork = reinterpret_cast<int4*>(base + L.work);
+  p->dwork2 = S.work2.empty() ? nullptr : reinterpret_cast<int4*>(base + L.work2);
+  p->drec = S.rows_cols.empty() ? nullptr : reinterpret_cast<int*>(base + L.rec);
+  p->dflags = reinterpret_cast<int*>(base + L.flags);
+  p->dbase = reinterpret_cast<int*>(base + L.ints);
+  p->dit0 = p->dbase + 1;
+  p->dnel = reinterpret_cast<long*>(base + L.nel);
+  p->dnpsi = reinterpret_cast<long*>(base + L.npsi);
+  stage_bind(p, S);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  p->last = st;
+  // the host vectors outlive the copies: the call ends with a stream sync
+  HIP_TRY(hipMemcpyAsync(p->dmodels, S.dm.data(), sizeof(DModel) * nmodels, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(p->dshots, S.ds.data(), sizeof(DShot) * nshots, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(p->dwork, S.work.data(), sizeof(int4) * S.work.size(), hipMemcpyHostToDevice, st));
+  if (p->dwork2)
+    HIP_TRY(hipMemcpyAsync(p->dwork2, S.work2.data(), sizeof(int4) * S.work2.size(),
+                           hipMemcpyHostToDevice, st));
+  if (p->drec)
+    HIP_TRY(hipMemcpyAsync(p->drec, S.rows_cols.data(), sizeof(int) * S.rows_cols.size(),
+                           hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(p->dnel, S.nel.data(), sizeof(long) * nshots, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(p->dnpsi, S.npsi.data(), sizeof(long) * nshots, hipMemcpyHostToDevice, st));
+  // flags and step counters start fresh every call (the scratch keeps the
+  // previous call's contents); cleared by kernels, not hipMemsetAsync
+  hipLaunchKernelGGL(k_fill_int, dim3((nshots + 255) / 256), dim3(256), 0, st, p->dflags, nshots, INT_MAX);
+  hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(1), 0, st, p->dbase, it0, p->dit0, it0);
+  if (nsteps > 0) {
+    const bool two = use_two(p, nsteps, 0);
+    if (two)
+      launch_pairs(p, st, nsteps);
+    else
+      launch_steps(p, st, nsteps);
+    end_of_run(p, st, it0 + nsteps, two ? ((nsteps / 2 + (nsteps & 1)) & 1) : -1);
+  }
+  HIP_TRY(hipGetLastError());
+  std::vector<int> f(nshots);
+  HIP_TRY(hipMemcpyAsync(f.data(), p->dflags, sizeof(int) * nshots, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  int first = INT_MAX;
+  for (int s = 0; s < nshots; ++s) {
+    first = std::min(first, f[s]);
+    if (per_shot) per_shot[s] = f[s] == INT_MAX ? -1 : f[s];
+  }
+  if (first_step) *first_step = first == INT_MAX ? -1 : first;
+  if (first != INT_MAX)
+    return fail(SDT_EUNSTABLE, "code became unstable and blew up at step " + std::to_string(first));
+  return SDT_OK;
+}
+
+long long sdt_noalloc_scratch_bytes(void) { return (long long)SDT_NOALLOC_SCRATCH_BYTES; }
 
 int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows) {
   Plan* p = static_cast<Plan*>(plan);

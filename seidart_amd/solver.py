@@ -184,6 +184,26 @@ class Simulation:
         self.step += nsteps
         return self
 
+    def run_noalloc(self, nsteps: int | None = None, stream=None):
+        """Advance the next ``nsteps`` steps through the allocation-free entry
+        point (``sdt_run_noalloc``: the same kernels and results as ``run``,
+        its descriptors and work lists in the library's fixed device
+        scratch, no device allocation) on this Simulation's buffers, and
+        report instability like ``check``.  Synchronises the stream."""
+        if nsteps is None:
+            nsteps = self.nt - self.step
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        first = C.c_int(-1)
+        per = (C.c_int * len(self.shots))()
+        rc = _hip.lib().sdt_run_noalloc(self.kid, self.prec, self._models, len(self.problems),
+                                        self._shots, len(self.shots), self.nt, self.step, nsteps,
+                                        C.c_void_p(stream.cuda_stream), C.byref(first), per)
+        self.step += nsteps
+        self.unstable = [int(v) for v in per]
+        _hip.check(rc)
+        return self
+
     def profile(self, nsteps: int, stream=None):
         """Run the next ``nsteps`` steps with every kernel bracketed by HIP
         events; return mean ms per launch of (first field kernel, second
