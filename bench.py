@@ -362,6 +362,35 @@ def copy_calibration(gib: float) -> int:
     return 0
 
 
+def probe_calibration(gib: float) -> int:
+    """Run the library's HBM probe (a copy of `gib` GiB with one 4-byte
+    element per lane, the stencils' access width) 10 times: the known-traffic
+    kernel the byte counters of the stencil kernels are calibrated against,
+    beside torch's vectorised copy (--copy-gib)."""
+    import ctypes as C
+    import torch
+    from seidart_amd import _hip
+    n = int(gib * 2**30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for _ in range(3):
+        _hip.check(_hip.lib().sdt_hbm_probe(C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), n, st))
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        _hip.check(_hip.lib().sdt_hbm_probe(C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), n, st))
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    assert torch.equal(a, b)
+    print(json.dumps({"probe_bytes_read": n * 4, "probe_bytes_written": n * 4, "ms": round(ms, 4),
+                      "GBs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -388,9 +417,13 @@ def main(argv=None):
                          "many seeded cells per parameter (slow)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="only time a plain device copy of this many GiB (HBM calibration)")
+    ap.add_argument("--probe-gib", type=float, default=0.0,
+                    help="only run the library's 4-byte-per-lane HBM probe copy of this many GiB")
     a = ap.parse_args(argv)
     if a.copy_gib > 0:
         return copy_calibration(a.copy_gib)
+    if a.probe_gib > 0:
+        return probe_calibration(a.probe_gib)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the batch shards by shot across GPUs by design; this build measures one GPU)", file=sys.stderr)

@@ -116,8 +116,11 @@ typedef struct sdt_model {
  *      calls of sdt_plan_run (a run may be split into segments).
  * field_b, psi_b: ELASTIC2 and EM2 (NULL elsewhere): a second state buffer
  *      of the same sizes.  Their single-pass step kernels read one buffer and
- *      writes the other; at the end of every sdt_plan_run the state is back
- *      in field / psi, with field_b / psi_b holding scratch or a copy.
+ *      write the other; at the end of every sdt_plan_run the state is back
+ *      in field / psi, with field_b / psi_b holding scratch or a copy.  A
+ *      caller that rewrites field / psi between runs writes the same values
+ *      into field_b / psi_b (the plan may reuse B's copy for a one-step run
+ *      that starts at an odd step).
  * isrc, jsrc, ksrc: 0-based padded source cell (jsrc ignored in 2-D).
  * src[c]: nt per-step increments added to component c at the source cell
  *      after the update (seismicfdtd.f95:359-361: s(it)*dt/rho(src);
@@ -225,6 +228,13 @@ long long sdt_noalloc_scratch_bytes(void);
  * out_ms[0] the first field update (stress / H), out_ms[1] the second
  * (velocity / E) and out_ms[2] the receiver gather. */
 int sdt_plan_profile(void* plan, int it0, int nsteps, void* stream, double* out_ms);
+
+/* Measurement hook (no counterpart in the reference): enqueue on `stream` a
+ * copy of nfloat floats from src to dst (device memory), one 4-byte element
+ * per lane, the access width of the stencil kernels.  Its traffic is known
+ * (4 * nfloat bytes read and written), so the HBM byte counters a profiler
+ * reads for it calibrate the counters read for the stencil kernels. */
+int sdt_hbm_probe(const void* src, void* dst, long long nfloat, void* stream);
 
 /* ------------------------------------------------------------------------
  * Adjoint (gradients of a trace misfit).  No counterpart in the reference,

@@ -76,7 +76,7 @@ EXPORTS = ["sdt_psi_elems", "sdt_plan_create", "sdt_plan_run", "sdt_plan_check",
            "sdt_state_elems", "sdt_adjoint_work_elems", "sdt_adjoint_setup",
            "sdt_plan_run_trail", "sdt_adjoint_run", "sdt_plan_set_tiles", "sdt_plan_get_tiles",
            "sdt_plan_set_steps_per_launch", "sdt_plan_get_steps_per_launch", "sdt_run_noalloc",
-           "sdt_noalloc_scratch_bytes"]
+           "sdt_noalloc_scratch_bytes", "sdt_hbm_probe"]
 
 
 def lib():
@@ -130,6 +130,8 @@ def lib():
     L.sdt_run_noalloc.restype = C.c_int
     L.sdt_noalloc_scratch_bytes.argtypes = []
     L.sdt_noalloc_scratch_bytes.restype = C.c_longlong
+    L.sdt_hbm_probe.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p]
+    L.sdt_hbm_probe.restype = C.c_int
     _lib = L
     return L
 

@@ -655,6 +655,9 @@ constexpr int EW2 = 58;  // owned columns per wavefront
 #ifndef SDT_EL2X2_PREFETCH
 #define SDT_EL2X2_PREFETCH 0
 #endif
+#ifndef SDT_EL2X2_WAVES_SPLIT
+#define SDT_EL2X2_WAVES_SPLIT 3  // the same for the split C-PML variant's edge kernel
+#endif
 #ifndef SDT_EL2X2_WAVES_INT
 #define SDT_EL2X2_WAVES_INT 5  // the same for the interior-only kernel
 #endif
@@ -1062,7 +1065,7 @@ __device__ __forceinline__ void el2x2_tile(const DModel* __restrict__ models,
 // wavefronts per SIMD; otherwise each wavefront picks its body.
 template <typename T, bool SPLIT, int WB, bool INT>
 // (the split variant's C-PML body needs a larger budget: fewer wavefronts)
-__global__ void __launch_bounds__(64 * WB, sizeof(T) == 4 ? (INT ? SDT_EL2X2_WAVES_INT : SPLIT ? 3 : SDT_EL2X2_WAVES)
+__global__ void __launch_bounds__(64 * WB, sizeof(T) == 4 ? (INT ? SDT_EL2X2_WAVES_INT : SPLIT ? SDT_EL2X2_WAVES_SPLIT : SDT_EL2X2_WAVES)
                                                           : (INT ? 3 : SPLIT ? 1 : 2))
 k_el2_step2(const DModel* __restrict__ models, const DShot* __restrict__ shots,
             const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel,
@@ -3168,6 +3171,20 @@ __global__ void k_fill_int(int* p, int n, int v) {
 }
 
 // ===========================================================================
+// HBM probe: a copy of known traffic (4 n bytes read, 4 n written) whose
+// loads and stores are one 4-byte element per lane, consecutive lanes on
+// consecutive addresses -- the access width of the stencil kernels' loads.
+// Measurement only: it calibrates the byte counters the stencils' traffic
+// is read from.
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_hbm_probe(const float* __restrict__ src,
+                                                   float* __restrict__ dst, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
+// ===========================================================================
 // Plan
 // ===========================================================================
 struct Plan {
@@ -3190,6 +3207,7 @@ struct Plan {
   int* dit0 = nullptr;         // first step of the current run of two-step launches
   int* drec = nullptr;         // receivers by row of every shot (DShot::rrow, rxid)
   bool graph_two = false;      // the cached graph holds two-step launches
+  bool b_holds = false;        // buffer B holds the current state too (ELASTIC2 / EM2)
   int* dflags = nullptr;  // per shot first unstable step (INT_MAX = none)
   int* dbase = nullptr;   // step index of rel = 0
   long* dnel = nullptr;   // per shot: field elements (ping-pong copy-back)
@@ -3663,21 +3681,33 @@ int validate(int kind, bool split, int prec, const sdt_model* models, int nmodel
 }
 
 // State after the last step of a run: EM H edges zeroed; ping-pong kinds
-// copied back into buffer A when the last step wrote buffer B.  After a run
-// of two-step launches (final_b: whether it ended in B, else -1) A always
-// holds the state, and so does B before an odd step, as after one-step runs.
+// copied back into buffer A when the last step wrote buffer B (one-step runs
+// ending before an odd step; final_b: whether a run of two-step launches
+// ended in B, else -1).  A always holds the state afterwards; b_holds
+// records whether B does too (it did when the last step wrote B).
 void end_of_run(Plan* p, hipStream_t st, int next_step, int final_b = -1) {
   const bool b = pingpong(p->kind) && (final_b < 0 ? (next_step & 1) != 0 : final_b != 0);
-  const bool to_b = final_b == 0 && (next_step & 1);
+  p->b_holds = b;
   if (p->prec == SDT_FP64) {
     if (b) launch_copy_back<double>(p, st);
-    if (to_b) launch_copy_back<double>(p, st, 1);
     launch_finalize<double>(p, st);
   } else {
     if (b) launch_copy_back<float>(p, st);
-    if (to_b) launch_copy_back<float>(p, st, 1);
     launch_finalize<float>(p, st);
   }
+}
+
+// Before a run: the one-step kernels read the state of step it from buffer
+// it & 1, so a one-step run that starts at an odd step needs the state in B.
+// Copy A into B unless the previous run left it there (a run of two-step
+// launches always starts from A and needs nothing).
+void start_of_run(Plan* p, hipStream_t st, int it0, bool two) {
+  if (!pingpong(p->kind) || two || !(it0 & 1) || p->b_holds) return;
+  if (p->prec == SDT_FP64)
+    launch_copy_back<double>(p, st, 1);
+  else
+    launch_copy_back<float>(p, st, 1);
+  p->b_holds = true;
 }
 
 long long state_elems(int kind, long long nx, long long ny, long long nz, long long lx,
@@ -4127,6 +4157,7 @@ int sdt_plan_run(void* plan, int it0, int nsteps, int graph_block, void* stream)
   p->last = st;
   if (nsteps == 0) return SDT_OK;
   const bool two = use_two(p, nsteps, graph_block);
+  start_of_run(p, st, it0, two);
   hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(1), 0, st, p->dbase, it0, p->dit0, it0);
   int done = 0;
   if (graph_block > 1 && nsteps >= graph_block) {
@@ -4168,6 +4199,7 @@ int sdt_plan_profile(void* plan, int it0, int nsteps, void* stream, double* out_
   for (int q = 0; q < 4; ++q) HIP_TRY(hipEventCreate(&ev[q]));
   double acc[3] = {0, 0, 0};
   const bool two = use_two(p, nsteps, 0);
+  start_of_run(p, st, it0, two);
   hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(1), 0, st, p->dbase, it0, p->dit0, it0);
   int rc = SDT_OK;
   if (two) {
@@ -4299,6 +4331,7 @@ int sdt_run_noalloc(int kind, int prec, const sdt_model* models, int nmodels,
   hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(1), 0, st, p->dbase, it0, p->dit0, it0);
   if (nsteps > 0) {
     const bool two = use_two(p, nsteps, 0);
+    start_of_run(p, st, it0, two);
     if (two)
       launch_pairs(p, st, nsteps);
     else
@@ -4321,6 +4354,15 @@ int sdt_run_noalloc(int kind, int prec, const sdt_model* models, int nmodels,
 }
 
 long long sdt_noalloc_scratch_bytes(void) { return (long long)SDT_NOALLOC_SCRATCH_BYTES; }
+
+int sdt_hbm_probe(const void* src, void* dst, long long nfloat, void* stream) {
+  if (!src || !dst || nfloat <= 0) return fail(SDT_EINVAL, "sdt_hbm_probe: null buffer or empty size");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_hbm_probe, dim3(256 * 8 * 4), dim3(256), 0, st, static_cast<const float*>(src),
+                     static_cast<float*>(dst), nfloat);
+  HIP_TRY(hipGetLastError());
+  return SDT_OK;
+}
 
 int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows) {
   Plan* p = static_cast<Plan*>(plan);

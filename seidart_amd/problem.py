@@ -75,6 +75,23 @@ PERMITTIVITY = ["e11", "e12", "e13", "e22", "e23", "e33"]
 CONDUCTIVITY = ["s11", "s12", "s13", "s22", "s23", "s33"]
 
 
+def field_extent(kind: str, name: str):
+    """Index (slices over the C-ordered (z, [y,] x) array) of the cells a
+    field owns in the legacy arrays: the EM fields are staggered with one
+    cell fewer along their own axis (Ex (nx-1) x nz, Ez nx x (nz-1), Hy
+    (nx-1) x (nz-1); electromagfdtd.f95:100-102, :668-673); the elastic
+    fields span the whole padded grid.  Cells outside stay zero."""
+    full = slice(None)
+    short = slice(0, -1)
+    if kind == "em2":
+        return {"Ex": (full, short), "Ez": (short, full), "Hy": (short, short)}[name]
+    if kind == "em25":
+        return {"Ex": (full, full, short), "Ey": (full, short, full), "Ez": (short, full, full),
+                "Hx": (short, short, full), "Hy": (short, full, short),
+                "Hz": (full, short, short)}[name]
+    return (full,) * (3 if kind == "elastic25" else 2)
+
+
 def ncomp(kind: str) -> int:
     return 2 if kind in ("elastic2", "em2") else 3
 

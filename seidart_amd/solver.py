@@ -16,8 +16,8 @@ import numpy as np
 import torch
 
 from . import _hip
-from .problem import FIELDS, KINDS, Problem, aux_b_x_half_by_row, em_dbh, kernel_coefs, ncomp, \
-    source_increments, stress_source_increments
+from .problem import FIELDS, KINDS, Problem, aux_b_x_half_by_row, em_dbh, field_extent, kernel_coefs, \
+    ncomp, source_increments, stress_source_increments
 
 __all__ = ["Simulation", "PRECISIONS"]
 
@@ -132,7 +132,14 @@ class Simulation:
                     bufs["psi" + tag] = st[len(self.names) * nel:] if npsi else st[:1]
                 for n in self.names:
                     if n in init:
-                        bufs[n].copy_(T(np.asarray(init[n], dtype=np.float64).reshape(p.field_shape)))
+                        # initial values only where the field lives in the
+                        # legacy arrays; the staggered EM fields' last column
+                        # or row stays zero
+                        a = np.asarray(init[n], dtype=np.float64).reshape(p.field_shape)
+                        inside = np.zeros_like(a)
+                        ext = field_extent(self.kind, n)
+                        inside[ext] = a[ext]
+                        bufs[n].copy_(T(inside))
                 inc = source_increments(p, sh)
                 if all_sources and self.kind in ("elastic2", "em2"):
                     inc = [np.zeros(p.nt) if (v is None and c != 1) else v for c, v in enumerate(inc)]
@@ -264,16 +271,18 @@ class Simulation:
         if steps < 1:
             return {}
         spls = (2, 1)
-        keep = [(b["state"].clone(), b["state_b"].clone(), b["trace"].clone()) for _, _, b in self.shots]
+        keep = [(b["state"].clone(), b["trace"].clone()) for _, _, b in self.shots]
         step0 = self.step
         stream = torch.cuda.current_stream(self.device)
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
         def restore(traces=False):
+            # state back in A, and in B too: the plan may take a one-step run
+            # at an odd step from B's copy
             self.step = step0
-            for (_, _, b), (a0, b0, t0) in zip(self.shots, keep):
+            for (_, _, b), (a0, t0) in zip(self.shots, keep):
                 b["state"].copy_(a0)
-                b["state_b"].copy_(b0)
+                b["state_b"].copy_(a0)
                 if traces:
                     b["trace"].copy_(t0)
 
@@ -304,8 +313,7 @@ class Simulation:
         (doing nothing) when fewer than ``graph_block`` steps remain."""
         if graph_block < 2 or self.nt - self.step < graph_block:
             return False
-        keep = [{k: b[k].clone() for k in ("state", "state_b", "trace") if k in b}
-                for _, _, b in self.shots]
+        keep = [{k: b[k].clone() for k in ("state", "trace")} for _, _, b in self.shots]
         step0 = self.step
         self.run(graph_block, graph_block=graph_block)
         torch.cuda.synchronize(self.device)
@@ -313,6 +321,8 @@ class Simulation:
         for (_, _, b), k in zip(self.shots, keep):
             for n, t in k.items():
                 b[n].copy_(t)
+            if "state_b" in b:  # B holds the state too (see autotune)
+                b["state_b"].copy_(b["state"])
         torch.cuda.synchronize(self.device)
         return True
 
