@@ -293,6 +293,13 @@ def config_lines(survey_steps: int = 200):
                          "alg_GBs": round(gcs * bpc, 1), "frac_hbm": round(gcs * bpc / HBM_PEAK_GBS, 4)}
         t32, t64 = traces["fp32"], traces["fp64"]
         rec["fp32_trace_rel_l2_vs_fp64"] = float(f"{np.linalg.norm(t32 - t64) / np.linalg.norm(t64):.3e}")
+        if p.kind == "elastic25" and K > 1000:
+            # the legacy 2.5-D elastic scheme grows without bound (energy x1e6
+            # and more between steps 1000 and 3000, in FP64 as in FP32), so
+            # FP32 and FP64 part ways late in the run; the bar holds early
+            a, b = t32[:1000], t64[:1000]
+            rec["fp32_trace_rel_l2_vs_fp64_first_1000_steps"] = float(
+                f"{np.linalg.norm(a - b) / np.linalg.norm(b):.3e}")
         out[name] = rec
         print(f"bench: {name} done", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()

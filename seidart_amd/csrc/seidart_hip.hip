@@ -656,7 +656,10 @@ constexpr int EW2 = 58;  // owned columns per wavefront
 #define SDT_EL2X2_PREFETCH 0
 #endif
 #ifndef SDT_EL2X2_WAVES_SPLIT
-#define SDT_EL2X2_WAVES_SPLIT 3  // the same for the split C-PML variant's edge kernel
+// the same for the split C-PML variant's edge kernel: at 4 wavefronts it
+// spills a few registers and is still faster than at 3 (1.178 against
+// 1.228 ms per two-step launch pair of the 96-shot layered workload)
+#define SDT_EL2X2_WAVES_SPLIT 4
 #endif
 #ifndef SDT_EL2X2_WAVES_INT
 #define SDT_EL2X2_WAVES_INT 5  // the same for the interior-only kernel
