@@ -288,8 +288,9 @@ int sdt_adjoint_work_elems(int kind, int nx, int ny, int nz, int lx, int ly, int
  * again only to change buffers or options).  grad_mask: bit q set =
  * accumulate coefficient slot q (0xfff = all 12 of ELASTIC2, 0xf = the 4 of
  * EM2; slots nobody differentiates cost nothing when left out).  flags: 0, or SDT_ADJ_REFERENCE to run the
- * unfused three-kernel adjoint (one thread per cell per phase, scratch
- * planes in `work`), kept as a cross-check of the fused kernel. */
+ * unfused three-kernel adjoint (ELASTIC2 and EM2: one thread per cell per
+ * phase, scratch planes in `work`), kept as a cross-check of the fused
+ * kernels. */
 #define SDT_ADJ_REFERENCE 1
 int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots, unsigned grad_mask,
                       int flags);

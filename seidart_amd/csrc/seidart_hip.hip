@@ -2868,6 +2868,204 @@ k_em2_adj_g(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   Lout[n + c] = Lout[n + c] + idx * ((i >= 1 ? W[3 * n + c - 1] : T(0)) - W[3 * n + c]);
 }
 
+// ---------------------------------------------------------------------------
+// Fused EM2 adjoint step: the transpose of k_em2_adj_{e,h,g} in one pass,
+// the same arithmetic in the same order.  A wavefront owns AW columns
+// (lanes 2..AW+1; lane l holds column xo - 2 + l) of eh rows of one shot and
+// marches down them (eh rows: the plan picks the height that gives it
+// enough tiles): at march row q it does E^T of row q, H^T of row q-1
+// (which reads the E^T results of rows q-1 and q and of column i+1) and the
+// E gather of row q-1 (the H^T results of rows q-2 and q-1 and of column
+// i-1).  E^T at column i reads the new Hy of column i-1, so two halo lanes on
+// the left and one on the right give exact results on the owned lanes; the
+// halo row above the tile is recomputed, so tiles are independent (lambda
+// ping-pongs).  Only owned cells write lambda_t and add to the gradient.
+// ---------------------------------------------------------------------------
+template <typename T, bool PML>
+__device__ __forceinline__ void em2_adj_tile(const DModel* __restrict__ models,
+                                             const DShot* __restrict__ shots,
+                                             const DAdj* __restrict__ adj, const int s, const int xo,
+                                             const int zo, const int eh, const int t, const int t0,
+                                             const unsigned gmask, const int lane, const int tb) {
+  const DShot& S = shots[s];
+  const DModel& M = models[S.model];
+  const DAdj& A = adj[s];
+  const int nx = M.nx, nz = M.nz;
+  const int i = xo - 2 + lane;
+  const bool incol = (i >= 0 && i < nx);
+  const int ic = incol ? i : (i < 0 ? 0 : nx - 1);
+  const bool own = incol && lane >= 2 && lane < 2 + AW;
+  const int zend = min(zo + eh, nz);
+  // 32-bit element offsets within one state / gradient array (checked at setup)
+  const uint32_t n = (uint32_t)nx * nz;
+  const int lx = M.lx, lz = M.lz;
+  const uint32_t SX = (uint32_t)nz * 2 * lx, SZ = (uint32_t)2 * lz * nx;
+  const A32<const T> Lin(A.lam[(t + 1) & 1]);
+  const A32<T> Lout(A.lam[t & 1]);
+  const A32<const T> u1(static_cast<const T*>(A.trail[tb]) + (long)(t + 1 - t0) * A.slot);
+  const A32<const T> u0(static_cast<const T*>(A.trail[tb]) + (long)(t - t0) * A.slot);
+  const A32<typename std::remove_pointer<decltype(GAcc<T>::of(A))>::type> G(GAcc<T>::of(A));
+  const Prof<T> P(&M.prof[0][0]);
+  const bool inx = PML && incol && in_slab(i, nx, lx);
+  const uint32_t xs = inx ? slab_pos(i, nx, lx) : 0;
+  const T ikx = P[PK][ic], ax = P[PA][ic], bx = P[PB][ic];
+  const T ikxh = P[PKH][ic], axh = P[PAH][ic], bxh = P[PBH][ic];
+  const auto ikz = P[12 + PK], az = P[12 + PA], bz = P[12 + PB];
+  const auto ikzh = P[12 + PKH], azh = P[12 + PAH], bzh = P[12 + PBH];
+  const A32<const T> CAX(M.coef[0]), CBX(M.coef[1]), CAZ(M.coef[2]), CBZ(M.coef[3]);
+  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dbh = T(M.scal[0]);
+  auto row = [&](int r) { return (uint32_t)(r < 0 ? 0 : (r >= nz ? nz - 1 : r)) * nx + ic; };
+  auto clampr = [&](int r) { return r < 0 ? 0 : (r >= nz ? nz - 1 : r); };
+
+  // carried from the previous march row: E^T of row q-1 (adjoints of dHy_dz
+  // at the Ex node, of dHy_dx at the Ez node, and the partial lambda_E), the
+  // H^T result la of row q-2, and u_{t+1}'s Hy of row q-1
+  T wx_p = 0, wz_p = 0, ex_p = 0, ez_p = 0, la_pp = 0;
+  T hy1_p = (incol && zo - 2 >= 0) ? u1[2 * n + row(zo - 2)] : T(0);
+  for (int q = zo - 1; q <= zend; ++q) {
+    // ---- E^T of row q (k_em2_adj_e) ----
+    const bool vq = incol && q >= 0 && q < nz;
+    const int qs = clampr(q);
+    const uint32_t c = row(q);
+    const T hy1 = vq ? u1[2 * n + c] : T(0);
+    const T hy1_w = lane_prev(hy1);  // u_{t+1} Hy(i-1, q)
+    const bool ownq = own && q >= zo && q < zend;
+    T wx = 0, wz = 0, exq = 0, ezq = 0;
+    if (vq) {
+      const bool inz = PML && in_slab(qs, nz, lz);
+      const uint32_t ox = 3 * n + (inx ? (uint32_t)qs * 2 * lx + xs : 0);
+      const uint32_t oz = 3 * n + 2 * SX + (inz ? (uint32_t)slab_pos(qs, nz, lz) * nx + ic : 0);
+      // Dirichlet ring of Ex and Ez (and cells outside their Fortran extent)
+      const bool edx = (i == 0 || i >= nx - 2 || q == 0 || q == nz - 1);
+      const bool edz = (i == 0 || i == nx - 1 || q == 0 || q >= nz - 2);
+      const T lex = edx ? T(0) : Lin[c], lez = edz ? T(0) : Lin[n + c];
+      if (ownq && A.gsrc && i == S.isrc && q == S.ksrc) {
+        A.gsrc[3 * (long)t + 0] = (double)lex;
+        A.gsrc[3 * (long)t + 2] = (double)lez;
+      }
+      if (q >= 1 && q <= nz - 2 && i <= nx - 2) {  // Ex node, :273-283
+        const T ld = CBX[c] * lex;
+        const T d = (hy1 - hy1_p) * idz;
+        T D = d;
+        wx = ld;
+        if (inz) {
+          const T lr = Lin[oz + SZ] + ld;
+          wx = ikzh[qs] * ld + azh[qs] * lr;
+          if (ownq) Lout[oz + SZ] = bzh[qs] * lr;
+          D = ikzh[qs] * d + u1[oz + SZ];
+        }
+        if (ownq) {
+          if (gmask & 1u) gadd(&G[c], (double)lex * (double)u0[c]);
+          if (gmask & 2u) gadd(&G[n + c], (double)lex * (double)D);
+        }
+        exq = CAX[c] * lex;
+      } else {
+        if (ownq && inz) Lout[oz + SZ] = Lin[oz + SZ];
+        exq = lex;
+      }
+      if (q <= nz - 2 && i >= 1 && i <= nx - 2) {  // Ez node, :285-295
+        const T le = CBZ[c] * lez;
+        const T e = (hy1 - hy1_w) * idx;
+        T E = e;
+        wz = le;
+        if (inx) {
+          const T lr = Lin[ox + SX] + le;
+          wz = ikxh * le + axh * lr;
+          if (ownq) Lout[ox + SX] = bxh * lr;
+          E = ikxh * e + u1[ox + SX];
+        }
+        if (ownq) {
+          if (gmask & 4u) gadd(&G[2 * n + c], (double)lez * (double)u0[n + c]);
+          if (gmask & 8u) gadd(&G[3 * n + c], (double)lez * (double)E);
+        }
+        ezq = CAZ[c] * lez;
+      } else {
+        if (ownq && inx) Lout[ox + SX] = Lin[ox + SX];
+        ezq = lez;
+      }
+    }
+
+    // ---- H^T of row r = q-1 (k_em2_adj_h) ----
+    const int r = q - 1;
+    const T wz_e = lane_next(wz_p);  // E^T adjoint of dHy_dx at the Ez node (i+1, r)
+    T la = 0, lb = 0;
+    const bool vr = incol && r >= 0 && r < nz && q >= zo;
+    if (vr) {
+      const bool ownr = own && r >= zo && r < zend;
+      const uint32_t cr = row(r);
+      const bool inz = PML && in_slab(r, nz, lz);
+      const uint32_t ox = 3 * n + (inx ? (uint32_t)r * 2 * lx + xs : 0);
+      const uint32_t oz = 3 * n + 2 * SX + (inz ? (uint32_t)slab_pos(r, nz, lz) * nx + ic : 0);
+      // Hy' feeds the E updates through dHy_dz (Ex node c, c+nx) and dHy_dx (Ez node c, c+1)
+      const T lh = Lin[2 * n + cr] + idz * (wx_p - (r + 1 < nz ? wx : T(0))) +
+                   idx * (wz_p - (i + 1 < nx ? wz_e : T(0)));
+      if (i <= nx - 2 && r <= nz - 2) {  // Hy node, :249-266
+        const bool ring = (i == 0 || i == nx - 2 || r == 0 || r == nz - 2);
+        if (ownr) Lout[2 * n + cr] = ring ? T(0) : lh;
+        const T w = dbh * lh;
+        la = w;
+        lb = w;
+        if (inz) {
+          const T lr = Lin[oz] + w;
+          la = ikz[r] * w + az[r] * lr;
+          if (ownr) Lout[oz] = bz[r] * lr;
+        }
+        if (inx) {
+          const T lr = Lin[ox] + w;
+          lb = ikx * w + ax * lr;
+          if (ownr) Lout[ox] = bx * lr;
+        }
+      } else if (ownr) {
+        Lout[2 * n + cr] = lh;
+        if (inz) Lout[oz] = Lin[oz];
+        if (inx) Lout[ox] = Lin[ox];
+      }
+    }
+
+    // ---- E gather of row r (k_em2_adj_g) ----
+    const T lb_w = lane_prev(lb);  // H^T adjoint of dEz_dx at the Hy node (i-1, r)
+    if (own && r >= zo && r < zend) {
+      const uint32_t cr = row(r);
+      Lout[cr] = ex_p + idz * ((r >= 1 ? la_pp : T(0)) - la);
+      Lout[n + cr] = ez_p + idx * ((i >= 1 ? lb_w : T(0)) - lb);
+    }
+
+    // ---- shift the windows ----
+    wx_p = wx; wz_p = wz; ex_p = exq; ez_p = ezq;
+    la_pp = la;
+    hy1_p = hy1;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64 * AWB)
+k_em2_adj(const DModel* __restrict__ models, const DShot* __restrict__ shots,
+          const DAdj* __restrict__ adj, const int4* __restrict__ work, int nitems,
+          const int* __restrict__ tbase, int k, unsigned gmask, const int* __restrict__ tsel) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int t0 = __builtin_amdgcn_readfirstlane(*tbase);
+  const int tb = __builtin_amdgcn_readfirstlane(*tsel);
+  const int t = t0 + k;
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const int4 wv = work[w];
+    const int s = __builtin_amdgcn_readfirstlane(wv.x);
+    const int xo = __builtin_amdgcn_readfirstlane(wv.y) + wave * AW;
+    const int zo = __builtin_amdgcn_readfirstlane(wv.z);
+    const int eh = __builtin_amdgcn_readfirstlane(wv.w);  // tile height
+    if (s < 0) continue;
+    const DModel& M = models[shots[s].model];
+    if (xo >= M.nx) continue;
+    // a wave whose columns and rows (halo included) avoid every C-PML slab
+    // runs the body without memory variables
+    const bool interior = xo - 2 >= M.lx && xo + 61 < M.nx - M.lx && zo - 1 >= M.lz &&
+                          zo + eh < M.nz - M.lz;
+    if (interior)
+      em2_adj_tile<T, false>(models, shots, adj, s, xo, zo, eh, t, t0, gmask, threadIdx.x & 63, tb);
+    else
+      em2_adj_tile<T, true>(models, shots, adj, s, xo, zo, eh, t, t0, gmask, threadIdx.x & 63, tb);
+  }
+}
+
 // ===========================================================================
 // EM25: electromag25, legacy/fortran/electromagfdtd.f95:619-1092
 // psi slabs: x [nz][ny][2lx]: dEz_dx dEy_dx dHz_dx dHy_dx
@@ -3782,6 +3980,11 @@ void launch_adjoint_em2(Plan* p, hipStream_t st, int khi, int klo) {
     if (p->maxrec > 0)
       hipLaunchKernelGGL(k_el2_adj_inject<T>, dim3((p->maxrec + 63) / 64, p->nshots), dim3(64), 0,
                          st, p->dmodels, p->dshots, p->dadj, p->nshots, tb, k);
+    if (!(p->adj_flags & SDT_ADJ_REFERENCE)) {
+      hipLaunchKernelGGL(k_em2_adj<T>, dim3(p->nadjwork), dim3(64 * AWB), 0, st, p->dmodels,
+                         p->dshots, p->dadj, p->dadjwork, p->nadjwork, tb, k, p->gmask, p->dtsel_adj);
+      continue;
+    }
     hipLaunchKernelGGL(k_em2_adj_e<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dadj, p->nshots,
                        tb, k, p->dtsel_adj, p->gmask);
     hipLaunchKernelGGL(k_em2_adj_h<T>, grd, blk, 0, st, p->dmodels, p->dshots, p->dadj, p->nshots,
@@ -4464,8 +4667,6 @@ int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots, unsigned 
   if (nshots != p->nshots) return fail(SDT_EINVAL, "adj must hold one entry per shot of the plan");
   if (p->kind == SDT_EM2 && (grad_mask & ~0xfu))
     return fail(SDT_EINVAL, "EM2 has 4 coefficient slots (grad_mask <= 0xf)");
-  if (p->kind == SDT_EM2 && (flags & SDT_ADJ_REFERENCE))
-    return fail(SDT_EINVAL, "SDT_ADJ_REFERENCE applies to ELASTIC2");
   std::vector<DAdj> h(nshots);
   for (int s = 0; s < nshots; ++s) {
     const sdt_adj_shot& a = adj[s];
@@ -4508,7 +4709,8 @@ int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots, unsigned 
   p->g32.clear();
   if (p->dflush) { (void)hipFree(p->dflush); p->dflush = nullptr; }
   p->nflush = 0;
-  if (p->prec == SDT_FP32 && !(flags & SDT_ADJ_REFERENCE)) {
+  // (the EM2 reference kernels stage in FP32 too; the elastic ones add in FP64)
+  if (p->prec == SDT_FP32 && (p->kind == SDT_EM2 || !(flags & SDT_ADJ_REFERENCE))) {
     std::vector<DFlush> fl;
     for (int s = 0; s < nshots; ++s) {
       const sdt_model& M = p->hmodels[p->hshots[s].model];
@@ -4535,14 +4737,30 @@ int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots, unsigned 
   if (!p->dtbase) HIP_TRY(hipMalloc(&p->dtbase, sizeof(int)));
   if (!p->dtsel_trail) HIP_TRY(hipMalloc(&p->dtsel_trail, sizeof(int)));
   if (!p->dtsel_adj) HIP_TRY(hipMalloc(&p->dtsel_adj, sizeof(int)));
-  if (!p->dadjwork && p->kind == SDT_ELASTIC2) {
-    // (shot, x0, z0) items of AWB * AW columns x AH rows; the items of one
-    // tile over all shots of its model share blockIdx % 8 (one XCD's L2)
+  if (!p->dadjwork) {
+    // (shot, x0, z0, rows) items of AWB * AW columns x rows; the items of one
+    // tile over all shots of its model share blockIdx % 8 (one XCD's L2).
+    // The elastic kernel walks AH rows; the EM kernel takes the tallest of
+    // AH, 16, 8, 4, 2 rows that gives the plan 1000 tiles or more (its
+    // single-shot grids are small and stay in the cache, so the walk's
+    // latency is what a tile costs).
+    const int bw = AW * AWB;
+    int eh = AH;
+    if (p->kind == SDT_EM2) {
+      for (int rows : {AH, 16, 8, 4, 2}) {
+        eh = rows;
+        long long tiles = 0;
+        for (const sdt_shot& S : p->hshots) {
+          const sdt_model& M = p->hmodels[S.model];
+          tiles += (long long)((M.nx + bw - 1) / bw) * ((M.nz + rows - 1) / rows);
+        }
+        if (tiles >= 1000) break;
+      }
+    }
     std::vector<int4> work;
     for (int m = 0; m < p->nmodels; ++m) {
       const sdt_model& M = p->hmodels[m];
-      const int bw = AW * AWB;
-      const int tx = (M.nx + bw - 1) / bw, tz = (M.nz + AH - 1) / AH, ntile = tx * tz;
+      const int tx = (M.nx + bw - 1) / bw, tz = (M.nz + eh - 1) / eh, ntile = tx * tz;
       std::vector<int> ms;
       for (int s = 0; s < p->nshots; ++s)
         if (p->hshots[s].model == m) ms.push_back(s);
@@ -4551,7 +4769,7 @@ int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots, unsigned 
         for (int s : ms)
           for (int tt = 0; tt < 8; ++tt) {
             const int t = tg * 8 + tt;
-            work.push_back(t < ntile ? make_int4(s, (t % tx) * bw, (t / tx) * AH, 0)
+            work.push_back(t < ntile ? make_int4(s, (t % tx) * bw, (t / tx) * eh, eh)
                                      : make_int4(-1, 0, 0, 0));
           }
     }

@@ -430,6 +430,38 @@ def test_em_dot_product():
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fused_em_adjoint_matches_reference_kernels(precision):
+    """The fused single-pass EM adjoint kernel against the three-kernel one
+    (E^T, H^T, gather): same lambda_0, gradients and source gradients up to
+    rounding, on two models of uneven size (one several tiles wide and tall,
+    one narrower than a tile) with two shots each, C-PML on every side."""
+    probs = [tiny_problem("em2", seed=90 + m, nt=70, nshots=2, nrec=7, **d)
+             for m, d in enumerate([dict(nx=300, nz=75, npml=6), dict(nx=57, nz=45, npml=5)])]
+    out = []
+    for ref in (False, True):
+        eng = _engine(probs, precision, every=24, reference=ref)
+        g = torch.Generator().manual_seed(11)
+        for s in range(eng.nshots):
+            eng.init[s].copy_(torch.randn(eng.init[s].numel(), generator=g, dtype=torch.float64))
+        tr = eng.forward()
+        ys = [torch.randn(t.shape, generator=g, dtype=torch.float64) for t in tr]
+        grad, gsrc = eng.backward(ys)
+        out.append(([x.cpu().numpy() for x in grad], [x.cpu().numpy() for x in gsrc],
+                    [eng.adjoint_state(s).double().cpu().numpy() for s in range(eng.nshots)]))
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    (g0, s0, l0), (g1, s1, l1) = out
+    for a, b in zip(g0, g1):
+        for q in range(4):
+            assert np.abs(b[q]).max() > 0, ("grad slot", q)
+            assert rel_l2(a[q], b[q]) < tol, ("grad slot", q, rel_l2(a[q], b[q]))
+    for a, b in zip(s0, s1):
+        assert rel_l2(a, b) < tol
+    for a, b in zip(l0, l1):
+        assert rel_l2(a, b) < tol
+
+
 def _em_problem():
     kw = dict(nx=60, nz=50, npml=8, nt=160, nshots=2, nrec=10)
     p = tiny_problem("em2", seed=63, **kw)
