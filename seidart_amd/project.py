@@ -344,14 +344,32 @@ def _extend2d(vals, geometry, cpml):
     return np.pad(np.asarray(vals, dtype=np.float64)[np.asarray(geometry)], cpml, mode="edge")
 
 
+# materials that Model.writemask marks as fluid (classes.py:556)
+FLUIDS = ("air", "water", "oil")
+
+
+def geometry_mask(domain, material) -> np.ndarray:
+    """Model.writemask (classes.py:549-606) for a 2-D domain: 1 on the cells
+    whose material is named air, water or oil, 0 elsewhere, the edge rows and
+    columns repeated into the C-PML.  Returns the padded (nx + 2c, nz + 2c)
+    array; ``geometry_mask.dat`` holds its transpose as one int32 record.
+    (Nothing in the reference calls writemask or reads the file; the kernels
+    do not use it.)"""
+    ids = [int(m.get("id", k)) for k, m in enumerate(material.material_list)
+           if str(m.get("name", "")) in FLUIDS]
+    mask = np.isin(np.asarray(domain.geometry), ids).astype(int)
+    return np.pad(mask, int(domain.cpml), mode="edge")
+
+
 def write_build_files(model, domain, material, workdir: str) -> list:
     """The files Model.build and pointsource write for a 2-D or 2.5-D run, in
     their formats (one Fortran record each, 2-D arrays as ``arr.T``):
     coefficient tables painted on the geometry (``c11.dat`` ... ``rho.dat`` or
     ``e11.dat`` ... ``s33.dat``), ``gamma_*.dat`` (attenuation x f0, with the
     sponge when gamma_max is set), ``density_gradient.dat`` (airsurf, written
-    for nmats > 1), the C-PML fields ``{sigma,kappa,alpha,acoef,bcoef}
-    [_half]_cpml.dat`` and the source series.  Returns the paths written."""
+    for nmats > 1), ``geometry_mask.dat`` (writemask, 2-D), the C-PML fields
+    ``{sigma,kappa,alpha,acoef,bcoef}[_half]_cpml.dat`` and the source
+    series.  Returns the paths written."""
     from .run import _sponge, air_density_gradient
     os.makedirs(workdir, exist_ok=True)
     P = lambda f: os.path.join(workdir, f)
@@ -383,6 +401,9 @@ def write_build_files(model, domain, material, workdir: str) -> list:
         air = names.index("air") if "air" in names else None
         grad = air_density_gradient(geom, air, int(getattr(model, "air_gradient_integer", 2) or 2))
         rec("density_gradient.dat", np.asarray(grad, dtype=np.float64))
+    if domain.dim != 3:
+        write_record(P("geometry_mask.dat"), geometry_mask(domain, material).T, dtype=np.int32)
+        out.append(P("geometry_mask.dat"))
     for half, suf in ((False, "_cpml.dat"), (True, "_half_cpml.dat")):
         for name, arr in zip(("sigma", "kappa", "alpha", "acoef", "bcoef"),
                              cpml_fields_2d(model, domain, half)):

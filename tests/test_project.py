@@ -87,6 +87,27 @@ def test_load_project_reads_tables_and_geometry(tmp_path):
     assert sm.is_seismic and not em.is_seismic and sm.f0 == 200.0
 
 
+def test_geometry_mask_marks_fluid_cells(tmp_path):
+    """writemask's file (classes.py:549-606): 1 on air, water and oil cells,
+    0 elsewhere, edges repeated into the C-PML, the transpose of the padded
+    (nx, nz) array in one int32 record."""
+    jf, geom = write_project(str(tmp_path), nx=30, nz=20, cpml=5)
+    d, mat, sm, em = project.load_project(jf)
+    m = project.geometry_mask(d, mat)
+    assert m.shape == (40, 30)
+    np.testing.assert_array_equal(m[5:35, 5:25], (geom == 2).astype(int))  # label 2 is air
+    np.testing.assert_array_equal(m[:5, 5:25], np.repeat(m[5:6, 5:25], 5, axis=0))
+    np.testing.assert_array_equal(m[5:35, :5], np.repeat(m[5:35, 5:6], 5, axis=1))
+    assert m[0, 0] == m[5, 5] and m[-1, -1] == m[34, 24]
+    assert set(np.unique(m)) == {0, 1}
+    project.build(sm, d, mat, write=False)
+    files = project.write_build_files(sm, d, mat, str(tmp_path / "build"))
+    path = str(tmp_path / "build" / "geometry_mask.dat")
+    assert path in files
+    got = fortranio.read_record(path, dtype=np.int32).reshape(30, 40)
+    np.testing.assert_array_equal(got, m.T)
+
+
 def test_dt_closed_forms(tmp_path):
     """dt = CFL / (vmax sqrt(sum 1/d^2)) (classes.py:469-485): isotropic media
     have vmax = the largest Vp along the axes (ice, 3840 m/s); EM vmax =
