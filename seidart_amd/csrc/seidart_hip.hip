@@ -1171,6 +1171,9 @@ __device__ __forceinline__ void gadd(__attribute__((address_space(1))) double* g
 __device__ __forceinline__ void gadd(__attribute__((address_space(1))) float* g, double v) {
   gadd((float*)g, v);
 }
+__device__ __forceinline__ void gadd(__attribute__((address_space(1))) float* g, float v) {
+  if (v != 0.0f) unsafeAtomicAdd((float*)g, v);
+}
 template <typename T>
 struct GAcc;
 template <>
@@ -1520,7 +1523,13 @@ k_el2_adj_g(const DModel* __restrict__ models, const DShot* __restrict__ shots,
 #endif
 constexpr int AW = 60;   // owned columns per wavefront
 constexpr int AWB = 4;   // wavefronts per block, side by side in x
-constexpr int AH = SDT_EL2_ROWS;
+#ifndef SDT_ADJ_ROWS
+#define SDT_ADJ_ROWS SDT_EL2_ROWS
+#endif
+constexpr int AH = SDT_ADJ_ROWS;
+#ifndef SDT_ADJ_F32PROD
+#define SDT_ADJ_F32PROD 0
+#endif
 
 template <typename T, bool SPLIT, bool PML>
 __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
@@ -1537,6 +1546,9 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
   const int ic = incol ? i : (i < 0 ? 0 : nx - 1);
   const bool own = incol && lane >= 2 && lane < 2 + AW;
   const bool sl = incol && lane >= 1 && lane <= 2 + AW;  // lanes of the stress phase
+  // type of the gradient products: FP64, or FP32 in FP32 plans when
+  // SDT_ADJ_F32PROD is set (the staging sum is FP32 either way)
+  using GT = typename std::conditional<sizeof(T) == 4 && SDT_ADJ_F32PROD, float, double>::type;
   const int zend = min(zo + AH, nz);
   // 32-bit element offsets within one state / gradient array (checked at setup)
   const uint32_t n = (uint32_t)nx * nz;
@@ -1656,8 +1668,8 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
       }
       pvx = omx * lvx;
       if (ownr) {
-        if (gmask & (1u << 8)) gadd(&G[8 * n + cr], (double)lvx * (double)(b1 + b2));
-        if (gmask & (1u << 10)) gadd(&G[10 * n + cr], (double)lvx * (double)vx_n);
+        if (gmask & (1u << 8)) gadd(&G[8 * n + cr], (GT)lvx * (GT)(b1 + b2));
+        if (gmask & (1u << 10)) gadd(&G[10 * n + cr], (GT)lvx * (GT)vx_n);
       }
     } else if (ownr) {
       if (inx) Lout[oxr + 2 * SX] = Lx2;
@@ -1682,8 +1694,8 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
       }
       pvz = omz * lvz;
       if (ownr) {
-        if (gmask & (1u << 9)) gadd(&G[9 * n + cr], (double)lvz * (double)(b1 + b2));
-        if (gmask & (1u << 11)) gadd(&G[11 * n + cr], (double)lvz * (double)vz_n);
+        if (gmask & (1u << 9)) gadd(&G[9 * n + cr], (GT)lvz * (GT)(b1 + b2));
+        if (gmask & (1u << 11)) gadd(&G[11 * n + cr], (GT)lvz * (GT)vz_n);
       }
     } else if (ownr) {
       if (inx) Lout[oxr + 3 * SX] = Lx3;
@@ -1736,10 +1748,10 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
           k4 = bzhq * l;
         }
         if (ownq) {
-          const double m = (double)mu;
-          if (gmask & (1u << 5)) gadd(&G[5 * n + cq], m * (double)g1);
-          if (gmask & (1u << 6)) gadd(&G[6 * n + cq], m * (double)g2);
-          if (gmask & (1u << 7)) gadd(&G[7 * n + cq], m * ((double)g3 + (double)g4));
+          const GT m = (GT)mu;
+          if (gmask & (1u << 5)) gadd(&G[5 * n + cq], m * (GT)g1);
+          if (gmask & (1u << 6)) gadd(&G[6 * n + cq], m * (GT)g2);
+          if (gmask & (1u << 7)) gadd(&G[7 * n + cq], m * ((GT)g3 + (GT)g4));
         }
       }
       if (q >= 1 && i <= nx - 2) {  // N^T
@@ -1770,11 +1782,11 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
           lm4 = bzhq * l;
         }
         if (ownq) {
-          const double mx = (double)mux, mz = (double)muz, sh = (double)d3 + (double)d4;
-          if (gmask & (1u << 0)) gadd(&G[cq], mx * (double)d1);
-          if (gmask & (1u << 1)) gadd(&G[n + cq], mx * (double)d2 + mz * (double)d1);
+          const GT mx = (GT)mux, mz = (GT)muz, sh = (GT)d3 + (GT)d4;
+          if (gmask & (1u << 0)) gadd(&G[cq], mx * (GT)d1);
+          if (gmask & (1u << 1)) gadd(&G[n + cq], mx * (GT)d2 + mz * (GT)d1);
           if (gmask & (1u << 2)) gadd(&G[2 * n + cq], mx * sh);
-          if (gmask & (1u << 3)) gadd(&G[3 * n + cq], mz * (double)d2);
+          if (gmask & (1u << 3)) gadd(&G[3 * n + cq], mz * (GT)d2);
           if (gmask & (1u << 4)) gadd(&G[4 * n + cq], mz * sh);
         }
       }
