@@ -1171,9 +1171,16 @@ __device__ __forceinline__ void gadd(__attribute__((address_space(1))) double* g
 __device__ __forceinline__ void gadd(__attribute__((address_space(1))) float* g, double v) {
   gadd((float*)g, v);
 }
+// gradient products in FP32 (not FP64) before the FP32 staging add: a
+// measured option, off by default
+#ifndef SDT_ADJ_F32PROD
+#define SDT_ADJ_F32PROD 0
+#endif
+#if SDT_ADJ_F32PROD
 __device__ __forceinline__ void gadd(__attribute__((address_space(1))) float* g, float v) {
   if (v != 0.0f) unsafeAtomicAdd((float*)g, v);
 }
+#endif
 template <typename T>
 struct GAcc;
 template <>
@@ -1527,108 +1534,134 @@ constexpr int AWB = 4;   // wavefronts per block, side by side in x
 #define SDT_ADJ_ROWS SDT_EL2_ROWS
 #endif
 constexpr int AH = SDT_ADJ_ROWS;
-#ifndef SDT_ADJ_F32PROD
-#define SDT_ADJ_F32PROD 0
-#endif
+
+// One march row of the fused adjoint, shared by the one-step kernel and by
+// both stages of the two-step kernel.  Inputs of march row r: lambda_{t+1}
+// of the velocities of row r, of the stresses of row q = r - 1, the memory
+// adjoints of row r (the stress-derivative memories the velocity update
+// uses) and of row q (the velocity-derivative memories, with the shear
+// loop's own four in the split variant).  Outputs: lambda_t of the
+// velocities of row r - 2, of the stresses of row q, and the memory adjoints
+// of rows r and q.
+template <typename T>
+struct AdjRow {
+  T lvx = 0, lvz = 0, lsxx = 0, lszz = 0, lsxz = 0;
+  T x2 = 0, x3 = 0, z2 = 0, z3 = 0;                                  // row r
+  T x0 = 0, x1 = 0, z0 = 0, z1 = 0, sx0 = 0, sx1 = 0, sz0 = 0, sz1 = 0;  // row q
+};
 
 template <typename T, bool SPLIT, bool PML>
-__device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
-                                             const DShot* __restrict__ shots,
-                                             const DAdj* __restrict__ adj, const int s, const int xo,
-                                             const int zo, const int t, const int t0,
-                                             const unsigned gmask, const int lane, const int tb) {
-  const DShot& S = shots[s];
-  const DModel& M = models[S.model];
-  const DAdj& A = adj[s];
-  const int nx = M.nx, nz = M.nz;
-  const int i = xo - 2 + lane;
-  const bool incol = (i >= 0 && i < nx);
-  const int ic = incol ? i : (i < 0 ? 0 : nx - 1);
-  const bool own = incol && lane >= 2 && lane < 2 + AW;
-  const bool sl = incol && lane >= 1 && lane <= 2 + AW;  // lanes of the stress phase
-  // type of the gradient products: FP64, or FP32 in FP32 plans when
-  // SDT_ADJ_F32PROD is set (the staging sum is FP32 either way)
+struct AdjStage {
   using GT = typename std::conditional<sizeof(T) == 4 && SDT_ADJ_F32PROD, float, double>::type;
-  const int zend = min(zo + AH, nz);
-  // 32-bit element offsets within one state / gradient array (checked at setup)
-  const uint32_t n = (uint32_t)nx * nz;
-  const int lx = M.lx, lz = M.lz;
-  const uint32_t SX = (uint32_t)nz * 2 * lx, SZ = (uint32_t)2 * lz * nx;
-  const uint32_t ZB = 5 * n + (SPLIT ? 6 : 4) * SX;
-  const A32<const T> Lin(A.lam[(t + 1) & 1]);
-  const A32<T> Lout(A.lam[t & 1]);
-  const A32<const T> u1(static_cast<const T*>(A.trail[tb]) + (long)(t + 1 - t0) * A.slot);
-  const A32<const T> u0(static_cast<const T*>(A.trail[tb]) + (long)(t - t0) * A.slot);
-  const A32<typename std::remove_pointer<decltype(GAcc<T>::of(A))>::type> G(GAcc<T>::of(A));
-  const Prof<T> P(&M.prof[0][0]);
-  const bool inx = PML && incol && in_slab(i, nx, lx);
-  const uint32_t xs = 5 * n + (inx ? slab_pos(i, nx, lx) : 0);
-  const T ikx = P[PK][ic], ax = P[PA][ic], bx = P[PB][ic];
-  const T ikxh = P[PKH][ic], axh = P[PAH][ic], bxh = P[PBH][ic];
-  const auto ikz = P[12 + PK], az = P[12 + PA], bz = P[12 + PB];
-  const auto ikzh = P[12 + PKH], azh = P[12 + PAH], bzh = P[12 + PBH];
-  const T idx = T(1.0 / M.dx), idz = T(1.0 / M.dz), dt = T(M.dt);
-  const A32<const T> N11(M.coef[0]), N13(M.coef[1]), N15(M.coef[2]);
-  const A32<const T> N33(M.coef[3]), N35(M.coef[4]);
-  const A32<const T> S15(M.coef[5]), S35(M.coef[6]), S55(M.coef[7]);
-  const A32<const T> DTX(M.coef[8]), DTZ(M.coef[9]);
-  const A32<const T> OMX(M.coef[10]), OMZ(M.coef[11]);
-  const auto brow = cp<T>(M.aux);
-  const bool mono = M.coef[2] != nullptr, atten = M.coef[10] != nullptr;
-  auto row = [&](int r) { return (uint32_t)(r < 0 ? 0 : (r >= nz ? nz - 1 : r)) * nx + ic; };
-  auto valid = [&](int r) { return incol && r >= 0 && r < nz; };
-  auto clampr = [&](int r) { return r < 0 ? 0 : (r >= nz ? nz - 1 : r); };
-
-  // register windows (rows relative to the march row r)
+  using GP = typename std::remove_pointer<decltype(GAcc<T>::of(std::declval<const DAdj&>()))>::type;
+  const DShot& S;
+  const DAdj& A;
+  const int nx, nz, i, ic, lx, lz;
+  const bool incol, sl, inx;
+  const uint32_t n, SX, SZ, ZB, xs;
+  const A32<const T> u1, u0;
+  const A32<GP> G;
+  T ikx, ax, bx, ikxh, axh, bxh, idx, idz, dt;
+  gcptr<T> ikz, az, bz, ikzh, azh, bzh, brow;
+  const A32<const T> N11, N13, N15, N33, N35, S15, S35, S55, DTX, DTZ, OMX, OMZ;
+  const bool mono, atten;
+  // windows (rows relative to the march row r)
   T A1 = 0, B1 = 0, C1 = 0, D1 = 0, D2 = 0;      // A, B, C, D of row r-1; D of r-2
   T pvx1 = 0, pvz1 = 0, pvx2 = 0, pvz2 = 0;      // partial lambda_v of rows r-1, r-2
   T vx_m = 0, vz_m = 0, vx_c = 0, vz_c = 0;      // u_t velocities of rows r-2, r-1
-  T sxz_p = valid(zo - 3) ? u1[4 * n + row(zo - 3)] : T(0);  // u_{t+1} sxz of row r-1
-  T szz_c = valid(zo - 2) ? u1[3 * n + row(zo - 2)] : T(0);  // u_{t+1} szz of row r
+  T sxz_p = 0, szz_c = 0;                        // u_{t+1} sxz of row r-1, szz of row r
   T E1p = 0, E2p = 0, E3p = 0, E4p = 0, F1p = 0, F2p = 0, F3p = 0, F4p = 0;  // row r-2
   T F2pp = 0, F4pp = 0;                                                      // row r-3
+  // per march row: rows, validity and memory offsets of rows r and q
+  int rs = 0, qs = 0;
+  bool vr = false, vq = false, inzr = false, inzq = false;
+  uint32_t cr = 0, cq = 0, cn = 0, oxr = 0, oxq = 0, ozr = 0, ozq = 0;
 
-  for (int r = zo - 2; r <= zend + 1; ++r) {
-    const int q = r - 1, pr = r - 2;
-    const uint32_t cr = row(r), cq = row(q), cn = row(r + 1);
-    const bool vr = valid(r), vq = valid(q);
-    const int rs = clampr(r), qs = clampr(q);
-    const bool inzr = PML && incol && in_slab(rs, nz, lz), inzq = PML && incol && in_slab(qs, nz, lz);
-    const uint32_t oxr = xs + (inx ? (uint32_t)rs * 2 * lx : 0), oxq = xs + (inx ? (uint32_t)qs * 2 * lx : 0);
-    const uint32_t ozr = ZB + (inzr ? (uint32_t)slab_pos(rs, nz, lz) * nx + ic : 0);
-    const uint32_t ozq = ZB + (inzq ? (uint32_t)slab_pos(qs, nz, lz) * nx + ic : 0);
+  // u1 = the trail slot of u_{t+1}, u0 that of u_t; r0 = the first march row
+  __device__ __forceinline__ AdjStage(const DModel& M, const DShot& S_, const DAdj& A_, int xo, int lane,
+                                      const T* u1p, const T* u0p, int r0)
+      : S(S_), A(A_), nx(M.nx), nz(M.nz), i(xo - 2 + lane),
+        ic(i >= 0 && i < M.nx ? i : (i < 0 ? 0 : M.nx - 1)), lx(M.lx), lz(M.lz),
+        incol(i >= 0 && i < M.nx), sl(incol && lane >= 1 && lane <= 2 + AW),
+        inx(PML && incol && in_slab(i, M.nx, M.lx)), n((uint32_t)M.nx * M.nz),
+        SX((uint32_t)M.nz * 2 * M.lx), SZ((uint32_t)2 * M.lz * M.nx),
+        ZB(5 * n + (SPLIT ? 6 : 4) * SX), xs(5 * n + (inx ? slab_pos(i, M.nx, M.lx) : 0)),
+        u1(u1p), u0(u0p), G(GAcc<T>::of(A_)), N11(M.coef[0]), N13(M.coef[1]), N15(M.coef[2]),
+        N33(M.coef[3]), N35(M.coef[4]), S15(M.coef[5]), S35(M.coef[6]), S55(M.coef[7]),
+        DTX(M.coef[8]), DTZ(M.coef[9]), OMX(M.coef[10]), OMZ(M.coef[11]),
+        mono(M.coef[2] != nullptr), atten(M.coef[10] != nullptr) {
+    const Prof<T> P(&M.prof[0][0]);
+    ikx = P[PK][ic]; ax = P[PA][ic]; bx = P[PB][ic];
+    ikxh = P[PKH][ic]; axh = P[PAH][ic]; bxh = P[PBH][ic];
+    ikz = P[12 + PK]; az = P[12 + PA]; bz = P[12 + PB];
+    ikzh = P[12 + PKH]; azh = P[12 + PAH]; bzh = P[12 + PBH];
+    brow = cp<T>(M.aux);
+    idx = T(1.0 / M.dx); idz = T(1.0 / M.dz); dt = T(M.dt);
+    sxz_p = valid(r0 - 1) ? u1[4 * n + rowi(r0 - 1)] : T(0);
+    szz_c = valid(r0) ? u1[3 * n + rowi(r0)] : T(0);
+  }
+  __device__ __forceinline__ uint32_t rowi(int r) const {
+    return (uint32_t)(r < 0 ? 0 : (r >= nz ? nz - 1 : r)) * nx + ic;
+  }
+  __device__ __forceinline__ bool valid(int r) const { return incol && r >= 0 && r < nz; }
+  __device__ __forceinline__ int clampr(int r) const { return r < 0 ? 0 : (r >= nz ? nz - 1 : r); }
+
+  // rows, validity and memory offsets of march row r
+  __device__ __forceinline__ void prep(int r) {
+    const int q = r - 1;
+    cr = rowi(r); cq = rowi(q); cn = rowi(r + 1);
+    vr = valid(r); vq = valid(q);
+    rs = clampr(r); qs = clampr(q);
+    inzr = PML && incol && in_slab(rs, nz, lz);
+    inzq = PML && incol && in_slab(qs, nz, lz);
+    oxr = xs + (inx ? (uint32_t)rs * 2 * lx : 0);
+    oxq = xs + (inx ? (uint32_t)qs * 2 * lx : 0);
+    ozr = ZB + (inzr ? (uint32_t)slab_pos(rs, nz, lz) * nx + ic : 0);
+    ozq = ZB + (inzq ? (uint32_t)slab_pos(qs, nz, lz) * nx + ic : 0);
+  }
+
+  // the inputs of the prepared row from a lambda state (slab loads only
+  // where the cell has that memory: a zero-width slab has no storage)
+  __device__ __forceinline__ void load(const A32<const T>& L, AdjRow<T>& in) const {
+    in.lvx = L[cr]; in.lvz = L[n + cr];
+    in.lsxx = L[2 * n + cq]; in.lszz = L[3 * n + cq]; in.lsxz = L[4 * n + cq];
+    in.x2 = inx ? L[oxr + 2 * SX] : T(0); in.x3 = inx ? L[oxr + 3 * SX] : T(0);
+    in.z2 = inzr ? L[ozr + 2 * SZ] : T(0); in.z3 = inzr ? L[ozr + 3 * SZ] : T(0);
+    in.x0 = inx ? L[oxq] : T(0); in.x1 = inx ? L[oxq + SX] : T(0);
+    in.z0 = inzq ? L[ozq] : T(0); in.z1 = inzq ? L[ozq + SZ] : T(0);
+    if (SPLIT) {
+      in.sx0 = inx ? L[oxq + 4 * SX] : T(0);
+      in.sx1 = inx ? L[oxq + 5 * SX] : T(0);
+      in.sz0 = inzq ? L[ozq + 4 * SZ] : T(0);
+      in.sz1 = inzq ? L[ozq + 5 * SZ] : T(0);
+    }
+  }
+
+  // One march row (prepared): V^T of row r, the stress gathers with S^T and
+  // N^T of row q (from march row qmin on), the velocity gather of row r-2.
+  // Gradient terms are added for the cells of rows [g0, g1) on the lanes
+  // where `own`; the source gradient of step tg likewise.
+  __device__ __forceinline__ void row(const int r, const AdjRow<T>& in, AdjRow<T>& o, const bool own,
+                                      const int g0, const int g1, const int qmin, const int tg,
+                                      const unsigned gmask) {
+    const int q = r - 1;
     // ---- loads ----
-    const T lvx0 = Lin[cr], lvz0 = Lin[n + cr];
     const T dtx = DTX[cr], dtz = DTZ[cr];
     const T omx = atten ? OMX[cr] : T(1), omz = atten ? OMZ[cr] : T(1);
     const T sxx1 = u1[2 * n + cr], sxz1 = u1[4 * n + cr], szz1n = u1[3 * n + cn];
     const T vx_n = vr ? u0[cr] : T(0), vz_n = vr ? u0[n + cr] : T(0);
-    const T lsxx0 = Lin[2 * n + cq], lszz0 = Lin[3 * n + cq], lsxz0 = Lin[4 * n + cq];
     const T n11 = N11[cq], n13 = N13[cq], n33 = N33[cq], s55 = S55[cq];
     const T n15 = mono ? N15[cq] : T(0), n35 = mono ? N35[cq] : T(0);
     const T s15 = mono ? S15[cq] : T(0), s35 = mono ? S35[cq] : T(0);
-    // psi and lambda_psi of the velocity row r (stress-derivative memory)
-    // (slab loads only where the cell has that memory: a zero-width slab
-    // has no storage to read)
-    const T Lx2 = inx ? Lin[oxr + 2 * SX] : T(0), Lx3 = inx ? Lin[oxr + 3 * SX] : T(0);
+    // psi of the velocity row r (stress-derivative memory) and of the stress
+    // row q (velocity-derivative memory)
     const T Ux2 = inx ? u1[oxr + 2 * SX] : T(0), Ux3 = inx ? u1[oxr + 3 * SX] : T(0);
-    const T Lz2 = inzr ? Lin[ozr + 2 * SZ] : T(0), Lz3 = inzr ? Lin[ozr + 3 * SZ] : T(0);
     const T Uz2 = inzr ? u1[ozr + 2 * SZ] : T(0), Uz3 = inzr ? u1[ozr + 3 * SZ] : T(0);
-    // of the stress row q (velocity-derivative memory)
     const uint32_t xq2 = SPLIT ? 4 * SX : 0, zq2 = SPLIT ? 4 * SZ : 0;  // shear loop's arrays
-    const T Lx0 = inx ? Lin[oxq] : T(0), Lx1 = inx ? Lin[oxq + SX] : T(0);
-    const T Lz0 = inzq ? Lin[ozq] : T(0), Lz1 = inzq ? Lin[ozq + SZ] : T(0);
     const T Ox0 = inx ? u0[oxq] : T(0), Ox1 = inx ? u0[oxq + SX] : T(0);
     const T Oz0 = inzq ? u0[ozq] : T(0), Oz1 = inzq ? u0[ozq + SZ] : T(0);
     const T Qx0 = inx ? u1[oxq + xq2] : T(0), Qx1 = inx ? u1[oxq + xq2 + SX] : T(0);
     const T Qz0 = inzq ? u1[ozq + zq2] : T(0), Qz1 = inzq ? u1[ozq + zq2 + SZ] : T(0);
-    T Lsx0 = 0, Lsx1 = 0, Lsz0 = 0, Lsz1 = 0;
-    if (SPLIT) {
-      Lsx0 = inx ? Lin[oxq + 4 * SX] : T(0);
-      Lsx1 = inx ? Lin[oxq + 5 * SX] : T(0);
-      Lsz0 = inzq ? Lin[ozq + 4 * SZ] : T(0);
-      Lsz1 = inzq ? Lin[ozq + 5 * SZ] : T(0);
-    }
     const T browq = SPLIT ? bxh : brow[qs];
     const T bzr = bz[rs], azr = az[rs], ikzr = ikz[rs], bzhr = bzh[rs], azhr = azh[rs], ikzhr = ikzh[rs];
     const T bzq = bz[qs], azq = az[qs], ikzq = ikz[qs], bzhq = bzh[qs], azhq = azh[qs], ikzhq = ikzh[qs];
@@ -1641,29 +1674,30 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
     const T F1p_e = lane_next(F1p), F3p_e = lane_next(F3p);
 
     // ---- V^T of row r: velocity updates, seismicfdtd.f95:317-375 ----
-    const bool ownr = own && r >= zo && r < zend;
+    const bool ownr = own && r >= g0 && r < g1;
     const bool edge = (i == 0 || i == nx - 1 || r == 0 || r == nz - 1);
-    const T lvx = (vr && !edge) ? lvx0 : T(0), lvz = (vr && !edge) ? lvz0 : T(0);
+    const T lvx = (vr && !edge) ? in.lvx : T(0), lvz = (vr && !edge) ? in.lvz : T(0);
     T An = 0, Bn = 0, Cn = 0, Dn = 0, pvx = lvx, pvz = lvz;
     if (ownr && i == S.isrc && r == S.ksrc && A.gsrc) {
-      A.gsrc[3 * (long)t + 0] = (double)lvx;
-      A.gsrc[3 * (long)t + 2] = (double)lvz;
+      A.gsrc[3 * (long)tg + 0] = (double)lvx;
+      A.gsrc[3 * (long)tg + 2] = (double)lvz;
     }
+    o.x2 = in.x2; o.z3 = in.z3; o.x3 = in.x3; o.z2 = in.z2;
     if (vr && i >= 1 && r >= 1) {
       const T w = dtx * lvx;
       An = w;
       Bn = w;
       T b1 = (sxx1 - sxx1_w) * idx, b2 = (sxz1 - sxz_p) * idz;
       if (inx) {
-        const T lr = Lx2 + w;
+        const T lr = in.x2 + w;
         An = ikx * w + ax * lr;
-        if (ownr) Lout[oxr + 2 * SX] = bx * lr;
+        o.x2 = bx * lr;
         b1 = ikx * b1 + Ux2;
       }
       if (inzr) {
-        const T lr = Lz3 + w;
+        const T lr = in.z3 + w;
         Bn = ikzr * w + azr * lr;
-        if (ownr) Lout[ozr + 3 * SZ] = bzr * lr;
+        o.z3 = bzr * lr;
         b2 = ikzr * b2 + Uz3;
       }
       pvx = omx * lvx;
@@ -1671,9 +1705,6 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
         if (gmask & (1u << 8)) gadd(&G[8 * n + cr], (GT)lvx * (GT)(b1 + b2));
         if (gmask & (1u << 10)) gadd(&G[10 * n + cr], (GT)lvx * (GT)vx_n);
       }
-    } else if (ownr) {
-      if (inx) Lout[oxr + 2 * SX] = Lx2;
-      if (inzr) Lout[ozr + 3 * SZ] = Lz3;
     }
     if (vr && i <= nx - 2 && r <= nz - 2) {
       const T w = dtz * lvz;
@@ -1681,15 +1712,15 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
       Dn = w;
       T b1 = (sxz1_e - sxz1) * idx, b2 = (szz1n - szz_c) * idz;
       if (inx) {
-        const T lr = Lx3 + w;
+        const T lr = in.x3 + w;
         Cn = ikxh * w + axh * lr;
-        if (ownr) Lout[oxr + 3 * SX] = bxh * lr;
+        o.x3 = bxh * lr;
         b1 = ikxh * b1 + Ux3;
       }
       if (inzr) {
-        const T lr = Lz2 + w;
+        const T lr = in.z2 + w;
         Dn = ikzhr * w + azhr * lr;
-        if (ownr) Lout[ozr + 2 * SZ] = bzhr * lr;
+        o.z2 = bzhr * lr;
         b2 = ikzhr * b2 + Uz2;
       }
       pvz = omz * lvz;
@@ -1697,25 +1728,21 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
         if (gmask & (1u << 9)) gadd(&G[9 * n + cr], (GT)lvz * (GT)(b1 + b2));
         if (gmask & (1u << 11)) gadd(&G[11 * n + cr], (GT)lvz * (GT)vz_n);
       }
-    } else if (ownr) {
-      if (inx) Lout[oxr + 3 * SX] = Lx3;
-      if (inzr) Lout[ozr + 2 * SZ] = Lz2;
     }
 
     // ---- stresses of row q: gathers, then S^T and N^T, :247-312 ----
     T Ee1 = 0, Ee2 = 0, Ee3 = 0, Ee4 = 0, Ff1 = 0, Ff2 = 0, Ff3 = 0, Ff4 = 0;
-    if (sl && vq && q >= zo - 1) {
-      const bool ownq = own && q >= zo && q < zend;
-      const T lsxx = lsxx0 + idx * (A1 - A1_e);
-      const T lszz = lszz0 + idz * (D2 - D1);
-      const T lsxz = lsxz0 + idz * (B1 - Bn) + idx * (C1_w - C1);
-      if (ownq) {
-        Lout[2 * n + cq] = lsxx;
-        Lout[3 * n + cq] = lszz;
-        Lout[4 * n + cq] = lsxz;
-      }
-      T lm1 = Lx0, lm3 = Lx1, lm2 = Lz0, lm4 = Lz1;     // normal loop's psi adjoints
-      T ls1 = Lsx0, ls3 = Lsx1, ls2 = Lsz0, ls4 = Lsz1;  // shear loop's (split)
+    o.lsxx = 0; o.lszz = 0; o.lsxz = 0;
+    o.x0 = in.x0; o.x1 = in.x1; o.z0 = in.z0; o.z1 = in.z1;
+    o.sx0 = in.sx0; o.sx1 = in.sx1; o.sz0 = in.sz0; o.sz1 = in.sz1;
+    if (sl && vq && q >= qmin) {
+      const bool ownq = own && q >= g0 && q < g1;
+      const T lsxx = in.lsxx + idx * (A1 - A1_e);
+      const T lszz = in.lszz + idz * (D2 - D1);
+      const T lsxz = in.lsxz + idz * (B1 - Bn) + idx * (C1_w - C1);
+      o.lsxx = lsxx; o.lszz = lszz; o.lsxz = lsxz;
+      T lm1 = in.x0, lm3 = in.x1, lm2 = in.z0, lm4 = in.z1;     // normal loop's psi adjoints
+      T ls1 = in.sx0, ls3 = in.sx1, ls2 = in.sz0, ls4 = in.sz1;  // shear loop's (split)
       T& k1 = SPLIT ? ls1 : lm1;
       T& k2 = SPLIT ? ls2 : lm2;
       T& k3 = SPLIT ? ls3 : lm3;
@@ -1790,34 +1817,13 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
           if (gmask & (1u << 4)) gadd(&G[4 * n + cq], mz * sh);
         }
       }
-      if (ownq) {
-        if (inx) {
-          Lout[oxq] = lm1;
-          Lout[oxq + SX] = lm3;
-          if (SPLIT) {
-            Lout[oxq + 4 * SX] = ls1;
-            Lout[oxq + 5 * SX] = ls3;
-          }
-        }
-        if (inzq) {
-          Lout[ozq] = lm2;
-          Lout[ozq + SZ] = lm4;
-          if (SPLIT) {
-            Lout[ozq + 4 * SZ] = ls2;
-            Lout[ozq + 5 * SZ] = ls4;
-          }
-        }
-      }
+      o.x0 = lm1; o.x1 = lm3; o.z0 = lm2; o.z1 = lm4;
+      o.sx0 = ls1; o.sx1 = ls3; o.sz0 = ls2; o.sz1 = ls4;
     }
 
-    // ---- velocities of row pr: gather of the stress-phase adjoints ----
-    if (own && pr >= zo && pr < zend) {
-      const uint32_t cp_ = row(pr);
-      Lout[cp_] = pvx2 + idx * (E1p_w - E1p) + idz * (E4p - Ee4) + idx * (F1p - F1p_e) +
-                  idz * (F4pp - F4p);
-      Lout[n + cp_] = pvz2 + idz * (E2p - Ee2) + idx * (E3p_w - E3p) + idz * (F2pp - F2p) +
-                      idx * (F3p - F3p_e);
-    }
+    // ---- velocities of row r-2: gather of the stress-phase adjoints ----
+    o.lvx = pvx2 + idx * (E1p_w - E1p) + idz * (E4p - Ee4) + idx * (F1p - F1p_e) + idz * (F4pp - F4p);
+    o.lvz = pvz2 + idz * (E2p - Ee2) + idx * (E3p_w - E3p) + idz * (F2pp - F2p) + idx * (F3p - F3p_e);
 
     // ---- shift the windows ----
     F2pp = F2p; F4pp = F4p;
@@ -1827,6 +1833,68 @@ __device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
     A1 = An; B1 = Bn; C1 = Cn; D2 = D1; D1 = Dn;
     vx_m = vx_c; vz_m = vz_c; vx_c = vx_n; vz_c = vz_n;
     sxz_p = sxz1; szz_c = szz1n;
+  }
+
+  // Store the outputs of the prepared march row r into a lambda state for
+  // the cells of rows [g0, g1) on the lanes where `own`.
+  __device__ __forceinline__ void store(const A32<T>& L, const int r, const AdjRow<T>& o, const bool own,
+                                        const int g0, const int g1) const {
+    const int q = r - 1, pr = r - 2;
+    if (own && r >= g0 && r < g1) {
+      if (inx) { L[oxr + 2 * SX] = o.x2; L[oxr + 3 * SX] = o.x3; }
+      if (inzr) { L[ozr + 3 * SZ] = o.z3; L[ozr + 2 * SZ] = o.z2; }
+    }
+    if (own && q >= g0 && q < g1) {
+      L[2 * n + cq] = o.lsxx;
+      L[3 * n + cq] = o.lszz;
+      L[4 * n + cq] = o.lsxz;
+      if (inx) {
+        L[oxq] = o.x0;
+        L[oxq + SX] = o.x1;
+        if (SPLIT) {
+          L[oxq + 4 * SX] = o.sx0;
+          L[oxq + 5 * SX] = o.sx1;
+        }
+      }
+      if (inzq) {
+        L[ozq] = o.z0;
+        L[ozq + SZ] = o.z1;
+        if (SPLIT) {
+          L[ozq + 4 * SZ] = o.sz0;
+          L[ozq + 5 * SZ] = o.sz1;
+        }
+      }
+    }
+    if (own && pr >= g0 && pr < g1) {
+      const uint32_t cp_ = rowi(pr);
+      L[cp_] = o.lvx;
+      L[n + cp_] = o.lvz;
+    }
+  }
+};
+
+template <typename T, bool SPLIT, bool PML>
+__device__ __forceinline__ void el2_adj_tile(const DModel* __restrict__ models,
+                                             const DShot* __restrict__ shots,
+                                             const DAdj* __restrict__ adj, const int s, const int xo,
+                                             const int zo, const int t, const int t0,
+                                             const unsigned gmask, const int lane, const int tb) {
+  const DShot& S = shots[s];
+  const DModel& M = models[S.model];
+  const DAdj& A = adj[s];
+  const int zend = min(zo + AH, M.nz);
+  const A32<const T> Lin(A.lam[(t + 1) & 1]);
+  const A32<T> Lout(A.lam[t & 1]);
+  const T* trail = static_cast<const T*>(A.trail[tb]);
+  AdjStage<T, SPLIT, PML> st(M, S, A, xo, lane, trail + (long)(t + 1 - t0) * A.slot,
+                             trail + (long)(t - t0) * A.slot, zo - 2);
+  const bool own = st.incol && lane >= 2 && lane < 2 + AW;
+  for (int r = zo - 2; r <= zend + 1; ++r) {
+    st.prep(r);
+    AdjRow<T> in, o;
+    st.load(Lin, in);
+    st.row(r, in, o, own, zo, zend, zo - 1, t, gmask);
+    st.store(Lout, r, o, own, zo, zend);
   }
 }
 
@@ -4789,6 +4857,7 @@ int sdt_adjoint_setup(void* plan, const sdt_adj_shot* adj, int nshots, unsigned 
     HIP_TRY(hipMemcpy(p->dadjwork, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice));
     p->nadjwork = (int)work.size();
   }
+
   return SDT_OK;
 }
 
