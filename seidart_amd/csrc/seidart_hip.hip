@@ -2437,11 +2437,17 @@ __device__ __forceinline__ A32<const T> cplane(const void* f, long nxy, int k) {
 #ifndef SDT_EL25V_WAVES
 #define SDT_EL25V_WAVES 4
 #endif
+// the C-PML body of the stress pass with two rows of loads in flight too
+// (measured no faster: 1.312 against 1.310 ms per FP32 aniso25 step, at more
+// registers)
+#ifndef SDT_EL25_PF_PML
+#define SDT_EL25_PF_PML 0
+#endif
 
 template <typename T, bool PML>
 __device__ __forceinline__ void el25_stress_tile(const DModel* __restrict__ models,
                                               const DShot* __restrict__ shots, const int4 wv,
-                                              const int it) {
+                                              const int it, T* __restrict__ cs) {
   const int s = __builtin_amdgcn_readfirstlane(wv.x);
   const DShot& S = shots[s];
   const DModel& M = models[S.model];
@@ -2474,15 +2480,26 @@ __device__ __forceinline__ void el25_stress_tile(const DModel* __restrict__ mode
   // loop ranges of the four legacy stress loops (seismicfdtd.f95:617, :677, :728, :771)
   const bool p1 = own && k >= 1 && i <= nx - 2, p2 = own && k >= 1 && i >= 1;
   const bool p3 = own && k <= nz - 2 && i >= 1, p4 = own && k <= nz - 2 && i <= nx - 2;
-  T C1[15], C2[6], C3[6], C4[6];
+  // The 33 coefficients of the four loops at this column and plane do not
+  // change along y: they are loaded once per work item into LDS, one slot per
+  // thread that only that thread reads back, and read where each loop uses
+  // them (volatile, so they are not held in registers for the whole walk);
+  // the registers go to a second set of row loads instead.  cs: the block's
+  // 33 * NT25 LDS slots.
+  {
+    T* w = cs + threadIdx.x;
 #pragma unroll
-  for (int q = 0; q < 15; ++q) C1[q] = p1 ? A32<const T>(M.coef[q])[p] : T(0);
+    for (int q = 0; q < 15; ++q) w[q * NT25] = p1 ? A32<const T>(M.coef[q])[p] : T(0);
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    C2[q] = p2 ? A32<const T>(M.coef[15 + q])[p] : T(0);
-    C3[q] = p3 ? A32<const T>(M.coef[21 + q])[p] : T(0);
-    C4[q] = p4 ? A32<const T>(M.coef[27 + q])[p] : T(0);
+    for (int q = 0; q < 6; ++q) {
+      w[(15 + q) * NT25] = p2 ? A32<const T>(M.coef[15 + q])[p] : T(0);
+      w[(21 + q) * NT25] = p3 ? A32<const T>(M.coef[21 + q])[p] : T(0);
+      w[(27 + q) * NT25] = p4 ? A32<const T>(M.coef[27 + q])[p] : T(0);
+    }
   }
+  // coefficient q at C[q * NT25], read through an LDS pointer (ds_read)
+  const __attribute__((address_space(3))) volatile T* C =
+      (const __attribute__((address_space(3))) volatile T*)(cs) + threadIdx.x;
   const int kd = k >= 1 ? k - 1 : 0, ku = k <= nz - 2 ? k + 1 : nz - 1;
   const A32<const T> VX = cplane<T>(S.f[0], nxy, k), VY = cplane<T>(S.f[1], nxy, k);
   const A32<const T> VZ = cplane<T>(S.f[2], nxy, k);
@@ -2502,14 +2519,27 @@ __device__ __forceinline__ void el25_stress_tile(const DModel* __restrict__ mode
   T vxm = 0, vym = 0, vzm = 0;
   if (j0 >= 1) { vxm = VX[r0 - nx]; vym = VY[r0 - nx]; vzm = VZ[r0 - nx]; }
   T vxc = VX[r0], vyc = VY[r0], vzc = VZ[r0];
-#pragma unroll 1
-  for (int j = j0; j < j1; ++j) {
-    const uint32_t c = (uint32_t)j * nx + ic;
+
+  // the loads of row j: velocities of row j+1 (n) and of planes k-1 (d) and
+  // k+1 (u), the row's stresses (rows past the last are clamped in bounds)
+  struct RowLd { T vxn, vyn, vzn, vxd, vyd, vzd, vxu, vyu, vzu, sxx, syy, szz, sxy, sxz, syz; };
+  auto load_row = [&](int j) {
+    RowLd R;
+    const uint32_t c = (uint32_t)(j < ny ? j : ny - 1) * nx + ic;
     const uint32_t cn = j + 1 < ny ? c + nx : c;
-    const T vxn = VX[cn], vyn = VY[cn], vzn = VZ[cn];      // row j+1 (n)
-    const T vxd = VXD[c], vyd = VYD[c], vzd = VZD[c];      // plane k-1 (d)
-    const T vxu = VXU[c], vyu = VYU[c], vzu = VZU[c];      // plane k+1 (u)
-    const T sxx0 = SXX[c], syy0 = SYY[c], szz0 = SZZ[c], sxy0 = SXY[c], sxz0 = SXZ[c], syz0 = SYZ[c];
+    R.vxn = VX[cn]; R.vyn = VY[cn]; R.vzn = VZ[cn];
+    R.vxd = VXD[c]; R.vyd = VYD[c]; R.vzd = VZD[c];
+    R.vxu = VXU[c]; R.vyu = VYU[c]; R.vzu = VZU[c];
+    R.sxx = SXX[c]; R.syy = SYY[c]; R.szz = SZZ[c];
+    R.sxy = SXY[c]; R.sxz = SXZ[c]; R.syz = SYZ[c];
+    return R;
+  };
+  auto row = [&](const int j, const RowLd& R) {
+    const uint32_t c = (uint32_t)j * nx + ic;
+    const T vxn = R.vxn, vyn = R.vyn, vzn = R.vzn;      // row j+1 (n)
+    const T vxd = R.vxd, vyd = R.vyd, vzd = R.vzd;      // plane k-1 (d)
+    const T vxu = R.vxu, vyu = R.vyu, vzu = R.vzu;      // plane k+1 (u)
+    const T sxx0 = R.sxx, syy0 = R.syy, szz0 = R.szz, sxy0 = R.sxy, sxz0 = R.sxz, syz0 = R.syz;
     const T vxe = lane_next(vxc), vye = lane_next(vyc), vze = lane_next(vzc);  // i+1 (e)
     const T vxw = lane_prev(vxc), vyw = lane_prev(vyc), vzw = lane_prev(vzc);  // i-1 (w)
     const bool iny = PML && in_slab(j, ny, ly);
@@ -2540,12 +2570,16 @@ __device__ __forceinline__ void el25_stress_tile(const DModel* __restrict__ mode
         mzz = bz_k * mzz + az_k * dvz_dz; mxz = bz_k * mxz + az_k * dvx_dz; myz = bz_k * myz + az_k * dvy_dz;
         dvz_dz = dvz_dz * ikz_k + mzz; dvx_dz = dvx_dz * ikz_k + mxz; dvy_dz = dvy_dz * ikz_k + myz;
       }
-      SXX[c] = sxx0 + (C1[0] * dvx_dx + C1[1] * dvy_dy + C1[2] * dvz_dz + C1[3] * (dvy_dz + dvz_dy) +
-                       C1[4] * (dvx_dz + dvz_dx) + C1[5] * (dvx_dy + dvz_dy)) * dt;  // A3 (c16)
-      SYY[c] = syy0 + (C1[1] * dvx_dx + C1[6] * dvy_dy + C1[7] * dvz_dz + C1[8] * (dvy_dz + dvz_dy) +
-                       C1[9] * (dvx_dz + dvz_dx) + C1[10] * (dvy_dx + dvx_dy)) * dt;
-      SZZ[c] = szz0 + (C1[2] * dvx_dx + C1[7] * dvy_dy + C1[11] * dvz_dz + C1[12] * (dvy_dz + dvz_dy) +
-                       C1[13] * (dvx_dz + dvz_dx) + C1[14] * (dvy_dx + dvx_dy)) * dt;
+      const T c0 = C[0], c1 = C[1 * NT25], c2 = C[2 * NT25], c3 = C[3 * NT25], c4 = C[4 * NT25];
+      const T c5 = C[5 * NT25], c6 = C[6 * NT25], c7 = C[7 * NT25], c8 = C[8 * NT25], c9 = C[9 * NT25];
+      const T c10 = C[10 * NT25], c11 = C[11 * NT25], c12 = C[12 * NT25], c13 = C[13 * NT25];
+      const T c14 = C[14 * NT25];
+      SXX[c] = sxx0 + (c0 * dvx_dx + c1 * dvy_dy + c2 * dvz_dz + c3 * (dvy_dz + dvz_dy) +
+                       c4 * (dvx_dz + dvz_dx) + c5 * (dvx_dy + dvz_dy)) * dt;  // A3 (c16)
+      SYY[c] = syy0 + (c1 * dvx_dx + c6 * dvy_dy + c7 * dvz_dz + c8 * (dvy_dz + dvz_dy) +
+                       c9 * (dvx_dz + dvz_dx) + c10 * (dvy_dx + dvx_dy)) * dt;
+      SZZ[c] = szz0 + (c2 * dvx_dx + c7 * dvy_dy + c11 * dvz_dz + c12 * (dvy_dz + dvz_dy) +
+                       c13 * (dvx_dz + dvz_dx) + c14 * (dvy_dx + dvx_dy)) * dt;
     }
     // loop 2: sigma xy (seismicfdtd.f95:677-725); dvz_dx and dvx_dz get no correction
     if (p2 && j <= ny - 2) {
@@ -2565,8 +2599,9 @@ __device__ __forceinline__ void el25_stress_tile(const DModel* __restrict__ mode
         mzz = bzh_k * mzz + azh_k * dvz_dz; mxz = bzh_k * mxz + azh_k * dvx_dz; myz = bzh_k * myz + azh_k * dvy_dz;
         dvz_dz = dvz_dz * ikzh_k + mzz; dvy_dz = dvy_dz * ikzh_k + myz;
       }
-      SXY[c] = sxy0 + (C2[0] * dvx_dx + C2[1] * dvy_dy + C2[2] * dvz_dz + C2[3] * (dvz_dy + dvy_dz) +
-                       C2[4] * (dvz_dx + dvx_dz) + C2[5] * (dvy_dx + dvx_dy)) * dt;
+      const __attribute__((address_space(3))) volatile T* K = C + 15 * NT25;
+      SXY[c] = sxy0 + (K[0] * dvx_dx + K[NT25] * dvy_dy + K[2 * NT25] * dvz_dz + K[3 * NT25] * (dvz_dy + dvy_dz) +
+                       K[4 * NT25] * (dvz_dx + dvx_dz) + K[5 * NT25] * (dvy_dx + dvx_dy)) * dt;
     }
     // loop 3: sigma xz (seismicfdtd.f95:728-768)
     if (p3 && j >= 1) {
@@ -2586,8 +2621,9 @@ __device__ __forceinline__ void el25_stress_tile(const DModel* __restrict__ mode
         mzz = bzh_k * mzz + azh_k * dvz_dz; mxz = bzh_k * mxz + azh_k * dvx_dz; myz = bzh_k * myz + azh_k * dvy_dz;
         dvz_dz = dvz_dz * ikzh_k + mzz; dvx_dz = dvx_dz * ikzh_k + mxz; dvy_dz = dvy_dz * ikzh_k + myz;
       }
-      SXZ[c] = sxz0 + (C3[0] * dvx_dx + C3[1] * dvy_dy + C3[2] * dvz_dz + C3[3] * (dvx_dz + dvz_dx) +
-                       C3[4] * (dvx_dz + dvz_dx) + C3[5] * (dvx_dy + dvy_dx)) * dt;  // A3 (c45)
+      const __attribute__((address_space(3))) volatile T* K = C + 21 * NT25;
+      SXZ[c] = sxz0 + (K[0] * dvx_dx + K[NT25] * dvy_dy + K[2 * NT25] * dvz_dz + K[3 * NT25] * (dvx_dz + dvz_dx) +
+                       K[4 * NT25] * (dvx_dz + dvz_dx) + K[5 * NT25] * (dvx_dy + dvy_dx)) * dt;  // A3 (c45)
     }
     // loop 4: sigma yz (seismicfdtd.f95:771-813)
     if (p4 && j <= ny - 2) {
@@ -2607,8 +2643,9 @@ __device__ __forceinline__ void el25_stress_tile(const DModel* __restrict__ mode
         mzz = bzh_k * mzz + azh_k * dvz_dz; mxz = bzh_k * mxz + azh_k * dvx_dz; myz = bzh_k * myz + azh_k * dvy_dz;
         dvz_dz = dvz_dz * ikzh_k + mzz; dvx_dz = dvx_dz * ikzh_k + mxz; dvy_dz = dvy_dz * ikzh_k + myz;
       }
-      SYZ[c] = syz0 + (C4[0] * dvx_dx + C4[1] * dvy_dy + C4[2] * dvz_dz + C4[3] * (dvy_dz + dvz_dy) +
-                       C4[4] * (dvx_dz + dvz_dx) + C4[5] * (dvy_dx + dvx_dy)) * dt;
+      const __attribute__((address_space(3))) volatile T* K = C + 27 * NT25;
+      SYZ[c] = syz0 + (K[0] * dvx_dx + K[NT25] * dvy_dy + K[2 * NT25] * dvz_dz + K[3 * NT25] * (dvy_dz + dvz_dy) +
+                       K[4 * NT25] * (dvx_dz + dvz_dx) + K[5 * NT25] * (dvy_dx + dvx_dy)) * dt;
     }
     if (own) {
       if (inx) { psi[ox] = mxx; psi[ox + L.SX] = myx; psi[ox + 2 * L.SX] = mzx; }
@@ -2625,6 +2662,23 @@ __device__ __forceinline__ void el25_stress_tile(const DModel* __restrict__ mode
     }
     vxm = vxc; vym = vyc; vzm = vzc;
     vxc = vxn; vyc = vyn; vzc = vzn;
+  };
+  // two rows per iteration: the loads of row j+1 are in flight while row j
+  // computes, and those of row j+2 while row j+1 does
+  if constexpr (!PML || SDT_EL25_PF_PML) {
+    RowLd A = load_row(j0);
+    int j = j0;
+#pragma unroll 1
+    for (; j + 1 < j1; j += 2) {
+      const RowLd B = load_row(j + 1);
+      row(j, A);
+      A = load_row(j + 2);
+      row(j + 1, B);
+    }
+    if (j < j1) row(j, A);
+  } else {
+#pragma unroll 1
+    for (int j = j0; j < j1; ++j) row(j, load_row(j));
   }
 }
 
@@ -2644,10 +2698,11 @@ k_el25_stress(const DModel* __restrict__ models, const DShot* __restrict__ shots
   const bool interior = xo - 1 >= M.lx && xo + 62 < M.nx - M.lx && k >= M.lz && k < M.nz - M.lz &&
                         j0 >= M.ly && j1 <= M.ny - M.ly;
   const int it = *base + rel;
+  __shared__ T cs[33 * NT25];  // the y-invariant coefficients (el25_stress_tile)
   if (interior)
-    el25_stress_tile<T, false>(models, shots, wv, it);
+    el25_stress_tile<T, false>(models, shots, wv, it, cs);
   else
-    el25_stress_tile<T, true>(models, shots, wv, it);
+    el25_stress_tile<T, true>(models, shots, wv, it, cs);
 }
 
 template <typename T, bool PML>
@@ -2708,15 +2763,25 @@ __device__ __forceinline__ void el25_vel_tile(const DModel* __restrict__ models,
   T sxym = 0, syzm = 0;
   if (j0 >= 1) { sxym = SXY[r0 - nx]; syzm = SYZ[r0 - nx]; }
   T syyc = SYY[r0];
-#pragma unroll 1
-  for (int j = j0; j < j1; ++j) {
-    const uint32_t c = (uint32_t)j * nx + ic;
+  // the loads of row j (rows past the last are clamped in bounds)
+  struct RowLd { T sxx, sxy, sxz, szz, syz, syyn, sxzd, syzd, szzu, vx, vy, vz; };
+  auto load_row = [&](int j) {
+    RowLd R;
+    const uint32_t c = (uint32_t)(j < ny ? j : ny - 1) * nx + ic;
     const uint32_t cn = j + 1 < ny ? c + nx : c;
-    const T sxx = SXX[c], sxy = SXY[c], sxz = SXZ[c], szz = SZZ[c], syz = SYZ[c];
-    const T syyn = SYY[cn];
-    const T sxzd = SXZD[c], syzd = SYZD[c], szzu = SZZU[c];
+    R.sxx = SXX[c]; R.sxy = SXY[c]; R.sxz = SXZ[c]; R.szz = SZZ[c]; R.syz = SYZ[c];
+    R.syyn = SYY[cn];
+    R.sxzd = SXZD[c]; R.syzd = SYZD[c]; R.szzu = SZZU[c];
+    R.vx = VX[c]; R.vy = VY[c]; R.vz = VZ[c];
+    return R;
+  };
+  auto row = [&](const int j, const RowLd& R) {
+    const uint32_t c = (uint32_t)j * nx + ic;
+    const T sxx = R.sxx, sxy = R.sxy, sxz = R.sxz, szz = R.szz, syz = R.syz;
+    const T syyn = R.syyn;
+    const T sxzd = R.sxzd, syzd = R.syzd, szzu = R.szzu;
     const T sxxw = lane_prev(sxx), sxye = lane_next(sxy), sxze = lane_next(sxz);
-    T vxn = VX[c], vyn = VY[c], vzn = VZ[c];
+    T vxn = R.vx, vyn = R.vy, vzn = R.vz;
     const bool iny = PML && in_slab(j, ny, ly);
     const long ox = L.bx + ((long)k * ny + j) * 2 * lx + xs;
     const long oy = L.by + ((long)k * 2 * ly + (iny ? slab_pos(j, ny, ly) : 0)) * nx + ic;
@@ -2760,6 +2825,23 @@ __device__ __forceinline__ void el25_vel_tile(const DModel* __restrict__ models,
       VZ[c] = vzn;
     }
     sxym = sxy; syzm = syz; syyc = syyn;
+  };
+  // two rows per iteration, as in the stress pass (FP64: one row at a time,
+  // two would cost a wavefront per SIMD)
+  if constexpr (sizeof(T) == 4) {
+    RowLd A = load_row(j0);
+    int j = j0;
+#pragma unroll 1
+    for (; j + 1 < j1; j += 2) {
+      const RowLd B = load_row(j + 1);
+      row(j, A);
+      A = load_row(j + 2);
+      row(j + 1, B);
+    }
+    if (j < j1) row(j, A);
+  } else {
+#pragma unroll 1
+    for (int j = j0; j < j1; ++j) row(j, load_row(j));
   }
 }
 
