@@ -249,7 +249,7 @@ def alg_bytes_per_step(problems, itemsize):
     return tot * itemsize
 
 
-def config_lines(survey_steps: int = 200):
+def config_lines(survey_steps: int = 200, ref_steps: int = 0, only=None):
     """Throughput and FP32 accuracy of the other configurations at one GPU.
     The reference configurations run their full length (plain_psv 2000
     steps, gpr 4000, aniso25 3000) in FP32 and FP64, after the hipGraph of
@@ -257,7 +257,9 @@ def config_lines(survey_steps: int = 200):
     configurations (gpr25, gpr_survey) run `survey_steps` steps.  Each line
     gives Gcell-updates/s, the algorithmic bytes per cell-update, the
     fraction of the HBM peak and the FP32 trace error against the FP64 run
-    over the same steps."""
+    over the same steps.  For profiling runs, `ref_steps` > 0 shortens the
+    reference configurations to that many steps and `only` (names) keeps
+    only those lines."""
     import torch
     from seidart_amd import configs
     from seidart_amd.solver import Simulation
@@ -268,6 +270,10 @@ def config_lines(survey_steps: int = 200):
              ("gpr25", configs.gpr25, survey_steps),
              ("gpr_survey", configs.gpr_survey, survey_steps)]
     for name, make, K in cases:
+        if only and name not in only:
+            continue
+        if K is None and ref_steps > 0:
+            K = ref_steps
         p = make() if K is None else make(nt=K)
         K = p.nt
         cells = p.nx * p.ny * p.nz * len(p.shots)
@@ -418,6 +424,11 @@ def main(argv=None):
     ap.add_argument("--config-steps", type=int, default=200,
                     help="steps of the measured lines that are not reference configurations "
                          "(gpr25, gpr_survey); the reference configurations run their full length")
+    ap.add_argument("--ref-config-steps", type=int, default=0,
+                    help="profiling runs: shorten the reference configurations to this many steps "
+                         "(0: their full length)")
+    ap.add_argument("--only-configs", default="",
+                    help="profiling runs: comma-separated names of the configuration lines to run")
     ap.add_argument("--fwi-steps", type=int, default=6000)
     ap.add_argument("--fwi-fd", type=int, default=0,
                     help="also check the FWI gradient against central differences at this "
@@ -611,7 +622,8 @@ def main(argv=None):
         print("bench: legacy C-PML leg done", file=sys.stderr, flush=True)
 
     fwi = None if a.no_fwi else fwi_gradient(nt=a.fwi_steps, fd_cells=a.fwi_fd, tiles=fixed)
-    others = None if a.no_configs else config_lines(a.config_steps)
+    others = None if a.no_configs else config_lines(
+        a.config_steps, a.ref_config_steps, [n for n in a.only_configs.split(",") if n] or None)
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s",
