@@ -1540,7 +1540,8 @@ k_el2_adj_g(const DModel* __restrict__ models, const DShot* __restrict__ shots,
 #ifndef SDT_ADJ_AW
 #define SDT_ADJ_AW 60
 #endif
-constexpr int AW = SDT_ADJ_AW;   // owned columns per wavefront (at most 60)
+constexpr int AW = SDT_ADJ_AW;   // owned columns per wavefront
+static_assert(AW >= 1 && AW <= 60, "adjoint wavefront: two halo lanes on each side");
 constexpr int AWB = 4;   // wavefronts per block, side by side in x
 #ifndef SDT_ADJ_ROWS
 #define SDT_ADJ_ROWS SDT_EL2_ROWS
@@ -2410,7 +2411,8 @@ __global__ void k_em2_edges(const DModel* __restrict__ models, const DShot* __re
 #ifndef SDT_EL25_EW
 #define SDT_EL25_EW 62
 #endif
-constexpr int EW25 = SDT_EL25_EW;      // owned columns per wavefront (at most 62)
+constexpr int EW25 = SDT_EL25_EW;      // owned columns per wavefront
+static_assert(EW25 >= 1 && EW25 <= 62, "2.5-D wavefront: one halo lane on each side");
 constexpr int KB25 = SDT_EL25_PLANES;  // z planes per block
 constexpr int XW25 = SDT_EL25_XW;      // wavefronts side by side in x per block
 constexpr int NT25 = 64 * KB25 * XW25;
