@@ -154,6 +154,36 @@ def test_fused_adjoint_matches_reference_kernels(precision, split):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_fused_adjoint_many_shots_per_gradient(precision):
+    """Eleven shots adding into one gradient array at once (in FP32 through
+    one staging array, with atomics): gradients, source gradients and
+    lambda_0 against the three-kernel adjoint."""
+    p = tiny_problem("elastic2", seed=87, nx=140, nz=70, npml=6, nt=60, gamma=1e-3, nshots=11, nrec=7)
+    out = []
+    for ref in (False, True):
+        eng = _engine(p, precision, every=20, reference=ref)
+        g = torch.Generator().manual_seed(12)
+        for s in range(eng.nshots):
+            eng.init[s].copy_(torch.randn(eng.init[s].numel(), generator=g, dtype=torch.float64))
+        tr = eng.forward()
+        ys = [torch.randn(t.shape, generator=g, dtype=torch.float64) for t in tr]
+        grad, gsrc = eng.backward(ys)
+        out.append(([x.cpu().numpy() for x in grad], [x.cpu().numpy() for x in gsrc],
+                    [eng.adjoint_state(s).double().cpu().numpy() for s in range(eng.nshots)]))
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    (g0, s0, l0), (g1, s1, l1) = out
+    assert len(g0) == 1
+    for q in range(12):
+        if np.abs(g1[0][q]).max() > 0:
+            assert rel_l2(g0[0][q], g1[0][q]) < tol, ("grad slot", q, rel_l2(g0[0][q], g1[0][q]))
+    for a, b in zip(s0, s1):
+        assert rel_l2(a, b) < tol
+    for a, b in zip(l0, l1):
+        assert rel_l2(a, b) < tol
+
+
+@pytest.mark.gpu
 def test_skipped_trivial_slots_give_same_gradients():
     """VTI without attenuation: the engine leaves the c15 / c35 and gamma
     slots off the device, yet its gradients (including those slots') equal
