@@ -22,7 +22,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .cpml import EPS0, MU0, AxisProfile, slab_width
+from .cpml import EPS0, slab_width
 
 __all__ = ["KINDS", "FIELDS", "STIFFNESS", "PERMITTIVITY", "CONDUCTIVITY", "Shot", "Problem",
            "kernel_coefs", "source_increments", "aux_b_x_half_by_row", "ncomp",

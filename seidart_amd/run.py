@@ -28,7 +28,6 @@ from __future__ import annotations
 import json
 import os
 import sys
-import warnings
 from dataclasses import dataclass
 
 import numpy as np
