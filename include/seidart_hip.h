@@ -173,7 +173,8 @@ int sdt_plan_run(void* plan, int it0, int nsteps, int graph_block, void* stream)
  * in the reference;
  * for an autotuner, seidart_amd.solver.Simulation.autotune): a block of
  * waves_per_block (1, 2 or 4) wavefronts side by side, each owning 61
- * columns (58 in the two-step ELASTIC2 kernel), walks `rows` rows (2..256).
+ * columns (56 in the two-step ELASTIC2 kernel, 60 in the two-step EM2
+ * one), walks `rows` rows (1..256).
  * Default: 4 wavefronts and the tallest of 32, 16, 8, 4 rows that gives the
  * plan at least 1024 blocks (else 2).  Results do not depend on the shape.
  * Synchronises the device when the shape changes. */

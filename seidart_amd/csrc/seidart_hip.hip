@@ -4758,7 +4758,7 @@ int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows) {
   if (!pingpong(p->kind)) return fail(SDT_EINVAL, "tile shapes apply to ELASTIC2 and EM2 plans");
   if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
     return fail(SDT_EINVAL, "waves_per_block must be 1, 2 or 4");
-  if (rows < 2 || rows > 256) return fail(SDT_EINVAL, "rows must be in [2, 256]");
+  if (rows < 1 || rows > 256) return fail(SDT_EINVAL, "rows must be in [1, 256]");
   if (waves_per_block == p->el2_wb && rows == p->el2_eh) return SDT_OK;
   // the old work list and the graphs that bake in the launch shape may be in flight
   HIP_TRY(hipDeviceSynchronize());

@@ -239,7 +239,7 @@ class Simulation:
         _hip.check(_hip.lib().sdt_plan_set_steps_per_launch(self._plan, int(steps)))
 
     # ------------------------------------------------------------------
-    TILE_CANDIDATES = tuple((wb, rows) for wb in (4, 2, 1) for rows in (2, 4, 8, 16, 32, 64))
+    TILE_CANDIDATES = tuple((wb, rows) for wb in (4, 2, 1) for rows in (1, 2, 4, 8, 16, 32, 64))
 
     def tiles(self):
         """Current (waves per block, rows) of the single-pass step kernel

@@ -141,7 +141,7 @@ def test_em2_tile_shapes_keep_results(oracle_lib):
     probs = [tiny_problem("em2", seed=95 + m, nt=60, nshots=2, **d)
              for m, d in enumerate([dict(nx=300, nz=75, npml=6), dict(nx=57, nz=45, npml=5)])]
     ref = _sim(probs, "fp32").run(graph_block=10).check().traces()
-    for wb, rows in ((1, 16), (2, 64), (4, 8), (1, 2), (4, 4)):
+    for wb, rows in ((1, 16), (2, 64), (4, 8), (1, 2), (4, 4), (1, 1), (4, 1)):
         got = _sim(probs, "fp32").set_tiles(wb, rows).run(graph_block=10).check().traces()
         for a, b in zip(ref, got):
             np.testing.assert_array_equal(a, b)
@@ -284,14 +284,14 @@ def test_tile_shapes_and_autotune_keep_results(oracle_lib):
     probs = [tiny_problem("elastic2", seed=90 + m, nt=60, gamma=1e-3, nshots=2, **d)
              for m, d in enumerate([dict(nx=300, nz=75, npml=6), dict(nx=57, nz=45, npml=5)])]
     ref = _sim(probs, "fp32").run(graph_block=10).check().traces()
-    for wb, rows in ((1, 16), (2, 64), (4, 8), (1, 2), (4, 4)):
+    for wb, rows in ((1, 16), (2, 64), (4, 8), (1, 2), (4, 4), (1, 1), (4, 1)):
         got = _sim(probs, "fp32").set_tiles(wb, rows).run(graph_block=10).check().traces()
         for a, b in zip(ref, got):
             np.testing.assert_array_equal(a, b)
     s = _sim(probs, "fp32")
     s.run(7)
     res = s.autotune(steps=4)
-    assert len(res) == 36 and (s.steps_per_launch, *s.tiles()) in res
+    assert len(res) == 2 * len(type(s).TILE_CANDIDATES) and (s.steps_per_launch, *s.tiles()) in res
     got = s.run(graph_block=10).check().traces()
     for a, b in zip(ref, got):
         np.testing.assert_array_equal(a, b)

@@ -72,7 +72,7 @@ def test_two_step_isotropic_without_attenuation():
 def test_two_step_tile_shapes():
     probs = _problems(30, seed=330)
     ref = _snap(_sim(probs, "fp32", 1).run().check())
-    for wb, rows in ((1, 2), (2, 3), (4, 16), (1, 64), (4, 256), (2, 32)):
+    for wb, rows in ((1, 2), (2, 3), (4, 16), (1, 64), (4, 256), (2, 32), (1, 1), (4, 1)):
         got = _snap(_sim(probs, "fp32", 2).set_tiles(wb, rows).run(graph_block=6).check())
         _assert_same(ref, got)
 
@@ -142,7 +142,7 @@ def test_em2_two_step_equals_one_step(precision):
     probs = _em_problems(61)
     probs[1].shots[1].src = (probs[1].nx - 3, 0, 2)   # source in the C-PML corner
     ref = _snap(_sim(probs, precision, 1).run(graph_block=10).check())
-    for gb, tiles in ((10, None), (0, (1, 2)), (6, (2, 16)), (4, (4, 64))):
+    for gb, tiles in ((10, None), (0, (1, 2)), (6, (2, 16)), (4, (4, 64)), (6, (2, 1))):
         s = _sim(probs, precision, 2)
         if tiles:
             s.set_tiles(*tiles)
