@@ -3876,7 +3876,6 @@ std::vector<int4> el25_work(const Plan* p, int planes, int xstep) {
 // 2: receiver gather) of one step.
 template <typename T>
 void launch_one(Plan* p, hipStream_t st, int rel, int which) {
-  const dim3 blk(TX, TZ);
   const dim3 grd(p->nwork);
   if (which == 2) {
     // the 2-D step kernels write their receiver samples themselves
