@@ -277,7 +277,7 @@ __device__ __forceinline__ T dpp_prev(T v) {
   }
 }
 
-template <typename T, bool TRAIL, bool SPLIT, bool PML>
+template <typename T, bool TRAIL, bool SPLIT, bool PML, bool R1>
 __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
                                          const DShot* __restrict__ shots, const int4 wi, const int it,
                                          const int lane, const DAdj* __restrict__ adj, const int slot,
@@ -292,12 +292,15 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
 // slot 0) and write the new state into the next slot (forward recomputation
 // for the adjoint).
 // SPLIT: the split C-PML variant (see SDT_SPLIT_CPML).
+// R1: tiles of one row (eh == 1), whose three row steps take their loads
+// all at once (a latency-bound shape: register budget of one wavefront per
+// SIMD).
 // The state of step it is read from buffer it & 1 (A = field/psi, B =
 // field_b/psi_b) and written to the other; with prun (the first step of a
 // run of two-step launches, k_el2_step2) it is read from buffer
 // ((it - *prun) / 2) & 1 instead, the buffer those launches leave it in.
-template <typename T, bool TRAIL, bool SPLIT, int WB>
-__global__ void __launch_bounds__(64 * WB, sizeof(T) == 4 ? SDT_EL2_WAVES_PER_SIMD : 3)
+template <typename T, bool TRAIL, bool SPLIT, int WB, bool R1 = false>
+__global__ void __launch_bounds__(64 * WB, R1 ? 1 : (sizeof(T) == 4 ? SDT_EL2_WAVES_PER_SIMD : 3))
 k_el2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
            const int4* __restrict__ work, int nitems, const int* __restrict__ base, int rel,
            const DAdj* __restrict__ adj, const int* __restrict__ tsel, int eh,
@@ -320,13 +323,13 @@ k_el2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
                           wi.z + eh + 1 < M.nz - M.lz;
     const int tb = TRAIL ? __builtin_amdgcn_readfirstlane(*tsel) : 0;
     if (interior)
-      el2_tile<T, TRAIL, SPLIT, false>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh, par);
+      el2_tile<T, TRAIL, SPLIT, false, R1>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh, par);
     else
-      el2_tile<T, TRAIL, SPLIT, true>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh, par);
+      el2_tile<T, TRAIL, SPLIT, true, R1>(models, shots, wi, it, threadIdx.x & 63, adj, rel, tb, eh, par);
   }
 }
 
-template <typename T, bool TRAIL, bool SPLIT, bool PML>
+template <typename T, bool TRAIL, bool SPLIT, bool PML, bool R1>
 __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
                                          const DShot* __restrict__ shots, const int4 wi, const int it,
                                          const int lane, const DAdj* __restrict__ adj, const int slot,
@@ -462,14 +465,8 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
   Row ahead{};
   if constexpr (PF) ahead = load_row(zo - 1);
 
-  for (int r = zo - 1; r <= zend; ++r) {
-    Row L;
-    if constexpr (PF) {
-      L = ahead;
-      ahead = load_row(r + 1);
-    } else {
-      L = load_row(r);
-    }
+  // row step r: the stresses of row r, then the velocities of row r - 1
+  auto step_row = [&](const int r, const Row& L) {
     const int rr = r - 1;                             // velocity row
     const uint32_t c = row(r), cv = row(rr);
     const bool rv = valid(r);
@@ -619,6 +616,25 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
     vz_m = vz_c;
     vx_c = vx_n;
     vz_c = vz_n;
+  };
+  if constexpr (R1) {
+    // one row (zend == zo + 1): the loads of rows zo-1, zo, zo+1 read only
+    // the previous state, so all three are in flight before the first step
+    const Row L0 = load_row(zo - 1), L1 = load_row(zo), L2 = load_row(zo + 1);
+    step_row(zo - 1, L0);
+    step_row(zo, L1);
+    step_row(zo + 1, L2);
+  } else {
+    for (int r = zo - 1; r <= zend; ++r) {
+      Row L;
+      if constexpr (PF) {
+        L = ahead;
+        ahead = load_row(r + 1);
+      } else {
+        L = load_row(r);
+      }
+      step_row(r, L);
+    }
   }
 }
 
@@ -3667,6 +3683,23 @@ template <typename T, bool TRAIL, bool SPLIT>
 void launch_el2(Plan* p, hipStream_t st, int rel, const int* tsel, const int* prun = nullptr) {
   const DAdj* adj = TRAIL ? p->dadj : nullptr;
   const dim3 g(p->el2_grid);
+  if (!TRAIL && p->el2_eh == 1) {  // tiles of one row: all loads at once
+    switch (p->el2_wb) {
+      case 1:
+        hipLaunchKernelGGL((k_el2_step<T, false, SPLIT, 1, true>), g, dim3(64), 0, st, p->dmodels,
+                           p->dshots, p->dwork, p->nwork, p->dbase, rel, adj, tsel, 1, prun);
+        break;
+      case 2:
+        hipLaunchKernelGGL((k_el2_step<T, false, SPLIT, 2, true>), g, dim3(128), 0, st, p->dmodels,
+                           p->dshots, p->dwork, p->nwork, p->dbase, rel, adj, tsel, 1, prun);
+        break;
+      default:
+        hipLaunchKernelGGL((k_el2_step<T, false, SPLIT, 4, true>), g, dim3(256), 0, st, p->dmodels,
+                           p->dshots, p->dwork, p->nwork, p->dbase, rel, adj, tsel, 1, prun);
+        break;
+    }
+    return;
+  }
   switch (p->el2_wb) {
     case 1:
       hipLaunchKernelGGL((k_el2_step<T, TRAIL, SPLIT, 1>), g, dim3(64), 0, st, p->dmodels,
