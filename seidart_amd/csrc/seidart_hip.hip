@@ -2291,16 +2291,27 @@ __device__ __forceinline__ void em2x2_tile(const DModel* __restrict__ models,
   const bool has_rec = nrec > 0 && S.rrow[ze] > S.rrow[zo];  // receivers on own rows (per tile)
   bool bad0 = false, bad1 = false;
 
-  for (int r = zo - 2; r <= ze; ++r) {
-    const bool actB = r >= zo - 1, actC = r >= zo, actD = r >= zo + 1;
+  // every load of row r (addresses clamped in bounds)
+  struct Row { T exn, ez, hy; CE k; MX m; };
+  auto load_row = [&](int r) {
+    Row L;
     const uint32_t c = cell(r);
-    const T ld_exn = exi[cell(r + 1)], ld_ez = ezi[c], ld_hy = hyi[c];
-    CE k;
-    k.cax = CAX[c];
-    k.cbx = CBX[c];
-    k.caz = CAZ[c];
-    k.cbz = CBZ[c];
-    MX m = load_m(r);
+    L.exn = exi[cell(r + 1)];
+    L.ez = ezi[c];
+    L.hy = hyi[c];
+    L.k.cax = CAX[c];
+    L.k.cbx = CBX[c];
+    L.k.caz = CAZ[c];
+    L.k.cbz = CBZ[c];
+    L.m = load_m(r);
+    return L;
+  };
+  // row step r: A, B of step it at row r, then C, D of step it+1 at row r-1
+  auto step_row = [&](const int r, const Row& L) {
+    const bool actB = r >= zo - 1, actC = r >= zo, actD = r >= zo + 1;
+    const T ld_exn = L.exn, ld_ez = L.ez, ld_hy = L.hy;
+    const CE k = L.k;
+    MX m = L.m;
     const bool rv = valid(r);
     const T ex0_n = valid(r + 1) ? ld_exn : T(0);
     const T ez0_c = rv ? ld_ez : T(0);
@@ -2342,7 +2353,8 @@ __device__ __forceinline__ void em2x2_tile(const DModel* __restrict__ models,
     hy2_p = hy2_c;
     k_p = k;
     m_p = m;
-  }
+  };
+  for (int r = zo - 2; r <= ze; ++r) step_row(r, load_row(r));
   if (bad0)
     check_blowup(HUGE_VAL, S.blowup, it);
   else if (bad1)

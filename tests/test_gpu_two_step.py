@@ -142,7 +142,8 @@ def test_em2_two_step_equals_one_step(precision):
     probs = _em_problems(61)
     probs[1].shots[1].src = (probs[1].nx - 3, 0, 2)   # source in the C-PML corner
     ref = _snap(_sim(probs, precision, 1).run(graph_block=10).check())
-    for gb, tiles in ((10, None), (0, (1, 2)), (6, (2, 16)), (4, (4, 64)), (6, (2, 1))):
+    for gb, tiles in ((10, None), (0, (1, 2)), (6, (2, 16)), (4, (4, 64)), (6, (2, 1)), (4, (4, 4)),
+                      (0, (2, 4))):
         s = _sim(probs, precision, 2)
         if tiles:
             s.set_tiles(*tiles)
