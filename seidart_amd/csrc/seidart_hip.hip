@@ -311,11 +311,12 @@ k_el2_step(const DModel* __restrict__ models, const DShot* __restrict__ shots,
   const int par = prun ? __builtin_amdgcn_readfirstlane(((it - *prun) >> 1) & 1) : (it & 1);
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
     const int4 wv = work[w];
+    // .w: the shot's model, so its record loads beside the shot's
     const int4 wi = make_int4(__builtin_amdgcn_readfirstlane(wv.x),
                               __builtin_amdgcn_readfirstlane(wv.y) + wave * EW,
-                              __builtin_amdgcn_readfirstlane(wv.z), 0);
+                              __builtin_amdgcn_readfirstlane(wv.z), __builtin_amdgcn_readfirstlane(wv.w));
     if (wi.x < 0) continue;
-    const DModel& M = models[shots[wi.x].model];
+    const DModel& M = models[wi.w];
     if (wi.y >= M.nx) continue;
     // a wave whose columns and rows (halo included) avoid every C-PML slab
     // runs the body without memory variables
@@ -338,7 +339,7 @@ __device__ __forceinline__ void el2_tile(const DModel* __restrict__ models,
   // the interior and C-PML bodies (and every tile shape) round identically.
 #pragma clang fp contract(off)
   const DShot& S = shots[wi.x];
-  const DModel& M = models[S.model];
+  const DModel& M = models[wi.w];  // the shot's model (carried by the work item)
   const int nx = M.nx, nz = M.nz;
   const int xo = wi.y, zo = wi.z;
   const int i = xo - 1 + lane;
@@ -3852,7 +3853,8 @@ std::vector<int> row_cuts(int nz, int eh, int zt, int zb) {
 // which: 0 every block, 1 the interior blocks of the two-step kernel, 2 the
 // others.  ew: owned columns per wavefront; two-step lists (ew = EW2, EW2E)
 // cut their rows at the interior limits of their kernel and carry each
-// tile's height in .w (0: the plan's rows).
+// tile's height in .w (0: the plan's rows); one-step lists (ew = EW) carry
+// the shot's model index in .w.
 std::vector<int4> el2_work(const Plan* p, int ew = EW, int which = 0) {
   std::vector<int4> work;
   const int bw = ew * p->el2_wb, eh = p->el2_eh;
@@ -3883,7 +3885,7 @@ std::vector<int4> el2_work(const Plan* p, int ew = EW, int which = 0) {
             continue;
           }
           const int z0 = cuts[tl[k] / tx], h = cuts[tl[k] / tx + 1] - z0;
-          work.push_back(make_int4(s, (tl[k] % tx) * bw, z0, ew == EW ? 0 : h));
+          work.push_back(make_int4(s, (tl[k] % tx) * bw, z0, ew == EW ? m : h));
         }
   }
   return work;
