@@ -69,6 +69,21 @@ def test_two_step_isotropic_without_attenuation():
     _assert_same(ref, got)
 
 
+@pytest.mark.parametrize("precision,split", [("fp32", False), ("fp64", False),
+                                             ("fp32", True), ("fp64", True)])
+def test_one_row_tiles_equal_taller_tiles(precision, split):
+    """Tiles of one row run the one-step kernel instance that issues all of a
+    tile's loads before its first row step; the state and traces equal those
+    of 8-row tiles bit for bit, for blocks of 1, 2 and 4 wavefronts, three
+    models, receivers on the outer ring and a source in a C-PML corner."""
+    probs = _problems(41, split=split, seed=340)
+    probs[1].shots[1].src = (probs[1].nx - 3, 0, 2)
+    ref = _snap(_sim(probs, precision, 1).set_tiles(4, 8).run(graph_block=8).check())
+    for wb in (1, 2, 4):
+        got = _snap(_sim(probs, precision, 1).set_tiles(wb, 1).run(graph_block=5).check())
+        _assert_same(ref, got)
+
+
 def test_two_step_tile_shapes():
     probs = _problems(30, seed=330)
     ref = _snap(_sim(probs, "fp32", 1).run().check())
