@@ -240,6 +240,7 @@ class Simulation:
 
     # ------------------------------------------------------------------
     TILE_CANDIDATES = tuple((wb, rows) for wb in (4, 2, 1) for rows in (1, 2, 4, 8, 16, 32, 64))
+    AUTOTUNE_RETIME = 3  # shapes autotune times a second time
 
     def tiles(self):
         """Current (waves per block, rows) of the single-pass step kernel
@@ -262,8 +263,10 @@ class Simulation:
         counter are restored afterwards, so the run continues as if nothing
         had happened.  Returns {(steps_per_launch, wb, rows): ms per step}
         and keeps the best (``steps`` is rounded up to a multiple of 4, so
-        both modes end a block with the state in the same buffer).  Other
-        kinds have a single shape: returns {}."""
+        both modes end a block with the state in the same buffer); the
+        ``AUTOTUNE_RETIME`` fastest shapes are timed again over three replays
+        and keep their fastest time.  Other kinds have a single shape:
+        returns {}."""
         if self.kind not in PINGPONG:
             return {}
         left = self.nt - self.step
@@ -286,19 +289,31 @@ class Simulation:
                 if traces:
                     b["trace"].copy_(t0)
 
-        res = {}
-        for spl in spls:
+        def timed(spl, wb, rows, reps):
+            # capture this shape, then the fastest of `reps` timed replays
             self.steps_per_launch = spl
-            for wb, rows in (candidates or self.TILE_CANDIDATES):
-                self.set_tiles(wb, rows)
-                self.run(steps, graph_block=steps)    # capture this shape
-                restore()
+            self.set_tiles(wb, rows)
+            self.run(steps, graph_block=steps)
+            restore()
+            t = float("inf")
+            for _ in range(reps):
                 ev0.record(stream)
                 self.run(steps, graph_block=steps)
                 ev1.record(stream)
                 ev1.synchronize()
-                res[(spl, wb, rows)] = ev0.elapsed_time(ev1) / steps
-                restore(traces=True)
+                t = min(t, ev0.elapsed_time(ev1) / steps)
+                restore()
+            restore(traces=True)
+            return t
+
+        res = {}
+        for spl in spls:
+            for wb, rows in (candidates or self.TILE_CANDIDATES):
+                res[(spl, wb, rows)] = timed(spl, wb, rows, 1)
+        # one replay can order shapes a few per cent apart wrongly: the
+        # fastest few are timed again, three replays each
+        for key in sorted(res, key=res.get)[:self.AUTOTUNE_RETIME]:
+            res[key] = min(res[key], timed(*key, 3))
         best = min(res, key=res.get)
         self.steps_per_launch = best[0]
         self.set_tiles(*best[1:])
