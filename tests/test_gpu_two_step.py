@@ -75,9 +75,14 @@ def test_one_row_tiles_equal_taller_tiles(precision, split):
     """Tiles of one row run the one-step kernel instance that issues all of a
     tile's loads before its first row step; the state and traces equal those
     of 8-row tiles bit for bit, for blocks of 1, 2 and 4 wavefronts, three
-    models, receivers on the outer ring and a source in a C-PML corner."""
+    models, receivers on the outer ring, a source in a C-PML corner and a
+    stress-rate (moment) source."""
     probs = _problems(41, split=split, seed=340)
     probs[1].shots[1].src = (probs[1].nx - 3, 0, 2)
+    p = probs[0]
+    t = np.arange(p.nt) * p.dt
+    w = np.exp(-((t - 15 * p.dt) / (5 * p.dt)) ** 2)
+    p.shots[0].moment = (1e3 * w, None, 2e3 * w, None, -5e2 * w, None)
     ref = _snap(_sim(probs, precision, 1).set_tiles(4, 8).run(graph_block=8).check())
     for wb in (1, 2, 4):
         got = _snap(_sim(probs, precision, 1).set_tiles(wb, 1).run(graph_block=5).check())
