@@ -468,10 +468,10 @@ def main(argv=None):
     fixed = tuple(int(v) for v in a.tiles.split("x")) if a.tiles else None
     if fixed:
         sim.set_tiles(*fixed)
-        tune = {(sim.steps_per_launch, *fixed): float("nan")}
+        tune = {(sim.steps_per_launch, *fixed, sim.tile_order): float("nan")}
     else:
         tune = sim.autotune(steps=10)   # tile shape of the step kernel (setup, untimed)
-    sim_spl, sim_tiles = sim.steps_per_launch, sim.tiles()
+    sim_spl, sim_tiles, sim_order = sim.steps_per_launch, sim.tiles(), sim.tile_order
     captured = sim.capture_graph(gb) if gb else False
     sim.run(W, graph_block=gb if W >= gb else 0)
     torch.cuda.synchronize()
@@ -549,8 +549,7 @@ def main(argv=None):
                     r = sp.autotune(steps=12)   # tile shape and steps per launch (setup, untimed)
                     best = min(r, key=r.get) if r else None
                 elif best:
-                    sp.steps_per_launch = best[0]
-                    sp.set_tiles(*best[1:])
+                    sp.use(best)
                 sp.capture_graph(50)
                 torch.cuda.synchronize()
                 t1 = time.perf_counter()
@@ -639,9 +638,10 @@ def main(argv=None):
                    "cells_per_step": cells_per_step, "shots": sum(shots),
                    "graph_block": gb if captured else 0,
                    "tiles": {"chosen": {"steps_per_launch": sim_spl, "waves": sim_tiles[0],
-                                        "rows": sim_tiles[1]},
+                                        "rows": sim_tiles[1], "order": sim_order},
                              "ms_per_step": {} if fixed else
-                             {f"{n}-step {w}x{r}": round(v, 4) for (n, w, r), v in tune.items()}}},
+                             {f"{n}-step {w}x{r} order {o}": round(v, 4)
+                              for (n, w, r, o), v in tune.items()}}},
         "gpu_ms_events": round(gpu_ms, 3),
         "roofline": roofline,
         "fp64_plain": plain,

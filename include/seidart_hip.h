@@ -181,6 +181,19 @@ int sdt_plan_run(void* plan, int it0, int nsteps, int graph_block, void* stream)
 int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows);
 int sdt_plan_get_tiles(void* plan, int* waves_per_block, int* rows);
 
+/* ELASTIC2 / EM2 order of the step kernels' work lists (no counterpart in
+ * the reference; the autotuner's third knob beside the tile shape and the
+ * steps per launch).  0 (the default) deals a model's tiles to the eight
+ * XCDs in turn, so every XCD covers the whole model; 1 lists the tiles
+ * column by column and gives each XCD a contiguous run of them, so a tile
+ * and the one below it, which read the same halo rows, run on one XCD one
+ * after the other, and each XCD keeps to one part of the model from step
+ * to step (small single-shot grids gain; the large batched plans do not).
+ * Either way all shots of a tile run on one XCD.  Results do not depend on
+ * the order.  Synchronises the device when the order changes. */
+int sdt_plan_set_tile_order(void* plan, int order);
+int sdt_plan_get_tile_order(void* plan, int* order);
+
 /* ELASTIC2 / EM2 steps per launch of the step kernel (no counterpart in the
  * reference): 2 (the default for plans of at least 4M cells per step,
  * summed over shots; smaller plans default to 1) advances two time steps in

@@ -75,7 +75,7 @@ EXPORTS = ["sdt_psi_elems", "sdt_plan_create", "sdt_plan_run", "sdt_plan_check",
            "sdt_plan_profile", "sdt_plan_destroy", "sdt_last_error", "sdt_version",
            "sdt_state_elems", "sdt_adjoint_work_elems", "sdt_adjoint_setup",
            "sdt_plan_run_trail", "sdt_adjoint_run", "sdt_plan_set_tiles", "sdt_plan_get_tiles",
-           "sdt_plan_set_steps_per_launch", "sdt_plan_get_steps_per_launch", "sdt_run_noalloc",
+           "sdt_plan_set_tile_order", "sdt_plan_get_tile_order", "sdt_plan_set_steps_per_launch", "sdt_plan_get_steps_per_launch", "sdt_run_noalloc",
            "sdt_noalloc_scratch_bytes", "sdt_hbm_probe"]
 
 
@@ -120,6 +120,10 @@ def lib():
     L.sdt_plan_set_tiles.restype = C.c_int
     L.sdt_plan_get_tiles.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.sdt_plan_get_tiles.restype = C.c_int
+    L.sdt_plan_set_tile_order.argtypes = [C.c_void_p, C.c_int]
+    L.sdt_plan_set_tile_order.restype = C.c_int
+    L.sdt_plan_get_tile_order.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.sdt_plan_get_tile_order.restype = C.c_int
     L.sdt_plan_set_steps_per_launch.argtypes = [C.c_void_p, C.c_int]
     L.sdt_plan_set_steps_per_launch.restype = C.c_int
     L.sdt_plan_get_steps_per_launch.argtypes = [C.c_void_p, C.POINTER(C.c_int)]

@@ -74,6 +74,11 @@ class AdjointEngine:
         coefficients may make them non-trivial; otherwise the kernels skip
         them (their gradients are still available)."""
         self.sim = Simulation(problems, precision, device, all_slots=all_slots, all_sources=True)
+        # the trail recomputation runs the one-step kernel over the plan's
+        # work list beside the adjoint kernel; there the column-strip order
+        # (sdt_plan_set_tile_order 1) costs more than it gains in the
+        # forward pass, so autotune keeps the default order
+        self.sim.TILE_ORDERS = (0,)
         if self.sim.kind not in NSLOTS:
             raise ValueError("the adjoint engine handles elastic2 and em2 problems")
         self.kind = self.sim.kind

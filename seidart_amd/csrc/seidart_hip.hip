@@ -3612,6 +3612,7 @@ struct Plan {
   int el2_grid = 0;  // persistent grid of the ELASTIC2 step kernel
   int el2_wb = SDT_EL2_WAVES_PER_BLOCK;  // tile: wavefronts per block
   int el2_eh = SDT_EL2_ROWS;             // tile: rows
+  int el2_order = 0;  // work-list order: 0 tiles dealt to the XCDs in turn, 1 column strips per XCD
   int spl = 1;                 // steps per launch of the ELASTIC2 step kernel (1 or 2)
   int4* dwork2 = nullptr;      // work list of the two-step kernel: interior blocks first
   int nwork2 = 0, nwork2i = 0;
@@ -3854,7 +3855,13 @@ std::vector<int> row_cuts(int nz, int eh, int zt, int zb) {
 // others.  ew: owned columns per wavefront; two-step lists (ew = EW2, EW2E)
 // cut their rows at the interior limits of their kernel and carry each
 // tile's height in .w (0: the plan's rows); one-step lists (ew = EW) carry
-// the shot's model index in .w.
+// the shot's model index in .w.  Order (p->el2_order): 0 deals a model's
+// tiles to the XCDs in turn (tile k of the list on XCD k % 8), every XCD
+// covering the whole model; 1 lists them column by column and gives each
+// XCD a contiguous run, so a tile and the one below it, which share halo
+// rows, run on one XCD one after the other, and each XCD keeps to one part
+// of the model from step to step.  Either way all shots of a tile run on
+// one XCD, one after another.
 std::vector<int4> el2_work(const Plan* p, int ew = EW, int which = 0) {
   std::vector<int4> work;
   const int bw = ew * p->el2_wb, eh = p->el2_eh;
@@ -3870,22 +3877,34 @@ std::vector<int4> el2_work(const Plan* p, int ew = EW, int which = 0) {
       if (p->hshots[s].model == m) ms.push_back(s);
     if (ms.empty()) continue;
     std::vector<int> tl;
-    for (int t = 0; t < ntile; ++t) {
+    auto take = [&](int t) {
       const int z0 = cuts[t / tx], h = cuts[t / tx + 1] - z0;
       if (which == 0 || el2x2_interior(M, (t % tx) * bw, z0, p->el2_wb, h) == (which == 1))
         tl.push_back(t);
+    };
+    if (p->el2_order) {
+      for (int x = 0; x < tx; ++x)
+        for (int z = 0; z < tz; ++z) take(z * tx + x);
+    } else {
+      for (int t = 0; t < ntile; ++t) take(t);
     }
     const int nt = (int)tl.size();
-    for (int tg = 0; tg < (nt + 7) / 8; ++tg)
+    // tile of slot tg of XCD tt (-1: none)
+    const int chunk = (nt + 7) / 8;
+    auto tile_at = [&](int tg, int tt) {
+      const int k = p->el2_order ? tt * chunk + tg : tg * 8 + tt;
+      return k < nt ? tl[k] : -1;
+    };
+    for (int tg = 0; tg < chunk; ++tg)
       for (int s : ms)
         for (int tt = 0; tt < 8; ++tt) {
-          const int k = tg * 8 + tt;
-          if (k >= nt) {
+          const int t = tile_at(tg, tt);
+          if (t < 0) {
             work.push_back(make_int4(-1, 0, 0, 0));
             continue;
           }
-          const int z0 = cuts[tl[k] / tx], h = cuts[tl[k] / tx + 1] - z0;
-          work.push_back(make_int4(s, (tl[k] % tx) * bw, z0, ew == EW ? m : h));
+          const int z0 = cuts[t / tx], h = cuts[t / tx + 1] - z0;
+          work.push_back(make_int4(s, (t % tx) * bw, z0, ew == EW ? m : h));
         }
   }
   return work;
@@ -4798,20 +4817,16 @@ int sdt_hbm_probe(const void* src, void* dst, long long nfloat, void* stream) {
   return SDT_OK;
 }
 
-int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows) {
-  Plan* p = static_cast<Plan*>(plan);
-  if (!p) return fail(SDT_EINVAL, "plan is null");
-  if (!pingpong(p->kind)) return fail(SDT_EINVAL, "tile shapes apply to ELASTIC2 and EM2 plans");
-  if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
-    return fail(SDT_EINVAL, "waves_per_block must be 1, 2 or 4");
-  if (rows < 1 || rows > 256) return fail(SDT_EINVAL, "rows must be in [1, 256]");
-  if (waves_per_block == p->el2_wb && rows == p->el2_eh) return SDT_OK;
+// Rebuild the 2-D work lists for a tile shape and order (all or nothing: on
+// a failure the plan keeps its tiles, order and work lists), dropping the
+// graphs that bake in the old launch shape.
+static int retile(Plan* p, int waves_per_block, int rows, int order) {
   // the old work list and the graphs that bake in the launch shape may be in flight
   HIP_TRY(hipDeviceSynchronize());
-  // all or nothing: on a failure the plan keeps its tiles and work lists
-  const int owb = p->el2_wb, oeh = p->el2_eh;
+  const int owb = p->el2_wb, oeh = p->el2_eh, oord = p->el2_order;
   p->el2_wb = waves_per_block;
   p->el2_eh = rows;
+  p->el2_order = order;
   const std::vector<int4> work = el2_work(p);
   int4* dw = nullptr;
   int rc = SDT_OK;
@@ -4828,6 +4843,7 @@ int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows) {
     if (dw) (void)hipFree(dw);
     p->el2_wb = owb;
     p->el2_eh = oeh;
+    p->el2_order = oord;
     return rc;
   }
   (void)hipFree(p->dwork);
@@ -4837,6 +4853,33 @@ int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows) {
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
   if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
   drop_adjoint_graphs(p);
+  return SDT_OK;
+}
+
+int sdt_plan_set_tiles(void* plan, int waves_per_block, int rows) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p) return fail(SDT_EINVAL, "plan is null");
+  if (!pingpong(p->kind)) return fail(SDT_EINVAL, "tile shapes apply to ELASTIC2 and EM2 plans");
+  if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
+    return fail(SDT_EINVAL, "waves_per_block must be 1, 2 or 4");
+  if (rows < 1 || rows > 256) return fail(SDT_EINVAL, "rows must be in [1, 256]");
+  if (waves_per_block == p->el2_wb && rows == p->el2_eh) return SDT_OK;
+  return retile(p, waves_per_block, rows, p->el2_order);
+}
+
+int sdt_plan_set_tile_order(void* plan, int order) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p) return fail(SDT_EINVAL, "plan is null");
+  if (!pingpong(p->kind)) return fail(SDT_EINVAL, "tile orders apply to ELASTIC2 and EM2 plans");
+  if (order != 0 && order != 1) return fail(SDT_EINVAL, "tile order must be 0 or 1");
+  if (order == p->el2_order) return SDT_OK;
+  return retile(p, p->el2_wb, p->el2_eh, order);
+}
+
+int sdt_plan_get_tile_order(void* plan, int* order) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p || !order) return fail(SDT_EINVAL, "null argument");
+  *order = p->el2_order;
   return SDT_OK;
 }
 

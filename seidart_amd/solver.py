@@ -240,6 +240,7 @@ class Simulation:
 
     # ------------------------------------------------------------------
     TILE_CANDIDATES = tuple((wb, rows) for wb in (4, 2, 1) for rows in (1, 2, 4, 8, 16, 32, 64))
+    TILE_ORDERS = (0, 1)  # work-list orders (sdt_plan_set_tile_order)
     AUTOTUNE_RETIME = 3  # shapes autotune times a second time
 
     def tiles(self):
@@ -253,16 +254,38 @@ class Simulation:
         _hip.check(_hip.lib().sdt_plan_set_tiles(self._plan, int(waves_per_block), int(rows)))
         return self
 
+    @property
+    def tile_order(self) -> int:
+        """Order of the step kernels' work lists: 0 deals the tiles to the
+        XCDs in turn, 1 gives each XCD a run of tiles taken column by column
+        (ELASTIC2, EM2); results do not depend on it."""
+        v = C.c_int()
+        _hip.check(_hip.lib().sdt_plan_get_tile_order(self._plan, C.byref(v)))
+        return v.value
+
+    @tile_order.setter
+    def tile_order(self, order: int):
+        _hip.check(_hip.lib().sdt_plan_set_tile_order(self._plan, int(order)))
+
+    def use(self, key):
+        """Apply an ``autotune`` key (steps per launch, waves per block,
+        rows, tile order)."""
+        spl, wb, rows, order = key
+        self.steps_per_launch = spl
+        self.tile_order = order
+        return self.set_tiles(wb, rows)
+
     def autotune(self, steps: int = 8, candidates=None):
-        """Pick the tile shape of the single-pass step kernel (ELASTIC2, EM2)
-        and the steps per launch (2 or 1) that run this plan's steps fastest.
+        """Pick the tile shape of the single-pass step kernel (ELASTIC2, EM2),
+        the order of its work list and the steps per launch (2 or 1) that run
+        this plan's steps fastest.
 
         Each candidate runs ``steps`` steps as one hipGraph block (captured
         once, then replayed and timed with HIP events on the stream, receiver
         gather included) from the current state; the state, traces and step
         counter are restored afterwards, so the run continues as if nothing
-        had happened.  Returns {(steps_per_launch, wb, rows): ms per step}
-        and keeps the best (``steps`` is rounded up to a multiple of 4, so
+        had happened.  Returns {(steps_per_launch, wb, rows, order): ms per
+        step} and keeps the best (``steps`` is rounded up to a multiple of 4, so
         both modes end a block with the state in the same buffer); the
         ``AUTOTUNE_RETIME`` fastest shapes are timed again over three replays
         and keep their fastest time.  Other kinds have a single shape:
@@ -289,10 +312,9 @@ class Simulation:
                 if traces:
                     b["trace"].copy_(t0)
 
-        def timed(spl, wb, rows, reps):
+        def timed(spl, wb, rows, order, reps):
             # capture this shape, then the fastest of `reps` timed replays
-            self.steps_per_launch = spl
-            self.set_tiles(wb, rows)
+            self.use((spl, wb, rows, order))
             self.run(steps, graph_block=steps)
             restore()
             t = float("inf")
@@ -307,16 +329,15 @@ class Simulation:
             return t
 
         res = {}
-        for spl in spls:
-            for wb, rows in (candidates or self.TILE_CANDIDATES):
-                res[(spl, wb, rows)] = timed(spl, wb, rows, 1)
+        for order in self.TILE_ORDERS:
+            for spl in spls:
+                for wb, rows in (candidates or self.TILE_CANDIDATES):
+                    res[(spl, wb, rows, order)] = timed(spl, wb, rows, order, 1)
         # one replay can order shapes a few per cent apart wrongly: the
         # fastest few are timed again, three replays each
         for key in sorted(res, key=res.get)[:self.AUTOTUNE_RETIME]:
             res[key] = min(res[key], timed(*key, 3))
-        best = min(res, key=res.get)
-        self.steps_per_launch = best[0]
-        self.set_tiles(*best[1:])
+        self.use(min(res, key=res.get))
         torch.cuda.synchronize(self.device)
         return res
 

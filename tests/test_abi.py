@@ -96,9 +96,11 @@ def test_adjoint_calls_reject_null_plan():
     assert L.sdt_plan_run_trail(None, 0, 1, 0, None) == _hip.SDT_EINVAL
     assert L.sdt_adjoint_run(None, 1, 0, 0, 0, None) == _hip.SDT_EINVAL
     assert L.sdt_plan_set_tiles(None, 4, 32) == _hip.SDT_EINVAL
+    assert L.sdt_plan_set_tile_order(None, 1) == _hip.SDT_EINVAL
     assert L.sdt_plan_set_steps_per_launch(None, 2) == _hip.SDT_EINVAL
     v = C.c_int(0)
     assert L.sdt_plan_get_steps_per_launch(None, C.byref(v)) == _hip.SDT_EINVAL
+    assert L.sdt_plan_get_tile_order(None, C.byref(v)) == _hip.SDT_EINVAL
 
 
 def _model(nx=20, nz=20):

@@ -7,6 +7,7 @@ import os
 
 import numpy as np
 import pytest
+import torch
 
 from golden.make_golden import CASES, case_problem
 from helpers import rel_l2, tiny_problem
@@ -291,10 +292,39 @@ def test_tile_shapes_and_autotune_keep_results(oracle_lib):
     s = _sim(probs, "fp32")
     s.run(7)
     res = s.autotune(steps=4)
-    assert len(res) == 2 * len(type(s).TILE_CANDIDATES) and (s.steps_per_launch, *s.tiles()) in res
+    n = 2 * len(type(s).TILE_ORDERS) * len(type(s).TILE_CANDIDATES)
+    assert len(res) == n and (s.steps_per_launch, *s.tiles(), s.tile_order) in res
     got = s.run(graph_block=10).check().traces()
     for a, b in zip(ref, got):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind", ["elastic2", "em2"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_tile_orders_keep_results(kind, precision):
+    """Both work-list orders (tiles dealt to the XCDs in turn, or column
+    strips per XCD) give bit-identical states and traces, for one and two
+    steps per launch, several tile shapes, two models and two shots each;
+    an order other than 0 or 1 is refused."""
+    probs = [tiny_problem(kind, seed=120 + m, nt=44, nshots=2, **d)
+             for m, d in enumerate([dict(nx=300, nz=75, npml=6), dict(nx=57, nz=45, npml=5)])]
+
+    def snap(s):
+        torch.cuda.synchronize()
+        return [b["state"].cpu().numpy().copy() for _, _, b in s.shots], s.traces()
+
+    ref = snap(_sim(probs, precision).run(graph_block=10).check())
+    for spl, wb, rows in ((1, 4, 1), (1, 2, 8), (2, 4, 16), (2, 1, 4)):
+        s = _sim(probs, precision)
+        s.steps_per_launch = spl
+        s.set_tiles(wb, rows)
+        s.tile_order = 1
+        assert s.tile_order == 1
+        got = snap(s.run(graph_block=10).check())
+        for a, b in zip(ref[0] + ref[1], got[0] + got[1]):
+            np.testing.assert_array_equal(a, b)
+    with pytest.raises(RuntimeError):
+        s.tile_order = 2
 
 
 @pytest.mark.parametrize("kind", ["elastic2", "em2", "elastic25", "em25"])
